@@ -1,0 +1,169 @@
+/* sparlab_gpu.h - the C ABI of the MI355X element-and-assembly path.
+ *
+ * This library replaces, behind plain pointers and sizes, the element loop of
+ * sparlab_form's Newton iteration:
+ *   - the incompatible-mode Hex8 element evaluation
+ *       detail::incompatible_elastoplastic_element
+ *       (reference src/fem/IncompatibleModes.cpp:577-600, dispatched from
+ *       src/fem/Elastoplastic.cpp:226-235, declared at
+ *       include/sparlab/fem/Elastoplastic.hpp:196-200), for a whole batch of
+ *       elements at once;
+ *   - the element loop and gather of NonlinearSystem::evaluate
+ *       (src/fem/NonlinearSystem.hpp:192-268) and the sparse assembly of
+ *       Assembler::build_pattern / scatter / assemble_elementwise
+ *       (src/fem/Assembler.cpp:86-215, 271-292);
+ *   - the penalty node-to-tool contact of ToolContact::evaluate
+ *       (src/fem/RigidTool.cpp:398-555) and add_node_block
+ *       (src/fem/Forming.cpp:280-297).
+ *
+ * Conventions (the reference's, include/sparlab/core/Types.hpp:31-48):
+ * FP64 throughout; node-major DOFs {ux, uy, uz} (DOF = 3 node + k); VTK
+ * hexahedron node order; Voigt order {11, 22, 33, 12, 23, 31} with engineering
+ * shear strains and tensorial stresses.
+ *
+ * Device pointers: every pointer argument of a *_device entry point is HIP
+ * device memory on the current device. The caller owns all memory.
+ *
+ * Errors: every entry point returns 0 on success or a negative
+ * SPARLAB_E_* code; sparlab_last_error() returns the message of the calling
+ * thread's last failure. Per-element failures are reported per element in a
+ * status array (SPARLAB_ELEMENT_* codes), never as a return code of the call,
+ * as the reference records a failure per element and raises one SolverError
+ * after its element loop (src/fem/NonlinearSystem.hpp:231-238).
+ */
+#ifndef SPARLAB_GPU_H
+#define SPARLAB_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes of the calls. */
+#define SPARLAB_OK 0
+#define SPARLAB_E_ARGUMENT (-1)   /* ConfigError / ModelError in the reference */
+#define SPARLAB_E_DEVICE (-2)     /* a HIP call failed */
+#define SPARLAB_E_UNSUPPORTED (-3)
+#define SPARLAB_E_SOLVER (-4)     /* SolverError (an element or a contact node failed) */
+#define SPARLAB_E_MESH (-5)       /* MeshError */
+
+/* Per-element status codes (status arrays). */
+#define SPARLAB_ELEMENT_OK 0
+#define SPARLAB_ELEMENT_CENTRE_JACOBIAN 1   /* MeshError: det J <= 0 at the centre */
+#define SPARLAB_ELEMENT_POINT_JACOBIAN 2    /* MeshError: det J <= 0 at a point */
+#define SPARLAB_ELEMENT_INVERTED 3          /* SolverError: det F <= 0 */
+#define SPARLAB_ELEMENT_DEGENERATE 4        /* SolverError: 1 + 2 mu <= 0 or non-finite E */
+#define SPARLAB_ELEMENT_RETURN_FAILED 5     /* SolverError: the return did not converge */
+#define SPARLAB_ELEMENT_SINGULAR_MODES 6    /* SolverError: K_aa singular */
+#define SPARLAB_ELEMENT_NOT_CONVERGED 7     /* SolverError: 25 local iterations */
+#define SPARLAB_ELEMENT_NO_DESCENT 8        /* SolverError: no step lowers |r_alpha| */
+#define SPARLAB_ELEMENT_NONFINITE_TRIAL 9   /* SolverError: non-finite trial stress */
+#define SPARLAB_ELEMENT_INTERNAL 10
+
+/* Kinematics (include/sparlab/fem/Elastoplastic.hpp: Kinematics). */
+#define SPARLAB_SMALL_STRAIN 0
+#define SPARLAB_FINITE 1
+#define SPARLAB_FINITE_LOGARITHMIC 2
+
+/* Material: Material + PlasticityParameters
+ * (include/sparlab/material/Material.hpp:148-257). yield_matrix is found by
+ * sparlab_material_finalize (Material::set_plasticity). */
+typedef struct sparlab_material {
+  double youngs_modulus;              /* E [Pa] */
+  double poisson_ratio;               /* nu */
+  double yield_stress;                /* sigma_y0 [Pa]; 0: elastic */
+  double hardening_modulus;           /* H, linear isotropic [Pa] */
+  double kinematic_hardening_modulus; /* H_kin, Prager [Pa] */
+  double saturation_stress;           /* Q, Voce [Pa] */
+  double saturation_rate;             /* delta, Voce [-] */
+  int32_t criterion;                  /* 0 von Mises, 1 Hill48 */
+  int32_t num_backstresses;           /* Chaboche terms, 0..4 */
+  double backstress_modulus[4];       /* C_i [Pa] */
+  double backstress_recovery[4];      /* gamma_i [-] */
+  int32_t kinematic_integration;      /* 0 exponential, 1 backward Euler */
+  int32_t hill_calibration;           /* 0 r-values, 1 stress ratios, 2 coefficients */
+  double hill_r[3];                   /* r0, r45, r90 */
+  double hill_stress_ratio[3];        /* sigma45/sigma0, sigma90/sigma0, sigma_b/sigma0 */
+  double hill_shear_lm[2];            /* L, M of the r-value and ratio calibrations */
+  double hill_coefficients[6];        /* F, G, H, L, M, N (in for 2, out otherwise) */
+  double rolling_direction[3];
+  double sheet_normal[3];
+  double yield_matrix[36];            /* out: P, tensorial Voigt, global axes */
+} sparlab_material;
+
+/* Integration rule: the n x n x n Gauss cube, or the n x n x m box rule
+ * through thickness_axis (src/elements/Hex8Kernels.hpp:125-130). */
+typedef struct sparlab_rule {
+  int32_t stiffness_points;  /* n: 1..4 (2 by default) */
+  int32_t thickness_points;  /* m: 0 (= n) or 1..7 */
+  int32_t thickness_axis;    /* 0 xi, 1 eta, 2 zeta */
+} sparlab_rule;
+
+/* Validates the material and fills yield_matrix and hill_coefficients
+ * (replaces Material::set_plasticity, src/material/Material.cpp:221-271 with
+ * derive_hill :135-217). */
+int sparlab_material_finalize(sparlab_material* material);
+
+/* Points of the rule (elastoplastic_points, src/fem/Elastoplastic.cpp:222-224),
+ * or a negative code. The element kernels take rules of up to 8 points. */
+int sparlab_rule_points(const sparlab_rule* rule);
+
+/* Doubles of history per integration point in the library's layout: plastic
+ * strain (6), kinematic history (6 per 6-vector: the radial return's back
+ * stress, or the general return's backstresses), equivalent plastic strain,
+ * loading flag. */
+int sparlab_state_fields(const sparlab_material* material);
+
+/* A batch of elements on a mesh (device pointers).
+ * coords: 3 num_nodes; conn: 8 num_elements (int32, VTK order); u: 3 num_nodes.
+ * state_in / state_out: fields x (num_elements x points), structure of
+ *   arrays: field f of point q of element e at [f * num_elements * points +
+ *   e * points + q] (the committed and the trial history,
+ *   src/fem/NonlinearSystem.hpp:328).
+ * alpha_in: committed mode parameters, 9 per element (may be null: zero);
+ * alpha_out: 9 per element.
+ * force: 24 per element; tangent: 576 per element, row-major (may be null for
+ *   a residual-only evaluation); energy: per element; yielding, iterations,
+ *   status: int32 per element; fail_value: per element (the refused Jacobian
+ *   determinant for SPARLAB_ELEMENT_*_JACOBIAN). */
+typedef struct sparlab_element_batch {
+  const double* coords;
+  const int32_t* conn;
+  const double* u;
+  int64_t num_nodes;
+  int64_t num_elements;
+  const double* state_in;
+  const double* alpha_in;
+  double* state_out;
+  double* alpha_out;
+  double* force;
+  double* tangent;
+  double* energy;
+  int32_t* yielding;
+  int32_t* iterations;
+  int32_t* status;
+  double* fail_value;
+} sparlab_element_batch;
+
+/* Evaluates every element of the batch (replaces the calls of
+ * incompatible_elastoplastic_element in the element loop of
+ * src/fem/NonlinearSystem.hpp:208-237; want_tangent as there). Asynchronous
+ * on `stream` (a hipStream_t; null: the default stream). Returns 0 once the
+ * work is queued; per-element outcomes are in status. */
+int sparlab_elements_evaluate_device(const sparlab_material* material, const sparlab_rule* rule,
+                                     int kinematics, int want_tangent,
+                                     const sparlab_element_batch* batch, void* stream);
+
+/* The message of the calling thread's last failed call. */
+const char* sparlab_last_error(void);
+
+/* The library version string. */
+const char* sparlab_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPARLAB_GPU_H */

@@ -1,0 +1,342 @@
+// The C ABI of include/sparlab_gpu.h: host-side validation, the material's
+// derived data, the integration rule, and the element-kernel launches.
+#include "sparlab_gpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "element_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& message) {
+  g_error = message;
+  return code;
+}
+
+bool finite_positive(double v) { return std::isfinite(v) && v > 0.0; }
+
+// Gauss-Legendre abscissae and weights on [-1, 1]: the closed forms for 1..4
+// points and Newton's iteration on P_n for 5..7
+// (src/elements/Quadrature.cpp:16-48, 100-139).
+std::vector<std::pair<double, double>> gauss_line(int n) {
+  switch (n) {
+    case 1: return {{0.0, 2.0}};
+    case 2: {
+      const double a = 1.0 / std::sqrt(3.0);
+      return {{-a, 1.0}, {a, 1.0}};
+    }
+    case 3: {
+      const double a = std::sqrt(3.0 / 5.0);
+      return {{-a, 5.0 / 9.0}, {0.0, 8.0 / 9.0}, {a, 5.0 / 9.0}};
+    }
+    case 4: {
+      const double r = std::sqrt(6.0 / 5.0);
+      const double inner = std::sqrt((3.0 - 2.0 * r) / 7.0);
+      const double outer = std::sqrt((3.0 + 2.0 * r) / 7.0);
+      const double s30 = std::sqrt(30.0);
+      return {{-outer, (18.0 - s30) / 36.0},
+              {-inner, (18.0 + s30) / 36.0},
+              {inner, (18.0 + s30) / 36.0},
+              {outer, (18.0 - s30) / 36.0}};
+    }
+    default: break;
+  }
+  std::vector<std::pair<double, double>> out(static_cast<std::size_t>(n));
+  const double pi = std::acos(-1.0);
+  auto legendre = [n](double x, double& pn, double& pm) {
+    double p0 = 1.0, p1 = x;
+    for (int k = 2; k <= n; ++k) {
+      const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+      p0 = p1;
+      p1 = p2;
+    }
+    pn = p1;
+    pm = p0;
+  };
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    double x = std::cos(pi * (i + 0.75) / (n + 0.5));
+    for (int it = 0; it < 100; ++it) {
+      double pn, pm;
+      legendre(x, pn, pm);
+      const double dp = n * (x * pn - pm) / (x * x - 1.0);
+      const double step = pn / dp;
+      x -= step;
+      if (std::abs(step) <= 1.0e-16) break;
+    }
+    if (2 * i + 1 == n) x = 0.0;
+    double pn, pm;
+    legendre(x, pn, pm);
+    const double dp = n * (x * pn - pm) / (x * x - 1.0);
+    const double w = 2.0 / ((1.0 - x * x) * dp * dp);
+    out[static_cast<std::size_t>(i)] = {-x, w};
+    out[static_cast<std::size_t>(n - 1 - i)] = {x, w};
+  }
+  return out;
+}
+
+// The Hex8 stiffness rule with xi fastest, then eta, then zeta
+// (src/elements/Quadrature.cpp:72-93, 165-219).
+int build_rule(const sparlab_rule& r, sbk::RuleArg& out, std::string& why) {
+  const int n = r.stiffness_points;
+  if (n < 1 || n > 4) { why = "stiffness_points must be 1..4"; return -1; }
+  const int m = (r.thickness_points <= 0 || r.thickness_points == n) ? n : r.thickness_points;
+  if (m < 1 || m > 7) { why = "thickness_points must be 0 or 1..7"; return -1; }
+  if (r.thickness_axis < 0 || r.thickness_axis > 2) { why = "thickness_axis must be 0, 1 or 2"; return -1; }
+  const auto plane = gauss_line(n);
+  const auto through = gauss_line(m);
+  const auto& rx = r.thickness_axis == 0 ? through : plane;
+  const auto& ry = r.thickness_axis == 1 ? through : plane;
+  const auto& rz = r.thickness_axis == 2 ? through : plane;
+  const std::size_t total = rx.size() * ry.size() * rz.size();
+  if (total > static_cast<std::size_t>(sbk::kMaxPoints)) {
+    why = "the element kernels take integration rules of at most 8 points (got " + std::to_string(total) + ")";
+    return -2;
+  }
+  int q = 0;
+  for (const auto& gk : rz)
+    for (const auto& gj : ry)
+      for (const auto& gi : rx) {
+        out.xi[q] = gi.first;
+        out.eta[q] = gj.first;
+        out.zeta[q] = gk.first;
+        out.w[q] = gi.second * gj.second * gk.second;
+        ++q;
+      }
+  out.n = q;
+  return 0;
+}
+
+int kinematic_terms(const sparlab_material& m) {
+  return m.num_backstresses + (m.kinematic_hardening_modulus > 0.0 ? 1 : 0);
+}
+bool general_law(const sparlab_material& m) { return m.criterion == 1 || m.num_backstresses > 0; }
+int kin_fields(const sparlab_material& m) { return general_law(m) ? kinematic_terms(m) : 1; }
+
+sbk::DevMaterial device_material(const sparlab_material& m) {
+  sbk::DevMaterial d{};
+  d.shear = m.youngs_modulus / (2.0 * (1.0 + m.poisson_ratio));
+  const double lame = m.youngs_modulus * m.poisson_ratio / ((1.0 + m.poisson_ratio) * (1.0 - 2.0 * m.poisson_ratio));
+  d.bulk = lame + 2.0 * d.shear / 3.0;
+  d.sy0 = m.yield_stress;
+  d.hard = m.hardening_modulus;
+  d.hkin = m.kinematic_hardening_modulus;
+  d.qsat = m.saturation_stress;
+  d.delta = m.saturation_rate;
+  std::memcpy(d.P, m.yield_matrix, sizeof(d.P));
+  d.general = general_law(m) ? 1 : 0;
+  d.terms = kinematic_terms(m);
+  for (int i = 0; i < d.terms; ++i) {
+    d.mod[i] = i < m.num_backstresses ? m.backstress_modulus[i] : m.kinematic_hardening_modulus;
+    d.rec[i] = i < m.num_backstresses ? m.backstress_recovery[i] : 0.0;
+  }
+  d.exponential = m.kinematic_integration == 0 ? 1 : 0;
+  d.enabled = m.yield_stress > 0.0 ? 1 : 0;
+  d.symmetric = 1;
+  for (int i = 0; i < m.num_backstresses; ++i)
+    if (m.backstress_recovery[i] > 0.0) d.symmetric = 0;
+  d.kin_fields = kin_fields(m);
+  return d;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sparlab_last_error(void) { return g_error.c_str(); }
+
+const char* sparlab_version(void) { return "sparlab-gpu 0.1 (gfx950)"; }
+
+int sparlab_material_finalize(sparlab_material* m) {
+  if (m == nullptr) return fail(SPARLAB_E_ARGUMENT, "null material");
+  if (!(m->youngs_modulus > 0.0)) return fail(SPARLAB_E_ARGUMENT, "the material needs a positive Young's modulus");
+  if (!(m->poisson_ratio > -1.0 && m->poisson_ratio < 0.5))
+    return fail(SPARLAB_E_ARGUMENT, "the material needs a Poisson ratio in (-1, 0.5)");
+  const double values[] = {m->yield_stress, m->hardening_modulus, m->kinematic_hardening_modulus,
+                           m->saturation_stress, m->saturation_rate};
+  for (double v : values)
+    if (!std::isfinite(v) || v < 0.0)
+      return fail(SPARLAB_E_ARGUMENT, "the material needs finite, non-negative plasticity parameters");
+  if (m->saturation_stress > 0.0 && !(m->saturation_rate > 0.0))
+    return fail(SPARLAB_E_ARGUMENT, "a Voce saturation stress needs a positive saturation rate");
+  if (m->num_backstresses < 0 || m->num_backstresses > 4)
+    return fail(SPARLAB_E_ARGUMENT, "at most 4 backstresses are supported");
+  for (int i = 0; i < m->num_backstresses; ++i)
+    if (!finite_positive(m->backstress_modulus[i]) || !std::isfinite(m->backstress_recovery[i]) ||
+        m->backstress_recovery[i] < 0.0)
+      return fail(SPARLAB_E_ARGUMENT, "a backstress needs a finite positive modulus and a non-negative recovery");
+  const bool enabled = m->yield_stress > 0.0;
+  if (!enabled && (m->hardening_modulus > 0.0 || m->kinematic_hardening_modulus > 0.0 ||
+                   m->saturation_stress > 0.0 || m->num_backstresses > 0))
+    return fail(SPARLAB_E_ARGUMENT, "hardening parameters need a yield stress");
+  if (!enabled && m->criterion != 0) return fail(SPARLAB_E_ARGUMENT, "a yield criterion needs a yield stress");
+  if (general_law(*m) && kinematic_terms(*m) > 4)
+    return fail(SPARLAB_E_ARGUMENT, "backstresses and Prager's modulus: at most 4 kinematic terms in all");
+  if (m->criterion != 1) {
+    // von Mises: (3/2) |dev xi|^2 (Material.cpp:92-102)
+    std::memset(m->yield_matrix, 0, sizeof(m->yield_matrix));
+    for (int i = 0; i < 3; ++i) {
+      for (int j = 0; j < 3; ++j) m->yield_matrix[i * 6 + j] = -0.5;
+      m->yield_matrix[i * 6 + i] = 1.0;
+      m->yield_matrix[(3 + i) * 6 + 3 + i] = 3.0;
+    }
+    return SPARLAB_OK;
+  }
+  // Hill48 (Material.cpp:135-217)
+  double F, G, H, L, M, N;
+  if (m->hill_calibration == 0) {
+    const double r0 = m->hill_r[0], r45 = m->hill_r[1], r90 = m->hill_r[2];
+    if (!finite_positive(r0) || !finite_positive(r45) || !finite_positive(r90))
+      return fail(SPARLAB_E_ARGUMENT, "the Hill48 r-values must be finite and positive");
+    G = 1.0 / (1.0 + r0);
+    H = r0 / (1.0 + r0);
+    F = r0 / (r90 * (1.0 + r0));
+    N = (r0 + r90) * (1.0 + 2.0 * r45) / (2.0 * r90 * (1.0 + r0));
+    L = m->hill_shear_lm[0];
+    M = m->hill_shear_lm[1];
+  } else if (m->hill_calibration == 1) {
+    const double s45 = m->hill_stress_ratio[0], s90 = m->hill_stress_ratio[1], sb = m->hill_stress_ratio[2];
+    if (!finite_positive(s45) || !finite_positive(s90) || !finite_positive(sb))
+      return fail(SPARLAB_E_ARGUMENT, "the Hill48 yield stress ratios must be finite and positive");
+    const double a = 1.0 / (s90 * s90), b = 1.0 / (sb * sb);
+    F = 0.5 * (a + b - 1.0);
+    G = 0.5 * (1.0 - a + b);
+    H = 0.5 * (1.0 + a - b);
+    N = 0.5 * (4.0 / (s45 * s45) - F - G);
+    L = m->hill_shear_lm[0];
+    M = m->hill_shear_lm[1];
+  } else {
+    F = m->hill_coefficients[0];
+    G = m->hill_coefficients[1];
+    H = m->hill_coefficients[2];
+    L = m->hill_coefficients[3];
+    M = m->hill_coefficients[4];
+    N = m->hill_coefficients[5];
+  }
+  for (double v : {F, G, H, L, M, N})
+    if (!std::isfinite(v)) return fail(SPARLAB_E_ARGUMENT, "the Hill48 coefficients must be finite");
+  if (!(L > 0.0 && M > 0.0 && N > 0.0) || !(F + H > 0.0) || !(F * G + G * H + H * F > 0.0))
+    return fail(SPARLAB_E_ARGUMENT, "the Hill48 coefficients do not make a convex yield surface");
+  const double* rdi = m->rolling_direction;
+  const double* ndi = m->sheet_normal;
+  const double nn = std::sqrt(ndi[0] * ndi[0] + ndi[1] * ndi[1] + ndi[2] * ndi[2]);
+  const double rn = std::sqrt(rdi[0] * rdi[0] + rdi[1] * rdi[1] + rdi[2] * rdi[2]);
+  if (!(nn > 0.0) || !(rn > 0.0) || !std::isfinite(nn) || !std::isfinite(rn))
+    return fail(SPARLAB_E_ARGUMENT, "the rolling direction and the sheet normal must be finite non-zero vectors");
+  double nd[3], rd[3];
+  for (int i = 0; i < 3; ++i) { nd[i] = ndi[i] / nn; rd[i] = rdi[i] / rn; }
+  const double c = rd[0] * nd[0] + rd[1] * nd[1] + rd[2] * nd[2];
+  for (int i = 0; i < 3; ++i) rd[i] -= c * nd[i];
+  const double r2 = std::sqrt(rd[0] * rd[0] + rd[1] * rd[1] + rd[2] * rd[2]);
+  if (!(r2 > 1.0e-6)) return fail(SPARLAB_E_ARGUMENT, "the rolling direction is parallel to the sheet normal");
+  for (int i = 0; i < 3; ++i) rd[i] /= r2;
+  const double td[3] = {nd[1] * rd[2] - nd[2] * rd[1], nd[2] * rd[0] - nd[0] * rd[2], nd[0] * rd[1] - nd[1] * rd[0]};
+  const double q[3][3] = {{rd[0], rd[1], rd[2]}, {td[0], td[1], td[2]}, {nd[0], nd[1], nd[2]}};
+  double pm[6][6] = {};
+  pm[0][0] = G + H;
+  pm[1][1] = F + H;
+  pm[2][2] = F + G;
+  pm[0][1] = pm[1][0] = -H;
+  pm[1][2] = pm[2][1] = -F;
+  pm[0][2] = pm[2][0] = -G;
+  pm[3][3] = 2.0 * N;
+  pm[4][4] = 2.0 * L;
+  pm[5][5] = 2.0 * M;
+  // Voigt map of sigma' = q sigma q^T, column k = the image of basis tensor k
+  static const int vi[6] = {0, 1, 2, 0, 1, 2};
+  static const int vj[6] = {0, 1, 2, 1, 2, 0};
+  double t[6][6];
+  for (int k = 0; k < 6; ++k) {
+    double e[3][3] = {};
+    e[vi[k]][vj[k]] = 1.0;
+    e[vj[k]][vi[k]] = 1.0;
+    double qe[3][3], r[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) qe[i][j] = q[i][0] * e[0][j] + q[i][1] * e[1][j] + q[i][2] * e[2][j];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) r[i][j] = qe[i][0] * q[j][0] + qe[i][1] * q[j][1] + qe[i][2] * q[j][2];
+    for (int row = 0; row < 6; ++row) t[row][k] = r[vi[row]][vj[row]];
+  }
+  double tp[6][6];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 6; ++k) s += t[k][i] * pm[k][j];
+      tp[i][j] = s;
+    }
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 6; ++k) s += tp[i][k] * t[k][j];
+      m->yield_matrix[i * 6 + j] = s;
+    }
+  const double coeffs[6] = {F, G, H, L, M, N};
+  std::memcpy(m->hill_coefficients, coeffs, sizeof(coeffs));
+  return SPARLAB_OK;
+}
+
+int sparlab_rule_points(const sparlab_rule* rule) {
+  if (rule == nullptr) return fail(SPARLAB_E_ARGUMENT, "null rule");
+  sbk::RuleArg r;
+  std::string why;
+  const int st = build_rule(*rule, r, why);
+  if (st == -1) return fail(SPARLAB_E_ARGUMENT, why);
+  if (st == -2) return fail(SPARLAB_E_UNSUPPORTED, why);
+  return r.n;
+}
+
+int sparlab_state_fields(const sparlab_material* m) {
+  if (m == nullptr) return fail(SPARLAB_E_ARGUMENT, "null material");
+  return sbk::state_fields(kin_fields(*m));
+}
+
+int sparlab_elements_evaluate_device(const sparlab_material* material, const sparlab_rule* rule, int kinematics,
+                                     int want_tangent, const sparlab_element_batch* b, void* stream) {
+  if (material == nullptr || rule == nullptr || b == nullptr) return fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (kinematics < 0 || kinematics > 2) return fail(SPARLAB_E_ARGUMENT, "kinematics must be 0, 1 or 2");
+  if (b->num_elements < 0 || b->num_nodes < 0) return fail(SPARLAB_E_ARGUMENT, "negative sizes");
+  if (b->num_elements == 0) return SPARLAB_OK;
+  if (b->coords == nullptr || b->conn == nullptr || b->u == nullptr || b->state_in == nullptr ||
+      b->state_out == nullptr || b->alpha_out == nullptr || b->force == nullptr || b->energy == nullptr ||
+      b->yielding == nullptr || b->iterations == nullptr || b->status == nullptr || b->fail_value == nullptr)
+    return fail(SPARLAB_E_ARGUMENT, "a required buffer of the element batch is null");
+  if (want_tangent && b->tangent == nullptr) return fail(SPARLAB_E_ARGUMENT, "want_tangent needs a tangent buffer");
+  if (kinematic_terms(*material) > 4) return fail(SPARLAB_E_ARGUMENT, "at most 4 kinematic terms");
+  sbk::ElementArgs a{};
+  std::string why;
+  const int rs = build_rule(*rule, a.rule, why);
+  if (rs == -1) return fail(SPARLAB_E_ARGUMENT, why);
+  if (rs == -2) return fail(SPARLAB_E_UNSUPPORTED, why);
+  a.coords = b->coords;
+  a.conn = b->conn;
+  a.u = b->u;
+  a.ne = b->num_elements;
+  a.kinematics = kinematics;
+  a.mat = device_material(*material);
+  a.state_in = b->state_in;
+  a.state_stride = b->num_elements * a.rule.n;
+  a.alpha_in = b->alpha_in;
+  a.alpha_out = b->alpha_out;
+  a.state_out = b->state_out;
+  a.force = b->force;
+  a.tangent = want_tangent ? b->tangent : nullptr;
+  a.energy = b->energy;
+  a.yielding = b->yielding;
+  a.iterations = b->iterations;
+  a.status = b->status;
+  a.fail_value = b->fail_value;
+  // The local solve writes every element's status; the condensation reads it.
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (sbk::launch_local_solve(a, s) != 0) return fail(SPARLAB_E_DEVICE, "the local-solve kernel did not launch");
+  if (sbk::launch_condense(a, want_tangent != 0, s) != 0)
+    return fail(SPARLAB_E_DEVICE, "the condensation kernel did not launch");
+  return SPARLAB_OK;
+}
+
+}  // extern "C"

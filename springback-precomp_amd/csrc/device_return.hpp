@@ -1,0 +1,527 @@
+// Device material-point returns (gfx950, FP64): the J2 radial return and the
+// closest-point return of Hill48 with Chaboche/Prager backstresses, each with
+// its consistent tangent, for one integration point per lane.
+//
+// Algorithm: src/material/Plasticity.cpp:66-171 (return_3d) and :185-471
+// (GeneralReturn, return_general_3d); the same tolerances and iteration
+// limits (Voce Newton: 60 iterations, 1e-13 radius, fallback 8 eps |xi_tr|;
+// general: 50 Newton iterations, 40 halvings, tol max(1e-13 sy, 32 eps scale),
+// stagnation within 4 tol, one polishing step).
+#pragma once
+
+#include "device_math.hpp"
+
+namespace sbk {
+
+// History of one point, in registers.
+struct PointState {
+  double ep[6];
+  double kin[kMaxTerms][6];  // radial return: kin[0] = beta; general: alpha_i
+  double abar;
+  bool loading;
+};
+
+struct ReturnOut {
+  double stress[6];
+  double C[6][6];
+  double energy;
+  bool yielding;
+};
+
+enum ReturnFail { kReturnOk = 0, kReturnNotConverged = 5, kReturnNonFinite = 9 };
+
+__device__ __forceinline__ double tensor_norm(const double v[6]) {
+  return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + 2.0 * (v[3] * v[3] + v[4] * v[4] + v[5] * v[5]));
+}
+__device__ __forceinline__ double dev_norm2(const double e[6]) {
+  const double mean = (e[0] + e[1] + e[2]) / 3.0;
+  double out = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) out = out + (e[i] - mean) * (e[i] - mean);
+#pragma unroll
+  for (int i = 3; i < 6; ++i) out = out + 0.5 * e[i] * e[i];
+  return out;
+}
+__device__ __forceinline__ double strain_norm2(const double e[6]) {
+  double out = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) out = out + e[i] * e[i];
+#pragma unroll
+  for (int i = 3; i < 6; ++i) out = out + 0.5 * e[i] * e[i];
+  return out;
+}
+__device__ __forceinline__ void elastic_C(double k, double g, double C[6][6]) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) C[i][j] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) C[i][j] = k - 2.0 * g / 3.0;
+    C[i][i] = k + 4.0 * g / 3.0;
+    C[3 + i][3 + i] = g;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// J2 radial return (Plasticity.cpp:66-171); the state is updated in place.
+// ---------------------------------------------------------------------------
+__device__ int return_radial(const DevMaterial& m, const double strain[6], PointState& st, ReturnOut& out) {
+  const double g = m.shear, k = m.bulk, hk = m.hkin;
+  const double sqrt23 = 0.816496580927726;  // sqrt(2/3), rounded as std::sqrt(2.0 / 3.0)
+  double el[6], s_tr[6], xi[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) el[i] = strain[i] - st.ep[i];
+  const double vol = el[0] + el[1] + el[2];
+  const double mean = vol / 3.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) s_tr[i] = 2.0 * g * (el[i] - mean);
+#pragma unroll
+  for (int i = 3; i < 6; ++i) s_tr[i] = g * el[i];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) xi[i] = s_tr[i] - st.kin[0][i];
+  const double xn = tensor_norm(xi);
+  const double an = st.abar;
+  const double rn = sqrt23 * yield_stress(m, an);
+  const double f_tr = xn - rn;
+  const bool was_loading = st.loading;
+  st.loading = false;
+  if (!m.enabled || f_tr <= 1.0e-12 * rn) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) out.stress[i] = s_tr[i] + (i < 3 ? k * vol : 0.0);
+    elastic_C(k, g, out.C);
+    out.energy = 0.5 * k * vol * vol + g * dev_norm2(el) + isotropic_energy(m, an) + hk / 3.0 * strain_norm2(st.ep);
+    out.yielding = false;
+    if (m.enabled && was_loading && f_tr > -1.0e-10 * rn && xn > 0.0) {
+      double n[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) n[i] = xi[i] / xn;
+      const double fac = 2.0 * g / (1.0 + (yield_slope(m, an) + hk) / (3.0 * g));
+#pragma unroll
+      for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) out.C[i][j] = out.C[i][j] - fac * (n[i] * n[j]);
+      st.loading = true;
+    }
+    return kReturnOk;
+  }
+  const double stiff = 2.0 * g + 2.0 / 3.0 * hk;
+  double dg = f_tr / (stiff + 2.0 / 3.0 * yield_slope(m, an));
+  if (m.qsat != 0.0) {
+    dg = 0.0;
+    bool conv = false;
+    for (int it = 0; it < 60; ++it) {
+      const double a = an + sqrt23 * dg;
+      const double res = xn - stiff * dg - sqrt23 * yield_stress(m, a);
+      if (fabs(res) <= 1.0e-13 * rn) { conv = true; break; }
+      dg = dg + res / (stiff + 2.0 / 3.0 * yield_slope(m, a));
+    }
+    if (!conv) {
+      const double a = an + sqrt23 * dg;
+      const double res = xn - stiff * dg - sqrt23 * yield_stress(m, a);
+      conv = fabs(res) <= 8.0 * kEps * xn;
+    }
+    if (!conv) return kReturnNotConverged;
+  }
+  const double alpha = an + sqrt23 * dg;
+  double n[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) n[i] = xi[i] / xn;
+  out.yielding = true;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) out.stress[i] = s_tr[i] - 2.0 * g * dg * n[i] + (i < 3 ? k * vol : 0.0);
+  st.loading = true;
+  st.abar = alpha;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) st.kin[0][i] = st.kin[0][i] + 2.0 / 3.0 * hk * dg * n[i];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) st.ep[i] = st.ep[i] + (i < 3 ? 1.0 : 2.0) * dg * n[i];
+  const double theta = 1.0 - 2.0 * g * dg / xn;
+  const double theta_bar = 1.0 / (1.0 + (yield_slope(m, alpha) + hk) / (3.0 * g)) - (1.0 - theta);
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) out.C[i][j] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) out.C[i][j] = k - 2.0 * g * theta / 3.0;
+    out.C[i][i] = k + 4.0 * g * theta / 3.0;
+    out.C[3 + i][3 + i] = g * theta;
+  }
+  const double f2 = 2.0 * g * theta_bar;
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) out.C[i][j] = out.C[i][j] - f2 * (n[i] * n[j]);
+  double eln[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) eln[i] = el[i] - (i < 3 ? 1.0 : 2.0) * dg * n[i];
+  out.energy = 0.5 * k * vol * vol + g * dev_norm2(eln) + isotropic_energy(m, alpha) + hk / 3.0 * strain_norm2(st.ep);
+  return kReturnOk;
+}
+
+// ---------------------------------------------------------------------------
+// The general closest-point return (Plasticity.cpp:246-471)
+// ---------------------------------------------------------------------------
+struct GenPoint {
+  double eq;        // sigma_bar(xi)
+  double m[6];      // P xi / sigma_bar (engineering)
+  double b, db;
+  double back_rate[6];
+  double res[7];
+};
+
+__device__ __forceinline__ void decay(const DevMaterial& mat, int i, double dl, double& theta, double& chi,
+                                      double& dtheta, double& dchi) {
+  const double gm = mat.rec[i];
+  if (gm == 0.0) {
+    theta = 1.0; chi = dl; dtheta = 0.0; dchi = 1.0;
+    return;
+  }
+  if (mat.exponential) {
+    theta = exp(-gm * dl);
+    chi = -expm1(-gm * dl) / gm;
+    dtheta = -gm * theta;
+    dchi = theta;
+  } else {
+    theta = 1.0 / (1.0 + gm * dl);
+    chi = theta * dl;
+    dtheta = -gm * theta * theta;
+    dchi = theta * theta;
+  }
+}
+
+__device__ __forceinline__ void p_times(const DevMaterial& m, const double x[6], double out[6]) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double s = m.P[i * 6] * x[0];
+#pragma unroll
+    for (int j = 1; j < 6; ++j) s = s + m.P[i * 6 + j] * x[j];
+    out[i] = s;
+  }
+}
+
+__device__ __forceinline__ bool gen_evaluate(const DevMaterial& mat, const PointState& c, const double s_tr[6],
+                                             const double xi[6], double dl, GenPoint& e) {
+  double px[6];
+  p_times(mat, xi, px);
+  double q = xi[0] * px[0];
+#pragma unroll
+  for (int i = 1; i < 6; ++i) q = q + xi[i] * px[i];
+  if (!(q > 0.0) || !isfinite(q)) return false;
+  e.eq = sqrt(q);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) e.m[i] = px[i] / e.eq;
+  double xt[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) { xt[i] = s_tr[i]; e.back_rate[i] = 0.0; }
+  e.b = 2.0 * mat.shear * dl;
+  e.db = 2.0 * mat.shear;
+#pragma unroll
+  for (int t = 0; t < kMaxTerms; ++t) {
+    if (t >= mat.terms) break;
+    double th, chi, dth, dchi;
+    decay(mat, t, dl, th, chi, dth, dchi);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) xt[i] = xt[i] - th * c.kin[t][i];
+    e.b = e.b + 2.0 / 3.0 * mat.mod[t] * chi;
+    e.db = e.db + 2.0 / 3.0 * mat.mod[t] * dchi;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) e.back_rate[i] = e.back_rate[i] + dth * c.kin[t][i];
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) e.res[i] = xi[i] - xt[i] + e.b * (i < 3 ? e.m[i] : e.m[i] * 0.5);
+  e.res[6] = e.eq - yield_stress(mat, c.abar + dl);
+  return true;
+}
+
+// flow_curvature: S^-1 (P - m m^T)
+__device__ __forceinline__ double flow_curv(const DevMaterial& mat, const GenPoint& e, int i, int j) {
+  const double v = mat.P[i * 6 + j] - e.m[i] * e.m[j];
+  return i < 3 ? v : v * 0.5;
+}
+
+__device__ __forceinline__ void gen_jacobian(const DevMaterial& mat, const PointState& c, const GenPoint& e,
+                                             double dl, double J[7][7]) {
+  const double r = e.b / e.eq;
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) J[i][k] = ((i == k) ? 1.0 : 0.0) + r * flow_curv(mat, e, i, k);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) J[i][6] = e.back_rate[i] + e.db * (i < 3 ? e.m[i] : e.m[i] * 0.5);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) J[6][k] = e.m[k];
+  J[6][6] = -yield_slope(mat, c.abar + dl);
+}
+
+// Partial-pivot LU of a 7 x 7 matrix in registers (first largest |a_ik| is
+// the pivot). Row swaps use selects so no register array is indexed at run time.
+__device__ __forceinline__ void lu7(double a[7][7], int piv[7]) {
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    int br = k;
+    double big = fabs(a[k][k]);
+#pragma unroll
+    for (int i = k + 1; i < 7; ++i) {
+      const double v = fabs(a[i][k]);
+      if (v > big) { big = v; br = i; }
+    }
+    piv[k] = br;
+    if (big != 0.0) {
+#pragma unroll
+      for (int i = k + 1; i < 7; ++i) {
+        if (i == br) {
+#pragma unroll
+          for (int j = 0; j < 7; ++j) { const double t = a[k][j]; a[k][j] = a[i][j]; a[i][j] = t; }
+        }
+      }
+#pragma unroll
+      for (int i = k + 1; i < 7; ++i) a[i][k] = a[i][k] / a[k][k];
+    }
+#pragma unroll
+    for (int i = k + 1; i < 7; ++i)
+#pragma unroll
+      for (int j = k + 1; j < 7; ++j) a[i][j] = a[i][j] - a[i][k] * a[k][j];
+  }
+}
+template <int NR>
+__device__ __forceinline__ void lu7_solve(const double a[7][7], const int piv[7], double b[7][NR]) {
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+#pragma unroll
+    for (int i = k + 1; i < 7; ++i) {
+      if (i == piv[k]) {
+#pragma unroll
+        for (int j = 0; j < NR; ++j) { const double t = b[k][j]; b[k][j] = b[i][j]; b[i][j] = t; }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+#pragma unroll
+    for (int i = 1; i < 7; ++i)
+#pragma unroll
+      for (int k = 0; k < i; ++k) b[i][j] = b[i][j] - a[i][k] * b[k][j];
+#pragma unroll
+    for (int i = 6; i >= 0; --i) {
+#pragma unroll
+      for (int k = i + 1; k < 7; ++k) b[i][j] = b[i][j] - a[i][k] * b[k][j];
+      b[i][j] = b[i][j] / a[i][i];
+    }
+  }
+}
+
+__device__ __forceinline__ double norm7(const double r[7]) {
+  double s = r[0] * r[0];
+#pragma unroll
+  for (int i = 1; i < 7; ++i) s = s + r[i] * r[i];
+  return sqrt(s);
+}
+__device__ __forceinline__ double maxabs7(const double r[7]) {
+  double s = fabs(r[0]);
+#pragma unroll
+  for (int i = 1; i < 7; ++i) s = fmax(s, fabs(r[i]));
+  return s;
+}
+
+// C_alg = C^e - 2G [m_t a^T + (dl / sigma_bar) S^-1 (P - m m^T) A],
+// [A; a^T] = J^-1 [D_dev; 0] (Plasticity.cpp:302-311).
+__device__ __forceinline__ void gen_tangent(const DevMaterial& mat, const PointState& c, const GenPoint& e,
+                                            double dl, double C[6][6]) {
+  const double g = mat.shear;
+  double J[7][7];
+  gen_jacobian(mat, c, e, dl, J);
+  int piv[7];
+  lu7(J, piv);
+  double da[7][6];
+#pragma unroll
+  for (int i = 0; i < 7; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) da[i][j] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) da[i][j] = -2.0 * g / 3.0;
+    da[i][i] = 4.0 * g / 3.0;
+    da[3 + i][3 + i] = g;
+  }
+  lu7_solve<6>(J, piv, da);
+  elastic_C(mat.bulk, g, C);
+  const double g2 = 2.0 * g;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const double mt = i < 3 ? e.m[i] : e.m[i] * 0.5;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) C[i][j] = C[i][j] - g2 * (mt * da[6][j]);
+  }
+  if (dl > 0.0) {
+    const double s = 2.0 * g * dl / e.eq;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        double acc = flow_curv(mat, e, i, 0) * da[0][j];
+#pragma unroll
+        for (int k = 1; k < 6; ++k) acc = acc + flow_curv(mat, e, i, k) * da[k][j];
+        C[i][j] = C[i][j] - s * acc;
+      }
+  }
+}
+
+__device__ __forceinline__ double kinematic_energy(const DevMaterial& mat, const PointState& st) {
+  double out = 0.0;
+#pragma unroll
+  for (int t = 0; t < kMaxTerms; ++t) {
+    if (t >= mat.terms) break;
+    const double nrm = tensor_norm(st.kin[t]);
+    out = out + 0.75 * nrm * nrm / mat.mod[t];
+  }
+  return out;
+}
+
+// return_general_3d; the state is updated in place (want_tangent = true, as
+// IncompatibleModes.cpp:232 asks for it on every pass).
+__device__ int return_general(const DevMaterial& mat, const double strain[6], PointState& st, ReturnOut& out) {
+  const double g = mat.shear, k = mat.bulk;
+  double el[6], s_tr[6], xi_tr[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) el[i] = strain[i] - st.ep[i];
+  const double vol = el[0] + el[1] + el[2];
+  const double mean = vol / 3.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) s_tr[i] = 2.0 * g * (el[i] - mean);
+#pragma unroll
+  for (int i = 3; i < 6; ++i) s_tr[i] = g * el[i];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) xi_tr[i] = s_tr[i];
+#pragma unroll
+  for (int t = 0; t < kMaxTerms; ++t) {
+    if (t >= mat.terms) break;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) xi_tr[i] = xi_tr[i] - st.kin[t][i];
+  }
+  const double an = st.abar;
+  const double sy_n = yield_stress(mat, an);
+  double px[6];
+  p_times(mat, xi_tr, px);
+  double q = xi_tr[0] * px[0];
+#pragma unroll
+  for (int i = 1; i < 6; ++i) q = q + xi_tr[i] * px[i];
+  const double eq_tr = sqrt(fmax(q, 0.0));
+  const double f_tr = eq_tr - sy_n;
+  if (!isfinite(f_tr)) return kReturnNonFinite;
+  const bool was_loading = st.loading;
+  st.loading = false;
+  if (!mat.enabled || f_tr <= 1.0e-12 * sy_n) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) out.stress[i] = s_tr[i] + (i < 3 ? k * vol : 0.0);
+    elastic_C(k, g, out.C);
+    out.energy = 0.5 * k * vol * vol + g * dev_norm2(el) + isotropic_energy(mat, an) + kinematic_energy(mat, st);
+    out.yielding = false;
+    if (mat.enabled && was_loading && f_tr > -1.0e-10 * sy_n && eq_tr > 0.0) {
+      st.loading = true;
+      GenPoint e;
+      if (gen_evaluate(mat, st, s_tr, xi_tr, 0.0, e)) gen_tangent(mat, st, e, 0.0, out.C);
+    }
+    return kReturnOk;
+  }
+  double xi[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) xi[i] = xi_tr[i];
+  double dl = 0.0;
+  GenPoint e;
+  gen_evaluate(mat, st, s_tr, xi, dl, e);
+  double scale = sy_n;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) scale = fmax(scale, fabs(s_tr[i]));
+#pragma unroll
+  for (int i = 0; i < 6; ++i) scale = fmax(scale, fabs(xi_tr[i]));
+  const double eps_scale = kEps * scale;
+  const double tol = fmax(1.0e-13 * sy_n, 32.0 * eps_scale);
+  bool conv = false;
+  for (int it = 0; it < 50; ++it) {
+    const double r_inf = maxabs7(e.res);
+    if (r_inf <= tol) { conv = true; break; }
+    double J[7][7];
+    gen_jacobian(mat, st, e, dl, J);
+    int piv[7];
+    lu7(J, piv);
+    double step[7][1];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) step[i][0] = -e.res[i];
+    lu7_solve<1>(J, piv, step);
+    const double r0 = norm7(e.res);
+    double t = 1.0;
+    bool accepted = false;
+    for (int ls = 0; ls < 40; ++ls) {
+      double xt[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) xt[i] = xi[i] + t * step[i][0];
+      const double dlt = fmax(dl + t * step[6][0], 0.0);
+      GenPoint tr;
+      if (gen_evaluate(mat, st, s_tr, xt, dlt, tr) && norm7(tr.res) <= (1.0 - 1.0e-4 * t) * r0) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) xi[i] = xt[i];
+        dl = dlt;
+        e = tr;
+        accepted = true;
+        break;
+      }
+      t = t * 0.5;
+    }
+    if (!accepted) { conv = r_inf <= 4.0 * tol; break; }
+  }
+  if (!conv) return kReturnNotConverged;
+  if (maxabs7(e.res) > 4.0 * eps_scale) {
+    double J[7][7];
+    gen_jacobian(mat, st, e, dl, J);
+    int piv[7];
+    lu7(J, piv);
+    double step[7][1];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) step[i][0] = -e.res[i];
+    lu7_solve<1>(J, piv, step);
+    double xp[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) xp[i] = xi[i] + step[i][0];
+    const double dlp = fmax(dl + step[6][0], 0.0);
+    GenPoint pol;
+    if (gen_evaluate(mat, st, s_tr, xp, dlp, pol) && maxabs7(pol.res) < maxabs7(e.res)) {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) xi[i] = xp[i];
+      dl = dlp;
+      e = pol;
+    }
+  }
+  out.yielding = true;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const double mt = i < 3 ? e.m[i] : e.m[i] * 0.5;
+    out.stress[i] = s_tr[i] - 2.0 * g * dl * mt + (i < 3 ? k * vol : 0.0);
+  }
+  // tangent from the committed backstresses (the Jacobian's back_rate uses alpha_n)
+  gen_tangent(mat, st, e, dl, out.C);
+  st.loading = true;
+  st.abar = an + dl;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) st.ep[i] = st.ep[i] + dl * e.m[i];
+#pragma unroll
+  for (int t = 0; t < kMaxTerms; ++t) {
+    if (t >= mat.terms) break;
+    double th, chi, dth, dchi;
+    decay(mat, t, dl, th, chi, dth, dchi);
+    const double f = 2.0 / 3.0 * mat.mod[t] * chi;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) st.kin[t][i] = th * st.kin[t][i] + f * (i < 3 ? e.m[i] : e.m[i] * 0.5);
+  }
+  double eln[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) eln[i] = el[i] - dl * e.m[i];
+  out.energy = 0.5 * k * vol * vol + g * dev_norm2(eln) + isotropic_energy(mat, st.abar) + kinematic_energy(mat, st);
+  return kReturnOk;
+}
+
+}  // namespace sbk

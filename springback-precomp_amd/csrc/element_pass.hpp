@@ -1,0 +1,260 @@
+// One integration point per lane: the element set-up and one pass of the
+// incompatible-mode Hex8 at the mode parameters alpha.
+//
+// Reference: src/fem/IncompatibleModes.cpp:134-192 (set_up), :195-222
+// (kinematics_at), :227-247 (respond), :252-275 (accumulate_local);
+// src/elements/Hex8Incompatible.cpp:73-83 (mode gradients).
+#pragma once
+
+#include "device_return.hpp"
+
+namespace sbk {
+
+enum Kin { kSmall = 0, kFinite = 1, kLog = 2 };
+
+// Failure codes written per element (0 = success).
+enum ElementStatus {
+  kOk = 0,
+  kCentreJacobian = 1,     // MeshError: det J <= 0 at the centre
+  kPointJacobian = 2,      // MeshError: det J <= 0 at a point
+  kInverted = 3,           // SolverError: det F <= 0 (logarithmic kinematics)
+  kDegenerate = 4,         // SolverError: non-finite E or 1 + 2 mu <= 0
+  kReturnFailed = 5,       // SolverError: the return did not converge
+  kSingularModes = 6,      // SolverError: K_aa singular
+  kLocalNotConverged = 7,  // SolverError: 25 local iterations
+  kNoDescent = 8,          // SolverError: no step lowers |r_alpha|
+  kNonFiniteTrial = 9,     // SolverError: non-finite trial stress
+  kUnsupported = 10,       // the launch does not fit the kernel
+};
+
+struct Rule8_unused {
+  double xi[kMaxPoints], eta[kMaxPoints], zeta[kMaxPoints], w[kMaxPoints];
+  int n;
+};
+
+// The alpha-independent data of the lane's point.
+struct PointGeom {
+  double g[3][11];  // [G | G~]
+  double hu[3][3];  // U G^T
+  double w;         // w_q det J
+};
+
+__device__ __forceinline__ void load_element(const double* __restrict__ coords, const int* __restrict__ conn,
+                                             const double* __restrict__ u, long long e, double x0[8][3],
+                                             double ue[24]) {
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const long long n = conn[e * 8 + a];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      x0[a][k] = coords[n * 3 + k];
+      ue[3 * a + k] = u[n * 3 + k];
+    }
+  }
+}
+
+// Returns 0 or kCentreJacobian / kPointJacobian (fail_value: the determinant).
+__device__ __forceinline__ int setup_point(const double x0[8][3], const double ue[24], double xi, double eta,
+                                           double zeta, double wq, PointGeom& pg, double& fail_value) {
+  double dn0[8][3];
+  shape_gradients(0.0, 0.0, 0.0, dn0);
+  double j0[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double s = x0[0][i] * dn0[0][j];
+#pragma unroll
+      for (int a = 1; a < 8; ++a) s = s + x0[a][i] * dn0[a][j];
+      j0[i][j] = s;
+    }
+  const double det0 = det3(j0);
+  if (!(det0 > 0.0)) { fail_value = det0; return kCentreJacobian; }
+  double ji0[3][3];
+  inverse3(j0, ji0);
+  double dn[8][3];
+  shape_gradients(xi, eta, zeta, dn);
+  double jq[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double s = x0[0][i] * dn[0][j];
+#pragma unroll
+      for (int a = 1; a < 8; ++a) s = s + x0[a][i] * dn[a][j];
+      jq[i][j] = s;
+    }
+  const double det = det3(jq);
+  if (!(det > 0.0)) { fail_value = det; return kPointJacobian; }
+  double ji[3][3];
+  inverse3(jq, ji);
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) pg.g[i][a] = dn[a][0] * ji[0][i] + dn[a][1] * ji[1][i] + dn[a][2] * ji[2][i];
+  const double ratio = det0 / det;
+  const double nat[3] = {xi, eta, zeta};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double f = ratio * -2.0 * nat[k];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) pg.g[i][8 + k] = f * ji0[k][i];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double s = ue[i] * pg.g[j][0];
+#pragma unroll
+      for (int a = 1; a < 8; ++a) s = s + ue[3 * a + i] * pg.g[j][a];
+      pg.hu[i][j] = s;
+    }
+  pg.w = wq * det;
+  return kOk;
+}
+
+__device__ __forceinline__ void load_state(const DevMaterial& m, const double* __restrict__ state, long long stride,
+                                           long long q, PointState& st) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i) st.ep[i] = state[i * stride + q];
+#pragma unroll
+  for (int t = 0; t < kMaxTerms; ++t) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) st.kin[t][i] = (t < m.kin_fields) ? state[(6 + 6 * t + i) * stride + q] : 0.0;
+  }
+  st.abar = state[(6 + 6 * m.kin_fields) * stride + q];
+  st.loading = state[(7 + 6 * m.kin_fields) * stride + q] != 0.0;
+}
+
+__device__ __forceinline__ void store_state(const DevMaterial& m, double* __restrict__ state, long long stride,
+                                            long long q, const PointState& st) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i) state[i * stride + q] = st.ep[i];
+#pragma unroll
+  for (int t = 0; t < kMaxTerms; ++t) {
+    if (t < m.kin_fields) {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) state[(6 + 6 * t + i) * stride + q] = st.kin[t][i];
+    }
+  }
+  state[(6 + 6 * m.kin_fields) * stride + q] = st.abar;
+  state[(7 + 6 * m.kin_fields) * stride + q] = st.loading ? 1.0 : 0.0;
+}
+
+// B_NL(F, [G | G~]) entry (c, 3a + k) (IncompatibleModes.cpp:110-121).
+__device__ __forceinline__ double bnl(const double F[3][3], const double g[3][11], int c, int a, int k) {
+  switch (c) {
+    case 0: return F[k][0] * g[0][a];
+    case 1: return F[k][1] * g[1][a];
+    case 2: return F[k][2] * g[2][a];
+    case 3: return F[k][0] * g[1][a] + F[k][1] * g[0][a];
+    case 4: return F[k][1] * g[2][a] + F[k][2] * g[1][a];
+    default: return F[k][2] * g[0][a] + F[k][0] * g[2][a];
+  }
+}
+
+// The response of the lane's point at alpha: F (I for small strain), the
+// stress conjugate to B_NL, the tangent between B_NL's, the point's new state,
+// energy density and yielding. Returns 0 or a failure code.
+struct PointResponse {
+  double F[3][3];
+  double h[3][3];
+  double S[6];
+  double mat[6][6];
+  double energy;
+  bool yielding;
+};
+
+template <int KIN>
+__device__ __forceinline__ int point_response(const DevMaterial& m, const PointGeom& pg, const double ue[24],
+                                              const double alpha[9], PointState& st, PointResponse& r) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      r.h[i][j] = pg.hu[i][j] + (alpha[i] * pg.g[j][8] + alpha[3 + i] * pg.g[j][9] + alpha[6 + i] * pg.g[j][10]);
+  double strain[6];
+  LogData L;
+  if constexpr (KIN == kSmall) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) r.F[i][j] = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double s1 = bnl(r.F, pg.g, c, 0, 0) * ue[0];
+#pragma unroll
+      for (int kk = 1; kk < 24; ++kk) s1 = s1 + bnl(r.F, pg.g, c, kk / 3, kk % 3) * ue[kk];
+      double s2 = bnl(r.F, pg.g, c, 8, 0) * alpha[0];
+#pragma unroll
+      for (int kk = 1; kk < 9; ++kk) s2 = s2 + bnl(r.F, pg.g, c, 8 + kk / 3, kk % 3) * alpha[kk];
+      strain[c] = s1 + s2;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) r.F[i][j] = (i == j) ? 1.0 + r.h[i][j] : r.h[i][j];
+    if constexpr (KIN == kLog) {
+      if (!(det3(r.F) > 0.0)) return kInverted;
+    }
+    double e3[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double hth = r.h[0][i] * r.h[0][j] + r.h[1][i] * r.h[1][j] + r.h[2][i] * r.h[2][j];
+        e3[i][j] = 0.5 * ((r.h[i][j] + r.h[j][i]) + hth);
+      }
+    double green[6] = {e3[0][0], e3[1][1], e3[2][2], 2.0 * e3[0][1], 2.0 * e3[1][2], 2.0 * e3[2][0]};
+    if constexpr (KIN == kLog) {
+      const int lf = logarithmic_strain(green, L);
+      if (lf != 0) return kDegenerate;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) strain[i] = L.strain[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) strain[i] = green[i];
+    }
+  }
+  ReturnOut ro;
+  const int rf = m.general ? return_general(m, strain, st, ro) : return_radial(m, strain, st, ro);
+  if (rf == kReturnNotConverged) return kReturnFailed;
+  if (rf == kReturnNonFinite) return kNonFiniteTrial;
+  r.energy = ro.energy;
+  r.yielding = ro.yielding;
+  if constexpr (KIN == kLog) {
+    double curv[6][6];
+    log_push(L, ro.stress, r.S, curv);
+    // P^T C P + T:L
+    double pc[6][6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        double s = L.P[0][i] * ro.C[0][j];
+#pragma unroll
+        for (int k = 1; k < 6; ++k) s = s + L.P[k][i] * ro.C[k][j];
+        pc[i][j] = s;
+      }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        double s = pc[i][0] * L.P[0][j];
+#pragma unroll
+        for (int k = 1; k < 6; ++k) s = s + pc[i][k] * L.P[k][j];
+        r.mat[i][j] = s + curv[i][j];
+      }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) r.S[i] = ro.stress[i];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) r.mat[i][j] = ro.C[i][j];
+  }
+  return kOk;
+}
+
+}  // namespace sbk

@@ -1,0 +1,282 @@
+// 9 x 9 dense algebra of the local Newton iteration, on a team's LDS slot.
+//
+// The reference factors K_aa with Eigen's FullPivLU (IncompatibleModes.cpp:335,
+// 461), tests the symmetric part with LLT (:491), and in its fallbacks uses a
+// 9 x 9 SelfAdjointEigenSolver (:510), partialPivLu (:513) and LLT solves (:527).
+// Here the full-pivot LU is team-parallel (lane p owns column/row k+p of each
+// step, the pivot is a DPP arg-max with the first-in-column-major tie rule),
+// everything else runs redundantly on the team's 8 lanes (identical data and
+// identical control flow, so the redundant LDS writes carry identical values).
+#pragma once
+
+#include "device_math.hpp"
+
+namespace sbk {
+
+struct LuSlot {
+  double a[81];   // row-major; the LU factors after coop_full_lu
+  int rt[9], ct[9];
+  int nonzero;
+  int pad;
+  double maxpivot;
+};
+
+// Eigen's FullPivLU pivoting: the largest |a_ij| of the remaining corner, the
+// first in column-major order on ties; invertible when every pivot exceeds
+// 9 eps times the largest pivot.
+__device__ inline bool coop_full_lu(LuSlot& s, int p) {
+  double maxpivot = 0.0;
+  int nonzero = 9;
+  for (int k = 0; k < 9; ++k) {
+    // candidates: lane p scans column k + p, lane 0 also column k + 8 (k = 0)
+    double best = -1.0;
+    int bidx = 1 << 30;
+    for (int jj = 0; jj < 2; ++jj) {
+      const int j = k + p + 8 * jj;
+      if (j < 9 && (jj == 0 || p == 0)) {
+        for (int i = k; i < 9; ++i) {
+          const double v = fabs(s.a[i * 9 + j]);
+          const int idx = j * 9 + i;
+          if (v > best || (v == best && idx < bidx)) { best = v; bidx = idx; }
+        }
+      }
+    }
+    // team arg-max
+#pragma unroll
+    for (int step = 0; step < 3; ++step) {
+      double ov;
+      int oi;
+      if (step == 0) { ov = dpp_f64<kXor1>(best); oi = dpp_i32<kXor1>(bidx); }
+      else if (step == 1) { ov = dpp_f64<kXor2>(best); oi = dpp_i32<kXor2>(bidx); }
+      else { ov = dpp_f64<kHalfMirror>(best); oi = dpp_i32<kHalfMirror>(bidx); }
+      if (ov > best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
+    }
+    if (best == 0.0) {
+      nonzero = k;
+      if (p == 0)
+        for (int i = k; i < 9; ++i) { s.rt[i] = i; s.ct[i] = i; }
+      break;
+    }
+    if (best > maxpivot) maxpivot = best;
+    const int br = bidx % 9, bc = bidx / 9;
+    if (p == 0) { s.rt[k] = br; s.ct[k] = bc; }
+    // row swap k <-> br: lane p swaps column p (lane 0 also column 8)
+    if (br != k) {
+      for (int jj = 0; jj < 2; ++jj) {
+        const int j = p + 8 * jj;
+        if (j < 9 && (jj == 0 || p == 0)) {
+          const double t = s.a[k * 9 + j];
+          s.a[k * 9 + j] = s.a[br * 9 + j];
+          s.a[br * 9 + j] = t;
+        }
+      }
+    }
+    team_sync();
+    if (bc != k) {
+      for (int ii = 0; ii < 2; ++ii) {
+        const int i = p + 8 * ii;
+        if (i < 9 && (ii == 0 || p == 0)) {
+          const double t = s.a[i * 9 + k];
+          s.a[i * 9 + k] = s.a[i * 9 + bc];
+          s.a[i * 9 + bc] = t;
+        }
+      }
+    }
+    team_sync();
+    if (k < 8) {
+      const int i = k + 1 + p;
+      if (i < 9) {
+        const double l = s.a[i * 9 + k] / s.a[k * 9 + k];
+        s.a[i * 9 + k] = l;
+        for (int j = k + 1; j < 9; ++j) s.a[i * 9 + j] = s.a[i * 9 + j] - l * s.a[k * 9 + j];
+      }
+    }
+    team_sync();
+  }
+  if (p == 0) { s.nonzero = nonzero; s.maxpivot = maxpivot; }
+  team_sync();
+  const double thr = maxpivot * (kEps * 9.0);
+  int rank = 0;
+  for (int i = 0; i < nonzero; ++i)
+    if (fabs(s.a[i * 9 + i]) > thr) ++rank;
+  return rank == 9;
+}
+
+// Solve A x = b with the factors (every lane, redundantly).
+__device__ inline void lu_solve(const LuSlot& s, const double b[9], double x[9]) {
+  double c[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) c[i] = b[i];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int r = s.rt[k];
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i)
+      if (i == r) { const double t = c[k]; c[k] = c[i]; c[i] = t; }
+  }
+#pragma unroll
+  for (int i = 1; i < 9; ++i)
+#pragma unroll
+    for (int k = 0; k < i; ++k) c[i] = c[i] - s.a[i * 9 + k] * c[k];
+#pragma unroll
+  for (int i = 8; i >= 0; --i) {
+#pragma unroll
+    for (int k = i + 1; k < 9; ++k) c[i] = c[i] - s.a[i * 9 + k] * c[k];
+    c[i] = c[i] / s.a[i * 9 + i];
+  }
+#pragma unroll
+  for (int k = 8; k >= 0; --k) {
+    const int q = s.ct[k];
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i)
+      if (i == q) { const double t = c[k]; c[k] = c[i]; c[i] = t; }
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) x[i] = c[i];
+}
+
+// Eigen's unblocked LLT on the lower triangle of w (in place, redundant);
+// false at the first non-positive pivot (the rest of w left as it was).
+__device__ inline bool llt9(double* w) {
+  for (int k = 0; k < 9; ++k) {
+    double x = w[k * 9 + k];
+    if (k > 0) {
+      double s = w[k * 9] * w[k * 9];
+      for (int j = 1; j < k; ++j) s = s + w[k * 9 + j] * w[k * 9 + j];
+      x = x - s;
+    }
+    if (x <= 0.0) return false;
+    x = sqrt(x);
+    w[k * 9 + k] = x;
+    if (k > 0)
+      for (int i = k + 1; i < 9; ++i) {
+        double s = w[i * 9] * w[k * 9];
+        for (int j = 1; j < k; ++j) s = s + w[i * 9 + j] * w[k * 9 + j];
+        w[i * 9 + k] = w[i * 9 + k] - s;
+      }
+    for (int i = k + 1; i < 9; ++i) w[i * 9 + k] = w[i * 9 + k] / x;
+  }
+  return true;
+}
+
+__device__ inline void llt9_solve(const double* w, const double b[9], double x[9]) {
+  double y[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) y[i] = b[i];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+#pragma unroll
+    for (int k = 0; k < i; ++k) y[i] = y[i] - w[i * 9 + k] * y[k];
+    y[i] = y[i] / w[i * 9 + i];
+  }
+#pragma unroll
+  for (int i = 8; i >= 0; --i) {
+#pragma unroll
+    for (int k = i + 1; k < 9; ++k) y[i] = y[i] - w[k * 9 + i] * y[k];
+    y[i] = y[i] / w[i * 9 + i];
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) x[i] = y[i];
+}
+
+// Eigenvalues (ascending) of the symmetric matrix in w, by cyclic Jacobi in
+// place (redundant; the fallback path only).
+__device__ inline void jacobi9_values(double* a, double ev[9]) {
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    bool any = false;
+    for (int p = 0; p < 8; ++p)
+      for (int q = p + 1; q < 9; ++q) {
+        const double apq = a[p * 9 + q];
+        if (apq == 0.0) continue;
+        const double g = 100.0 * fabs(apq);
+        const double app = fabs(a[p * 9 + p]), aqq = fabs(a[q * 9 + q]);
+        if (app + g == app && aqq + g == aqq) {
+          a[p * 9 + q] = 0.0;
+          a[q * 9 + p] = 0.0;
+          continue;
+        }
+        any = true;
+        const double h = a[q * 9 + q] - a[p * 9 + p];
+        double t;
+        if (fabs(h) + g == fabs(h)) {
+          t = apq / h;
+        } else {
+          const double theta = 0.5 * h / apq;
+          t = 1.0 / (fabs(theta) + sqrt(1.0 + theta * theta));
+          if (theta < 0.0) t = -t;
+        }
+        const double c = 1.0 / sqrt(1.0 + t * t);
+        const double s = t * c;
+        const double tau = s / (1.0 + c);
+        a[p * 9 + p] = a[p * 9 + p] - t * apq;
+        a[q * 9 + q] = a[q * 9 + q] + t * apq;
+        a[p * 9 + q] = 0.0;
+        a[q * 9 + p] = 0.0;
+        for (int r = 0; r < 9; ++r) {
+          if (r == p || r == q) continue;
+          const double gr = a[r * 9 + p], hr = a[r * 9 + q];
+          a[r * 9 + p] = gr - s * (hr + gr * tau);
+          a[p * 9 + r] = a[r * 9 + p];
+          a[r * 9 + q] = hr + s * (gr - hr * tau);
+          a[q * 9 + r] = a[r * 9 + q];
+        }
+      }
+    if (!any) break;
+  }
+  for (int i = 0; i < 9; ++i) ev[i] = a[i * 9 + i];
+  for (int i = 1; i < 9; ++i)
+    for (int j = i; j > 0 && ev[j] < ev[j - 1]; --j) {
+      const double t = ev[j];
+      ev[j] = ev[j - 1];
+      ev[j - 1] = t;
+    }
+}
+
+// Partial-pivot LU solve of the 9 x 9 system in w (in place, redundant).
+__device__ inline void plu9_solve(double* a, const double b[9], double x[9]) {
+  int piv[9];
+  for (int k = 0; k < 9; ++k) {
+    int br = k;
+    double big = fabs(a[k * 9 + k]);
+    for (int i = k + 1; i < 9; ++i) {
+      const double v = fabs(a[i * 9 + k]);
+      if (v > big) { big = v; br = i; }
+    }
+    piv[k] = br;
+    if (big != 0.0) {
+      if (br != k)
+        for (int j = 0; j < 9; ++j) {
+          const double t = a[k * 9 + j];
+          a[k * 9 + j] = a[br * 9 + j];
+          a[br * 9 + j] = t;
+        }
+      for (int i = k + 1; i < 9; ++i) a[i * 9 + k] = a[i * 9 + k] / a[k * 9 + k];
+    }
+    if (k < 8)
+      for (int i = k + 1; i < 9; ++i)
+        for (int j = k + 1; j < 9; ++j) a[i * 9 + j] = a[i * 9 + j] - a[i * 9 + k] * a[k * 9 + j];
+  }
+  double c[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) c[i] = b[i];
+  for (int k = 0; k < 9; ++k) {
+    const int r = piv[k];
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+      if (i == r && i > k) { const double t = c[k]; c[k] = c[i]; c[i] = t; }
+  }
+#pragma unroll
+  for (int i = 1; i < 9; ++i)
+#pragma unroll
+    for (int k = 0; k < i; ++k) c[i] = c[i] - a[i * 9 + k] * c[k];
+#pragma unroll
+  for (int i = 8; i >= 0; --i) {
+#pragma unroll
+    for (int k = i + 1; k < 9; ++k) c[i] = c[i] - a[i * 9 + k] * c[k];
+    c[i] = c[i] / a[i * 9 + i];
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) x[i] = c[i];
+}
+
+}  // namespace sbk
