@@ -156,6 +156,132 @@ int sparlab_elements_evaluate_device(const sparlab_material* material, const spa
                                      int kinematics, int want_tangent,
                                      const sparlab_element_batch* batch, void* stream);
 
+/* ------------------------------------------------------------------------
+ * The nonlinear system: the drop-in for detail::NonlinearSystem
+ * (src/fem/NonlinearSystem.hpp:85-334) with the Assembler's cached pattern
+ * (src/fem/Assembler.cpp:86-173) on the device. The system owns device copies
+ * of the mesh, the pattern, the committed and trial history; the caller owns
+ * u and the outputs.
+ * ------------------------------------------------------------------------ */
+typedef struct sparlab_system sparlab_system;
+
+/* The scalars of an evaluation (NonlinearSystem::Evaluation,
+ * src/fem/NonlinearSystem.hpp:27-47). gross = |sum_e |f_e|| (no external load:
+ * f_ext = 0 in the forming analysis). */
+typedef struct sparlab_evaluation {
+  double energy;
+  double gross;
+  int64_t yielding_points;
+  int64_t failed_elements;
+  int64_t first_failed_element;   /* -1 when none */
+  int32_t first_failure_status;   /* SPARLAB_ELEMENT_* */
+  int32_t pad;
+  double first_failure_value;
+} sparlab_evaluation;
+
+/* Builds the system of a Hex8 mesh (host arrays: coords 3 num_nodes,
+ * conn 8 num_elements, VTK order): uploads the mesh, builds the pattern
+ * (Assembler::build_pattern) and its gather maps, and starts the history
+ * virgin (NonlinearSystem::initialise_history, src/fem/NonlinearSystem.hpp:
+ * 273-311). SPARLAB_E_MESH for a node index out of range or repeated. */
+int sparlab_system_create(const double* coords, int64_t num_nodes, const int32_t* conn, int64_t num_elements,
+                          const sparlab_material* material, const sparlab_rule* rule, int kinematics,
+                          sparlab_system** out);
+void sparlab_system_destroy(sparlab_system* system);
+
+/* The pattern: nnz, and host copies of row_ptr (3 num_nodes + 1) and col_idx
+ * (nnz). The matrix is structurally symmetric, so these are also the
+ * reference's column-major outer and inner arrays. */
+int64_t sparlab_system_nnz(const sparlab_system* system);
+int sparlab_system_pattern(const sparlab_system* system, int32_t* row_ptr, int32_t* col_idx);
+/* Per node: the index of the first value of its diagonal 3 x 3 block (the
+ * entries add_node_block writes, src/fem/Forming.cpp:280-297); row k of the
+ * block starts k * (row length) later. */
+int sparlab_system_diagonal_blocks(const sparlab_system* system, int32_t* first_value);
+
+/* NonlinearSystem::evaluate(u, 0, want_tangent) on device pointers: u (3 nn),
+ * fint (3 nn), values (nnz; with a tangent). column_major = 0 writes the
+ * row-major values of K; 1 the reference's column-major values (they differ
+ * only for a non-symmetric K). Asynchronous on stream; the trial history is
+ * kept for sparlab_system_commit. */
+int sparlab_system_evaluate_device(sparlab_system* system, const double* u, int want_tangent, int column_major,
+                                   double* fint, double* values, void* stream);
+/* Assembler::assemble_elementwise / the force gather of NonlinearSystem for
+ * given element arrays (device pointers): fe (24 per element) into fint
+ * (3 nn) and, when gross is non-null, gross[0] = |sum_e |f_e|| (device);
+ * ke (576 per element, row-major) into values (nnz). Either part may be
+ * skipped with null outputs. Every value is the sum of its contributions in
+ * ascending element order, the order of the reference's serial scatter. */
+int sparlab_system_assemble_device(sparlab_system* system, const double* fe, const double* ke, int column_major,
+                                   double* fint, double* values, double* gross, void* stream);
+/* Waits for the last evaluation and reads its scalars. SPARLAB_E_SOLVER when
+ * an element failed (the reference's SolverError after its element loop). */
+int sparlab_system_result(sparlab_system* system, sparlab_evaluation* out, void* stream);
+/* The same from host arrays (includes the PCIe copies). */
+int sparlab_system_evaluate(sparlab_system* system, const double* u, int want_tangent, int column_major,
+                            double* fint, double* values, sparlab_evaluation* out);
+/* NonlinearSystem::commit: the last evaluation's trial history becomes the
+ * committed one (a buffer swap). */
+int sparlab_system_commit(sparlab_system* system);
+/* committed_all / internal_all (trial = 0) or the last trial (trial = 1), in
+ * the library's layout (sparlab_state_fields x points x elements, and 9 per
+ * element); set_committed / set_internal. Host arrays; null skips. */
+int sparlab_system_history(const sparlab_system* system, int trial, double* states, double* alpha);
+int sparlab_system_set_history(sparlab_system* system, const double* states, const double* alpha);
+/* The last evaluation's element forces (24 per element) and tangents (576). */
+int sparlab_system_element_outputs(const sparlab_system* system, double* fe, double* ke);
+
+/* The pattern of a Hex8 mesh on the host, without a device (the reference's
+ * Assembler::build_pattern, src/fem/Assembler.cpp:86-173): returns nnz (or a
+ * negative code); with row_ptr (3 num_nodes + 1) and col_idx (nnz) non-null,
+ * fills them, and block_offset (64 per element, may be null) with the
+ * reference's per-element block offsets. */
+int64_t sparlab_pattern_build(int64_t num_nodes, int64_t num_elements, const int32_t* conn, int32_t* row_ptr,
+                              int32_t* col_idx, int32_t* block_offset);
+
+/* ------------------------------------------------------------------------
+ * Penalty contact of slave nodes with a rigid tool (ToolContact::evaluate,
+ * src/fem/RigidTool.cpp:398-555, for one tool over a list of slave nodes).
+ * ------------------------------------------------------------------------ */
+typedef struct sparlab_tool {
+  int32_t shape;               /* 0 plane, 1 sphere, 2 cylinder */
+  int32_t pad;
+  double centre[3];            /* c(t), the tool's position at the evaluation */
+  double centre0[3];           /* c(t0), the increment's start (nodes new to contact) */
+  double radius;
+  double normal[3];            /* plane */
+  double axis[3];              /* cylinder */
+  double friction;             /* mu (0: frictionless) */
+  double tangential_ratio;     /* kappa_T / kappa */
+} sparlab_tool;
+
+/* Device pointers. coords, u, start: 3 per node (start: u at the start of the
+ * increment). Per slave: nodes (int32), kappa_area (kappa_j A_j,
+ * src/fem/RigidTool.cpp:298-321), the committed friction history
+ * (hist_present int32, hist_force 3, hist_relative 3; hist_present may be
+ * null: no history). Outputs: residual (3 per node) -= the contact forces;
+ * blocks (9 per slave, row-major, with want_tangent); active (int32 per
+ * slave); the trial history (trial_present, trial_force 3, trial_relative 3);
+ * work: 8 doubles and one int32 per slave; scalars (10 doubles): gross,
+ * round_off, resultant[3], max_penetration, slipping, asymmetric blocks,
+ * active, nodes through the tool (> 0: the reference's SolverError). */
+int sparlab_contact_evaluate_device(const sparlab_tool* tool, int friction_symmetric, int want_tangent,
+                                    const double* coords, const double* u, const double* start,
+                                    int64_t num_slaves, const int32_t* nodes, const double* kappa_area,
+                                    const int32_t* hist_present, const double* hist_force,
+                                    const double* hist_relative, double* residual, double* blocks,
+                                    int32_t* active, int32_t* trial_present, double* trial_force,
+                                    double* trial_relative, double* work, int32_t* work_status,
+                                    double* scalars, void* stream);
+
+/* add_node_block (src/fem/Forming.cpp:280-297) for every active slave: its
+ * block into the assembled values at the node's diagonal block
+ * (first_value, row_len per slave: sparlab_system_diagonal_blocks and 3 times
+ * the node's neighbour count); column_major as in sparlab_system_evaluate. */
+int sparlab_add_node_blocks_device(const double* blocks, const int32_t* active, const int32_t* first_value,
+                                   const int32_t* row_len, int64_t num_slaves, int column_major, double* values,
+                                   void* stream);
+
 /* The message of the calling thread's last failed call. */
 const char* sparlab_last_error(void);
 

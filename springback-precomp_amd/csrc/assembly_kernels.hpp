@@ -1,0 +1,37 @@
+// Launch arguments of the assembly kernels (device pointers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace sbk {
+
+struct AssemblyArgs {
+  const double* ke;            // 576 per element, row-major
+  const int* block_ptr;        // num_blocks + 1
+  const int* block_contrib;    // e * 64 + a_local * 8 + b_local
+  const int* block_pos;        // first value of the block's first row
+  const int* block_row_len;    // stride between the block's rows
+  long long num_blocks;
+  int column_major;            // 0: row-major values of K; 1: the reference's column-major values
+  double* values;              // nnz
+};
+
+struct ForceArgs {
+  const double* fe;            // 24 per element
+  const int* node_ptr;         // num_nodes + 1
+  const int* node_incidence;   // e * 8 + a_local
+  long long num_nodes;
+  double* fint;                // 3 num_nodes
+  double* gross;               // 3 num_nodes: sum_e |f_e| per DOF
+};
+
+int launch_assemble_values(const AssemblyArgs& a, hipStream_t s);
+int launch_gather_force(const ForceArgs& a, hipStream_t s);
+// out[0] = sum x (square = 0) or sqrt(sum x^2) (square = 1), fixed-order tree
+// over nparts contiguous slices (nparts <= 1024 partial sums).
+int launch_reduce(const double* x, long long n, int square, double* partial, int nparts, double* out,
+                  hipStream_t s);
+// summary[0] failed elements, [1] lowest failed element, [2] yielding points
+int launch_status_summary(const int* status, const int* yielding, long long n, long long* summary, hipStream_t s);
+
+}  // namespace sbk

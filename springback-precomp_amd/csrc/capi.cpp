@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "contact_kernels.hpp"
 #include "element_kernels.hpp"
 
 namespace {
@@ -146,6 +147,28 @@ sbk::DevMaterial device_material(const sparlab_material& m) {
 }
 
 }  // namespace
+
+namespace sbk {
+
+int capi_fail(int code, const std::string& message) { return fail(code, message); }
+
+// Validates the arguments the element kernels share and fills the launch
+// record's rule and material.
+int capi_prepare_elements(const sparlab_material* material, const sparlab_rule* rule, int kinematics,
+                          ElementArgs& a) {
+  if (material == nullptr || rule == nullptr) return fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (kinematics < 0 || kinematics > 2) return fail(SPARLAB_E_ARGUMENT, "kinematics must be 0, 1 or 2");
+  if (kinematic_terms(*material) > 4) return fail(SPARLAB_E_ARGUMENT, "at most 4 kinematic terms");
+  std::string why;
+  const int rs = build_rule(*rule, a.rule, why);
+  if (rs == -1) return fail(SPARLAB_E_ARGUMENT, why);
+  if (rs == -2) return fail(SPARLAB_E_UNSUPPORTED, why);
+  a.kinematics = kinematics;
+  a.mat = device_material(*material);
+  return SPARLAB_OK;
+}
+
+}  // namespace sbk
 
 extern "C" {
 
@@ -307,18 +330,13 @@ int sparlab_elements_evaluate_device(const sparlab_material* material, const spa
       b->yielding == nullptr || b->iterations == nullptr || b->status == nullptr || b->fail_value == nullptr)
     return fail(SPARLAB_E_ARGUMENT, "a required buffer of the element batch is null");
   if (want_tangent && b->tangent == nullptr) return fail(SPARLAB_E_ARGUMENT, "want_tangent needs a tangent buffer");
-  if (kinematic_terms(*material) > 4) return fail(SPARLAB_E_ARGUMENT, "at most 4 kinematic terms");
   sbk::ElementArgs a{};
-  std::string why;
-  const int rs = build_rule(*rule, a.rule, why);
-  if (rs == -1) return fail(SPARLAB_E_ARGUMENT, why);
-  if (rs == -2) return fail(SPARLAB_E_UNSUPPORTED, why);
+  const int prep = sbk::capi_prepare_elements(material, rule, kinematics, a);
+  if (prep != SPARLAB_OK) return prep;
   a.coords = b->coords;
   a.conn = b->conn;
   a.u = b->u;
   a.ne = b->num_elements;
-  a.kinematics = kinematics;
-  a.mat = device_material(*material);
   a.state_in = b->state_in;
   a.state_stride = b->num_elements * a.rule.n;
   a.alpha_in = b->alpha_in;
@@ -333,9 +351,78 @@ int sparlab_elements_evaluate_device(const sparlab_material* material, const spa
   a.fail_value = b->fail_value;
   // The local solve writes every element's status; the condensation reads it.
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (sbk::launch_local_solve(a, s) != 0) return fail(SPARLAB_E_DEVICE, "the local-solve kernel did not launch");
-  if (sbk::launch_condense(a, want_tangent != 0, s) != 0)
-    return fail(SPARLAB_E_DEVICE, "the condensation kernel did not launch");
+  int err = sbk::launch_local_solve(a, s);
+  if (err != 0)
+    return fail(SPARLAB_E_DEVICE, std::string("the local-solve kernel did not launch: ") +
+                                      hipGetErrorString(static_cast<hipError_t>(err)));
+  err = sbk::launch_condense(a, want_tangent != 0, s);
+  if (err != 0)
+    return fail(SPARLAB_E_DEVICE, std::string("the condensation kernel did not launch: ") +
+                                      hipGetErrorString(static_cast<hipError_t>(err)));
+  return SPARLAB_OK;
+}
+
+int sparlab_contact_evaluate_device(const sparlab_tool* tool, int friction_symmetric, int want_tangent,
+                                    const double* coords, const double* u, const double* start,
+                                    int64_t num_slaves, const int32_t* nodes, const double* kappa_area,
+                                    const int32_t* hist_present, const double* hist_force,
+                                    const double* hist_relative, double* residual, double* blocks,
+                                    int32_t* active, int32_t* trial_present, double* trial_force,
+                                    double* trial_relative, double* work, int32_t* work_status,
+                                    double* scalars, void* stream) {
+  if (tool == nullptr || coords == nullptr || u == nullptr || start == nullptr || residual == nullptr ||
+      active == nullptr || trial_present == nullptr || trial_force == nullptr || trial_relative == nullptr ||
+      work == nullptr || work_status == nullptr || scalars == nullptr || (num_slaves > 0 && (nodes == nullptr || kappa_area == nullptr)))
+    return fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (want_tangent && blocks == nullptr) return fail(SPARLAB_E_ARGUMENT, "want_tangent needs a blocks buffer");
+  if (tool->shape < 0 || tool->shape > 2) return fail(SPARLAB_E_ARGUMENT, "tool shape must be 0, 1 or 2");
+  if (tool->shape != 0 && !(tool->radius > 0.0)) return fail(SPARLAB_E_ARGUMENT, "a sphere or cylinder needs a radius");
+  if (hist_present != nullptr && (hist_force == nullptr || hist_relative == nullptr))
+    return fail(SPARLAB_E_ARGUMENT, "a friction history needs its forces and positions");
+  sbk::ContactArgs a{};
+  a.tool.shape = tool->shape;
+  for (int k = 0; k < 3; ++k) {
+    a.tool.centre[k] = tool->centre[k];
+    a.tool.centre0[k] = tool->centre0[k];
+    a.tool.normal[k] = tool->normal[k];
+    a.tool.axis[k] = tool->axis[k];
+  }
+  a.tool.radius = tool->radius;
+  a.tool.friction = tool->friction;
+  a.tool.tangential_ratio = tool->tangential_ratio;
+  a.friction_symmetric = friction_symmetric;
+  a.want_tangent = want_tangent;
+  a.coords = coords;
+  a.u = u;
+  a.start = start;
+  a.num_slaves = num_slaves;
+  a.nodes = nodes;
+  a.kappa_area = kappa_area;
+  a.hist_present = hist_present;
+  a.hist_force = hist_force;
+  a.hist_relative = hist_relative;
+  a.residual = residual;
+  a.blocks = blocks;
+  a.active = active;
+  a.trial_present = trial_present;
+  a.trial_force = trial_force;
+  a.trial_relative = trial_relative;
+  a.node_terms = work;
+  a.status = work_status;
+  if (sbk::launch_contact(a, scalars, static_cast<hipStream_t>(stream)) != 0)
+    return fail(SPARLAB_E_DEVICE, "the contact kernel did not launch");
+  return SPARLAB_OK;
+}
+
+int sparlab_add_node_blocks_device(const double* blocks, const int32_t* active, const int32_t* first_value,
+                                   const int32_t* row_len, int64_t num_slaves, int column_major, double* values,
+                                   void* stream) {
+  if (num_slaves > 0 && (blocks == nullptr || active == nullptr || first_value == nullptr || row_len == nullptr ||
+                         values == nullptr))
+    return fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (sbk::launch_add_node_blocks(blocks, active, first_value, row_len, num_slaves, column_major, values,
+                                  static_cast<hipStream_t>(stream)) != 0)
+    return fail(SPARLAB_E_DEVICE, "the node-block kernel did not launch");
   return SPARLAB_OK;
 }
 

@@ -1092,6 +1092,7 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
 // Launchers
 // ---------------------------------------------------------------------------
 int launch_local_solve(const ElementArgs& a, hipStream_t stream) {
+  (void)hipGetLastError();
   const unsigned int blocks = static_cast<unsigned int>((a.ne + kChunk - 1) / kChunk);
   if (blocks == 0) return 0;
   switch (a.kinematics) {
@@ -1099,10 +1100,11 @@ int launch_local_solve(const ElementArgs& a, hipStream_t stream) {
     case kFinite: hipLaunchKernelGGL(local_solve_kernel<kFinite>, dim3(blocks), dim3(64), 0, stream, a); break;
     default: hipLaunchKernelGGL(local_solve_kernel<kLog>, dim3(blocks), dim3(64), 0, stream, a); break;
   }
-  return hipGetLastError() == hipSuccess ? 0 : 1;
+  return static_cast<int>(hipGetLastError());
 }
 
 int launch_condense(const ElementArgs& a, bool tangent, hipStream_t stream) {
+  (void)hipGetLastError();
   const unsigned int blocks = static_cast<unsigned int>((a.ne + kWaveTeams - 1) / kWaveTeams);
   if (blocks == 0) return 0;
 #define SBK_LAUNCH(K, T) hipLaunchKernelGGL((condense_kernel<K, T>), dim3(blocks), dim3(64), 0, stream, a)
@@ -1120,7 +1122,7 @@ int launch_condense(const ElementArgs& a, bool tangent, hipStream_t stream) {
     }
   }
 #undef SBK_LAUNCH
-  return hipGetLastError() == hipSuccess ? 0 : 1;
+  return static_cast<int>(hipGetLastError());
 }
 
 }  // namespace sbk

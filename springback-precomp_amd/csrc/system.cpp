@@ -1,0 +1,322 @@
+// The nonlinear system on the device: the drop-in for the element loop of
+// detail::NonlinearSystem (src/fem/NonlinearSystem.hpp:192-268) with its
+// committed history (:119-182) and the Assembler's pattern and scatter
+// (src/fem/Assembler.cpp:86-215, 271-292).
+//
+// Device-resident: the mesh, the pattern and gather maps, the committed and
+// the trial history (swapped on commit), and the per-element outputs that the
+// assembly gathers. One evaluation = the element kernels, the force gather,
+// the value gather (with a tangent) and three small reductions.
+#include "sparlab_gpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "assembly.hpp"
+#include "assembly_kernels.hpp"
+#include "element_kernels.hpp"
+
+namespace sbk {
+int capi_fail(int code, const std::string& message);
+int capi_prepare_elements(const sparlab_material* material, const sparlab_rule* rule, int kinematics,
+                          ElementArgs& a);
+}  // namespace sbk
+
+struct sparlab_system {
+  sparlab_material material;
+  sparlab_rule rule;
+  int kinematics = 2;
+  int points = 0;
+  int fields = 0;
+  int64_t nn = 0, ne = 0;
+  sbk::HostPattern pattern;
+  sbk::ElementArgs args{};
+  // device memory
+  double* coords = nullptr;
+  int32_t* conn = nullptr;
+  double* state[2] = {nullptr, nullptr};  // committed = state[cur], trial = state[1 - cur]
+  double* alpha[2] = {nullptr, nullptr};
+  int cur = 0;
+  double *fe = nullptr, *ke = nullptr, *energy = nullptr, *fail_value = nullptr, *gross = nullptr;
+  int32_t *yielding = nullptr, *iterations = nullptr, *status = nullptr;
+  int32_t *block_ptr = nullptr, *block_contrib = nullptr, *block_pos = nullptr, *block_row_len = nullptr;
+  int32_t *node_ptr = nullptr, *node_incidence = nullptr, *diag_block = nullptr;
+  double* partial = nullptr;    // 1024 partial sums
+  double* scalars = nullptr;    // [0] energy, [1] gross
+  long long* summary = nullptr; // failed count, lowest failed element, yielding points
+  bool evaluated = false;
+  bool last_tangent = false;
+};
+
+namespace {
+
+template <class T>
+bool dalloc(T*& p, std::size_t count) {
+  if (count == 0) count = 1;
+  return hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)) == hipSuccess;
+}
+template <class T>
+bool upload(T*& p, const std::vector<T>& v) {
+  if (!dalloc(p, v.size())) return false;
+  return v.empty() || hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) == hipSuccess;
+}
+template <class T>
+void dfree(T*& p) {
+  if (p) hipFree(p);
+  p = nullptr;
+}
+
+void release(sparlab_system* s) {
+  dfree(s->coords); dfree(s->conn);
+  for (int i = 0; i < 2; ++i) { dfree(s->state[i]); dfree(s->alpha[i]); }
+  dfree(s->fe); dfree(s->ke); dfree(s->energy); dfree(s->fail_value); dfree(s->gross);
+  dfree(s->yielding); dfree(s->iterations); dfree(s->status);
+  dfree(s->block_ptr); dfree(s->block_contrib); dfree(s->block_pos); dfree(s->block_row_len);
+  dfree(s->node_ptr); dfree(s->node_incidence); dfree(s->diag_block);
+  dfree(s->partial); dfree(s->scalars); dfree(s->summary);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sparlab_system_create(const double* coords, int64_t num_nodes, const int32_t* conn, int64_t num_elements,
+                          const sparlab_material* material, const sparlab_rule* rule, int kinematics,
+                          sparlab_system** out) {
+  if (out == nullptr || coords == nullptr || conn == nullptr) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  *out = nullptr;
+  auto* s = new sparlab_system();
+  s->material = *material;
+  s->rule = *rule;
+  s->kinematics = kinematics;
+  s->nn = num_nodes;
+  s->ne = num_elements;
+  int st = sbk::capi_prepare_elements(material, rule, kinematics, s->args);
+  if (st != SPARLAB_OK) { delete s; return st; }
+  s->points = s->args.rule.n;
+  s->fields = sbk::state_fields(s->args.mat.kin_fields);
+  std::string why;
+  if (!sbk::build_pattern(num_nodes, num_elements, conn, s->pattern, &why)) {
+    delete s;
+    return sbk::capi_fail(SPARLAB_E_MESH, why);
+  }
+  const std::size_t ne = static_cast<std::size_t>(num_elements), nn = static_cast<std::size_t>(num_nodes);
+  const std::size_t hist = static_cast<std::size_t>(s->fields) * ne * static_cast<std::size_t>(s->points);
+  bool ok = dalloc(s->coords, 3 * nn) && dalloc(s->conn, 8 * ne) && dalloc(s->state[0], hist) &&
+            dalloc(s->state[1], hist) && dalloc(s->alpha[0], 9 * ne) && dalloc(s->alpha[1], 9 * ne) &&
+            dalloc(s->fe, 24 * ne) && dalloc(s->ke, 576 * ne) && dalloc(s->energy, ne) &&
+            dalloc(s->fail_value, ne) && dalloc(s->gross, 3 * nn) && dalloc(s->yielding, ne) &&
+            dalloc(s->iterations, ne) && dalloc(s->status, ne) && dalloc(s->partial, 1024) &&
+            dalloc(s->scalars, 2) && dalloc(s->summary, 3) && upload(s->block_ptr, s->pattern.block_ptr) &&
+            upload(s->block_contrib, s->pattern.block_contrib) && upload(s->block_pos, s->pattern.block_pos) &&
+            upload(s->block_row_len, s->pattern.block_row_len) && upload(s->node_ptr, s->pattern.node_ptr) &&
+            upload(s->node_incidence, s->pattern.node_incidence) && upload(s->diag_block, s->pattern.diag_block);
+  ok = ok && hipMemcpy(s->coords, coords, 3 * nn * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(s->conn, conn, 8 * ne * sizeof(int32_t), hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemset(s->state[0], 0, hist * sizeof(double)) == hipSuccess &&
+       hipMemset(s->alpha[0], 0, 9 * ne * sizeof(double)) == hipSuccess;
+  if (!ok) {
+    release(s);
+    delete s;
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "device allocation or upload failed");
+  }
+  *out = s;
+  return SPARLAB_OK;
+}
+
+void sparlab_system_destroy(sparlab_system* s) {
+  if (!s) return;
+  release(s);
+  delete s;
+}
+
+int64_t sparlab_system_nnz(const sparlab_system* s) { return s ? static_cast<int64_t>(s->pattern.col_idx.size()) : -1; }
+
+int sparlab_system_pattern(const sparlab_system* s, int32_t* row_ptr, int32_t* col_idx) {
+  if (!s || !row_ptr || !col_idx) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  std::memcpy(row_ptr, s->pattern.row_ptr.data(), s->pattern.row_ptr.size() * sizeof(int32_t));
+  std::memcpy(col_idx, s->pattern.col_idx.data(), s->pattern.col_idx.size() * sizeof(int32_t));
+  return SPARLAB_OK;
+}
+
+int64_t sparlab_pattern_build(int64_t num_nodes, int64_t num_elements, const int32_t* conn, int32_t* row_ptr,
+                              int32_t* col_idx, int32_t* block_offset) {
+  if (conn == nullptr && num_elements > 0) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null connectivity");
+  sbk::HostPattern p;
+  std::string why;
+  if (!sbk::build_pattern(num_nodes, num_elements, conn, p, &why)) return sbk::capi_fail(SPARLAB_E_MESH, why);
+  if (row_ptr && col_idx) {
+    std::memcpy(row_ptr, p.row_ptr.data(), p.row_ptr.size() * sizeof(int32_t));
+    std::memcpy(col_idx, p.col_idx.data(), p.col_idx.size() * sizeof(int32_t));
+  }
+  if (block_offset) std::memcpy(block_offset, p.block_offset.data(), p.block_offset.size() * sizeof(int32_t));
+  return static_cast<int64_t>(p.col_idx.size());
+}
+
+int sparlab_system_diagonal_blocks(const sparlab_system* s, int32_t* first_value) {
+  if (!s || !first_value) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  for (std::size_t n = 0; n < s->pattern.diag_block.size(); ++n)
+    first_value[n] = s->pattern.block_pos[static_cast<std::size_t>(s->pattern.diag_block[n])];
+  return SPARLAB_OK;
+}
+
+int sparlab_system_evaluate_device(sparlab_system* s, const double* u, int want_tangent, int column_major,
+                                   double* fint, double* values, void* stream) {
+  if (!s || !u || !fint) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (want_tangent && !values) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "want_tangent needs a values buffer");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  sbk::ElementArgs a = s->args;
+  a.coords = s->coords;
+  a.conn = s->conn;
+  a.u = u;
+  a.ne = s->ne;
+  a.state_in = s->state[s->cur];
+  a.state_stride = s->ne * s->points;
+  a.alpha_in = s->alpha[s->cur];
+  a.alpha_out = s->alpha[1 - s->cur];
+  a.state_out = s->state[1 - s->cur];
+  a.force = s->fe;
+  a.tangent = want_tangent ? s->ke : nullptr;
+  a.energy = s->energy;
+  a.yielding = s->yielding;
+  a.iterations = s->iterations;
+  a.status = s->status;
+  a.fail_value = s->fail_value;
+  if (sbk::launch_local_solve(a, st) != 0 || sbk::launch_condense(a, want_tangent != 0, st) != 0)
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "element kernel launch failed");
+  sbk::ForceArgs f{s->fe, s->node_ptr, s->node_incidence, static_cast<long long>(s->nn), fint, s->gross};
+  if (sbk::launch_gather_force(f, st) != 0) return sbk::capi_fail(SPARLAB_E_DEVICE, "force gather launch failed");
+  if (want_tangent) {
+    sbk::AssemblyArgs g{s->ke, s->block_ptr, s->block_contrib, s->block_pos, s->block_row_len,
+                        static_cast<long long>(s->pattern.block_pos.size()), column_major ? 1 : 0, values};
+    if (sbk::launch_assemble_values(g, st) != 0) return sbk::capi_fail(SPARLAB_E_DEVICE, "assembly launch failed");
+  }
+  const int parts = static_cast<int>(std::min<int64_t>(1024, (3 * s->nn + 4095) / 4096 + 1));
+  if (sbk::launch_reduce(s->gross, 3 * s->nn, 1, s->partial, parts, s->scalars + 1, st) != 0 ||
+      sbk::launch_reduce(s->energy, s->ne, 0, s->partial, static_cast<int>(std::min<int64_t>(1024, s->ne / 4096 + 1)),
+                         s->scalars, st) != 0 ||
+      sbk::launch_status_summary(s->status, s->yielding, s->ne, s->summary, st) != 0)
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "reduction launch failed");
+  s->evaluated = true;
+  s->last_tangent = want_tangent != 0;
+  return SPARLAB_OK;
+}
+
+int sparlab_system_assemble_device(sparlab_system* s, const double* fe, const double* ke, int column_major,
+                                   double* fint, double* values, double* gross, void* stream) {
+  if (!s || (!fe && fint) || (!ke && values)) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (fint) {
+    sbk::ForceArgs f{fe, s->node_ptr, s->node_incidence, static_cast<long long>(s->nn), fint, s->gross};
+    if (sbk::launch_gather_force(f, st) != 0) return sbk::capi_fail(SPARLAB_E_DEVICE, "force gather launch failed");
+    const int parts = static_cast<int>(std::min<int64_t>(1024, (3 * s->nn + 4095) / 4096 + 1));
+    if (gross && sbk::launch_reduce(s->gross, 3 * s->nn, 1, s->partial, parts, gross, st) != 0)
+      return sbk::capi_fail(SPARLAB_E_DEVICE, "reduction launch failed");
+  }
+  if (values) {
+    sbk::AssemblyArgs g{ke, s->block_ptr, s->block_contrib, s->block_pos, s->block_row_len,
+                        static_cast<long long>(s->pattern.block_pos.size()), column_major ? 1 : 0, values};
+    if (sbk::launch_assemble_values(g, st) != 0) return sbk::capi_fail(SPARLAB_E_DEVICE, "assembly launch failed");
+  }
+  return SPARLAB_OK;
+}
+
+int sparlab_system_result(sparlab_system* s, sparlab_evaluation* out, void* stream) {
+  if (!s || !out) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (!s->evaluated) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "no evaluation to read");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  double sc[2];
+  long long sum[3];
+  if (hipMemcpyAsync(sc, s->scalars, sizeof(sc), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(sum, s->summary, sizeof(sum), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "reading the evaluation failed");
+  out->energy = sc[0];
+  out->gross = sc[1];
+  out->failed_elements = sum[0];
+  out->yielding_points = sum[2];
+  out->first_failed_element = sum[0] > 0 ? sum[1] : -1;
+  out->first_failure_status = 0;
+  out->first_failure_value = 0.0;
+  if (sum[0] > 0) {
+    int32_t code = 0;
+    double v = 0.0;
+    hipMemcpy(&code, s->status + sum[1], sizeof(code), hipMemcpyDeviceToHost);
+    hipMemcpy(&v, s->fail_value + sum[1], sizeof(v), hipMemcpyDeviceToHost);
+    out->first_failure_status = code;
+    out->first_failure_value = v;
+  }
+  return sum[0] > 0 ? SPARLAB_E_SOLVER : SPARLAB_OK;
+}
+
+int sparlab_system_evaluate(sparlab_system* s, const double* u_host, int want_tangent, int column_major,
+                            double* fint_host, double* values_host, sparlab_evaluation* out) {
+  if (!s || !u_host || !fint_host || !out) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  const std::size_t nd = 3 * static_cast<std::size_t>(s->nn);
+  const std::size_t nnz = s->pattern.col_idx.size();
+  double *du = nullptr, *df = nullptr, *dv = nullptr;
+  if (!dalloc(du, nd) || !dalloc(df, nd) || (want_tangent && !dalloc(dv, nnz))) {
+    dfree(du); dfree(df); dfree(dv);
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "device allocation failed");
+  }
+  int rc = SPARLAB_OK;
+  if (hipMemcpy(du, u_host, nd * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    rc = sbk::capi_fail(SPARLAB_E_DEVICE, "upload of u failed");
+  } else {
+    rc = sparlab_system_evaluate_device(s, du, want_tangent, column_major, df, dv, nullptr);
+    if (rc == SPARLAB_OK) {
+      rc = sparlab_system_result(s, out, nullptr);
+      if (hipMemcpy(fint_host, df, nd * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+          (want_tangent && values_host &&
+           hipMemcpy(values_host, dv, nnz * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
+        rc = sbk::capi_fail(SPARLAB_E_DEVICE, "download failed");
+      else if (rc == SPARLAB_E_SOLVER)
+        sbk::capi_fail(SPARLAB_E_SOLVER, "Hex8 element " + std::to_string(out->first_failed_element) +
+                                             " failed with status " + std::to_string(out->first_failure_status));
+    }
+  }
+  dfree(du); dfree(df); dfree(dv);
+  return rc;
+}
+
+int sparlab_system_commit(sparlab_system* s) {
+  if (!s || !s->evaluated) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "commit needs an evaluation");
+  s->cur = 1 - s->cur;
+  s->evaluated = false;
+  return SPARLAB_OK;
+}
+
+int sparlab_system_history(const sparlab_system* s, int trial, double* states, double* alpha) {
+  if (!s) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null system");
+  const int k = trial ? 1 - s->cur : s->cur;
+  const std::size_t hist = static_cast<std::size_t>(s->fields) * s->ne * s->points;
+  if (hipDeviceSynchronize() != hipSuccess) return sbk::capi_fail(SPARLAB_E_DEVICE, "synchronize failed");
+  if ((states && hipMemcpy(states, s->state[k], hist * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) ||
+      (alpha && hipMemcpy(alpha, s->alpha[k], 9 * s->ne * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "download failed");
+  return SPARLAB_OK;
+}
+
+int sparlab_system_set_history(sparlab_system* s, const double* states, const double* alpha) {
+  if (!s) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null system");
+  const std::size_t hist = static_cast<std::size_t>(s->fields) * s->ne * s->points;
+  if ((states && hipMemcpy(s->state[s->cur], states, hist * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) ||
+      (alpha && hipMemcpy(s->alpha[s->cur], alpha, 9 * s->ne * sizeof(double), hipMemcpyHostToDevice) != hipSuccess))
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "upload failed");
+  return SPARLAB_OK;
+}
+
+int sparlab_system_element_outputs(const sparlab_system* s, double* fe, double* ke) {
+  if (!s) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null system");
+  if (hipDeviceSynchronize() != hipSuccess) return sbk::capi_fail(SPARLAB_E_DEVICE, "synchronize failed");
+  if ((fe && hipMemcpy(fe, s->fe, 24 * s->ne * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) ||
+      (ke && hipMemcpy(ke, s->ke, 576 * s->ne * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "download failed");
+  return SPARLAB_OK;
+}
+
+}  // extern "C"

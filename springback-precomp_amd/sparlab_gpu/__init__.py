@@ -93,6 +93,11 @@ def lib():
     """The loaded C-ABI library; raises if it has not been built."""
     global _lib
     if _lib is None:
+        # One HIP runtime per process: torch's libamdhip64 (SONAME
+        # libamdhip64.so.7) is loaded first, so the library's dependency of the
+        # same SONAME binds to it and torch's device pointers and streams are
+        # the library's.
+        import torch  # noqa: F401
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: run springback-precomp_amd/build.py "
                                "(or __graft_entry__.build()) first")
@@ -304,3 +309,245 @@ def raise_first_failure(batch):
     if bad.size:
         e = int(bad[0])
         raise SolverError(f"Hex8 element {e}: {ELEMENT_STATUS.get(int(st[e]), 'unknown')}", e, int(st[e]))
+
+
+# ---------------------------------------------------------------------------
+# Sparsity pattern, nonlinear system, contact
+# ---------------------------------------------------------------------------
+class CEvaluation(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("gross", ctypes.c_double), ("yielding_points", ctypes.c_int64),
+                ("failed_elements", ctypes.c_int64), ("first_failed_element", ctypes.c_int64),
+                ("first_failure_status", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("first_failure_value", ctypes.c_double)]
+
+
+class CTool(ctypes.Structure):
+    _fields_ = [("shape", ctypes.c_int32), ("pad", ctypes.c_int32), ("centre", ctypes.c_double * 3),
+                ("centre0", ctypes.c_double * 3), ("radius", ctypes.c_double), ("normal", ctypes.c_double * 3),
+                ("axis", ctypes.c_double * 3), ("friction", ctypes.c_double),
+                ("tangential_ratio", ctypes.c_double)]
+
+
+def _np_ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def pattern(num_nodes, conn):
+    """The reference's sparsity pattern (host): row_ptr, col_idx, block_offset.
+    Structurally symmetric: also the reference's column-major outer/inner."""
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    L = lib()
+    L.sparlab_pattern_build.restype = ctypes.c_int64
+    L.sparlab_pattern_build.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p]
+    ne = conn.shape[0]
+    nnz = _check(L.sparlab_pattern_build(num_nodes, ne, _np_ptr(conn), None, None, None))
+    row_ptr = np.zeros(3 * num_nodes + 1, dtype=np.int32)
+    col_idx = np.zeros(nnz, dtype=np.int32)
+    bo = np.zeros(64 * ne, dtype=np.int32)
+    _check(L.sparlab_pattern_build(num_nodes, ne, _np_ptr(conn), _np_ptr(row_ptr), _np_ptr(col_idx), _np_ptr(bo)))
+    return row_ptr, col_idx, bo
+
+
+class Evaluation:
+    """NonlinearSystem::Evaluation (src/fem/NonlinearSystem.hpp:27-47) on the host."""
+
+    def __init__(self, internal, values, scalars):
+        self.internal = internal
+        self.external = np.zeros_like(internal)
+        self.residual = internal.copy()
+        self.tangent_values = values
+        self.energy = scalars.energy
+        self.gross = scalars.gross
+        self.yielding_points = scalars.yielding_points
+
+
+class NonlinearSystem:
+    """The device nonlinear system (sparlab_system_*): the element loop, the
+    gather and the assembly of detail::NonlinearSystem with the Assembler's
+    pattern, and the committed history."""
+
+    def __init__(self, coords, conn, material, rule=None, kinematics=FINITE_LOGARITHMIC):
+        import torch  # noqa: F401  (HIP runtime loaded through torch)
+        self.coords = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 3)
+        self.conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, 8)
+        self.material = material
+        self.rule = rule or Rule()
+        self.kinematics = kinematics
+        L = lib()
+        L.sparlab_system_create.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
+                                            ctypes.POINTER(CMaterial), ctypes.POINTER(CRule), ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_void_p)]
+        L.sparlab_system_destroy.argtypes = [ctypes.c_void_p]
+        L.sparlab_system_nnz.restype = ctypes.c_int64
+        L.sparlab_system_nnz.argtypes = [ctypes.c_void_p]
+        for name in ("sparlab_system_pattern",):
+            getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.sparlab_system_diagonal_blocks.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.sparlab_system_evaluate_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.sparlab_system_result.argtypes = [ctypes.c_void_p, ctypes.POINTER(CEvaluation), ctypes.c_void_p]
+        L.sparlab_system_evaluate.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(CEvaluation)]
+        L.sparlab_system_commit.argtypes = [ctypes.c_void_p]
+        L.sparlab_system_history.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.sparlab_system_set_history.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.sparlab_system_element_outputs.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        h = ctypes.c_void_p()
+        _check(L.sparlab_system_create(_np_ptr(self.coords), self.coords.shape[0], _np_ptr(self.conn),
+                                       self.conn.shape[0], ctypes.byref(material.c), ctypes.byref(self.rule.c),
+                                       kinematics, ctypes.byref(h)))
+        self.h = h
+        self.num_nodes = self.coords.shape[0]
+        self.num_elements = self.conn.shape[0]
+        self.points = self.rule.points
+        self.nnz = L.sparlab_system_nnz(h)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            lib().sparlab_system_destroy(h)
+            self.h = None
+
+    def pattern(self):
+        row_ptr = np.zeros(3 * self.num_nodes + 1, dtype=np.int32)
+        col_idx = np.zeros(self.nnz, dtype=np.int32)
+        _check(lib().sparlab_system_pattern(self.h, _np_ptr(row_ptr), _np_ptr(col_idx)))
+        return row_ptr, col_idx
+
+    def diagonal_blocks(self):
+        out = np.zeros(self.num_nodes, dtype=np.int32)
+        _check(lib().sparlab_system_diagonal_blocks(self.h, _np_ptr(out)))
+        return out
+
+    def evaluate(self, u, lam=0.0, want_tangent=True, column_major=False):
+        """NonlinearSystem::evaluate(u, lambda, want_tangent) from host arrays."""
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        fint = np.zeros(3 * self.num_nodes)
+        values = np.zeros(self.nnz) if want_tangent else None
+        ev = CEvaluation()
+        rc = lib().sparlab_system_evaluate(self.h, _np_ptr(u), int(want_tangent), int(column_major), _np_ptr(fint),
+                                           _np_ptr(values), ctypes.byref(ev))
+        if rc == -4:
+            raise SolverError(f"Hex8 element {ev.first_failed_element}: "
+                              f"{ELEMENT_STATUS.get(ev.first_failure_status, 'unknown')}",
+                              ev.first_failed_element, ev.first_failure_status)
+        _check(rc)
+        return Evaluation(fint, values, ev)
+
+    def evaluate_device(self, u, fint, values=None, want_tangent=True, column_major=False, stream=None):
+        """Queues an evaluation on device tensors (u, fint: 3 nn; values: nnz)."""
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream(u.device)
+        _check(lib().sparlab_system_evaluate_device(self.h, ctypes.c_void_p(u.data_ptr()), int(want_tangent),
+                                                    int(column_major), ctypes.c_void_p(fint.data_ptr()),
+                                                    None if values is None else ctypes.c_void_p(values.data_ptr()),
+                                                    ctypes.c_void_p(s.cuda_stream)))
+
+    def assemble_device(self, fe=None, ke=None, fint=None, values=None, gross=None, column_major=False,
+                        stream=None):
+        """Gathers given element arrays (device tensors) into fint / values."""
+        import torch
+        L = lib()
+        L.sparlab_system_assemble_device.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int] + [ctypes.c_void_p] * 4
+        ref = next(t for t in (fe, ke) if t is not None)
+        s = stream if stream is not None else torch.cuda.current_stream(ref.device)
+        p = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+        _check(L.sparlab_system_assemble_device(self.h, p(fe), p(ke), int(column_major), p(fint), p(values), p(gross),
+                                                ctypes.c_void_p(s.cuda_stream)))
+
+    def result(self, stream=None):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        ev = CEvaluation()
+        rc = lib().sparlab_system_result(self.h, ctypes.byref(ev), ctypes.c_void_p(s.cuda_stream))
+        if rc == -4:
+            raise SolverError(f"Hex8 element {ev.first_failed_element}: "
+                              f"{ELEMENT_STATUS.get(ev.first_failure_status, 'unknown')}",
+                              ev.first_failed_element, ev.first_failure_status)
+        _check(rc)
+        return ev
+
+    def commit(self):
+        _check(lib().sparlab_system_commit(self.h))
+
+    def history(self, trial=False):
+        """(states (E, P, 39) reference records, alpha (E, 9))."""
+        nf = self.material.state_fields
+        st = np.zeros((nf, self.num_elements * self.points))
+        al = np.zeros((self.num_elements, 9))
+        _check(lib().sparlab_system_history(self.h, int(trial), _np_ptr(st), _np_ptr(al)))
+        return states_to_reference(st, self.material, self.num_elements, self.points), al
+
+    def set_history(self, states_ref=None, alpha=None):
+        st = None if states_ref is None else np.ascontiguousarray(states_from_reference(states_ref, self.material))
+        al = None if alpha is None else np.ascontiguousarray(alpha, dtype=np.float64)
+        _check(lib().sparlab_system_set_history(self.h, _np_ptr(st), _np_ptr(al)))
+
+    def element_outputs(self):
+        fe = np.zeros((self.num_elements, 24))
+        ke = np.zeros((self.num_elements, 24, 24))
+        _check(lib().sparlab_system_element_outputs(self.h, _np_ptr(fe), _np_ptr(ke)))
+        return fe, ke
+
+
+def make_tool(shape="sphere", centre=(0, 0, 0), centre0=None, radius=0.0, normal=(0, 0, 1), axis=(0, 0, 1),
+              friction=0.0, tangential_ratio=1.0):
+    t = CTool()
+    t.shape = {"plane": 0, "sphere": 1, "cylinder": 2}[shape]
+    centre0 = centre if centre0 is None else centre0
+    for i in range(3):
+        t.centre[i], t.centre0[i], t.normal[i], t.axis[i] = centre[i], centre0[i], normal[i], axis[i]
+    t.radius, t.friction, t.tangential_ratio = radius, friction, tangential_ratio
+    return t
+
+
+def contact_evaluate_device(tool, coords, u, start, nodes, kappa_area, history=None, residual=None,
+                            want_tangent=True, friction_symmetric=False, stream=None):
+    """ToolContact::evaluate for one tool on device tensors. Returns a dict of
+    device tensors (residual, blocks, active, trial history) and host scalars."""
+    import torch
+    dev = coords.device
+    ns = nodes.shape[0]
+    nn = coords.shape[0] if coords.dim() == 2 else coords.shape[0] // 3
+    f64 = dict(dtype=torch.float64, device=dev)
+    i32 = dict(dtype=torch.int32, device=dev)
+    if residual is None:
+        residual = torch.zeros(3 * nn, **f64)
+    out = dict(residual=residual, blocks=torch.zeros((ns, 3, 3), **f64), active=torch.zeros(ns, **i32),
+               trial_present=torch.zeros(ns, **i32), trial_force=torch.zeros((ns, 3), **f64),
+               trial_relative=torch.zeros((ns, 3), **f64))
+    work = torch.zeros((max(ns, 1), 8), **f64)
+    work_status = torch.zeros(max(ns, 1), **i32)
+    scalars = torch.zeros(10, **f64)
+    hp, hf, hr = (None, None, None) if history is None else history
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    L = lib()
+    p = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+    L.sparlab_contact_evaluate_device.argtypes = [ctypes.POINTER(CTool), ctypes.c_int, ctypes.c_int] + \
+        [ctypes.c_void_p] * 3 + [ctypes.c_int64] + [ctypes.c_void_p] * 14 + [ctypes.c_void_p]
+    _check(L.sparlab_contact_evaluate_device(ctypes.byref(tool), int(friction_symmetric), int(want_tangent),
+                                             p(coords), p(u), p(start), ns, p(nodes), p(kappa_area), p(hp), p(hf),
+                                             p(hr), p(residual), p(out["blocks"]), p(out["active"]),
+                                             p(out["trial_present"]), p(out["trial_force"]),
+                                             p(out["trial_relative"]), p(work), p(work_status), p(scalars),
+                                             ctypes.c_void_p(s.cuda_stream)))
+    sc = scalars.cpu().numpy()
+    out.update(gross=sc[0], round_off=sc[1], resultant=sc[2:5], max_penetration=sc[5], n_slipping=int(sc[6]),
+               symmetric=sc[7] == 0, n_active=int(sc[8]), through=int(sc[9]))
+    if out["through"] > 0:
+        raise SolverError("a node lies less than half the radius from the tool's centre or axis: "
+                          "the tool is through the surface")
+    return out
+
+
+def add_node_blocks_device(blocks, active, first_value, row_len, values, column_major=False, stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream(values.device)
+    L = lib()
+    L.sparlab_add_node_blocks_device.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int64, ctypes.c_int,
+                                                                         ctypes.c_void_p, ctypes.c_void_p]
+    _check(L.sparlab_add_node_blocks_device(ctypes.c_void_p(blocks.data_ptr()), ctypes.c_void_p(active.data_ptr()),
+                                            ctypes.c_void_p(first_value.data_ptr()),
+                                            ctypes.c_void_p(row_len.data_ptr()), active.shape[0], int(column_major),
+                                            ctypes.c_void_p(values.data_ptr()), ctypes.c_void_p(s.cuda_stream)))

@@ -91,3 +91,34 @@ def element_arrays(w):
     """Per-element views for the oracle: x0 (ne, 8, 3) and ue (ne, 24)."""
     conn = w["conn"]
     return w["coords"][conn], w["u"][conn].reshape(conn.shape[0], 24)
+
+
+def structured_hex_mesh(nx, ny, nz, lx, ly, lz, x0=0.0, y0=0.0, z0=0.0):
+    """The reference's make_structured_hex_mesh (src/mesh/StructuredMesh.cpp:62-113):
+    node n = k nnx nny + j nnx + i, elements x fastest, VTK corner order."""
+    nnx, nny, nnz = nx + 1, ny + 1, nz + 1
+    dx, dy, dz = lx / nx, ly / ny, lz / nz
+    k, j, i = np.meshgrid(np.arange(nnz), np.arange(nny), np.arange(nnx), indexing="ij")
+    coords = np.stack([x0 + i.ravel() * dx, y0 + j.ravel() * dy, z0 + k.ravel() * dz], axis=1).astype(np.float64)
+    layer = nnx * nny
+    ek, ej, ei = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    n0 = (ek * layer + ej * nnx + ei).ravel()
+    conn = np.stack([n0, n0 + 1, n0 + nnx + 1, n0 + nnx,
+                     n0 + layer, n0 + layer + 1, n0 + layer + nnx + 1, n0 + layer + nnx], axis=1).astype(np.int32)
+    return coords, conn
+
+
+def assembly_workload(nx=100, ny=100, nz=4, seed=99, contact_nodes=10000):
+    """The coloured-assembly + contact workload: a structured nx x ny x nz Hex8
+    plate with random FP64 element forces and non-symmetric tangents (seeded),
+    and penalty contact of top-surface nodes with a sphere R = 5 mm at random
+    seeded offsets (SI units: a 0.1 x 0.1 x 0.004 m plate)."""
+    rng = np.random.default_rng(seed)
+    coords, conn = structured_hex_mesh(nx, ny, nz, 0.1, 0.1, 0.004, -0.05, -0.05, -0.004)
+    ne = conn.shape[0]
+    fe = rng.normal(size=(ne, 24))
+    ke = rng.normal(size=(ne, 24, 24))
+    nnx, nny = nx + 1, ny + 1
+    top = np.arange(nz * nnx * nny, (nz + 1) * nnx * nny, dtype=np.int32)
+    nodes = np.sort(rng.choice(top, size=min(contact_nodes, top.size), replace=False)).astype(np.int32)
+    return dict(coords=coords, conn=conn, fe=fe, ke=ke, contact_nodes=nodes)

@@ -1,0 +1,133 @@
+"""The C ABI without a device: the library loads and exports every symbol
+include/sparlab_gpu.h declares; the host-side parts (material derivation,
+integration rules, state layout, the sparsity pattern) against the oracle.
+
+The pattern is the reference's (src/fem/Assembler.cpp:86-173); its known
+answers: the brute-force pattern of all element DOF pairs (the triplet path,
+Assembler.cpp:228-259), the closed-form nnz of a structured grid
+(9 prod_d (3 n_d - 2)), structural symmetry, and the oracle's restatement bit
+for bit.
+"""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import sparlab_gpu as G
+from sparlab_gpu import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "sparlab_gpu.h")).read()
+    return sorted(set(re.findall(r"\b(sparlab_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = G.lib()
+    names = declared_symbols()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(lib, name), f"{name} is declared in include/sparlab_gpu.h but not exported"
+    assert lib.sparlab_version().decode().startswith("sparlab-gpu")
+
+
+def test_material_derivation_matches_oracle():
+    for hill in (dict(r0=0.7, r45=0.6, r90=0.8), dict(r0=1.9, r45=1.5, r90=2.3, rolling_direction=(1.0, 0.5, 0.0)),
+                 dict(r0=1.9, r45=1.5, r90=2.3, rolling_direction=(1.0, 0.4, 0.3), sheet_normal=(-0.3, 0.1, 1.0))):
+        m = G.Material(70e9, 0.33, yield_stress=95e6, hill=hill)
+        p, _ = O.hill_yield_matrix(hill["r0"], hill["r45"], hill["r90"], rd=hill.get("rolling_direction", (1, 0, 0)),
+                                   nd=hill.get("sheet_normal", (0, 0, 1)))
+        assert np.abs(m.yield_matrix - p).max() <= 1e-15 * np.abs(p).max()
+    vm = G.Material(200e9, 0.3, yield_stress=250e6)
+    assert np.array_equal(vm.yield_matrix, O.von_mises_yield_matrix())
+    with pytest.raises(RuntimeError):
+        G.Material(200e9, 0.3, hardening_modulus=1e9)          # hardening without a yield stress
+    with pytest.raises(RuntimeError):
+        G.Material(200e9, 0.6)                                  # nu outside (-1, 0.5)
+    with pytest.raises(RuntimeError):
+        G.Material(200e9, 0.3, yield_stress=1e8, hill=dict(r0=-1.0))
+
+
+def test_rules_and_state_layout():
+    for rule in ((2, 0, 2), (2, 2, 2), (1, 0, 2), (2, 1, 2), (2, 1, 0)):
+        r = G.Rule(*rule)
+        assert r.points == O.rule_points(*rule)
+    with pytest.raises(RuntimeError):
+        G.Rule(2, 5, 2).points            # 20 points: more than the kernels take
+    with pytest.raises(RuntimeError):
+        G.Rule(5, 0, 2).points
+    m = W.microbench_material()
+    assert m.state_fields == 8 + 6 * 2
+    ref = W.microbench(16, seed=1)["states"]
+    back = G.states_to_reference(G.states_from_reference(ref, m), m, 16, 8)
+    assert np.array_equal(back[:, :, 0:6], ref[:, :, 0:6])
+    assert np.array_equal(back[:, :, 12:24], ref[:, :, 12:24])
+    assert np.array_equal(back[:, :, 36], ref[:, :, 36])
+
+
+def brute_force_pattern(nn, conn):
+    rows = set()
+    for e in range(conn.shape[0]):
+        dofs = [3 * n + k for n in conn[e] for k in range(3)]
+        for i in dofs:
+            for j in dofs:
+                rows.add((i, j))
+    rows = sorted(rows)
+    row_ptr = np.zeros(3 * nn + 1, dtype=np.int64)
+    for i, _ in rows:
+        row_ptr[i + 1] += 1
+    return np.cumsum(row_ptr), np.array([j for _, j in rows])
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (3, 2, 2), (5, 4, 3)])
+def test_pattern_brute_force_and_closed_form(dims):
+    nx, ny, nz = dims
+    coords, conn = W.structured_hex_mesh(nx, ny, nz, 1.0, 1.0, 1.0)
+    nn = coords.shape[0]
+    row_ptr, col_idx, bo = G.pattern(nn, conn)
+    rp, ci = brute_force_pattern(nn, conn)
+    assert np.array_equal(row_ptr, rp) and np.array_equal(col_idx, ci)
+    # 9 prod_d (3 n_d - 2) with n_d = d + 1 nodes per direction
+    assert col_idx.size == 9 * np.prod([3 * (n + 1) - 2 for n in dims])
+    o_outer, o_inner, o_bo = O.pattern(nn, conn)
+    assert np.array_equal(row_ptr, o_outer) and np.array_equal(col_idx, o_inner) and np.array_equal(bo, o_bo)
+
+
+def test_pattern_of_the_assembly_workload():
+    """100 x 100 x 4: 51 005 nodes, nnz = 9 * 301 * 301 * 13 = 10 600 317."""
+    coords, conn = W.structured_hex_mesh(100, 100, 4, 0.1, 0.1, 0.004)
+    assert coords.shape[0] == 51005
+    row_ptr, col_idx, _ = G.pattern(coords.shape[0], conn)
+    assert col_idx.size == 10_600_317
+    o_outer, o_inner, _ = O.pattern(coords.shape[0], conn)
+    assert np.array_equal(row_ptr, o_outer) and np.array_equal(col_idx, o_inner)
+
+
+def test_pattern_of_an_unstructured_mesh():
+    """A perturbed numbering: elements in random order, nodes renumbered."""
+    coords, conn = W.structured_hex_mesh(4, 3, 2, 1.0, 1.0, 1.0)
+    rng = np.random.default_rng(4)
+    perm = rng.permutation(coords.shape[0]).astype(np.int32)
+    conn = perm[conn][rng.permutation(conn.shape[0])]
+    nn = coords.shape[0]
+    row_ptr, col_idx, bo = G.pattern(nn, conn)
+    rp, ci = brute_force_pattern(nn, conn)
+    assert np.array_equal(row_ptr, rp) and np.array_equal(col_idx, ci)
+    o_outer, o_inner, o_bo = O.pattern(nn, conn)
+    assert np.array_equal(bo, o_bo)
+
+
+def test_pattern_refuses_bad_connectivity():
+    coords, conn = W.structured_hex_mesh(1, 1, 1, 1.0, 1.0, 1.0)
+    bad = conn.copy()
+    bad[0, 3] = 99
+    with pytest.raises(RuntimeError):
+        G.pattern(coords.shape[0], bad)
+    bad = conn.copy()
+    bad[0, 3] = bad[0, 2]
+    with pytest.raises(RuntimeError):
+        G.pattern(coords.shape[0], bad)
