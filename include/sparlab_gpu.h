@@ -127,7 +127,13 @@ int sparlab_state_fields(const sparlab_material* material);
  * force: 24 per element; tangent: 576 per element, row-major (may be null for
  *   a residual-only evaluation); energy: per element; yielding, iterations,
  *   status: int32 per element; fail_value: per element (the refused Jacobian
- *   determinant for SPARLAB_ELEMENT_*_JACOBIAN). */
+ *   determinant for SPARLAB_ELEMENT_*_JACOBIAN).
+ * workspace: sparlab_element_workspace_doubles(num_elements, rule) doubles
+ *   of device scratch (each point's stress and tangent of its final local
+ *   pass and each element's K_aa factors, passed from the local solve to the
+ *   condensation); the reference keeps the same per-call data in a
+ *   thread_local workspace (src/fem/IncompatibleModes.cpp:66-71). Calls that
+ *   share a workspace must not overlap. */
 typedef struct sparlab_element_batch {
   const double* coords;
   const int32_t* conn;
@@ -145,7 +151,12 @@ typedef struct sparlab_element_batch {
   int32_t* iterations;
   int32_t* status;
   double* fail_value;
+  double* workspace;
 } sparlab_element_batch;
+
+/* Doubles of workspace an element batch needs (86 per integration point and
+ * 108 per element), or a negative code for a rule the kernels do not take. */
+int64_t sparlab_element_workspace_doubles(int64_t num_elements, const sparlab_rule* rule);
 
 /* Evaluates every element of the batch (replaces the calls of
  * incompatible_elastoplastic_element in the element loop of

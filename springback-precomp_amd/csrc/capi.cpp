@@ -319,6 +319,16 @@ int sparlab_state_fields(const sparlab_material* m) {
   return sbk::state_fields(kin_fields(*m));
 }
 
+int64_t sparlab_element_workspace_doubles(int64_t num_elements, const sparlab_rule* rule) {
+  if (rule == nullptr || num_elements < 0) return fail(SPARLAB_E_ARGUMENT, "null rule or negative size");
+  sbk::RuleArg r{};
+  std::string why;
+  const int rs = build_rule(*rule, r, why);
+  if (rs == -1) return fail(SPARLAB_E_ARGUMENT, why);
+  if (rs == -2) return fail(SPARLAB_E_UNSUPPORTED, why);
+  return static_cast<int64_t>(sbk::workspace_doubles(num_elements, r.n));
+}
+
 int sparlab_elements_evaluate_device(const sparlab_material* material, const sparlab_rule* rule, int kinematics,
                                      int want_tangent, const sparlab_element_batch* b, void* stream) {
   if (material == nullptr || rule == nullptr || b == nullptr) return fail(SPARLAB_E_ARGUMENT, "null argument");
@@ -327,7 +337,8 @@ int sparlab_elements_evaluate_device(const sparlab_material* material, const spa
   if (b->num_elements == 0) return SPARLAB_OK;
   if (b->coords == nullptr || b->conn == nullptr || b->u == nullptr || b->state_in == nullptr ||
       b->state_out == nullptr || b->alpha_out == nullptr || b->force == nullptr || b->energy == nullptr ||
-      b->yielding == nullptr || b->iterations == nullptr || b->status == nullptr || b->fail_value == nullptr)
+      b->yielding == nullptr || b->iterations == nullptr || b->status == nullptr || b->fail_value == nullptr ||
+      b->workspace == nullptr)
     return fail(SPARLAB_E_ARGUMENT, "a required buffer of the element batch is null");
   if (want_tangent && b->tangent == nullptr) return fail(SPARLAB_E_ARGUMENT, "want_tangent needs a tangent buffer");
   sbk::ElementArgs a{};
@@ -349,6 +360,7 @@ int sparlab_elements_evaluate_device(const sparlab_material* material, const spa
   a.iterations = b->iterations;
   a.status = b->status;
   a.fail_value = b->fail_value;
+  sbk::set_workspace(a, b->workspace);
   // The local solve writes every element's status; the condensation reads it.
   hipStream_t s = static_cast<hipStream_t>(stream);
   int err = sbk::launch_local_solve(a, s);

@@ -8,17 +8,20 @@
 // point) per element:
 //   local_solve_kernel  the local Newton iteration of the 9 mode parameters
 //                       from the committed ones, as a per-team state machine
-//                       that requests one pass at a time; each wavefront owns a
-//                       chunk of elements and a team that finishes takes the
-//                       next one, so the divergent iteration counts do not idle
-//                       the wavefront. Writes alpha and a status per element.
-//   condense_kernel     the pass at the converged alpha, K_aa's full-pivot LU,
-//                       the coupling blocks K_ua, K_au, K_uu and the static
-//                       condensation f* = f_u - K_ua K_aa^-1 r_a,
-//                       K* = K_uu - K_ua K_aa^-1 K_au; writes f*, K*, the
-//                       points' trial states, energy and yielding count.
-// The pass at the converged alpha is the same arithmetic on the same data as
-// the last pass of the local iteration, so both kernels agree bit for bit.
+//                       (kept in LDS) that requests one pass at a time; each
+//                       wavefront owns a chunk of elements and a team that
+//                       finishes takes the next one, so the divergent
+//                       iteration counts do not idle the wavefront. Every pass
+//                       writes the points' trial states; a pass that meets the
+//                       stopping test also writes the points' stress S and
+//                       tangent (the point workspace), energy and yielding.
+//                       The last such pass of a converged element is its
+//                       final pass. Writes alpha and a status per element.
+//   condense_kernel     reads S and the tangent of the final pass back, forms
+//                       r_a, K_aa (the same arithmetic as the pass) and its
+//                       full-pivot LU, the coupling blocks K_ua, K_au, K_uu
+//                       and the static condensation
+//                       f* = f_u - K_ua K_aa^-1 r_a, K* = K_uu - K_ua K_aa^-1 K_au.
 #include "element_kernels.hpp"
 #include "element_pass.hpp"
 #include "team_linalg.hpp"
@@ -37,161 +40,6 @@ constexpr double kLocalShift = 1.0e-3;
 constexpr double kLocalMarquardt = 1.0e-6;
 constexpr double kLocalTolerance = 1.0e-8;
 constexpr double kLocalRoundOff = 1.0e-13;
-
-// ---------------------------------------------------------------------------
-// One pass of a team: the local sums r_alpha, scale, floor and K_aa
-// (IncompatibleModes.cpp:252-275), reduced over the team.
-// ---------------------------------------------------------------------------
-struct LocalSums {
-  double ra[9];
-  double scale_max, floor_max;  // max over the 9 totals
-};
-
-template <int KIN>
-__device__ __forceinline__ int team_local_pass(const ElementArgs& A, long long e, int p, const PointGeom& pg,
-                                               const double ue[24], const double alpha[9], LuSlot& kaa_out,
-                                               LocalSums& out) {
-  const DevMaterial& m = A.mat;
-  const bool has_point = p < A.rule.n;
-  PointResponse r;
-  int fail = 0;
-  if (has_point) {
-    PointState st;
-    load_state(m, A.state_in, A.state_stride, e * A.rule.n + p, st);
-    fail = point_response<KIN>(m, pg, ue, alpha, st, r);
-  }
-  // The reference evaluates every point's kinematics before any return, so
-  // the reported failure is the first kinematic one, else the first return.
-  const int stage = (fail == kReturnFailed || fail == kNonFiniteTrial) ? 1 : 0;
-  const int first = team_imin(fail != 0 ? (stage << 12) | (p << 8) | fail : 1 << 20);
-  if (first != (1 << 20)) return first & 0xFF;
-  // B_alpha (6 x 9) of this point and the pieces of the sums
-  double ba[6][9];
-  double w = 0.0;
-  if (has_point) {
-    w = pg.w;
-#pragma unroll
-    for (int c = 0; c < 6; ++c)
-#pragma unroll
-      for (int n = 0; n < 9; ++n) ba[c][n] = bnl(r.F, pg.g, c, 8 + n / 3, n % 3);
-  } else {
-#pragma unroll
-    for (int c = 0; c < 6; ++c)
-#pragma unroll
-      for (int n = 0; n < 9; ++n) ba[c][n] = 0.0;
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      r.S[c] = 0.0;
-#pragma unroll
-      for (int d = 0; d < 6; ++d) r.mat[c][d] = 0.0;
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) r.h[i][j] = 0.0;
-  }
-  double hmax = 0.0, mmax = 0.0;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) hmax = fmax(hmax, fabs(r.h[i][j]));
-#pragma unroll
-  for (int i = 0; i < 6; ++i)
-#pragma unroll
-    for (int j = 0; j < 6; ++j) mmax = fmax(mmax, fabs(r.mat[i][j]));
-  const double ff = w * mmax * hmax * (1.0 + hmax);
-  double smax = 0.0, fmaxv = 0.0;
-#pragma unroll
-  for (int n = 0; n < 9; ++n) {
-    double sr = ba[0][n] * r.S[0], ss = fabs(ba[0][n]) * fabs(r.S[0]), sf = fabs(ba[0][n]);
-#pragma unroll
-    for (int c = 1; c < 6; ++c) {
-      sr = sr + ba[c][n] * r.S[c];
-      ss = ss + fabs(ba[c][n]) * fabs(r.S[c]);
-      sf = sf + fabs(ba[c][n]);
-    }
-    out.ra[n] = team_sum(w * sr);
-    smax = fmax(smax, team_sum(w * ss));
-    fmaxv = fmax(fmaxv, team_sum(ff * sf));
-  }
-  out.scale_max = smax;
-  out.floor_max = fmaxv;
-  // K_aa = sum_q w B_a^T mat B_a (+ the geometric alpha-alpha block)
-  double y[6][9];
-#pragma unroll
-  for (int c = 0; c < 6; ++c)
-#pragma unroll
-    for (int n = 0; n < 9; ++n) {
-      double s = r.mat[c][0] * ba[0][n];
-#pragma unroll
-      for (int d = 1; d < 6; ++d) s = s + r.mat[c][d] * ba[d][n];
-      y[c][n] = s;
-    }
-  double geo[3][3];
-  if constexpr (KIN != kSmall) {
-    // gt^T S gt with gt = G~ (3 x 3)
-    const double S3[3][3] = {{r.S[0], r.S[3], r.S[5]}, {r.S[3], r.S[1], r.S[4]}, {r.S[5], r.S[4], r.S[2]}};
-    double gs[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-        gs[a][j] = pg.g[0][8 + a] * S3[0][j] + pg.g[1][8 + a] * S3[1][j] + pg.g[2][8 + a] * S3[2][j];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b)
-        geo[a][b] = gs[a][0] * pg.g[0][8 + b] + gs[a][1] * pg.g[1][8 + b] + gs[a][2] * pg.g[2][8 + b];
-  }
-#pragma unroll
-  for (int mm = 0; mm < 9; ++mm)
-#pragma unroll
-    for (int n = 0; n < 9; ++n) {
-      double s = ba[0][mm] * y[0][n];
-#pragma unroll
-      for (int c = 1; c < 6; ++c) s = s + ba[c][mm] * y[c][n];
-      double v = w * s;
-      if constexpr (KIN != kSmall) {
-        if (mm % 3 == n % 3) v = v + w * geo[mm / 3][n / 3];
-      }
-      const double tot = team_sum(v);
-      if (((mm * 9 + n) & 7) == p) kaa_out.a[mm * 9 + n] = tot;
-    }
-  team_sync();
-  return 0;
-}
-
-// ---------------------------------------------------------------------------
-// The local Newton iteration as a state machine (IncompatibleModes.cpp:320-546)
-// ---------------------------------------------------------------------------
-enum Phase : int {
-  kIdle = 0,
-  kFirst,       // pass(alpha), must succeed
-  kLsReach,     // line search: try_pass(alpha + reach d)
-  kLsFalsi,     // line search: try_pass(alpha + step d), regula falsi
-  kLsFinal,     // line search: pass(alpha + s0 d), must succeed
-  kSnTrial,     // search on |r_a|: try_pass(alpha + trial d)
-  kSnFinal,     // search on |r_a|: pass(alpha + best d), must succeed
-  kIndefFull,   // indefinite sym(K_aa): try_pass(alpha + newton)
-  kLm,          // Levenberg-Marquardt: try_pass(alpha + d)
-};
-enum LsContext : int { kLsSpd = 0, kLsShifted = 1 };
-enum SnContext : int { kSnFromLs = 0, kSnAfterShift = 1 };
-
-struct TeamSlot {
-  LuSlot lu;         // K_aa and its factors
-  double k0[81];     // K_aa of the iteration (the fallbacks need it)
-  double w[81];      // work matrix of the fallbacks
-};
-
-struct Machine {
-  int phase, ls_ctx, sn_ctx, iterations, halving, search, side;
-  bool last_is_best, moved;
-  double alpha[9], d[9], newton[9], r0[9];
-  double residual, s;  // s: the trial is alpha + s d
-  double phi0, reach, s0, f0, s1, f1, step;
-  double best_step, best, trial_step, mu;
-};
 
 __device__ __forceinline__ double norm9(const double v[9]) {
   double s = v[0] * v[0];
@@ -216,6 +64,221 @@ __device__ __forceinline__ void axpy9(double a[9], double s, const double d[9]) 
   for (int i = 0; i < 9; ++i) a[i] = a[i] + s * d[i];
 }
 
+// ---------------------------------------------------------------------------
+// One pass of a team: the local sums r_alpha, scale, floor and K_aa
+// (IncompatibleModes.cpp:252-275), reduced over the team.
+// ---------------------------------------------------------------------------
+struct LocalSums {
+  double ra[9];
+  double scale_max, floor_max;  // max over the 9 totals
+  bool within_tol;              // max |r_a| <= tol scale + round-off floor (:398, :469)
+  bool stop;                    // within_tol or r_a = 0: the stopping test (:469)
+};
+
+// A point record of the workspace: S (6) then the tangent (36, row-major).
+constexpr int kWsStress = 0;
+constexpr int kWsTangent = 6;
+// An element record: LU factors (81), r_a (9), row and column transpositions.
+constexpr int kErFactors = 0;
+constexpr int kErResidual = 81;
+constexpr int kErRows = 90;
+constexpr int kErCols = 99;
+// A geometry record: F (9), [G | G~] (3 x 11, row-major), w.
+constexpr int kGrF = 0;
+constexpr int kGrG = 9;
+constexpr int kGrW = 42;
+
+// B_a (6 x 9) of a point: column n is mode n / 3, direction n % 3.
+__device__ __forceinline__ void mode_operator(const double F[3][3], const double gm[3][3], double ba[6][9]) {
+#pragma unroll
+  for (int c = 0; c < 6; ++c)
+#pragma unroll
+    for (int n = 0; n < 9; ++n) ba[c][n] = bnl_mode(F, gm, c, n / 3, n % 3);
+}
+
+// The team's sums of a pass from each lane's point response (w = 0 and zero
+// response on lanes without a point); the K_aa totals go to kaa (row-major).
+template <int KIN>
+__device__ __forceinline__ void team_sums(const PointResponse& r, const double gm[3][3], double w, int p,
+                                          double* kaa, LocalSums& out, bool want_kaa_only) {
+  double ba[6][9];
+  mode_operator(r.F, gm, ba);
+  if (!want_kaa_only) {
+    double hmax = 0.0, mmax = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) hmax = fmax(hmax, fabs(r.h[i][j]));
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) mmax = fmax(mmax, fabs(r.mat[i][j]));
+    const double ff = w * mmax * hmax * (1.0 + hmax);
+    double smax = 0.0, fmaxv = 0.0;
+#pragma unroll
+    for (int n = 0; n < 9; ++n) {
+      double sr = ba[0][n] * r.S[0], ss = fabs(ba[0][n]) * fabs(r.S[0]), sf = fabs(ba[0][n]);
+#pragma unroll
+      for (int c = 1; c < 6; ++c) {
+        sr = sr + ba[c][n] * r.S[c];
+        ss = ss + fabs(ba[c][n]) * fabs(r.S[c]);
+        sf = sf + fabs(ba[c][n]);
+      }
+      out.ra[n] = team_sum(w * sr);
+      smax = fmax(smax, team_sum(w * ss));
+      fmaxv = fmax(fmaxv, team_sum(ff * sf));
+    }
+    out.scale_max = smax;
+    out.floor_max = fmaxv;
+    const double largest = maxabs9(out.ra);
+    out.within_tol = largest <= kLocalTolerance * smax + kLocalRoundOff * fmaxv;
+    out.stop = out.within_tol || largest == 0.0;
+  } else {
+#pragma unroll
+    for (int n = 0; n < 9; ++n) {
+      double sr = ba[0][n] * r.S[0];
+#pragma unroll
+      for (int c = 1; c < 6; ++c) sr = sr + ba[c][n] * r.S[c];
+      out.ra[n] = team_sum(w * sr);
+    }
+  }
+  // K_aa = sum_q w B_a^T mat B_a (+ the geometric alpha-alpha block)
+  double y[6][9];
+#pragma unroll
+  for (int c = 0; c < 6; ++c)
+#pragma unroll
+    for (int n = 0; n < 9; ++n) {
+      double s = r.mat[c][0] * ba[0][n];
+#pragma unroll
+      for (int d = 1; d < 6; ++d) s = s + r.mat[c][d] * ba[d][n];
+      y[c][n] = s;
+    }
+  double geo[3][3];
+  if constexpr (KIN != kSmall) {
+    // gt^T S gt with gt = G~ (3 x 3)
+    const double S3[3][3] = {{r.S[0], r.S[3], r.S[5]}, {r.S[3], r.S[1], r.S[4]}, {r.S[5], r.S[4], r.S[2]}};
+    double gs[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) gs[a][j] = gm[0][a] * S3[0][j] + gm[1][a] * S3[1][j] + gm[2][a] * S3[2][j];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) geo[a][b] = gs[a][0] * gm[0][b] + gs[a][1] * gm[1][b] + gs[a][2] * gm[2][b];
+  }
+#pragma unroll
+  for (int mm = 0; mm < 9; ++mm)
+#pragma unroll
+    for (int n = 0; n < 9; ++n) {
+      double s = ba[0][mm] * y[0][n];
+#pragma unroll
+      for (int c = 1; c < 6; ++c) s = s + ba[c][mm] * y[c][n];
+      double v = w * s;
+      if constexpr (KIN != kSmall) {
+        if (mm % 3 == n % 3) v = v + w * geo[mm / 3][n / 3];
+      }
+      const double tot = team_sum(v);
+      if (((mm * 9 + n) & 7) == p) kaa[mm * 9 + n] = tot;
+    }
+  team_sync();
+}
+
+__device__ __forceinline__ void zero_response(PointResponse& r) {
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    r.S[c] = 0.0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) r.mat[c][d] = 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) { r.h[i][j] = 0.0; r.F[i][j] = 0.0; }
+  r.energy = 0.0;
+  r.yielding = false;
+}
+
+// One pass at alpha; returns 0 or the team's first failure (a kinematic one
+// before any return failure, as the reference evaluates every point's
+// kinematics first). Writes the trial states of a successful pass, and the
+// workspace, energy and yielding of a pass that meets the stopping test.
+template <int KIN>
+__device__ __forceinline__ int team_local_pass(const ElementArgs& A, long long e, int p, const ModeGeom& mg,
+                                               const double alpha[9], double* kaa, LocalSums& out) {
+  const DevMaterial& m = A.mat;
+  const bool has_point = p < A.rule.n;
+  const long long q = e * A.rule.n + p;
+  PointResponse r;
+  int fail = 0;
+  if (has_point) {
+    PointState st;
+    load_state(m, A.state_in, A.state_stride, q, st);
+    fail = point_response<KIN>(m, mg, alpha, st, r);
+    if (fail == 0) store_state(m, A.state_out, A.state_stride, q, st);
+  } else {
+    zero_response(r);
+  }
+  const int stage = (fail == kReturnFailed || fail == kNonFiniteTrial) ? 1 : 0;
+  const int first = team_imin(fail != 0 ? (stage << 12) | (p << 8) | fail : 1 << 20);
+  if (first != (1 << 20)) return first & 0xFF;
+  const double w = has_point ? mg.w : 0.0;
+  const double energy = team_sum(w * r.energy);
+  const int yielding = team_isum(r.yielding ? 1 : 0);
+  team_sums<KIN>(r, mg.gm, w, p, kaa, out, false);
+  if (out.stop) {
+    if (has_point) {
+      double* rec = A.point_ws + q * kPointRecord;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) rec[kWsStress + c] = r.S[c];
+#pragma unroll
+      for (int i = 0; i < 36; ++i) rec[kWsTangent + i] = r.mat[i / 6][i % 6];
+    }
+    if (p == 0) {
+      A.energy[e] = energy;
+      A.yielding[e] = yielding;
+    }
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// The local Newton iteration as a state machine (IncompatibleModes.cpp:320-546)
+// ---------------------------------------------------------------------------
+enum Phase : int {
+  kIdle = 0,
+  kFirst,       // pass(alpha), must succeed
+  kLsReach,     // line search: try_pass(alpha + reach d)
+  kLsFalsi,     // line search: try_pass(alpha + step d), regula falsi
+  kLsFinal,     // line search: pass(alpha + s0 d), must succeed
+  kSnTrial,     // search on |r_a|: try_pass(alpha + trial d)
+  kSnFinal,     // search on |r_a|: pass(alpha + best d), must succeed
+  kIndefFull,   // indefinite sym(K_aa): try_pass(alpha + newton)
+  kLm,          // Levenberg-Marquardt: try_pass(alpha + d)
+};
+enum LsContext : int { kLsSpd = 0, kLsShifted = 1 };
+enum SnContext : int { kSnFromLs = 0, kSnAfterShift = 1 };
+
+struct Machine {
+  int phase, ls_ctx, sn_ctx, iterations, halving, search, side;
+  bool last_is_best, moved;
+  double alpha[9], d[9], newton[9], r0[9];
+  double residual, s;  // s: the trial is alpha + s d
+  double phi0, reach, s0, f0, s1, f1, step;
+  double best_step, best, trial_step, mu;
+};
+
+// A team's LDS: K_aa and its factors; K_aa of the iteration (the fallbacks
+// need it); the machine. The fallbacks' work matrix is lu.a: the factors are
+// dead once Newton's direction is solved, and a pass rewrites lu.a before the
+// next factorisation.
+struct TeamSlot {
+  LuSlot lu;
+  double k0[81];
+  Machine m;
+  __device__ double* w() { return lu.a; }
+};
+
 // The outcome of a step of the machine: keep going (a new trial is
 // requested), converged, or failed with a status.
 enum StepResult { kContinue = 0, kConverged = 1, kFailed = 2 };
@@ -224,14 +287,12 @@ enum StepResult { kContinue = 0, kConverged = 1, kFailed = 2 };
 // test, K_aa's LU, Newton's direction and the choice of the search.
 __device__ int machine_top(Machine& M, TeamSlot& T, const LocalSums& cur, int p, int& status, double& value) {
   const double largest = maxabs9(cur.ra);
-  const double scale = cur.scale_max;
-  const double floor = kLocalRoundOff * cur.floor_max;
   // K_aa is in T.lu.a (the pass wrote it); keep a copy for the fallbacks.
   for (int i = p; i < 81; i += 8) T.k0[i] = T.lu.a[i];
   team_sync();
   const bool inv = coop_full_lu(T.lu, p);
   if (!inv) { status = kSingularModes; value = largest; return kFailed; }
-  if (largest <= kLocalTolerance * scale + floor || largest == 0.0) return kConverged;
+  if (cur.stop) return kConverged;
   if (M.iterations == kMaxLocalIterations || !isfinite(largest)) {
     status = kLocalNotConverged;
     value = largest;
@@ -247,10 +308,10 @@ __device__ int machine_top(Machine& M, TeamSlot& T, const LocalSums& cur, int p,
   // sym(K_aa) positive definite?
   for (int i = p; i < 81; i += 8) {
     const int r = i / 9, c = i % 9;
-    T.w[i] = 0.5 * (T.k0[r * 9 + c] + T.k0[c * 9 + r]);
+    T.w()[i] = 0.5 * (T.k0[r * 9 + c] + T.k0[c * 9 + r]);
   }
   team_sync();
-  const bool spd = llt9(T.w);
+  const bool spd = llt9(T.w());
   team_sync();
   if (spd) {
     // line_search(newton, r0), SPD context
@@ -287,11 +348,11 @@ __device__ __forceinline__ void lm_start(Machine& M, TeamSlot& T, int p) {
     const int r = i / 9, c = i % 9;
     double s = T.k0[r] * T.k0[c];
     for (int k = 1; k < 9; ++k) s = s + T.k0[k * 9 + r] * T.k0[k * 9 + c];
-    T.w[i] = s;
+    T.w()[i] = s;
   }
   team_sync();
-  double dmax = T.w[0];
-  for (int i = 1; i < 9; ++i) dmax = fmax(dmax, T.w[i * 10]);
+  double dmax = T.w()[0];
+  for (int i = 1; i < 9; ++i) dmax = fmax(dmax, T.w()[i * 10]);
   M.mu = kLocalMarquardt * dmax;
   M.moved = false;
   M.search = 0;
@@ -315,11 +376,11 @@ __device__ int lm_next(Machine& M, TeamSlot& T, int p, int& status, double& valu
       const int r = i / 9, c = i % 9;
       double s = T.k0[r] * T.k0[c];
       for (int k = 1; k < 9; ++k) s = s + T.k0[k * 9 + r] * T.k0[k * 9 + c];
-      T.w[i] = (r == c) ? s + M.mu : s;
+      T.w()[i] = (r == c) ? s + M.mu : s;
     }
-    llt9(T.w);
+    llt9(T.w());
     double x[9];
-    llt9_solve(T.w, grad, x);
+    llt9_solve(T.w(), grad, x);
     team_sync();
 #pragma unroll
     for (int i = 0; i < 9; ++i) M.d[i] = -x[i];
@@ -361,8 +422,7 @@ __device__ int machine_step(Machine& M, TeamSlot& T, bool ok, const LocalSums& c
         break;
       } else {
         const double full = norm9(cur.ra);
-        if (full <= kFullStepReduction * M.residual ||
-            maxabs9(cur.ra) <= kLocalTolerance * cur.scale_max + kLocalRoundOff * cur.floor_max) {
+        if (full <= kFullStepReduction * M.residual || cur.within_tol) {
           axpy9(M.alpha, M.reach, M.d);
           M.residual = full;
           next = kLsTrue;
@@ -482,17 +542,17 @@ __device__ int machine_step(Machine& M, TeamSlot& T, bool ok, const LocalSums& c
       // shifted direction: (K_aa + mu I) d = -r0, mu from sym's spectrum
       for (int i = 0; i < 81; ++i) {
         const int r = i / 9, c = i % 9;
-        T.w[i] = 0.5 * (T.k0[r * 9 + c] + T.k0[c * 9 + r]);
+        T.w()[i] = 0.5 * (T.k0[r * 9 + c] + T.k0[c * 9 + r]);
       }
       double ev[9];
-      jacobi9_values(T.w, ev);
+      jacobi9_values(T.w(), ev);
       double top = fabs(ev[0]);
       for (int i = 1; i < 9; ++i) top = fmax(top, fabs(ev[i]));
       const double mu = kLocalShift * top - ev[0];
       team_sync();
-      for (int i = 0; i < 81; ++i) T.w[i] = (i % 10 == 0) ? T.k0[i] + mu : T.k0[i];
+      for (int i = 0; i < 81; ++i) T.w()[i] = (i % 10 == 0) ? T.k0[i] + mu : T.k0[i];
       double x[9];
-      plu9_solve(T.w, M.r0, x);
+      plu9_solve(T.w(), M.r0, x);
       team_sync();
 #pragma unroll
       for (int i = 0; i < 9; ++i) M.d[i] = -x[i];
@@ -573,21 +633,22 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
   const int t = lane >> 3;
   const int p = lane & 7;
   TeamSlot& T = slots[t];
+  Machine& M = T.m;
   const long long c0 = static_cast<long long>(blockIdx.x) * kChunk;
   const long long c1 = c0 + kChunk < A.ne ? c0 + kChunk : A.ne;
   long long next = c0;
   long long e = -1;
-  Machine M;
-  M.phase = kIdle;
-  PointGeom pg;
-  double ue[24];
+  if (p == 0) M.phase = kIdle;
+  team_sync();
+  ModeGeom mg;
   for (;;) {
     // Hand the chunk's next elements to the idle teams, in team order.
-    const unsigned long long leaders = __ballot(M.phase == kIdle && p == 0);
+    const bool idle = M.phase == kIdle;
+    const unsigned long long leaders = __ballot(idle && p == 0);
     const int before = __popcll(leaders & ((1ull << (t * 8)) - 1ull));
     const int free_teams = __popcll(leaders);
     bool fresh = false;
-    if (M.phase == kIdle) {
+    if (idle) {
       const long long cand = next + before;
       if (cand < c1) {
         e = cand;
@@ -597,35 +658,43 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
       }
     }
     next += free_teams;
-    if (__ballot(M.phase != kIdle || fresh) == 0ull) break;
+    if (__ballot(!idle || fresh) == 0ull) break;
     if (fresh) {
-      double x0[8][3];
+      double x0[8][3], ue[24];
       load_element(A.coords, A.conn, A.u, e, x0, ue);
       double fv = 0.0;
       int st = kOk;
-      if (p < A.rule.n) st = setup_point(x0, ue, A.rule.xi[p], A.rule.eta[p], A.rule.zeta[p], A.rule.w[p], pg, fv);
+      PointGeom pg;
+      if (p < A.rule.n) {
+        st = setup_point(x0, ue, A.rule.xi[p], A.rule.eta[p], A.rule.zeta[p], A.rule.w[p], pg, fv);
+        mode_geom<KIN>(pg, ue, mg);
+      }
       // the first failing point in point order, as the reference's set-up loop
       const int code = team_imin(st != kOk ? (p << 8) | st : 1 << 20);
       if (code != (1 << 20)) {
         if (p == (code >> 8)) { A.status[e] = code & 0xFF; A.fail_value[e] = fv; }
         continue;  // the team stays idle and takes another element
       }
+      if (p == 0) {
 #pragma unroll
-      for (int i = 0; i < 9; ++i) M.alpha[i] = A.alpha_in ? A.alpha_in[e * 9 + i] : 0.0;
-      M.iterations = 0;
-      M.s = 0.0;
-      M.phase = kFirst;
+        for (int i = 0; i < 9; ++i) M.alpha[i] = A.alpha_in ? A.alpha_in[e * 9 + i] : 0.0;
+        M.iterations = 0;
+        M.s = 0.0;
+        M.phase = kFirst;
+      }
+      team_sync();
     }
-    if (M.phase != kIdle) {
+    if (!idle || fresh) {
       double trial[9];
-#pragma unroll
-      for (int i = 0; i < 9; ++i) trial[i] = M.alpha[i] + M.s * M.d[i];
       if (M.phase == kFirst) {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) { trial[i] = M.alpha[i]; M.d[i] = 0.0; }
+        for (int i = 0; i < 9; ++i) trial[i] = M.alpha[i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) trial[i] = M.alpha[i] + M.s * M.d[i];
       }
       LocalSums cur;
-      const int fail = team_local_pass<KIN>(A, e, p, pg, ue, trial, T.lu, cur);
+      const int fail = team_local_pass<KIN>(A, e, p, mg, trial, T.lu.a, cur);
       const bool ok = fail == 0;
       int status = ok ? kOk : fail;
       double value = 0.0;
@@ -637,28 +706,104 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
         }
         if (p < 9) A.alpha_out[e * 9 + p] = M.alpha[p];
         if (p == 0) A.alpha_out[e * 9 + 8] = M.alpha[8];
-        M.phase = kIdle;
+        // K_aa's factors and r_a of the final iteration, for the condensation
+        double* er = A.elem_ws + e * kElemRecord;
+        for (int i = p; i < 81; i += 8) er[kErFactors + i] = T.lu.a[i];
+        er[kErResidual + p] = cur.ra[p];
+        er[kErRows + p] = static_cast<double>(T.lu.rt[p]);
+        er[kErCols + p] = static_cast<double>(T.lu.ct[p]);
+        if (p == 0) {
+          er[kErResidual + 8] = cur.ra[8];
+          er[kErRows + 8] = static_cast<double>(T.lu.rt[8]);
+          er[kErCols + 8] = static_cast<double>(T.lu.ct[8]);
+        }
+        team_sync();
+        if (p == 0) M.phase = kIdle;
       } else if (res == kFailed) {
         if (p == 0) {
           A.status[e] = status == kOk ? kUnsupported : status;
           A.fail_value[e] = value;
           A.iterations[e] = M.iterations;
         }
-        M.phase = kIdle;
+        team_sync();
+        if (p == 0) M.phase = kIdle;
       }
+      team_sync();
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// The final pass and the condensation
+// The condensation (IncompatibleModes.cpp:279-306, 549-561)
 // ---------------------------------------------------------------------------
+// A team's LDS: K_aa's factors, then the coupling K_aa^-1 K_au (9 x 24), then
+// the rows of K* for the symmetrisation.
 struct CondenseSlot {
-  LuSlot lu;
-  double region[216];  // staging of one point, then K_aa^-1 K_au, then K rows
+  union {
+    LuSlot lu;
+    double region[216];
+  };
 };
 
-template <int KIN, bool TANGENT>
+// B_NL(F, [G | G~]) entry (c, 3a + k) with F and G read from a geometry record.
+__device__ __forceinline__ double bnl_rec(const double* gr, int c, int a, int k) {
+  const double* F = gr + kGrF;
+  const double* g = gr + kGrG;
+  switch (c) {
+    case 0: return F[k * 3 + 0] * g[0 * 11 + a];
+    case 1: return F[k * 3 + 1] * g[1 * 11 + a];
+    case 2: return F[k * 3 + 2] * g[2 * 11 + a];
+    case 3: return F[k * 3 + 0] * g[1 * 11 + a] + F[k * 3 + 1] * g[0 * 11 + a];
+    case 4: return F[k * 3 + 1] * g[2 * 11 + a] + F[k * 3 + 2] * g[1 * 11 + a];
+    default: return F[k * 3 + 2] * g[0 * 11 + a] + F[k * 3 + 0] * g[2 * 11 + a];
+  }
+}
+
+// z = B_u[:, 3a..3a+2]^T mat of one point (mat row-major in the point record).
+__device__ __forceinline__ void node_times_tangent(const double bu[6][3], const double* pr, double z[3][6]) {
+#pragma unroll
+  for (int d = 0; d < 6; ++d) {
+    double mc[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) mc[c] = pr[kWsTangent + c * 6 + d];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double s = bu[0][k] * mc[0];
+#pragma unroll
+      for (int c = 1; c < 6; ++c) s = s + bu[c][k] * mc[c];
+      z[k][d] = s;
+    }
+  }
+}
+
+__device__ __forceinline__ void node_operator(const double* gr, int a, double bu[6][3]) {
+#pragma unroll
+  for (int c = 0; c < 6; ++c)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) bu[c][k] = bnl_rec(gr, c, a, k);
+}
+
+// (S G)_a: row a of G^T S (3), S tensorial Voigt.
+__device__ __forceinline__ void stress_times_gradient(const double* gr, const double* pr, int a, double gsa[3]) {
+  const double* g = gr + kGrG;
+  const double* S = pr + kWsStress;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const double S3j0 = j == 0 ? S[0] : (j == 1 ? S[3] : S[5]);
+    const double S3j1 = j == 0 ? S[3] : (j == 1 ? S[1] : S[4]);
+    const double S3j2 = j == 0 ? S[5] : (j == 1 ? S[4] : S[2]);
+    gsa[j] = g[0 * 11 + a] * S3j0 + g[1 * 11 + a] * S3j1 + g[2 * 11 + a] * S3j2;
+  }
+}
+
+// One team (8 lanes) per element. Lane p first forms the geometry of point p
+// at the converged alpha (F with the pass's own arithmetic) into the
+// workspace; then lane a owns rows 3a..3a+2 of f*, K_ua, K_uu and K* (and
+// columns 3a..3a+2 of K_au), each accumulated point by point in point order
+// from the geometry records and the point records of the final pass. The
+// accumulations run as separate point loops (K_ua; K_au; K_uu in two column
+// halves) so that few accumulators are live at a time.
+template <int KIN, bool TANGENT, bool SYM>
 __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
   __shared__ CondenseSlot slots[kWaveTeams];
   const int lane = threadIdx.x;
@@ -668,424 +813,245 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
   const long long e = static_cast<long long>(blockIdx.x) * kWaveTeams + t;
   if (e >= A.ne) return;
   if (A.status[e] != kOk) return;
-  const DevMaterial& m = A.mat;
-  const bool has_point = p < A.rule.n;
-  double x0[8][3], ue[24];
-  load_element(A.coords, A.conn, A.u, e, x0, ue);
-  PointGeom pg;
-  double fv = 0.0;
-  if (has_point) setup_point(x0, ue, A.rule.xi[p], A.rule.eta[p], A.rule.zeta[p], A.rule.w[p], pg, fv);
-  double alpha[9];
+  const int n = A.rule.n;
+  const long long q0 = e * n;
+  // geometry of the lane's point
+  if (p < n) {
+    double x0[8][3], ue[24];
+    load_element(A.coords, A.conn, A.u, e, x0, ue);
+    PointGeom pg;
+    double fv = 0.0;
+    setup_point(x0, ue, A.rule.xi[p], A.rule.eta[p], A.rule.zeta[p], A.rule.w[p], pg, fv);
+    double alpha[9], gm[3][3], h[3][3], F[3][3];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) alpha[i] = A.alpha_out[e * 9 + i];
-  // the pass at the converged alpha
-  PointResponse r;
-  PointState st;
-  int fail = 0;
-  if (has_point) {
-    load_state(m, A.state_in, A.state_stride, e * A.rule.n + p, st);
-    fail = point_response<KIN>(m, pg, ue, alpha, st, r);
-  } else {
-    r.energy = 0.0;
-    r.yielding = false;
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      r.S[c] = 0.0;
-#pragma unroll
-      for (int d = 0; d < 6; ++d) r.mat[c][d] = 0.0;
-    }
+    for (int i = 0; i < 9; ++i) alpha[i] = A.alpha_out[e * 9 + i];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) { r.F[i][j] = 0.0; r.h[i][j] = 0.0; }
+      for (int j = 0; j < 3; ++j) gm[i][j] = pg.g[i][8 + j];
+    mode_kinematics<KIN>(pg.hu, gm, alpha, h, F);
+    double* gr = A.geom_ws + (q0 + p) * kGeomRecord;
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < 9; ++i) gr[kGrF + i] = F[i / 3][i % 3];
 #pragma unroll
-      for (int j = 0; j < 11; ++j) pg.g[i][j] = 0.0;
-    pg.w = 0.0;
+    for (int i = 0; i < 33; ++i) gr[kGrG + i] = pg.g[i / 11][i % 11];
+    gr[kGrW] = pg.w;
   }
-  if (team_or(fail != 0 ? 1 : 0)) {
-    if (p == 0) A.status[e] = kUnsupported;
-    return;
+  // K_aa's factors of the final local iteration
+  double* er = A.elem_ws + e * kElemRecord;
+  for (int i = p; i < 81; i += 8) T.lu.a[i] = er[kErFactors + i];
+  T.lu.rt[p] = static_cast<int>(er[kErRows + p]);
+  T.lu.ct[p] = static_cast<int>(er[kErCols + p]);
+  if (p == 0) {
+    T.lu.rt[8] = static_cast<int>(er[kErRows + 8]);
+    T.lu.ct[8] = static_cast<int>(er[kErCols + 8]);
   }
-  const double w = has_point ? pg.w : 0.0;
-  // sums of the pass: r (33), energy, yielding; K_aa for the LU
-  double ba[6][9];
+  team_sync();
+  const int a = p;
+  // f_u rows and K_ua of node a
+  double ru[3], kua[3][9];
 #pragma unroll
-  for (int c = 0; c < 6; ++c)
+  for (int k = 0; k < 3; ++k) {
+    ru[k] = 0.0;
 #pragma unroll
-    for (int n = 0; n < 9; ++n) ba[c][n] = has_point ? bnl(r.F, pg.g, c, 8 + n / 3, n % 3) : 0.0;
-  double ra[9];
-#pragma unroll
-  for (int n = 0; n < 9; ++n) {
-    double s = ba[0][n] * r.S[0];
-#pragma unroll
-    for (int c = 1; c < 6; ++c) s = s + ba[c][n] * r.S[c];
-    ra[n] = team_sum(w * s);
+    for (int j = 0; j < 9; ++j) kua[k][j] = 0.0;
   }
-  double ru[3];  // f_u rows of this lane's node a = p
-#pragma unroll
-  for (int k = 0; k < 3; ++k) ru[k] = 0.0;
-#pragma unroll
-  for (int a = 0; a < 8; ++a)
+  for (int q = 0; q < n; ++q) {
+    const double* gr = A.geom_ws + (q0 + q) * kGeomRecord;
+    const double* pr = A.point_ws + (q0 + q) * kPointRecord;
+    const double w = gr[kGrW];
+    double bu[6][3];
+    node_operator(gr, a, bu);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      double s = has_point ? bnl(r.F, pg.g, 0, a, k) * r.S[0] : 0.0;
+      double s = bu[0][k] * pr[kWsStress];
 #pragma unroll
-      for (int c = 1; c < 6; ++c) s = s + (has_point ? bnl(r.F, pg.g, c, a, k) * r.S[c] : 0.0);
-      const double tot = team_sum(w * s);
-      if (a == p) ru[k] = tot;
+      for (int c = 1; c < 6; ++c) s = s + bu[c][k] * pr[kWsStress + c];
+      ru[k] = ru[k] + w * s;
     }
-  const double energy = team_sum(w * (has_point ? r.energy : 0.0));
-  const int yielding = team_isum(has_point && r.yielding ? 1 : 0);
-  {
-    double y[6][9];
-#pragma unroll
-    for (int c = 0; c < 6; ++c)
-#pragma unroll
-      for (int n = 0; n < 9; ++n) {
-        double s = r.mat[c][0] * ba[0][n];
-#pragma unroll
-        for (int d = 1; d < 6; ++d) s = s + r.mat[c][d] * ba[d][n];
-        y[c][n] = s;
-      }
-    double geo[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    if constexpr (KIN != kSmall) {
-      const double S3[3][3] = {{r.S[0], r.S[3], r.S[5]}, {r.S[3], r.S[1], r.S[4]}, {r.S[5], r.S[4], r.S[2]}};
-      double gs[3][3];
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-          gs[a][j] = pg.g[0][8 + a] * S3[0][j] + pg.g[1][8 + a] * S3[1][j] + pg.g[2][8 + a] * S3[2][j];
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b)
-          geo[a][b] = gs[a][0] * pg.g[0][8 + b] + gs[a][1] * pg.g[1][8 + b] + gs[a][2] * pg.g[2][8 + b];
-    }
-#pragma unroll
-    for (int mm = 0; mm < 9; ++mm)
-#pragma unroll
-      for (int n = 0; n < 9; ++n) {
-        double s = ba[0][mm] * y[0][n];
-#pragma unroll
-        for (int c = 1; c < 6; ++c) s = s + ba[c][mm] * y[c][n];
-        double v = w * s;
-        if constexpr (KIN != kSmall) {
-          if (mm % 3 == n % 3) v = v + w * geo[mm / 3][n / 3];
-        }
-        const double tot = team_sum(v);
-        if (((mm * 9 + n) & 7) == p) T.lu.a[mm * 9 + n] = tot;
-      }
-  }
-  team_sync();
-  coop_full_lu(T.lu, p);
-  // trial states of the points
-  if (has_point) store_state(m, A.state_out, A.state_stride, e * A.rule.n + p, st);
-  if (p == 0) {
-    A.energy[e] = energy;
-    A.yielding[e] = yielding;
-  }
-  // Coupling blocks, point by point through the staging region: lane a owns
-  // rows 3a..3a+2 of K_ua (and K_uu) and columns 3a..3a+2 of K_au.
-  const int a = p;
-  double kua[3][9];
-  double kau[9][3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-#pragma unroll
-    for (int n = 0; n < 9; ++n) { kua[k][n] = 0.0; kau[n][k] = 0.0; }
-  const bool nonsym = !m.symmetric;
-  for (int q = 0; q < A.rule.n; ++q) {
-    if (p == q) {
-      double* s = T.region;
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) s[i * 3 + j] = r.F[i][j];
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 11; ++j) s[9 + i * 11 + j] = pg.g[i][j];
-#pragma unroll
-      for (int i = 0; i < 36; ++i) s[42 + i] = r.mat[i / 6][i % 6];
-#pragma unroll
-      for (int i = 0; i < 6; ++i) s[78 + i] = r.S[i];
-      s[84] = pg.w;
-    }
-    team_sync();
-    double Fq[3][3], gq[3][11], Mq[6][6], Sq[6];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) Fq[i][j] = T.region[i * 3 + j];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 11; ++j) gq[i][j] = T.region[9 + i * 11 + j];
-#pragma unroll
-    for (int i = 0; i < 36; ++i) Mq[i / 6][i % 6] = T.region[42 + i];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) Sq[i] = T.region[78 + i];
-    const double wq = T.region[84];
-    team_sync();
-    // z_k = B_u[:, 3a+k]^T mat (6)
     double z[3][6];
+    node_times_tangent(bu, pr, z);
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
+    for (int nn = 0; nn < 9; ++nn) {
+      double bcol[6];
 #pragma unroll
-      for (int d = 0; d < 6; ++d) {
-        double s = bnl(Fq, gq, 0, a, k) * Mq[0][d];
-#pragma unroll
-        for (int c = 1; c < 6; ++c) s = s + bnl(Fq, gq, c, a, k) * Mq[c][d];
-        z[k][d] = s;
-      }
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-      for (int n = 0; n < 9; ++n) {
-        double s = z[k][0] * bnl(Fq, gq, 0, 8 + n / 3, n % 3);
-#pragma unroll
-        for (int d = 1; d < 6; ++d) s = s + z[k][d] * bnl(Fq, gq, d, 8 + n / 3, n % 3);
-        kua[k][n] = kua[k][n] + wq * s;
-      }
-    if (nonsym && TANGENT) {
-      // K_au[:, 3a+k] = B_a^T (mat B_u[:, 3a+k])
+      for (int c = 0; c < 6; ++c) bcol[c] = bnl_rec(gr, c, 8 + nn / 3, nn % 3);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        double y[6];
+        double s = z[k][0] * bcol[0];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-          double s = Mq[c][0] * bnl(Fq, gq, 0, a, k);
-#pragma unroll
-          for (int d = 1; d < 6; ++d) s = s + Mq[c][d] * bnl(Fq, gq, d, a, k);
-          y[c] = s;
-        }
-#pragma unroll
-        for (int n = 0; n < 9; ++n) {
-          double s = bnl(Fq, gq, 0, 8 + n / 3, n % 3) * y[0];
-#pragma unroll
-          for (int c = 1; c < 6; ++c) s = s + bnl(Fq, gq, c, 8 + n / 3, n % 3) * y[c];
-          kau[n][k] = kau[n][k] + wq * s;
-        }
+        for (int d = 1; d < 6; ++d) s = s + z[k][d] * bcol[d];
+        kua[k][nn] = kua[k][nn] + w * s;
       }
     }
     if constexpr (KIN != kSmall) {
-      // geometric (G_aug^T S G_aug) (x) I: block (a, b) for the modes b = 8..10
+      // geometric (G_aug^T S G_aug) (x) I: block (a, 8 + bm)
+      const double* g = gr + kGrG;
       double gsa[3];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const double S3j0 = j == 0 ? Sq[0] : (j == 1 ? Sq[3] : Sq[5]);
-        const double S3j1 = j == 0 ? Sq[3] : (j == 1 ? Sq[1] : Sq[4]);
-        const double S3j2 = j == 0 ? Sq[5] : (j == 1 ? Sq[4] : Sq[2]);
-        gsa[j] = gq[0][a] * S3j0 + gq[1][a] * S3j1 + gq[2][a] * S3j2;
-      }
+      stress_times_gradient(gr, pr, a, gsa);
 #pragma unroll
       for (int bm = 0; bm < 3; ++bm) {
-        const double geo = gsa[0] * gq[0][8 + bm] + gsa[1] * gq[1][8 + bm] + gsa[2] * gq[2][8 + bm];
-        const double v = wq * geo;
+        const double geo = gsa[0] * g[0 * 11 + 8 + bm] + gsa[1] * g[1 * 11 + 8 + bm] + gsa[2] * g[2 * 11 + 8 + bm];
+        const double v = w * geo;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          kua[k][3 * bm + k] = kua[k][3 * bm + k] + v;
-          if (nonsym && TANGENT) kau[3 * bm + k][k] = kau[3 * bm + k][k] + v;
-        }
+        for (int k = 0; k < 3; ++k) kua[k][3 * bm + k] = kua[k][3 * bm + k] + v;
       }
     }
   }
-  // The force: f* = f_u - K_ua K_aa^-1 r_a
-  double corr[9];
-  lu_solve(T.lu, ra, corr);
+  // f* = f_u - K_ua K_aa^-1 r_a
+  {
+    double ra[9], corr[9];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    double s = kua[k][0] * corr[0];
+    for (int i = 0; i < 9; ++i) ra[i] = er[kErResidual + i];
+    lu_solve(T.lu, ra, corr);
 #pragma unroll
-    for (int n = 1; n < 9; ++n) s = s + kua[k][n] * corr[n];
-    A.force[e * 24 + 3 * a + k] = ru[k] - s;
+    for (int k = 0; k < 3; ++k) {
+      double s = kua[k][0] * corr[0];
+#pragma unroll
+      for (int nn = 1; nn < 9; ++nn) s = s + kua[k][nn] * corr[nn];
+      A.force[e * 24 + 3 * a + k] = ru[k] - s;
+    }
   }
   if constexpr (!TANGENT) return;
-  // coupling = K_aa^-1 K_au (or K_ua^T): lane a solves its columns 3a..3a+2
-  double cpl[9][3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    double col[9], x[9];
-#pragma unroll
-    for (int n = 0; n < 9; ++n) col[n] = nonsym ? kau[n][k] : kua[k][n];
-    lu_solve(T.lu, col, x);
-#pragma unroll
-    for (int n = 0; n < 9; ++n) cpl[n][k] = x[n];
-  }
-  // K_uu rows of node a, point by point again
-  double kuu[3][24];
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-#pragma unroll
-    for (int j = 0; j < 24; ++j) kuu[k][j] = 0.0;
-  team_sync();
-  for (int q = 0; q < A.rule.n; ++q) {
-    if (p == q) {
-      double* s = T.region;
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) s[i * 3 + j] = r.F[i][j];
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 11; ++j) s[9 + i * 11 + j] = pg.g[i][j];
-#pragma unroll
-      for (int i = 0; i < 36; ++i) s[42 + i] = r.mat[i / 6][i % 6];
-#pragma unroll
-      for (int i = 0; i < 6; ++i) s[78 + i] = r.S[i];
-      s[84] = pg.w;
-    }
-    team_sync();
-    double Fq[3][3], gq[3][11], Mq[6][6], Sq[6];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) Fq[i][j] = T.region[i * 3 + j];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 11; ++j) gq[i][j] = T.region[9 + i * 11 + j];
-#pragma unroll
-    for (int i = 0; i < 36; ++i) Mq[i / 6][i % 6] = T.region[42 + i];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) Sq[i] = T.region[78 + i];
-    const double wq = T.region[84];
-    team_sync();
-    double z[3][6];
+  // the coupling columns 3a..3a+2: K_aa^-1 K_ua^T, or K_aa^-1 K_au with
+  // K_au[:, 3a + k] = sum_q w B_a^T mat B_u[:, 3a + k] (+ geometric)
+  double col[3][9];
+  if constexpr (SYM) {
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
-      for (int d = 0; d < 6; ++d) {
-        double s = bnl(Fq, gq, 0, a, k) * Mq[0][d];
+      for (int nn = 0; nn < 9; ++nn) col[k][nn] = kua[k][nn];
+  } else {
 #pragma unroll
-        for (int c = 1; c < 6; ++c) s = s + bnl(Fq, gq, c, a, k) * Mq[c][d];
-        z[k][d] = s;
-      }
+    for (int k = 0; k < 3; ++k)
 #pragma unroll
-    for (int b = 0; b < 8; ++b)
+      for (int nn = 0; nn < 9; ++nn) col[k][nn] = 0.0;
+    for (int q = 0; q < n; ++q) {
+      const double* gr = A.geom_ws + (q0 + q) * kGeomRecord;
+      const double* pr = A.point_ws + (q0 + q) * kPointRecord;
+      const double w = gr[kGrW];
+      double bu[6][3];
+      node_operator(gr, a, bu);
+      // y = mat B_u[:, 3a + k]
+      double y[3][6];
 #pragma unroll
-      for (int kb = 0; kb < 3; ++kb) {
-        double bcol[6];
-#pragma unroll
-        for (int d = 0; d < 6; ++d) bcol[d] = bnl(Fq, gq, d, b, kb);
+      for (int c = 0; c < 6; ++c)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          double s = z[k][0] * bcol[0];
+          double s = pr[kWsTangent + c * 6] * bu[0][k];
 #pragma unroll
-          for (int d = 1; d < 6; ++d) s = s + z[k][d] * bcol[d];
-          kuu[k][3 * b + kb] = kuu[k][3 * b + kb] + wq * s;
+          for (int d = 1; d < 6; ++d) s = s + pr[kWsTangent + c * 6 + d] * bu[d][k];
+          y[k][c] = s;
+        }
+#pragma unroll
+      for (int nn = 0; nn < 9; ++nn) {
+        double bcol[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) bcol[c] = bnl_rec(gr, c, 8 + nn / 3, nn % 3);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          double s = bcol[0] * y[k][0];
+#pragma unroll
+          for (int c = 1; c < 6; ++c) s = s + bcol[c] * y[k][c];
+          col[k][nn] = col[k][nn] + w * s;
         }
       }
-    if constexpr (KIN != kSmall) {
-      double gsa[3];
+      if constexpr (KIN != kSmall) {
+        const double* g = gr + kGrG;
+        double gsa[3];
+        stress_times_gradient(gr, pr, a, gsa);
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const double S3j0 = j == 0 ? Sq[0] : (j == 1 ? Sq[3] : Sq[5]);
-        const double S3j1 = j == 0 ? Sq[3] : (j == 1 ? Sq[1] : Sq[4]);
-        const double S3j2 = j == 0 ? Sq[5] : (j == 1 ? Sq[4] : Sq[2]);
-        gsa[j] = gq[0][a] * S3j0 + gq[1][a] * S3j1 + gq[2][a] * S3j2;
-      }
+        for (int bm = 0; bm < 3; ++bm) {
+          const double geo = gsa[0] * g[0 * 11 + 8 + bm] + gsa[1] * g[1 * 11 + 8 + bm] + gsa[2] * g[2 * 11 + 8 + bm];
+          const double v = w * geo;
 #pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        const double geo = gsa[0] * gq[0][b] + gsa[1] * gq[1][b] + gsa[2] * gq[2][b];
-        const double v = wq * geo;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) kuu[k][3 * b + k] = kuu[k][3 * b + k] + v;
+          for (int k = 0; k < 3; ++k) col[k][3 * bm + k] = col[k][3 * bm + k] + v;
+        }
       }
     }
   }
-  // publish the coupling columns, then K* rows = K_uu - K_ua coupling
+  double cpl[3][9];
 #pragma unroll
-  for (int n = 0; n < 9; ++n)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) T.region[n * 24 + 3 * a + k] = cpl[n][k];
+  for (int k = 0; k < 3; ++k) lu_solve(T.lu, col[k], cpl[k]);
+  // publish the coupling (9 x 24)
   team_sync();
 #pragma unroll
-  for (int k = 0; k < 3; ++k)
+  for (int nn = 0; nn < 9; ++nn)
 #pragma unroll
-    for (int j = 0; j < 24; ++j) {
-      double s = kua[k][0] * T.region[j];
-#pragma unroll
-      for (int n = 1; n < 9; ++n) s = s + kua[k][n] * T.region[n * 24 + j];
-      kuu[k][j] = kuu[k][j] - s;
-    }
+    for (int k = 0; k < 3; ++k) T.region[nn * 24 + 3 * a + k] = cpl[k][nn];
   team_sync();
-  if (!nonsym) {
-    // K = (K + K^T) / 2 (IncompatibleModes.cpp:557), node block by node block.
-    // Sweep 1, round rr: lane rr publishes its (untouched) rows; every lane
-    // a > rr averages its block (a, rr) with the transpose of block (rr, a).
-    // Lane rr's blocks (rr, c) with c > rr are untouched until sweep 2.
-#pragma unroll
-    for (int rr = 0; rr < 7; ++rr) {
-      if (p == rr) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-          for (int j = 0; j < 24; ++j) T.region[k * 24 + j] = kuu[k][j];
-      }
-      team_sync();
-      if (a > rr) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-          for (int k2 = 0; k2 < 3; ++k2)
-            kuu[k][3 * rr + k2] = 0.5 * (kuu[k][3 * rr + k2] + T.region[k2 * 24 + 3 * a + k]);
-      }
-      team_sync();
-    }
-    // Diagonal blocks: both halves are this lane's own entries.
-    {
-      double dgb[3][3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int k2 = 0; k2 < 3; ++k2) {
-          double mine = 0.0, other = 0.0;
-#pragma unroll
-          for (int b = 0; b < 8; ++b)
-            if (b == a) { mine = kuu[k][3 * b + k2]; other = kuu[k2][3 * b + k]; }
-          dgb[k][k2] = 0.5 * (mine + other);
-        }
-#pragma unroll
-      for (int b = 0; b < 8; ++b)
-        if (b == a) {
-#pragma unroll
-          for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int k2 = 0; k2 < 3; ++k2) kuu[k][3 * b + k2] = dgb[k][k2];
-        }
-    }
-    // Sweep 2, round rr: every lane a > rr publishes its averaged block
-    // (a, rr); lane rr copies the transposes into its blocks (rr, a).
-#pragma unroll
-    for (int rr = 0; rr < 7; ++rr) {
-      if (a > rr) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-          for (int k2 = 0; k2 < 3; ++k2) T.region[a * 9 + k * 3 + k2] = kuu[k][3 * rr + k2];
-      }
-      team_sync();
-      if (p == rr) {
-#pragma unroll
-        for (int c = rr + 1; c < 8; ++c)
-#pragma unroll
-          for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int k2 = 0; k2 < 3; ++k2) kuu[k2][3 * c + k] = T.region[c * 9 + k * 3 + k2];
-      }
-      team_sync();
-    }
-  }
   double* out = A.tangent + e * 576 + (3 * a) * 24;
+  // K_uu rows of node a in two column halves, K* = K_uu - K_ua coupling
 #pragma unroll
-  for (int k = 0; k < 3; ++k)
+  for (int half = 0; half < 2; ++half) {
+    double kuu[3][12];
 #pragma unroll
-    for (int j = 0; j < 24; ++j) out[k * 24 + j] = kuu[k][j];
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int j = 0; j < 12; ++j) kuu[k][j] = 0.0;
+    for (int q = 0; q < n; ++q) {
+      const double* gr = A.geom_ws + (q0 + q) * kGeomRecord;
+      const double* pr = A.point_ws + (q0 + q) * kPointRecord;
+      const double w = gr[kGrW];
+      double bu[6][3];
+      node_operator(gr, a, bu);
+      double z[3][6];
+      node_times_tangent(bu, pr, z);
+#pragma unroll
+      for (int bb = 0; bb < 4; ++bb)
+#pragma unroll
+        for (int kb = 0; kb < 3; ++kb) {
+          double bcol[6];
+#pragma unroll
+          for (int d = 0; d < 6; ++d) bcol[d] = bnl_rec(gr, d, 4 * half + bb, kb);
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            double s = z[k][0] * bcol[0];
+#pragma unroll
+            for (int d = 1; d < 6; ++d) s = s + z[k][d] * bcol[d];
+            kuu[k][3 * bb + kb] = kuu[k][3 * bb + kb] + w * s;
+          }
+        }
+      if constexpr (KIN != kSmall) {
+        const double* g = gr + kGrG;
+        double gsa[3];
+        stress_times_gradient(gr, pr, a, gsa);
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb) {
+          const int b = 4 * half + bb;
+          const double geo = gsa[0] * g[0 * 11 + b] + gsa[1] * g[1 * 11 + b] + gsa[2] * g[2 * 11 + b];
+          const double v = w * geo;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) kuu[k][3 * bb + k] = kuu[k][3 * bb + k] + v;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int jj = 0; jj < 12; ++jj) {
+        const int j = 12 * half + jj;
+        double s = kua[k][0] * T.region[j];
+#pragma unroll
+        for (int nn = 1; nn < 9; ++nn) s = s + kua[k][nn] * T.region[nn * 24 + j];
+        out[k * 24 + j] = kuu[k][jj] - s;
+      }
+  }
+  if constexpr (SYM) {
+    // K = (K + K^T) / 2 (IncompatibleModes.cpp:557) through the output rows
+    team_sync();
+    double ks[3][24];
+    const double* base = A.tangent + e * 576;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int j = 0; j < 24; ++j) ks[k][j] = 0.5 * (out[k * 24 + j] + base[j * 24 + 3 * a + k]);
+    team_sync();
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int j = 0; j < 24; ++j) out[k * 24 + j] = ks[k][j];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1103,25 +1069,25 @@ int launch_local_solve(const ElementArgs& a, hipStream_t stream) {
   return static_cast<int>(hipGetLastError());
 }
 
+template <int KIN>
+void launch_condense_kin(const ElementArgs& a, bool tangent, unsigned int blocks, hipStream_t stream) {
+  if (!tangent)
+    hipLaunchKernelGGL((condense_kernel<KIN, false, true>), dim3(blocks), dim3(64), 0, stream, a);
+  else if (a.mat.symmetric)
+    hipLaunchKernelGGL((condense_kernel<KIN, true, true>), dim3(blocks), dim3(64), 0, stream, a);
+  else
+    hipLaunchKernelGGL((condense_kernel<KIN, true, false>), dim3(blocks), dim3(64), 0, stream, a);
+}
+
 int launch_condense(const ElementArgs& a, bool tangent, hipStream_t stream) {
   (void)hipGetLastError();
   const unsigned int blocks = static_cast<unsigned int>((a.ne + kWaveTeams - 1) / kWaveTeams);
   if (blocks == 0) return 0;
-#define SBK_LAUNCH(K, T) hipLaunchKernelGGL((condense_kernel<K, T>), dim3(blocks), dim3(64), 0, stream, a)
-  if (tangent) {
-    switch (a.kinematics) {
-      case kSmall: SBK_LAUNCH(kSmall, true); break;
-      case kFinite: SBK_LAUNCH(kFinite, true); break;
-      default: SBK_LAUNCH(kLog, true); break;
-    }
-  } else {
-    switch (a.kinematics) {
-      case kSmall: SBK_LAUNCH(kSmall, false); break;
-      case kFinite: SBK_LAUNCH(kFinite, false); break;
-      default: SBK_LAUNCH(kLog, false); break;
-    }
+  switch (a.kinematics) {
+    case kSmall: launch_condense_kin<kSmall>(a, tangent, blocks, stream); break;
+    case kFinite: launch_condense_kin<kFinite>(a, tangent, blocks, stream); break;
+    default: launch_condense_kin<kLog>(a, tangent, blocks, stream); break;
   }
-#undef SBK_LAUNCH
   return static_cast<int>(hipGetLastError());
 }
 

@@ -36,7 +36,34 @@ struct ElementArgs {
   int* iterations;    // local Newton iterations per element
   int* status;        // per element: 0 or an ElementStatus
   double* fail_value; // per element: the determinant of a refused Jacobian
+  // Workspace (device scratch of one call), three regions:
+  //   point_ws  [npts][42]: S (6), then the tangent (36, row-major) of each
+  //             point's final local pass (written by the local solve)
+  //   geom_ws   [npts][44]: F (9), [G | G~] (33), w of each point at the
+  //             converged alpha (condensation scratch)
+  //   elem_ws   [ne][108]:  K_aa's full-pivot LU factors (81, row-major),
+  //             r_a (9), row and column transpositions (9 + 9, as doubles)
+  //             of the final local iteration
+  // with npts = ne * rule.n, point q = e * rule.n + p.
+  double* point_ws;
+  double* geom_ws;
+  double* elem_ws;
 };
+
+constexpr int kPointRecord = 42;
+constexpr int kGeomRecord = 44;
+constexpr int kElemRecord = 108;
+// Doubles of workspace for ne elements of `points` points each.
+inline long long workspace_doubles(long long ne, int points) {
+  return ne * points * (kPointRecord + kGeomRecord) + ne * kElemRecord;
+}
+// Points the three regions of ElementArgs into one workspace.
+inline void set_workspace(ElementArgs& a, double* ws) {
+  const long long npts = a.ne * a.rule.n;
+  a.point_ws = ws;
+  a.geom_ws = ws + npts * kPointRecord;
+  a.elem_ws = ws + npts * (kPointRecord + kGeomRecord);
+}
 
 int launch_local_solve(const ElementArgs& a, hipStream_t stream);
 int launch_condense(const ElementArgs& a, bool tangent, hipStream_t stream);

@@ -152,6 +152,67 @@ __device__ __forceinline__ double bnl(const double F[3][3], const double g[3][11
     default: return F[k][2] * g[0][a] + F[k][0] * g[2][a];
   }
 }
+// The same entry for mode b (column 3 (8 + b) + k), from the mode gradients
+// gm[i][b] = G~[i][b] alone.
+__device__ __forceinline__ double bnl_mode(const double F[3][3], const double gm[3][3], int c, int b, int k) {
+  switch (c) {
+    case 0: return F[k][0] * gm[0][b];
+    case 1: return F[k][1] * gm[1][b];
+    case 2: return F[k][2] * gm[2][b];
+    case 3: return F[k][0] * gm[1][b] + F[k][1] * gm[0][b];
+    case 4: return F[k][1] * gm[2][b] + F[k][2] * gm[1][b];
+    default: return F[k][2] * gm[0][b] + F[k][0] * gm[2][b];
+  }
+}
+
+// What a pass at alpha needs of the lane's point: the mode gradients, the
+// nodal displacement gradient and the weight (and for small strain the nodal
+// part of the strain, B_u ue); 19 (25) doubles instead of the 43 of PointGeom.
+struct ModeGeom {
+  double gm[3][3];  // gm[i][b] = G~[i][b]
+  double hu[3][3];  // U G^T
+  double w;
+  double eps_u[6];  // small strain only
+};
+
+template <int KIN>
+__device__ __forceinline__ void mode_geom(const PointGeom& pg, const double ue[24], ModeGeom& mg) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) { mg.gm[i][j] = pg.g[i][8 + j]; mg.hu[i][j] = pg.hu[i][j]; }
+  mg.w = pg.w;
+  if constexpr (KIN == kSmall) {
+    double I3[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double s1 = bnl(I3, pg.g, c, 0, 0) * ue[0];
+#pragma unroll
+      for (int kk = 1; kk < 24; ++kk) s1 = s1 + bnl(I3, pg.g, c, kk / 3, kk % 3) * ue[kk];
+      mg.eps_u[c] = s1;
+    }
+  }
+}
+
+// H = hu + A G~^T (IncompatibleModes.cpp:195-205); F = I + H except for small
+// strain (F = I). Every kernel forms F with this function, so a pass and the
+// condensation at the same alpha see the same bits.
+template <int KIN>
+__device__ __forceinline__ void mode_kinematics(const double hu[3][3], const double gm[3][3], const double alpha[9],
+                                                double h[3][3], double F[3][3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      h[i][j] = hu[i][j] + (alpha[i] * gm[j][0] + alpha[3 + i] * gm[j][1] + alpha[6 + i] * gm[j][2]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      if constexpr (KIN == kSmall) F[i][j] = (i == j) ? 1.0 : 0.0;
+      else F[i][j] = (i == j) ? 1.0 + h[i][j] : h[i][j];
+    }
+}
 
 // The response of the lane's point at alpha: F (I for small strain), the
 // stress conjugate to B_NL, the tangent between B_NL's, the point's new state,
@@ -166,35 +227,20 @@ struct PointResponse {
 };
 
 template <int KIN>
-__device__ __forceinline__ int point_response(const DevMaterial& m, const PointGeom& pg, const double ue[24],
-                                              const double alpha[9], PointState& st, PointResponse& r) {
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-      r.h[i][j] = pg.hu[i][j] + (alpha[i] * pg.g[j][8] + alpha[3 + i] * pg.g[j][9] + alpha[6 + i] * pg.g[j][10]);
+__device__ __forceinline__ int point_response(const DevMaterial& m, const ModeGeom& mg, const double alpha[9],
+                                              PointState& st, PointResponse& r) {
+  mode_kinematics<KIN>(mg.hu, mg.gm, alpha, r.h, r.F);
   double strain[6];
   LogData L;
   if constexpr (KIN == kSmall) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) r.F[i][j] = (i == j) ? 1.0 : 0.0;
-#pragma unroll
     for (int c = 0; c < 6; ++c) {
-      double s1 = bnl(r.F, pg.g, c, 0, 0) * ue[0];
+      double s2 = bnl_mode(r.F, mg.gm, c, 0, 0) * alpha[0];
 #pragma unroll
-      for (int kk = 1; kk < 24; ++kk) s1 = s1 + bnl(r.F, pg.g, c, kk / 3, kk % 3) * ue[kk];
-      double s2 = bnl(r.F, pg.g, c, 8, 0) * alpha[0];
-#pragma unroll
-      for (int kk = 1; kk < 9; ++kk) s2 = s2 + bnl(r.F, pg.g, c, 8 + kk / 3, kk % 3) * alpha[kk];
-      strain[c] = s1 + s2;
+      for (int kk = 1; kk < 9; ++kk) s2 = s2 + bnl_mode(r.F, mg.gm, c, kk / 3, kk % 3) * alpha[kk];
+      strain[c] = mg.eps_u[c] + s2;
     }
   } else {
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) r.F[i][j] = (i == j) ? 1.0 + r.h[i][j] : r.h[i][j];
     if constexpr (KIN == kLog) {
       if (!(det3(r.F) > 0.0)) return kInverted;
     }

@@ -45,6 +45,7 @@ struct sparlab_system {
   int32_t *yielding = nullptr, *iterations = nullptr, *status = nullptr;
   int32_t *block_ptr = nullptr, *block_contrib = nullptr, *block_pos = nullptr, *block_row_len = nullptr;
   int32_t *node_ptr = nullptr, *node_incidence = nullptr, *diag_block = nullptr;
+  double* ws = nullptr;         // the element kernels' point workspace
   double* partial = nullptr;    // 1024 partial sums
   double* scalars = nullptr;    // [0] energy, [1] gross
   long long* summary = nullptr; // failed count, lowest failed element, yielding points
@@ -77,7 +78,7 @@ void release(sparlab_system* s) {
   dfree(s->yielding); dfree(s->iterations); dfree(s->status);
   dfree(s->block_ptr); dfree(s->block_contrib); dfree(s->block_pos); dfree(s->block_row_len);
   dfree(s->node_ptr); dfree(s->node_incidence); dfree(s->diag_block);
-  dfree(s->partial); dfree(s->scalars); dfree(s->summary);
+  dfree(s->ws); dfree(s->partial); dfree(s->scalars); dfree(s->summary);
 }
 
 }  // namespace
@@ -110,7 +111,9 @@ int sparlab_system_create(const double* coords, int64_t num_nodes, const int32_t
             dalloc(s->state[1], hist) && dalloc(s->alpha[0], 9 * ne) && dalloc(s->alpha[1], 9 * ne) &&
             dalloc(s->fe, 24 * ne) && dalloc(s->ke, 576 * ne) && dalloc(s->energy, ne) &&
             dalloc(s->fail_value, ne) && dalloc(s->gross, 3 * nn) && dalloc(s->yielding, ne) &&
-            dalloc(s->iterations, ne) && dalloc(s->status, ne) && dalloc(s->partial, 1024) &&
+            dalloc(s->iterations, ne) && dalloc(s->status, ne) &&
+            dalloc(s->ws, static_cast<std::size_t>(sbk::workspace_doubles(num_elements, s->points))) &&
+            dalloc(s->partial, 1024) &&
             dalloc(s->scalars, 2) && dalloc(s->summary, 3) && upload(s->block_ptr, s->pattern.block_ptr) &&
             upload(s->block_contrib, s->pattern.block_contrib) && upload(s->block_pos, s->pattern.block_pos) &&
             upload(s->block_row_len, s->pattern.block_row_len) && upload(s->node_ptr, s->pattern.node_ptr) &&
@@ -186,6 +189,7 @@ int sparlab_system_evaluate_device(sparlab_system* s, const double* u, int want_
   a.iterations = s->iterations;
   a.status = s->status;
   a.fail_value = s->fail_value;
+  sbk::set_workspace(a, s->ws);
   if (sbk::launch_local_solve(a, st) != 0 || sbk::launch_condense(a, want_tangent != 0, st) != 0)
     return sbk::capi_fail(SPARLAB_E_DEVICE, "element kernel launch failed");
   sbk::ForceArgs f{s->fe, s->node_ptr, s->node_incidence, static_cast<long long>(s->nn), fint, s->gross};

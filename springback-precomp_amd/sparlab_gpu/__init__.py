@@ -82,11 +82,16 @@ class CElementBatch(ctypes.Structure):
         ("state_out", ctypes.c_void_p), ("alpha_out", ctypes.c_void_p),
         ("force", ctypes.c_void_p), ("tangent", ctypes.c_void_p), ("energy", ctypes.c_void_p),
         ("yielding", ctypes.c_void_p), ("iterations", ctypes.c_void_p), ("status", ctypes.c_void_p),
-        ("fail_value", ctypes.c_void_p),
+        ("fail_value", ctypes.c_void_p), ("workspace", ctypes.c_void_p),
     ]
 
 
 _lib = None
+
+
+def workspace_doubles(num_elements, points):
+    """Doubles of device scratch an element batch needs (sparlab_element_workspace_doubles)."""
+    return num_elements * points * (42 + 44) + num_elements * 108
 
 
 def lib():
@@ -107,6 +112,8 @@ def lib():
         _lib.sparlab_material_finalize.argtypes = [ctypes.POINTER(CMaterial)]
         _lib.sparlab_rule_points.argtypes = [ctypes.POINTER(CRule)]
         _lib.sparlab_state_fields.argtypes = [ctypes.POINTER(CMaterial)]
+        _lib.sparlab_element_workspace_doubles.restype = ctypes.c_int64
+        _lib.sparlab_element_workspace_doubles.argtypes = [ctypes.c_int64, ctypes.POINTER(CRule)]
         _lib.sparlab_elements_evaluate_device.argtypes = [
             ctypes.POINTER(CMaterial), ctypes.POINTER(CRule), ctypes.c_int, ctypes.c_int,
             ctypes.POINTER(CElementBatch), ctypes.c_void_p]
@@ -277,6 +284,8 @@ class ElementBatch:
         self.iterations = torch.empty((ne,), **i32)
         self.status = torch.empty((ne,), **i32)
         self.fail_value = torch.empty((ne,), **f64)
+        # device scratch of the element kernels (86 doubles per point, 108 per element)
+        self.workspace = torch.empty((workspace_doubles(ne, points),), **f64)
         for name in ("coords", "u", "state_in"):
             t = getattr(self, name)
             if not (t.dtype == torch.float64 and t.is_contiguous() and t.device == dev):
@@ -287,7 +296,7 @@ class ElementBatch:
             _ptr(coords), _ptr(conn), _ptr(u), coords.shape[0], ne, _ptr(state_in), _ptr(alpha_in),
             _ptr(self.state_out), _ptr(self.alpha_out), _ptr(self.force), _ptr(self.tangent),
             _ptr(self.energy), _ptr(self.yielding), _ptr(self.iterations), _ptr(self.status),
-            _ptr(self.fail_value))
+            _ptr(self.fail_value), _ptr(self.workspace))
 
 
 def evaluate_elements(material, rule, batch, kinematics=FINITE_LOGARITHMIC, want_tangent=True, stream=None):
