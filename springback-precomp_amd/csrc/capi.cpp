@@ -120,6 +120,43 @@ int kinematic_terms(const sparlab_material& m) {
 bool general_law(const sparlab_material& m) { return m.criterion == 1 || m.num_backstresses > 0; }
 int kin_fields(const sparlab_material& m) { return general_law(m) ? kinematic_terms(m) : 1; }
 
+// Eigen-decomposition of the symmetric 6 x 6 matrix a by cyclic Jacobi
+// rotations (a is destroyed): a = V diag(l) V^T, V row-major, column k the
+// eigenvector of l[k].
+void symmetric_eigen6(double a[6][6], double l[6], double V[36]) {
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) V[i * 6 + j] = i == j ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < 6; ++p)
+      for (int q = p + 1; q < 6; ++q) off += a[p][q] * a[p][q];
+    if (off == 0.0) break;
+    for (int p = 0; p < 5; ++p)
+      for (int q = p + 1; q < 6; ++q) {
+        if (a[p][q] == 0.0) continue;
+        const double theta = 0.5 * (a[q][q] - a[p][p]) / a[p][q];
+        const double t = (theta >= 0.0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for (int k = 0; k < 6; ++k) {
+          const double akp = a[k][p], akq = a[k][q];
+          a[k][p] = c * akp - sn * akq;
+          a[k][q] = sn * akp + c * akq;
+        }
+        for (int k = 0; k < 6; ++k) {
+          const double apk = a[p][k], aqk = a[q][k];
+          a[p][k] = c * apk - sn * aqk;
+          a[q][k] = sn * apk + c * aqk;
+        }
+        for (int k = 0; k < 6; ++k) {
+          const double vkp = V[k * 6 + p], vkq = V[k * 6 + q];
+          V[k * 6 + p] = c * vkp - sn * vkq;
+          V[k * 6 + q] = sn * vkp + c * vkq;
+        }
+      }
+  }
+  for (int k = 0; k < 6; ++k) l[k] = a[k][k];
+}
+
 sbk::DevMaterial device_material(const sparlab_material& m) {
   sbk::DevMaterial d{};
   d.shear = m.youngs_modulus / (2.0 * (1.0 + m.poisson_ratio));
@@ -131,6 +168,16 @@ sbk::DevMaterial device_material(const sparlab_material& m) {
   d.qsat = m.saturation_stress;
   d.delta = m.saturation_rate;
   std::memcpy(d.P, m.yield_matrix, sizeof(d.P));
+  {
+    // D P D with D = diag(1, 1, 1, 2^-1/2 (x3)), for the block solve of the
+    // general return's Jacobian
+    double dpd[6][6];
+    const double dd[6] = {1.0, 1.0, 1.0, std::sqrt(0.5), std::sqrt(0.5), std::sqrt(0.5)};
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 6; ++j) dpd[i][j] = 0.5 * (dd[i] * m.yield_matrix[i * 6 + j] * dd[j] +
+                                                     dd[j] * m.yield_matrix[j * 6 + i] * dd[i]);
+    symmetric_eigen6(dpd, d.L, d.V);
+  }
   d.general = general_law(m) ? 1 : 0;
   d.terms = kinematic_terms(m);
   for (int i = 0; i < d.terms; ++i) {

@@ -31,6 +31,9 @@ struct DevMaterial {
   double shear, bulk;            // G, K = lambda + 2G/3
   double sy0, hard, hkin, qsat, delta;
   double P[36];                  // yield matrix, tensorial Voigt, global axes
+  // D P D = V diag(L) V^T with D = diag(1, 1, 1, 2^-1/2, 2^-1/2, 2^-1/2):
+  // V row-major (column k is eigenvector k), L the eigenvalues
+  double V[36], L[6];
   double mod[kMaxTerms], rec[kMaxTerms];  // kinematic terms: Chaboche, then Prager
   int general;                   // Hill48 or backstresses: closest-point return
   int terms;                     // kinematic terms of the general return
@@ -104,8 +107,17 @@ __device__ __forceinline__ int team_imin(int v) {
   v = min(v, dpp_i32<kHalfMirror>(v));
   return v;
 }
-// Orders the team's lanes' LDS traffic (one wavefront: in-order LDS queue).
+// Orders the team's lanes' LDS traffic. A team lives in one wavefront, whose
+// LDS instructions execute in issue order, so wavefront-scope fences suffice:
+// they keep the compiler from moving LDS accesses across and cost no waits.
 __device__ __forceinline__ void team_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// Orders global-memory traffic between the lanes of a team (the stores are
+// waited for, so other lanes' loads see them).
+__device__ __forceinline__ void team_sync_global() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -244,7 +256,6 @@ struct LogData {
   double mu[3];      // principal Green-Lagrange strains
   double first[3][3];
   double strain[6];  // E_log, engineering
-  double P[6][6];    // dE_log = P dE
 };
 
 __device__ __forceinline__ double log_divided(double mx, double my) {
@@ -300,6 +311,13 @@ __device__ __forceinline__ int logarithmic_strain(const double e[6], LogData& L)
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = a; b < 3; ++b) L.first[a][b] = L.first[b][a] = log_divided(L.mu[a], L.mu[b]);
+  return 0;
+}
+
+// The projection dE_log = P dE (LogarithmicStrain.cpp:128-141), formed from the
+// axes and first divided differences when it is needed (after the return, so
+// it is not held across it).
+__device__ __forceinline__ void log_projection(const LogData& L, double P[6][6]) {
   double m[6][6];
   principal_rotation(L.q, m);
   double sd[6];
@@ -312,9 +330,8 @@ __device__ __forceinline__ int logarithmic_strain(const double e[6], LogData& L)
       double s = m[0][r] * sd[0] * m[0][c];
 #pragma unroll
       for (int k = 1; k < 6; ++k) s = s + m[k][r] * sd[k] * m[k][c];
-      L.P[r][c] = c < 3 ? s : s * 0.5;
+      P[r][c] = c < 3 ? s : s * 0.5;
     }
-  return 0;
 }
 
 __device__ __forceinline__ double log_divided2(const LogData& L, int a, int b, int c) {

@@ -243,75 +243,86 @@ __device__ __forceinline__ double flow_curv(const DevMaterial& mat, const GenPoi
   return i < 3 ? v : v * 0.5;
 }
 
-__device__ __forceinline__ void gen_jacobian(const DevMaterial& mat, const PointState& c, const GenPoint& e,
-                                             double dl, double J[7][7]) {
-  const double r = e.b / e.eq;
+// The closest-point Jacobian (Plasticity.cpp:278-300)
+//   J = [[A, c], [m^T, -h]],  A = I + r S^-1 (P - m m^T),  r = b / sigma_bar,
+//   c = back_rate + db S^-1 m,  h = the yield slope,  S^-1 = diag(1, 1, 1, 1/2, 1/2, 1/2),
+// solved by blocks instead of a pivoted 7 x 7 LU: B = I + r S^-1 P =
+// D V (I + r L) V^T D^-1 with D P D = V L V^T (the material's, from the host),
+// A^-1 = B^-1 + B^-1 u m^T B^-1 / (1 - m^T B^-1 u) with u = r S^-1 m
+// (Sherman-Morrison), and the Schur complement m^T A^-1 c + h for the plastic
+// multiplier. A is similar to an SPD matrix with eigenvalues >= 1 (P - m m^T is
+// the Hessian of the convex sigma_bar times sigma_bar), so no pivoting is needed.
+struct JacSolver {
+  double m[6];   // flow direction
+  double d[6];   // 1 / (1 + r L_k)
+  double z[6];   // B^-1 u
+  double sm;     // 1 / (1 - m^T z)
+  double wc[6];  // A^-1 c
+  double schur;  // 1 / (m^T A^-1 c + h)
+};
+
+__device__ __forceinline__ double dot6(const double a[6], const double b[6]) {
+  double s = a[0] * b[0];
 #pragma unroll
-  for (int i = 0; i < 6; ++i)
-#pragma unroll
-    for (int k = 0; k < 6; ++k) J[i][k] = ((i == k) ? 1.0 : 0.0) + r * flow_curv(mat, e, i, k);
-#pragma unroll
-  for (int i = 0; i < 6; ++i) J[i][6] = e.back_rate[i] + e.db * (i < 3 ? e.m[i] : e.m[i] * 0.5);
-#pragma unroll
-  for (int k = 0; k < 6; ++k) J[6][k] = e.m[k];
-  J[6][6] = -yield_slope(mat, c.abar + dl);
+  for (int i = 1; i < 6; ++i) s = s + a[i] * b[i];
+  return s;
 }
 
-// Partial-pivot LU of a 7 x 7 matrix in registers (first largest |a_ik| is
-// the pivot). Row swaps use selects so no register array is indexed at run time.
-__device__ __forceinline__ void lu7(double a[7][7], int piv[7]) {
+__device__ __forceinline__ void b_inverse(const DevMaterial& mat, const double d[6], const double v[6], double out[6]) {
+  constexpr double kRoot2 = 1.4142135623730951;
+  constexpr double kInvRoot2 = 0.7071067811865476;
+  const double dv[6] = {v[0], v[1], v[2], kRoot2 * v[3], kRoot2 * v[4], kRoot2 * v[5]};
+  double t[6];
 #pragma unroll
-  for (int k = 0; k < 7; ++k) {
-    int br = k;
-    double big = fabs(a[k][k]);
+  for (int k = 0; k < 6; ++k) {
+    double s = mat.V[k] * dv[0];
 #pragma unroll
-    for (int i = k + 1; i < 7; ++i) {
-      const double v = fabs(a[i][k]);
-      if (v > big) { big = v; br = i; }
-    }
-    piv[k] = br;
-    if (big != 0.0) {
+    for (int i = 1; i < 6; ++i) s = s + mat.V[i * 6 + k] * dv[i];
+    t[k] = d[k] * s;
+  }
 #pragma unroll
-      for (int i = k + 1; i < 7; ++i) {
-        if (i == br) {
+  for (int i = 0; i < 6; ++i) {
+    double s = mat.V[i * 6] * t[0];
 #pragma unroll
-          for (int j = 0; j < 7; ++j) { const double t = a[k][j]; a[k][j] = a[i][j]; a[i][j] = t; }
-        }
-      }
-#pragma unroll
-      for (int i = k + 1; i < 7; ++i) a[i][k] = a[i][k] / a[k][k];
-    }
-#pragma unroll
-    for (int i = k + 1; i < 7; ++i)
-#pragma unroll
-      for (int j = k + 1; j < 7; ++j) a[i][j] = a[i][j] - a[i][k] * a[k][j];
+    for (int k = 1; k < 6; ++k) s = s + mat.V[i * 6 + k] * t[k];
+    out[i] = i < 3 ? s : kInvRoot2 * s;
   }
 }
-template <int NR>
-__device__ __forceinline__ void lu7_solve(const double a[7][7], const int piv[7], double b[7][NR]) {
+
+__device__ __forceinline__ void a_inverse(const DevMaterial& mat, const JacSolver& J, const double v[6], double out[6]) {
+  b_inverse(mat, J.d, v, out);
+  const double f = dot6(J.m, out) * J.sm;
 #pragma unroll
-  for (int k = 0; k < 7; ++k) {
+  for (int i = 0; i < 6; ++i) out[i] = out[i] + J.z[i] * f;
+}
+
+__device__ __forceinline__ void jac_setup(const DevMaterial& mat, const GenPoint& e, double dl, double abar,
+                                          JacSolver& J) {
+  const double r = e.b / e.eq;
 #pragma unroll
-    for (int i = k + 1; i < 7; ++i) {
-      if (i == piv[k]) {
+  for (int i = 0; i < 6; ++i) J.m[i] = e.m[i];
 #pragma unroll
-        for (int j = 0; j < NR; ++j) { const double t = b[k][j]; b[k][j] = b[i][j]; b[i][j] = t; }
-      }
-    }
-  }
+  for (int k = 0; k < 6; ++k) J.d[k] = 1.0 / (1.0 + r * mat.L[k]);
+  double u[6];
 #pragma unroll
-  for (int j = 0; j < NR; ++j) {
+  for (int i = 0; i < 6; ++i) u[i] = r * (i < 3 ? e.m[i] : e.m[i] * 0.5);
+  b_inverse(mat, J.d, u, J.z);
+  J.sm = 1.0 / (1.0 - dot6(J.m, J.z));
+  double c[6];
 #pragma unroll
-    for (int i = 1; i < 7; ++i)
+  for (int i = 0; i < 6; ++i) c[i] = e.back_rate[i] + e.db * (i < 3 ? e.m[i] : e.m[i] * 0.5);
+  a_inverse(mat, J, c, J.wc);
+  J.schur = 1.0 / (dot6(J.m, J.wc) + yield_slope(mat, abar + dl));
+}
+
+// [x; y] = J^-1 [b1; b2]
+__device__ __forceinline__ void jac_solve(const DevMaterial& mat, const JacSolver& J, const double b1[6], double b2,
+                                          double x[6], double& y) {
+  double w1[6];
+  a_inverse(mat, J, b1, w1);
+  y = (dot6(J.m, w1) - b2) * J.schur;
 #pragma unroll
-      for (int k = 0; k < i; ++k) b[i][j] = b[i][j] - a[i][k] * b[k][j];
-#pragma unroll
-    for (int i = 6; i >= 0; --i) {
-#pragma unroll
-      for (int k = i + 1; k < 7; ++k) b[i][j] = b[i][j] - a[i][k] * b[k][j];
-      b[i][j] = b[i][j] / a[i][i];
-    }
-  }
+  for (int i = 0; i < 6; ++i) x[i] = w1[i] - J.wc[i] * y;
 }
 
 __device__ __forceinline__ double norm7(const double r[7]) {
@@ -328,46 +339,47 @@ __device__ __forceinline__ double maxabs7(const double r[7]) {
 }
 
 // C_alg = C^e - 2G [m_t a^T + (dl / sigma_bar) S^-1 (P - m m^T) A],
-// [A; a^T] = J^-1 [D_dev; 0] (Plasticity.cpp:302-311).
+// [A; a^T] = J^-1 [D_dev; 0] (Plasticity.cpp:302-311), one column at a time.
 __device__ __forceinline__ void gen_tangent(const DevMaterial& mat, const PointState& c, const GenPoint& e,
                                             double dl, double C[6][6]) {
   const double g = mat.shear;
-  double J[7][7];
-  gen_jacobian(mat, c, e, dl, J);
-  int piv[7];
-  lu7(J, piv);
-  double da[7][6];
-#pragma unroll
-  for (int i = 0; i < 7; ++i)
-#pragma unroll
-    for (int j = 0; j < 6; ++j) da[i][j] = 0.0;
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-#pragma unroll
-    for (int j = 0; j < 3; ++j) da[i][j] = -2.0 * g / 3.0;
-    da[i][i] = 4.0 * g / 3.0;
-    da[3 + i][3 + i] = g;
-  }
-  lu7_solve<6>(J, piv, da);
+  JacSolver J;
+  jac_setup(mat, e, dl, c.abar, J);
   elastic_C(mat.bulk, g, C);
   const double g2 = 2.0 * g;
+  const double s = 2.0 * g * dl / e.eq;
+#pragma unroll 1
+  for (int j = 0; j < 6; ++j) {
+    double b1[6];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) {
-    const double mt = i < 3 ? e.m[i] : e.m[i] * 0.5;
+    for (int i = 0; i < 6; ++i) {
+      if (i < 3 && j < 3) b1[i] = (i == j) ? 4.0 * g / 3.0 : -2.0 * g / 3.0;
+      else b1[i] = (i == j) ? g : 0.0;
+    }
+    double x[6], y;
+    jac_solve(mat, J, b1, 0.0, x, y);
+    double col[6];
 #pragma unroll
-    for (int j = 0; j < 6; ++j) C[i][j] = C[i][j] - g2 * (mt * da[6][j]);
-  }
-  if (dl > 0.0) {
-    const double s = 2.0 * g * dl / e.eq;
+    for (int i = 0; i < 6; ++i) {
+      double cij = 0.0;
+#pragma unroll
+      for (int jj = 0; jj < 6; ++jj) if (jj == j) cij = C[i][jj];
+      const double mt = i < 3 ? e.m[i] : e.m[i] * 0.5;
+      col[i] = cij - g2 * (mt * y);
+    }
+    if (dl > 0.0) {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        double acc = flow_curv(mat, e, i, 0) * x[0];
+#pragma unroll
+        for (int k = 1; k < 6; ++k) acc = acc + flow_curv(mat, e, i, k) * x[k];
+        col[i] = col[i] - s * acc;
+      }
+    }
 #pragma unroll
     for (int i = 0; i < 6; ++i)
 #pragma unroll
-      for (int j = 0; j < 6; ++j) {
-        double acc = flow_curv(mat, e, i, 0) * da[0][j];
-#pragma unroll
-        for (int k = 1; k < 6; ++k) acc = acc + flow_curv(mat, e, i, k) * da[k][j];
-        C[i][j] = C[i][j] - s * acc;
-      }
+      for (int jj = 0; jj < 6; ++jj) if (jj == j) C[i][jj] = col[i];
   }
 }
 
@@ -445,14 +457,18 @@ __device__ int return_general(const DevMaterial& mat, const double strain[6], Po
   for (int it = 0; it < 50; ++it) {
     const double r_inf = maxabs7(e.res);
     if (r_inf <= tol) { conv = true; break; }
-    double J[7][7];
-    gen_jacobian(mat, st, e, dl, J);
-    int piv[7];
-    lu7(J, piv);
     double step[7][1];
+    {
+      JacSolver J;
+      jac_setup(mat, e, dl, st.abar, J);
+      double b1[6], x[6], y;
 #pragma unroll
-    for (int i = 0; i < 7; ++i) step[i][0] = -e.res[i];
-    lu7_solve<1>(J, piv, step);
+      for (int i = 0; i < 6; ++i) b1[i] = -e.res[i];
+      jac_solve(mat, J, b1, -e.res[6], x, y);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) step[i][0] = x[i];
+      step[6][0] = y;
+    }
     const double r0 = norm7(e.res);
     double t = 1.0;
     bool accepted = false;
@@ -476,14 +492,18 @@ __device__ int return_general(const DevMaterial& mat, const double strain[6], Po
   }
   if (!conv) return kReturnNotConverged;
   if (maxabs7(e.res) > 4.0 * eps_scale) {
-    double J[7][7];
-    gen_jacobian(mat, st, e, dl, J);
-    int piv[7];
-    lu7(J, piv);
     double step[7][1];
+    {
+      JacSolver J;
+      jac_setup(mat, e, dl, st.abar, J);
+      double b1[6], x[6], y;
 #pragma unroll
-    for (int i = 0; i < 7; ++i) step[i][0] = -e.res[i];
-    lu7_solve<1>(J, piv, step);
+      for (int i = 0; i < 6; ++i) b1[i] = -e.res[i];
+      jac_solve(mat, J, b1, -e.res[6], x, y);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) step[i][0] = x[i];
+      step[6][0] = y;
+    }
     double xp[6];
 #pragma unroll
     for (int i = 0; i < 6; ++i) xp[i] = xi[i] + step[i][0];

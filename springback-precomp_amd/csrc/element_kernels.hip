@@ -846,7 +846,7 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
     T.lu.rt[8] = static_cast<int>(er[kErRows + 8]);
     T.lu.ct[8] = static_cast<int>(er[kErCols + 8]);
   }
-  team_sync();
+  team_sync_global();  // the geometry records (global) and the factors (LDS)
   const int a = p;
   // f_u rows and K_ua of node a
   double ru[3], kua[3][9];
@@ -1039,14 +1039,14 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
   }
   if constexpr (SYM) {
     // K = (K + K^T) / 2 (IncompatibleModes.cpp:557) through the output rows
-    team_sync();
+    team_sync_global();
     double ks[3][24];
     const double* base = A.tangent + e * 576;
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
       for (int j = 0; j < 24; ++j) ks[k][j] = 0.5 * (out[k * 24 + j] + base[j * 24 + 3 * a + k]);
-    team_sync();
+    team_sync_global();
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll

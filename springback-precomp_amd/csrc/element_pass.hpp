@@ -273,23 +273,25 @@ __device__ __forceinline__ int point_response(const DevMaterial& m, const ModeGe
     double curv[6][6];
     log_push(L, ro.stress, r.S, curv);
     // P^T C P + T:L
+    double lp[6][6];
+    log_projection(L, lp);
     double pc[6][6];
 #pragma unroll
     for (int i = 0; i < 6; ++i)
 #pragma unroll
       for (int j = 0; j < 6; ++j) {
-        double s = L.P[0][i] * ro.C[0][j];
+        double s = lp[0][i] * ro.C[0][j];
 #pragma unroll
-        for (int k = 1; k < 6; ++k) s = s + L.P[k][i] * ro.C[k][j];
+        for (int k = 1; k < 6; ++k) s = s + lp[k][i] * ro.C[k][j];
         pc[i][j] = s;
       }
 #pragma unroll
     for (int i = 0; i < 6; ++i)
 #pragma unroll
       for (int j = 0; j < 6; ++j) {
-        double s = pc[i][0] * L.P[0][j];
+        double s = pc[i][0] * lp[0][j];
 #pragma unroll
-        for (int k = 1; k < 6; ++k) s = s + pc[i][k] * L.P[k][j];
+        for (int k = 1; k < 6; ++k) s = s + pc[i][k] * lp[k][j];
         r.mat[i][j] = s + curv[i][j];
       }
   } else {
