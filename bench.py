@@ -8,14 +8,17 @@ backstresses and Voce hardening, committed history inside the yield surface
 iteration of the modes, the condensed force f* (24) and tangent K* (24 x 24),
 the points' trial history. Inputs are resident in HBM before timing.
 
-Roofline: the element evaluation is FP64-compute-bound; its algorithmic FLOPs
-per element are counted by the oracle's op-counting instantiation
-(add/sub/mul/div/sqrt and each libm call = 1) on a sample of the same inputs,
-and divided by the HIP-event time of the element kernels.
+Roofline: the element evaluation is FP64-compute-bound. Its algorithmic
+FLOPs per element are counted by the oracle's op-counting instantiation
+(add/sub/mul/div/sqrt and each libm call = 1) on the first 1000 elements of the
+same inputs, split at the end of the local iteration. The dominant kernel is
+the local solve: achieved = its FLOPs per element x elements / its HIP-event
+time over the timed steps (events on the launch stream between the two
+kernels); the condensation kernel and the whole evaluation are reported beside.
 
 CPU baseline: the oracle's element loop (OpenMP schedule(static), one element
-per call, the reference's NonlinearSystem.hpp:208-237 structure) on a bounded
-sample of the same workload, on this host's cores.
+per call, the reference's NonlinearSystem.hpp:208-237 structure) on the same
+workload (all of it unless --cpu-sample bounds it), on this host's cores.
 """
 import argparse
 import json
@@ -34,8 +37,9 @@ HBM_PEAK_GBS = 8000.0
 
 
 def cpu_baseline_leg(workload, sample, threads):
-    """The oracle on the host: evals/s on `threads` cores, and the algorithmic
-    FLOPs per element (op-counting oracle) on the same sample."""
+    """The oracle on the host: evals/s on `threads` cores over the first
+    `sample` elements, and the algorithmic FLOPs per element (op-counting
+    oracle, split at the end of the local iteration) on its first 1000."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     from parity import oracle_material
@@ -50,9 +54,11 @@ def cpu_baseline_leg(workload, sample, threads):
     r = O.element_batch(x0, ue, om, st, kinematics=2, want_tangent=True, threads=threads)
     dt = time.perf_counter() - t0
     nflop = min(sample, 1000)
-    flops = [O.element_flops(x0[i], ue[i], om, states=st[i], kinematics=2) for i in range(nflop)]
+    parts = np.array([O.element_flops_split(x0[i], ue[i], om, states=st[i], kinematics=2) for i in range(nflop)],
+                     dtype=np.float64)
     return dict(evals_per_s=sample / dt, seconds=dt, failures=int((r["status"] != 0).sum()),
-                flops_per_elem=float(np.mean(flops)), flops_sample=nflop,
+                flops_per_elem=float(parts[:, 0].mean()), flops_local=float(parts[:, 1].mean()),
+                flops_condense=float(parts[:, 2].mean()), flops_sample=nflop,
                 plastic_fraction=float((r["scalars"][:, 1] > 0).mean()))
 
 
@@ -62,12 +68,12 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--elements", type=int, default=1_000_000)
-    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--cpu-sample", type=int, default=0, help="elements of the CPU leg (default: all)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
-              "(element evaluations are independent; see DESIGN.md for how they would shard)", file=sys.stderr)
+              "(element evaluations are independent and would shard by element ranges)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -97,39 +103,56 @@ def main():
     plastic = float((batch.yielding.cpu().numpy() > 0).mean())
     mean_iters = float(batch.iterations.cpu().numpy().mean())
 
+    # Timed region: K steps, each the local solve then the condensation, with
+    # HIP events on the launch stream between the two kernels.
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     start.record(stream)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        G.evaluate_elements(m, rule, batch, G.FINITE_LOGARITHMIC, True, stream)
+    for k in range(args.steps):
+        ev[k][0].record(stream)
+        G.evaluate_elements(m, rule, batch, G.FINITE_LOGARITHMIC, True, stream, stages=G.LOCAL_SOLVE)
+        ev[k][1].record(stream)
+        G.evaluate_elements(m, rule, batch, G.FINITE_LOGARITHMIC, True, stream, stages=G.CONDENSATION)
+        ev[k][2].record(stream)
     stop.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = start.elapsed_time(stop) / args.steps
+    ms_local = sum(e[0].elapsed_time(e[1]) for e in ev) / args.steps
+    ms_cond = sum(e[1].elapsed_time(e[2]) for e in ev) / args.steps
     value = ne / (ms * 1e-3)
-    print(f"# {ms:.3f} ms/step (event), {wall / args.steps * 1e3:.3f} ms/step (wall); failures {failures}, "
-          f"plastic elements {plastic:.3f}, mean local iterations {mean_iters:.2f}", file=sys.stderr, flush=True)
+    print(f"# {ms:.3f} ms/step (event), {wall / args.steps * 1e3:.3f} ms/step (wall); local solve {ms_local:.3f} ms, "
+          f"condensation {ms_cond:.3f} ms; failures {failures}, plastic elements {plastic:.3f}, "
+          f"mean local iterations {mean_iters:.2f}", file=sys.stderr, flush=True)
 
     cpu = None
     roof = None
     if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0))
-        threads = min(threads, 16)
-        cb = cpu_baseline_leg(w, min(args.cpu_sample, ne), threads)
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        sample = ne if args.cpu_sample <= 0 else min(args.cpu_sample, ne)
+        cb = cpu_baseline_leg(w, sample, threads)
         cpu = {"value": cb["evals_per_s"], "unit": "element evals/s", "cores": threads, "kind": "port",
-               "sample": f"first {min(args.cpu_sample, ne)} elements of the same workload, OpenMP "
-                         f"static over elements, {cb['seconds']:.1f} s wall"}
-        flops = cb["flops_per_elem"]
-        achieved = flops * ne / (ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "pipe": "fp64 (vector ALU; MI355X FP64 vector = matrix peak)",
+               "sample": f"the first {sample} elements of the same workload (all of it by default), the oracle's "
+                         f"element loop with OpenMP schedule(static) on {threads} threads, {cb['seconds']:.1f} s wall"}
+        fl, fc = cb["flops_local"], cb["flops_condense"]
+        achieved = fl * ne / (ms_local * 1e-3) / 1e12
+        roof = {"bound": "mfma", "pipe": "fp64 (vector ALU; MI355X FP64 vector peak = FP64 matrix peak)",
                 "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                "flops_per_elem": flops, "flops_sample": cb["flops_sample"],
-                "kernel": "local_solve_kernel + condense_kernel (one element evaluation)"}
-        print(f"# cpu oracle: {cb['evals_per_s']:.0f} evals/s on {threads} threads; "
-              f"{flops:.0f} FLOP/elem; oracle plastic fraction {cb['plastic_fraction']:.3f}", file=sys.stderr)
+                "kernel": "local_solve_kernel (the dominant kernel)",
+                "flops_per_elem": fl, "flops_sample": cb["flops_sample"],
+                "kernel_ms": ms_local,
+                "condensation": {"kernel": "condense_kernel", "flops_per_elem": fc, "kernel_ms": ms_cond,
+                                 "achieved": fc * ne / (ms_cond * 1e-3) / 1e12},
+                "element_evaluation": {"flops_per_elem": cb["flops_per_elem"], "ms": ms,
+                                       "achieved": cb["flops_per_elem"] * ne / (ms * 1e-3) / 1e12}}
+        print(f"# cpu oracle: {cb['evals_per_s']:.0f} evals/s on {threads} threads; {cb['flops_per_elem']:.0f} "
+              f"FLOP/elem ({fl:.0f} local + {fc:.0f} condensation); oracle plastic fraction "
+              f"{cb['plastic_fraction']:.3f}", file=sys.stderr)
 
     line = {
         "metric": METRIC, "value": value, "unit": "element evals/s", "n_gpus": 1, "steps": args.steps,

@@ -167,6 +167,14 @@ int sparlab_elements_evaluate_device(const sparlab_material* material, const spa
                                      int kinematics, int want_tangent,
                                      const sparlab_element_batch* batch, void* stream);
 
+/* The same evaluation in its two stages (stages: 1 the local iteration of
+ * the modes, src/fem/IncompatibleModes.cpp:320-546; 2 the coupling blocks and
+ * the condensation, :549-561, which needs stage 1 of the same batch to have
+ * run; 3 both), so that each stage can be timed or overlapped on its own. */
+int sparlab_elements_evaluate_stage_device(const sparlab_material* material, const sparlab_rule* rule,
+                                           int kinematics, int want_tangent,
+                                           const sparlab_element_batch* batch, int stages, void* stream);
+
 /* ------------------------------------------------------------------------
  * The nonlinear system: the drop-in for detail::NonlinearSystem
  * (src/fem/NonlinearSystem.hpp:85-334) with the Assembler's cached pattern

@@ -639,6 +639,7 @@ class IncompatibleHex8 {
         throw SolverError(os.str());
       }
     }
+    flop_mark() = flop_counter();
     accumulate_coupling(want_tangent, sums);
     const Vec<T, 9> correction = lu.solve(r_alpha());
     for (int i = 0; i < 24; ++i) {

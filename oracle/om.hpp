@@ -42,6 +42,12 @@ inline std::uint64_t& flop_counter() {
   static thread_local std::uint64_t count = 0;
   return count;
 }
+// The counter's value when the local iteration of an element has converged
+// (the split between the local solve and the condensation).
+inline std::uint64_t& flop_mark() {
+  static thread_local std::uint64_t mark = 0;
+  return mark;
+}
 
 struct Counted {
   double v = 0.0;

@@ -235,6 +235,25 @@ long long oracle_element_flops(const double* x0, const double* ue, const double*
   return st == 0 ? n : -1;
 }
 
+// The same count split at the end of the local iteration: local[0] = the
+// FLOPs up to convergence of the modes, local[1] = the coupling blocks and
+// the condensation. Returns the total or -1.
+long long oracle_element_flops_split(const double* x0, const double* ue, const double* states_in,
+                                     const double* alpha_in, const double* mat, int in_plane, int thickness,
+                                     int axis, int kinematics, int want_tangent, long long* parts) {
+  const Rule rule = make_rule(in_plane, thickness, axis);
+  std::vector<double> f(24), k(576), a(9), so(static_cast<std::size_t>(rule.n) * kStateRecord);
+  double sc[5];
+  flop_counter() = 0;
+  flop_mark() = 0;
+  const int st = element_impl<Counted>(x0, ue, states_in, alpha_in, read_material(mat), rule, kinematics,
+                                       want_tangent, f.data(), k.data(), a.data(), so.data(), sc, nullptr, 0);
+  const long long n = static_cast<long long>(flop_counter());
+  parts[0] = static_cast<long long>(flop_mark());
+  parts[1] = n - parts[0];
+  return st == 0 ? n : -1;
+}
+
 // A batch of independent elements, OpenMP schedule(static) over elements as
 // NonlinearSystem.hpp:208-237 (the CPU baseline). Element e reads
 // x0[e*24], ue[e*24], states_in[e*P*39], alpha_in[e*9] (alpha_in may be null)

@@ -376,10 +376,12 @@ int64_t sparlab_element_workspace_doubles(int64_t num_elements, const sparlab_ru
   return static_cast<int64_t>(sbk::workspace_doubles(num_elements, r.n));
 }
 
-int sparlab_elements_evaluate_device(const sparlab_material* material, const sparlab_rule* rule, int kinematics,
-                                     int want_tangent, const sparlab_element_batch* b, void* stream) {
+int sparlab_elements_evaluate_stage_device(const sparlab_material* material, const sparlab_rule* rule,
+                                           int kinematics, int want_tangent, const sparlab_element_batch* b,
+                                           int stages, void* stream) {
   if (material == nullptr || rule == nullptr || b == nullptr) return fail(SPARLAB_E_ARGUMENT, "null argument");
   if (kinematics < 0 || kinematics > 2) return fail(SPARLAB_E_ARGUMENT, "kinematics must be 0, 1 or 2");
+  if (stages < 1 || stages > 3) return fail(SPARLAB_E_ARGUMENT, "stages must be 1, 2 or 3");
   if (b->num_elements < 0 || b->num_nodes < 0) return fail(SPARLAB_E_ARGUMENT, "negative sizes");
   if (b->num_elements == 0) return SPARLAB_OK;
   if (b->coords == nullptr || b->conn == nullptr || b->u == nullptr || b->state_in == nullptr ||
@@ -410,15 +412,24 @@ int sparlab_elements_evaluate_device(const sparlab_material* material, const spa
   sbk::set_workspace(a, b->workspace);
   // The local solve writes every element's status; the condensation reads it.
   hipStream_t s = static_cast<hipStream_t>(stream);
-  int err = sbk::launch_local_solve(a, s);
-  if (err != 0)
-    return fail(SPARLAB_E_DEVICE, std::string("the local-solve kernel did not launch: ") +
-                                      hipGetErrorString(static_cast<hipError_t>(err)));
-  err = sbk::launch_condense(a, want_tangent != 0, s);
-  if (err != 0)
-    return fail(SPARLAB_E_DEVICE, std::string("the condensation kernel did not launch: ") +
-                                      hipGetErrorString(static_cast<hipError_t>(err)));
+  if (stages & 1) {
+    const int err = sbk::launch_local_solve(a, s);
+    if (err != 0)
+      return fail(SPARLAB_E_DEVICE, std::string("the local-solve kernel did not launch: ") +
+                                        hipGetErrorString(static_cast<hipError_t>(err)));
+  }
+  if (stages & 2) {
+    const int err = sbk::launch_condense(a, want_tangent != 0, s);
+    if (err != 0)
+      return fail(SPARLAB_E_DEVICE, std::string("the condensation kernel did not launch: ") +
+                                        hipGetErrorString(static_cast<hipError_t>(err)));
+  }
   return SPARLAB_OK;
+}
+
+int sparlab_elements_evaluate_device(const sparlab_material* material, const sparlab_rule* rule, int kinematics,
+                                     int want_tangent, const sparlab_element_batch* b, void* stream) {
+  return sparlab_elements_evaluate_stage_device(material, rule, kinematics, want_tangent, b, 3, stream);
 }
 
 int sparlab_contact_evaluate_device(const sparlab_tool* tool, int friction_symmetric, int want_tangent,

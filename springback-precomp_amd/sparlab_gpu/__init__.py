@@ -117,6 +117,9 @@ def lib():
         _lib.sparlab_elements_evaluate_device.argtypes = [
             ctypes.POINTER(CMaterial), ctypes.POINTER(CRule), ctypes.c_int, ctypes.c_int,
             ctypes.POINTER(CElementBatch), ctypes.c_void_p]
+        _lib.sparlab_elements_evaluate_stage_device.argtypes = [
+            ctypes.POINTER(CMaterial), ctypes.POINTER(CRule), ctypes.c_int, ctypes.c_int,
+            ctypes.POINTER(CElementBatch), ctypes.c_int, ctypes.c_void_p]
     return _lib
 
 
@@ -299,15 +302,21 @@ class ElementBatch:
             _ptr(self.fail_value), _ptr(self.workspace))
 
 
-def evaluate_elements(material, rule, batch, kinematics=FINITE_LOGARITHMIC, want_tangent=True, stream=None):
-    """Queues the evaluation of every element of `batch` (see ElementBatch)."""
+LOCAL_SOLVE = 1
+CONDENSATION = 2
+
+
+def evaluate_elements(material, rule, batch, kinematics=FINITE_LOGARITHMIC, want_tangent=True, stream=None,
+                      stages=LOCAL_SOLVE | CONDENSATION):
+    """Queues the evaluation of every element of `batch` (see ElementBatch);
+    `stages` selects the local solve, the condensation or both."""
     import torch
     if want_tangent and batch.tangent is None:
         raise ValueError("the batch was allocated without a tangent buffer")
     s = stream if stream is not None else torch.cuda.current_stream(batch.coords.device)
-    _check(lib().sparlab_elements_evaluate_device(ctypes.byref(material.c), ctypes.byref(rule.c), kinematics,
-                                                  int(want_tangent), ctypes.byref(batch.c),
-                                                  ctypes.c_void_p(s.cuda_stream)))
+    _check(lib().sparlab_elements_evaluate_stage_device(ctypes.byref(material.c), ctypes.byref(rule.c), kinematics,
+                                                        int(want_tangent), ctypes.byref(batch.c), int(stages),
+                                                        ctypes.c_void_p(s.cuda_stream)))
     return batch
 
 

@@ -165,6 +165,21 @@ def element_flops(x0, ue, mat, states=None, alpha=None, rule=(2, 0, 2), kinemati
                                       rule[2], kinematics, int(want_tangent))
 
 
+def element_flops_split(x0, ue, mat, states=None, alpha=None, rule=(2, 0, 2), kinematics=2, want_tangent=True):
+    """(FLOPs of the local iteration, FLOPs of the coupling blocks and condensation)."""
+    x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(24)
+    ue = np.ascontiguousarray(ue, dtype=np.float64).reshape(24)
+    npts = rule_points(*rule)
+    states = virgin_states(npts) if states is None else np.ascontiguousarray(states, dtype=np.float64)
+    a_in = None if alpha is None else np.ascontiguousarray(alpha, dtype=np.float64)
+    parts = np.zeros(2, dtype=np.int64)
+    f = lib().oracle_element_flops_split
+    f.restype = ctypes.c_longlong
+    total = f(dp(x0), dp(ue), dp(states), dp(a_in), dp(mat), rule[0], rule[1], rule[2], kinematics,
+              int(want_tangent), parts.ctypes.data_as(ctypes.c_void_p))
+    return int(total), int(parts[0]), int(parts[1])
+
+
 def element_batch(x0, ue, mat, states, alpha=None, rule=(2, 0, 2), kinematics=2, want_tangent=True,
                   threads=0):
     """x0: (E, 8, 3); ue: (E, 24); states: (E, P, 39); alpha: (E, 9) or None."""
