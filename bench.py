@@ -62,6 +62,70 @@ def cpu_baseline_leg(workload, sample, threads):
                 plastic_fraction=float((r["scalars"][:, 1] > 0).mean()))
 
 
+def batched_leg(G, W, torch, dev, steps):
+    """The batched form beside the plain drop-in on the 64-problem forming
+    workload (workloads.batched_problems: 4 part families, 2 000-40 000 Hex8
+    each, one forming increment from a virgin history): the plain drop-in
+    evaluates each problem's own system in turn, the batched form evaluates
+    the union in one captured launch sequence. Each step is one Newton
+    iteration's evaluation of all problems (element kernels, force gather,
+    CSR assembly, scalars)."""
+    probs = W.batched_problems()
+    m = W.microbench_material()
+    rule = G.Rule(2, 0, 2)
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64).reshape(-1), device=dev)
+    ne = int(sum(p["conn"].shape[0] for p in probs))
+    stream = torch.cuda.current_stream(dev)
+    # plain: one system per problem, evaluated one after another
+    plain = []
+    for p in probs:
+        s = G.NonlinearSystem(p["coords"], p["conn"], m, rule)
+        u = t(p["u"])
+        plain.append((s, u, torch.zeros_like(u), torch.zeros(s.nnz, dtype=torch.float64, device=dev)))
+    for s, u, f, v in plain:
+        s.evaluate_device(u, f, v, stream=stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        for s, u, f, v in plain:
+            s.evaluate_device(u, f, v, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_plain = e0.elapsed_time(e1) / steps
+    plain_ok = True
+    for s, _, _, _ in plain:
+        try:
+            s.result(stream)
+        except G.SolverError:
+            plain_ok = False
+    del plain
+    # batched: the union, autotuned, graph-captured
+    b = G.BatchedSystem([(p["coords"], p["conn"]) for p in probs], m, rule)
+    u = t(np.concatenate([p["u"] for p in probs]))
+    fint = torch.zeros_like(u)
+    values = torch.zeros(b.system.nnz, dtype=torch.float64, device=dev)
+    chunk = b.autotune(u, stream)
+    b.evaluate_device(u, fint, values, stream=stream)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        b.evaluate_device(u, fint, values, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_batched = e0.elapsed_time(e1) / steps
+    sc = b.result(stream)
+    return {"workload": "64 forming problems (dome, truncated cone, elliptic cone, pyramid; Sobol over footprint "
+                        "15-20 mm, depth 2-4 mm), 1 mm sheet, 2 Hex8 layers, 2 000-40 000 elements each, one "
+                        "forming increment from a virgin history, Hill48 + 2 Chaboche + Voce, log strain",
+            "problems": len(probs), "elements": ne, "steps": steps,
+            "plain_ms": ms_plain, "batched_ms": ms_batched,
+            "plain_evals_per_s": ne / (ms_plain * 1e-3), "batched_evals_per_s": ne / (ms_batched * 1e-3),
+            "speedup": ms_plain / ms_batched, "elements_per_wavefront": chunk,
+            "failures": int(sc["failed"].sum()) + (0 if plain_ok else 1),
+            "yielding_points": int(sc["yielding"].sum())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +134,7 @@ def main():
     ap.add_argument("--elements", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=0, help="elements of the CPU leg (default: all)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
@@ -154,6 +219,16 @@ def main():
               f"FLOP/elem ({fl:.0f} local + {fc:.0f} condensation); oracle plastic fraction "
               f"{cb['plastic_fraction']:.3f}", file=sys.stderr)
 
+    batched = None
+    if not args.no_batched:
+        del batch
+        torch.cuda.empty_cache()
+        batched = batched_leg(G, W, torch, dev, max(1, min(args.steps, 5)))
+        print(f"# batched form: {batched['elements']} elements in {batched['problems']} problems; plain "
+              f"{batched['plain_ms']:.2f} ms, batched {batched['batched_ms']:.2f} ms per evaluation of all "
+              f"({batched['speedup']:.2f}x, {batched['elements_per_wavefront']} elements per wavefront)",
+              file=sys.stderr, flush=True)
+
     line = {
         "metric": METRIC, "value": value, "unit": "element evals/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -162,7 +237,7 @@ def main():
                                "log strain, Hill48 + 2 Chaboche + Voce, f_int + K_t + trial history",
                    "elements": ne, "integration_points": npts, "plastic_fraction": plastic,
                    "mean_local_iterations": mean_iters, "failures": failures},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "batched": batched,
     }
     print(json.dumps(line))
 

@@ -259,6 +259,51 @@ int64_t sparlab_pattern_build(int64_t num_nodes, int64_t num_elements, const int
                               int32_t* col_idx, int32_t* block_offset);
 
 /* ------------------------------------------------------------------------
+ * The batched form: many independent problems (forming meshes) advanced
+ * together, each a NonlinearSystem (src/fem/NonlinearSystem.hpp:85-334) of
+ * its own. Their union is one block-diagonal system evaluated by one launch
+ * sequence, captured in a HIP graph and replayed every Newton iteration.
+ * ------------------------------------------------------------------------ */
+typedef struct sparlab_batch sparlab_batch;
+
+/* Builds the batch from host arrays: problem p owns nodes
+ * [node_offset[p], node_offset[p + 1]) of coords (3 per node) and elements
+ * [element_offset[p], element_offset[p + 1]) of conn (8 per element, node
+ * indices local to the problem, VTK order); one material, rule and kinematics
+ * for all. The union's DOFs, CSR rows and element outputs follow problem
+ * order, so problem p's DOFs start at 3 node_offset[p] and its CSR values
+ * are one contiguous slice (sparlab_batch_value_offsets). */
+int sparlab_batch_create(int32_t num_problems, const int64_t* node_offset, const int64_t* element_offset,
+                         const double* coords, const int32_t* conn, const sparlab_material* material,
+                         const sparlab_rule* rule, int kinematics, sparlab_batch** out);
+void sparlab_batch_destroy(sparlab_batch* batch);
+/* The union's system: pattern, history, commit and the plain entry points
+ * (owned by the batch). */
+sparlab_system* sparlab_batch_system(sparlab_batch* batch);
+/* num_problems + 1 offsets of each problem's first CSR value. */
+int sparlab_batch_value_offsets(const sparlab_batch* batch, int64_t* value_offset);
+/* Every problem's NonlinearSystem::evaluate at once (device pointers u, fint
+ * 3 per union node; values nnz with a tangent), and per-problem scalars.
+ * The first call for a given (u, fint, values, want_tangent, column_major,
+ * committed buffers) captures the launches into a HIP graph; later calls
+ * with the same pointers replay it, so the buffers must stay allocated while
+ * the batch lives (write new displacements into the same u). */
+int sparlab_batch_evaluate_device(sparlab_batch* batch, const double* u, int want_tangent, int column_major,
+                                  double* fint, double* values, void* stream);
+/* Capture and replay (enabled = 1, the default) or direct launches (0). */
+int sparlab_batch_set_graphs(sparlab_batch* batch, int enabled);
+/* Waits for the last batched evaluation; per problem (host arrays of
+ * num_problems, each may be null): energy, gross, yielding points, failed
+ * elements, the lowest failed element (local index, or -1).
+ * SPARLAB_E_SOLVER when any element failed. */
+int sparlab_batch_result(sparlab_batch* batch, double* energy, double* gross, int64_t* yielding,
+                         int64_t* failed, int64_t* first_failed, void* stream);
+/* Times the local solve at 8, 16, ..., 256 elements per wavefront on the
+ * batch at displacement u (device) and keeps the fastest (written to
+ * *chosen when non-null). The timing runs write the trial buffers only. */
+int sparlab_batch_autotune(sparlab_batch* batch, const double* u, int* chosen, void* stream);
+
+/* ------------------------------------------------------------------------
  * Penalty contact of slave nodes with a rigid tool (ToolContact::evaluate,
  * src/fem/RigidTool.cpp:398-555, for one tool over a list of slave nodes).
  * ------------------------------------------------------------------------ */

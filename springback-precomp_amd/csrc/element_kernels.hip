@@ -29,7 +29,7 @@
 namespace sbk {
 
 constexpr int kWaveTeams = 8;
-constexpr int kChunk = 64;  // elements per wavefront in the local solve
+constexpr int kDefaultChunk = 64;  // elements per wavefront in the local solve
 
 // IncompatibleModes.hpp:98-127
 constexpr int kMaxLocalIterations = 25;
@@ -634,8 +634,9 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
   const int p = lane & 7;
   TeamSlot& T = slots[t];
   Machine& M = T.m;
-  const long long c0 = static_cast<long long>(blockIdx.x) * kChunk;
-  const long long c1 = c0 + kChunk < A.ne ? c0 + kChunk : A.ne;
+  const long long chunk = A.chunk;
+  const long long c0 = static_cast<long long>(blockIdx.x) * chunk;
+  const long long c1 = c0 + chunk < A.ne ? c0 + chunk : A.ne;
   long long next = c0;
   long long e = -1;
   if (p == 0) M.phase = kIdle;
@@ -1057,9 +1058,12 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-int launch_local_solve(const ElementArgs& a, hipStream_t stream) {
+int launch_local_solve(const ElementArgs& args, hipStream_t stream) {
   (void)hipGetLastError();
-  const unsigned int blocks = static_cast<unsigned int>((a.ne + kChunk - 1) / kChunk);
+  ElementArgs a = args;
+  if (a.chunk <= 0) a.chunk = kDefaultChunk;
+  if (a.chunk % kWaveTeams != 0) return static_cast<int>(hipErrorInvalidValue);
+  const unsigned int blocks = static_cast<unsigned int>((a.ne + a.chunk - 1) / a.chunk);
   if (blocks == 0) return 0;
   switch (a.kinematics) {
     case kSmall: hipLaunchKernelGGL(local_solve_kernel<kSmall>, dim3(blocks), dim3(64), 0, stream, a); break;

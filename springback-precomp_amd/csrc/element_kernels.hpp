@@ -48,6 +48,8 @@ struct ElementArgs {
   double* point_ws;
   double* geom_ws;
   double* elem_ws;
+  // elements per wavefront of the local solve (a multiple of 8; 0: 64)
+  int chunk;
 };
 
 constexpr int kPointRecord = 42;

@@ -16,42 +16,8 @@
 #include <string>
 #include <vector>
 
-#include "assembly.hpp"
 #include "assembly_kernels.hpp"
-#include "element_kernels.hpp"
-
-namespace sbk {
-int capi_fail(int code, const std::string& message);
-int capi_prepare_elements(const sparlab_material* material, const sparlab_rule* rule, int kinematics,
-                          ElementArgs& a);
-}  // namespace sbk
-
-struct sparlab_system {
-  sparlab_material material;
-  sparlab_rule rule;
-  int kinematics = 2;
-  int points = 0;
-  int fields = 0;
-  int64_t nn = 0, ne = 0;
-  sbk::HostPattern pattern;
-  sbk::ElementArgs args{};
-  // device memory
-  double* coords = nullptr;
-  int32_t* conn = nullptr;
-  double* state[2] = {nullptr, nullptr};  // committed = state[cur], trial = state[1 - cur]
-  double* alpha[2] = {nullptr, nullptr};
-  int cur = 0;
-  double *fe = nullptr, *ke = nullptr, *energy = nullptr, *fail_value = nullptr, *gross = nullptr;
-  int32_t *yielding = nullptr, *iterations = nullptr, *status = nullptr;
-  int32_t *block_ptr = nullptr, *block_contrib = nullptr, *block_pos = nullptr, *block_row_len = nullptr;
-  int32_t *node_ptr = nullptr, *node_incidence = nullptr, *diag_block = nullptr;
-  double* ws = nullptr;         // the element kernels' point workspace
-  double* partial = nullptr;    // 1024 partial sums
-  double* scalars = nullptr;    // [0] energy, [1] gross
-  long long* summary = nullptr; // failed count, lowest failed element, yielding points
-  bool evaluated = false;
-  bool last_tangent = false;
-};
+#include "system_impl.hpp"
 
 namespace {
 
@@ -67,7 +33,7 @@ bool upload(T*& p, const std::vector<T>& v) {
 }
 template <class T>
 void dfree(T*& p) {
-  if (p) hipFree(p);
+  if (p) (void)hipFree(p);
   p = nullptr;
 }
 
@@ -82,6 +48,50 @@ void release(sparlab_system* s) {
 }
 
 }  // namespace
+
+namespace sbk {
+
+int system_enqueue(sparlab_system* s, const double* u, int want_tangent, int column_major, double* fint,
+                   double* values, hipStream_t st) {
+  ElementArgs a = s->args;
+  a.coords = s->coords;
+  a.conn = s->conn;
+  a.u = u;
+  a.ne = s->ne;
+  a.state_in = s->state[s->cur];
+  a.state_stride = s->ne * s->points;
+  a.alpha_in = s->alpha[s->cur];
+  a.alpha_out = s->alpha[1 - s->cur];
+  a.state_out = s->state[1 - s->cur];
+  a.force = s->fe;
+  a.tangent = want_tangent ? s->ke : nullptr;
+  a.energy = s->energy;
+  a.yielding = s->yielding;
+  a.iterations = s->iterations;
+  a.status = s->status;
+  a.fail_value = s->fail_value;
+  set_workspace(a, s->ws);
+  if (launch_local_solve(a, st) != 0 || launch_condense(a, want_tangent != 0, st) != 0)
+    return capi_fail(SPARLAB_E_DEVICE, "element kernel launch failed");
+  ForceArgs f{s->fe, s->node_ptr, s->node_incidence, static_cast<long long>(s->nn), fint, s->gross};
+  if (launch_gather_force(f, st) != 0) return capi_fail(SPARLAB_E_DEVICE, "force gather launch failed");
+  if (want_tangent) {
+    AssemblyArgs g{s->ke, s->block_ptr, s->block_contrib, s->block_pos, s->block_row_len,
+                        static_cast<long long>(s->pattern.block_pos.size()), column_major ? 1 : 0, values};
+    if (launch_assemble_values(g, st) != 0) return capi_fail(SPARLAB_E_DEVICE, "assembly launch failed");
+  }
+  const int parts = static_cast<int>(std::min<int64_t>(1024, (3 * s->nn + 4095) / 4096 + 1));
+  if (launch_reduce(s->gross, 3 * s->nn, 1, s->partial, parts, s->scalars + 1, st) != 0 ||
+      launch_reduce(s->energy, s->ne, 0, s->partial, static_cast<int>(std::min<int64_t>(1024, s->ne / 4096 + 1)),
+                         s->scalars, st) != 0 ||
+      launch_status_summary(s->status, s->yielding, s->ne, s->summary, st) != 0)
+    return capi_fail(SPARLAB_E_DEVICE, "reduction launch failed");
+  s->evaluated = true;
+  s->last_tangent = want_tangent != 0;
+  return SPARLAB_OK;
+}
+
+}  // namespace sbk
 
 extern "C" {
 
@@ -171,43 +181,7 @@ int sparlab_system_evaluate_device(sparlab_system* s, const double* u, int want_
                                    double* fint, double* values, void* stream) {
   if (!s || !u || !fint) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
   if (want_tangent && !values) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "want_tangent needs a values buffer");
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  sbk::ElementArgs a = s->args;
-  a.coords = s->coords;
-  a.conn = s->conn;
-  a.u = u;
-  a.ne = s->ne;
-  a.state_in = s->state[s->cur];
-  a.state_stride = s->ne * s->points;
-  a.alpha_in = s->alpha[s->cur];
-  a.alpha_out = s->alpha[1 - s->cur];
-  a.state_out = s->state[1 - s->cur];
-  a.force = s->fe;
-  a.tangent = want_tangent ? s->ke : nullptr;
-  a.energy = s->energy;
-  a.yielding = s->yielding;
-  a.iterations = s->iterations;
-  a.status = s->status;
-  a.fail_value = s->fail_value;
-  sbk::set_workspace(a, s->ws);
-  if (sbk::launch_local_solve(a, st) != 0 || sbk::launch_condense(a, want_tangent != 0, st) != 0)
-    return sbk::capi_fail(SPARLAB_E_DEVICE, "element kernel launch failed");
-  sbk::ForceArgs f{s->fe, s->node_ptr, s->node_incidence, static_cast<long long>(s->nn), fint, s->gross};
-  if (sbk::launch_gather_force(f, st) != 0) return sbk::capi_fail(SPARLAB_E_DEVICE, "force gather launch failed");
-  if (want_tangent) {
-    sbk::AssemblyArgs g{s->ke, s->block_ptr, s->block_contrib, s->block_pos, s->block_row_len,
-                        static_cast<long long>(s->pattern.block_pos.size()), column_major ? 1 : 0, values};
-    if (sbk::launch_assemble_values(g, st) != 0) return sbk::capi_fail(SPARLAB_E_DEVICE, "assembly launch failed");
-  }
-  const int parts = static_cast<int>(std::min<int64_t>(1024, (3 * s->nn + 4095) / 4096 + 1));
-  if (sbk::launch_reduce(s->gross, 3 * s->nn, 1, s->partial, parts, s->scalars + 1, st) != 0 ||
-      sbk::launch_reduce(s->energy, s->ne, 0, s->partial, static_cast<int>(std::min<int64_t>(1024, s->ne / 4096 + 1)),
-                         s->scalars, st) != 0 ||
-      sbk::launch_status_summary(s->status, s->yielding, s->ne, s->summary, st) != 0)
-    return sbk::capi_fail(SPARLAB_E_DEVICE, "reduction launch failed");
-  s->evaluated = true;
-  s->last_tangent = want_tangent != 0;
-  return SPARLAB_OK;
+  return sbk::system_enqueue(s, u, want_tangent, column_major, fint, values, static_cast<hipStream_t>(stream));
 }
 
 int sparlab_system_assemble_device(sparlab_system* s, const double* fe, const double* ke, int column_major,
@@ -249,8 +223,9 @@ int sparlab_system_result(sparlab_system* s, sparlab_evaluation* out, void* stre
   if (sum[0] > 0) {
     int32_t code = 0;
     double v = 0.0;
-    hipMemcpy(&code, s->status + sum[1], sizeof(code), hipMemcpyDeviceToHost);
-    hipMemcpy(&v, s->fail_value + sum[1], sizeof(v), hipMemcpyDeviceToHost);
+    if (hipMemcpy(&code, s->status + sum[1], sizeof(code), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&v, s->fail_value + sum[1], sizeof(v), hipMemcpyDeviceToHost) != hipSuccess)
+      return sbk::capi_fail(SPARLAB_E_DEVICE, "reading the first failure failed");
     out->first_failure_status = code;
     out->first_failure_value = v;
   }

@@ -1,0 +1,53 @@
+// The device nonlinear system's state, shared by the plain system
+// (system.cpp) and the batched form (batch.cpp).
+#pragma once
+
+#include "sparlab_gpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "assembly.hpp"
+#include "element_kernels.hpp"
+
+namespace sbk {
+int capi_fail(int code, const std::string& message);
+int capi_prepare_elements(const sparlab_material* material, const sparlab_rule* rule, int kinematics,
+                          ElementArgs& a);
+}  // namespace sbk
+
+struct sparlab_system {
+  sparlab_material material;
+  sparlab_rule rule;
+  int kinematics = 2;
+  int points = 0;
+  int fields = 0;
+  int64_t nn = 0, ne = 0;
+  sbk::HostPattern pattern;
+  sbk::ElementArgs args{};
+  // device memory
+  double* coords = nullptr;
+  int32_t* conn = nullptr;
+  double* state[2] = {nullptr, nullptr};  // committed = state[cur], trial = state[1 - cur]
+  double* alpha[2] = {nullptr, nullptr};
+  int cur = 0;
+  double *fe = nullptr, *ke = nullptr, *energy = nullptr, *fail_value = nullptr, *gross = nullptr;
+  int32_t *yielding = nullptr, *iterations = nullptr, *status = nullptr;
+  int32_t *block_ptr = nullptr, *block_contrib = nullptr, *block_pos = nullptr, *block_row_len = nullptr;
+  int32_t *node_ptr = nullptr, *node_incidence = nullptr, *diag_block = nullptr;
+  double* ws = nullptr;         // the element kernels' workspace
+  double* partial = nullptr;    // 1024 partial sums
+  double* scalars = nullptr;    // [0] energy, [1] gross
+  long long* summary = nullptr; // failed count, lowest failed element, yielding points
+  bool evaluated = false;
+  bool last_tangent = false;
+};
+
+namespace sbk {
+// The launches of one evaluation of the system on `stream` (what
+// sparlab_system_evaluate_device queues).
+int system_enqueue(sparlab_system* s, const double* u, int want_tangent, int column_major, double* fint,
+                   double* values, hipStream_t stream);
+}  // namespace sbk

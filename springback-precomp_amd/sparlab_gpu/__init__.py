@@ -385,8 +385,9 @@ class NonlinearSystem:
     gather and the assembly of detail::NonlinearSystem with the Assembler's
     pattern, and the committed history."""
 
-    def __init__(self, coords, conn, material, rule=None, kinematics=FINITE_LOGARITHMIC):
+    def __init__(self, coords, conn, material, rule=None, kinematics=FINITE_LOGARITHMIC, _handle=None):
         import torch  # noqa: F401  (HIP runtime loaded through torch)
+        self.owned = _handle is None
         self.coords = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 3)
         self.conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, 8)
         self.material = material
@@ -411,10 +412,13 @@ class NonlinearSystem:
         L.sparlab_system_history.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.sparlab_system_set_history.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.sparlab_system_element_outputs.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
-        h = ctypes.c_void_p()
-        _check(L.sparlab_system_create(_np_ptr(self.coords), self.coords.shape[0], _np_ptr(self.conn),
-                                       self.conn.shape[0], ctypes.byref(material.c), ctypes.byref(self.rule.c),
-                                       kinematics, ctypes.byref(h)))
+        if _handle is None:
+            h = ctypes.c_void_p()
+            _check(L.sparlab_system_create(_np_ptr(self.coords), self.coords.shape[0], _np_ptr(self.conn),
+                                           self.conn.shape[0], ctypes.byref(material.c), ctypes.byref(self.rule.c),
+                                           kinematics, ctypes.byref(h)))
+        else:
+            h = _handle
         self.h = h
         self.num_nodes = self.coords.shape[0]
         self.num_elements = self.conn.shape[0]
@@ -423,9 +427,9 @@ class NonlinearSystem:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h:
+        if h and getattr(self, "owned", True):
             lib().sparlab_system_destroy(h)
-            self.h = None
+        self.h = None
 
     def pattern(self):
         row_ptr = np.zeros(3 * self.num_nodes + 1, dtype=np.int32)
@@ -507,6 +511,101 @@ class NonlinearSystem:
         ke = np.zeros((self.num_elements, 24, 24))
         _check(lib().sparlab_system_element_outputs(self.h, _np_ptr(fe), _np_ptr(ke)))
         return fe, ke
+
+
+class BatchedSystem:
+    """The batched form (sparlab_batch_*): many independent problems, each a
+    NonlinearSystem of its own, evaluated together as one block-diagonal
+    union system whose launches are captured in a HIP graph.
+
+    problems: list of (coords (n_p, 3), conn (e_p, 8) with problem-local node
+    indices). `system` is the union's NonlinearSystem (pattern, history,
+    commit); problem p's DOFs are [3 node_offset[p], 3 node_offset[p + 1]) and
+    its CSR values [value_offset[p], value_offset[p + 1]).
+    """
+
+    def __init__(self, problems, material, rule=None, kinematics=FINITE_LOGARITHMIC):
+        import torch  # noqa: F401
+        L = lib()
+        L.sparlab_batch_create.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.POINTER(CMaterial), ctypes.POINTER(CRule),
+                                           ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.sparlab_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.sparlab_batch_system.restype = ctypes.c_void_p
+        L.sparlab_batch_system.argtypes = [ctypes.c_void_p]
+        L.sparlab_batch_value_offsets.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.sparlab_batch_evaluate_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.sparlab_batch_set_graphs.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.sparlab_batch_result.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 6
+        L.sparlab_batch_autotune.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        rule = rule or Rule()
+        coords = [np.ascontiguousarray(c, dtype=np.float64).reshape(-1, 3) for c, _ in problems]
+        conns = [np.ascontiguousarray(k, dtype=np.int32).reshape(-1, 8) for _, k in problems]
+        self.num_problems = len(problems)
+        self.node_offset = np.concatenate([[0], np.cumsum([c.shape[0] for c in coords])]).astype(np.int64)
+        self.element_offset = np.concatenate([[0], np.cumsum([k.shape[0] for k in conns])]).astype(np.int64)
+        ucoords = np.ascontiguousarray(np.concatenate(coords))
+        lconn = np.ascontiguousarray(np.concatenate(conns))
+        h = ctypes.c_void_p()
+        _check(L.sparlab_batch_create(self.num_problems, _np_ptr(self.node_offset), _np_ptr(self.element_offset),
+                                      _np_ptr(ucoords), _np_ptr(lconn), ctypes.byref(material.c),
+                                      ctypes.byref(rule.c), kinematics, ctypes.byref(h)))
+        self.h = h
+        uconn = lconn + np.repeat(self.node_offset[:-1], [k.shape[0] for k in conns])[:, None].astype(np.int32)
+        self.system = NonlinearSystem(ucoords, uconn, material, rule, kinematics,
+                                      _handle=ctypes.c_void_p(L.sparlab_batch_system(h)))
+        self.value_offset = np.zeros(self.num_problems + 1, dtype=np.int64)
+        _check(L.sparlab_batch_value_offsets(h, _np_ptr(self.value_offset)))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            sys_ = getattr(self, "system", None)
+            if sys_ is not None:
+                sys_.h = None
+            lib().sparlab_batch_destroy(h)
+            self.h = None
+
+    def set_graphs(self, enabled):
+        _check(lib().sparlab_batch_set_graphs(self.h, int(enabled)))
+
+    def autotune(self, u, stream=None):
+        """Picks the local solve's elements per wavefront on this batch at u (device tensor)."""
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream(u.device)
+        chosen = ctypes.c_int(0)
+        _check(lib().sparlab_batch_autotune(self.h, ctypes.c_void_p(u.data_ptr()), ctypes.byref(chosen),
+                                            ctypes.c_void_p(s.cuda_stream)))
+        return chosen.value
+
+    def evaluate_device(self, u, fint, values=None, want_tangent=True, column_major=False, stream=None):
+        """Every problem's evaluation (device tensors over the union; the same
+        tensors every call, as the captured graph keeps their addresses)."""
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream(u.device)
+        _check(lib().sparlab_batch_evaluate_device(self.h, ctypes.c_void_p(u.data_ptr()), int(want_tangent),
+                                                   int(column_major), ctypes.c_void_p(fint.data_ptr()),
+                                                   None if values is None else ctypes.c_void_p(values.data_ptr()),
+                                                   ctypes.c_void_p(s.cuda_stream)))
+
+    def result(self, stream=None):
+        """Per-problem scalars of the last evaluation: dict of arrays (energy,
+        gross, yielding, failed, first_failed)."""
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        n = self.num_problems
+        out = dict(energy=np.zeros(n), gross=np.zeros(n), yielding=np.zeros(n, np.int64),
+                   failed=np.zeros(n, np.int64), first_failed=np.zeros(n, np.int64))
+        rc = lib().sparlab_batch_result(self.h, _np_ptr(out["energy"]), _np_ptr(out["gross"]),
+                                        _np_ptr(out["yielding"]), _np_ptr(out["failed"]),
+                                        _np_ptr(out["first_failed"]), ctypes.c_void_p(s.cuda_stream))
+        if rc not in (0, -4):
+            _check(rc)
+        return out
+
+    def commit(self):
+        self.system.commit()
 
 
 def make_tool(shape="sphere", centre=(0, 0, 0), centre0=None, radius=0.0, normal=(0, 0, 1), axis=(0, 0, 1),

@@ -122,3 +122,53 @@ def assembly_workload(nx=100, ny=100, nz=4, seed=99, contact_nodes=10000):
     top = np.arange(nz * nnx * nny, (nz + 1) * nnx * nny, dtype=np.int32)
     nodes = np.sort(rng.choice(top, size=min(contact_nodes, top.size), replace=False)).astype(np.int32)
     return dict(coords=coords, conn=conn, fe=fe, ke=ke, contact_nodes=nodes)
+
+
+FAMILIES = ("dome", "truncated_cone", "elliptic_cone", "pyramid")
+
+
+def _family_depth(family, x, y, half, depth, shape):
+    """Downward deflection profile of a part family over the footprint
+    (centred at 0, half-width `half`), scaled to `depth` at the centre."""
+    if family == "dome":
+        r = np.hypot(x, y) / (0.9 * half)
+        return -depth * np.clip(1.0 - r * r, 0.0, None)
+    if family == "truncated_cone":
+        r = np.hypot(x, y)
+        top = shape * half
+        return -depth * np.clip((0.9 * half - r) / (0.9 * half - top), 0.0, 1.0)
+    if family == "elliptic_cone":
+        r = np.hypot(x / (0.9 * half), y / (0.9 * half * (0.55 + 0.4 * shape)))
+        return -depth * np.clip(1.0 - r, 0.0, 1.0)
+    r = np.maximum(np.abs(x), np.abs(y)) / (0.9 * half)
+    return -depth * np.clip(1.0 - r, 0.0, 1.0)
+
+
+def batched_problems(count=64, seed=20241203, min_elements=2000, max_elements=40000, layers=2, step=0.03):
+    """The batched form's workload: `count` sheet-forming problems from a
+    scrambled Sobol draw (scipy.stats.qmc, seeded) over footprint
+    15-20 mm, depth 2-4 mm, a family shape parameter and the mesh size, a
+    quarter of them in each part family (dome, truncated cone, elliptic cone,
+    pyramid); 1 mm sheets meshed `layers` Hex8 through the thickness with
+    element counts spread over [min_elements, max_elements]. Each problem's
+    displacement is one forming increment, `step` times its family's
+    deflection profile (every node of a column moves alike), from a virgin
+    history. Returns a list of dicts: coords (n, 3), conn (e, 8) local, u (n, 3),
+    family, footprint, depth."""
+    from scipy.stats import qmc
+    pts = qmc.Sobol(d=4, scramble=True, seed=seed).random(count)
+    out = []
+    for i in range(count):
+        family = FAMILIES[i % len(FAMILIES)]
+        footprint = 0.015 + 0.005 * pts[i, 0]
+        depth = 0.002 + 0.002 * pts[i, 1]
+        shape = pts[i, 2]
+        target = min_elements + (max_elements - min_elements) * pts[i, 3]
+        n = max(2, int(round(np.sqrt(target / layers))))
+        coords, conn = structured_hex_mesh(n, n, layers, footprint, footprint, 0.001,
+                                           -footprint / 2, -footprint / 2, -0.001)
+        w = _family_depth(family, coords[:, 0], coords[:, 1], footprint / 2, depth, shape)
+        u = np.zeros_like(coords)
+        u[:, 2] = step * w
+        out.append(dict(coords=coords, conn=conn, u=u, family=family, footprint=footprint, depth=depth))
+    return out

@@ -131,3 +131,19 @@ def test_pattern_refuses_bad_connectivity():
     bad[0, 3] = bad[0, 2]
     with pytest.raises(RuntimeError):
         G.pattern(coords.shape[0], bad)
+
+
+def test_union_pattern_is_block_diagonal():
+    """The batched form's premise: problems that share no nodes give a union
+    pattern whose rows of problem p are problem p's pattern shifted by its
+    first DOF, and whose values of problem p are one contiguous slice."""
+    meshes = [W.structured_hex_mesh(*d, 1.0, 1.0, 1.0) for d in ((2, 2, 1), (3, 1, 2), (1, 1, 1), (4, 3, 1))]
+    node_off = np.cumsum([0] + [c.shape[0] for c, _ in meshes])
+    uconn = np.concatenate([k + node_off[i] for i, (_, k) in enumerate(meshes)]).astype(np.int32)
+    row_ptr, col_idx, _ = G.pattern(int(node_off[-1]), uconn)
+    for i, (c, k) in enumerate(meshes):
+        rp, ci, _ = G.pattern(c.shape[0], k)
+        r0, r1 = 3 * node_off[i], 3 * node_off[i + 1]
+        v0, v1 = row_ptr[r0], row_ptr[r1]
+        assert np.array_equal(row_ptr[r0:r1 + 1] - v0, rp)
+        assert np.array_equal(col_idx[v0:v1] - 3 * node_off[i], ci)
