@@ -250,6 +250,32 @@ int sparlab_system_set_history(sparlab_system* system, const double* states, con
 /* The last evaluation's element forces (24 per element) and tangents (576). */
 int sparlab_system_element_outputs(const sparlab_system* system, double* fe, double* ke);
 
+/* Replace the nodal reference coordinates (host, 3 per node) or the material
+ * of a system (the pattern and the history are kept; the material must keep
+ * the history layout). For gradient-based shape compensation and
+ * calibration loops. */
+int sparlab_system_set_coords(sparlab_system* system, const double* coords);
+int sparlab_system_set_material(sparlab_system* system, const sparlab_material* material);
+
+/* The differentiation form (no reference counterpart): vector-Jacobian
+ * products of the last evaluation's assembled internal force, at its
+ * converged modes and the fixed committed history, for a cotangent v
+ * (device, 3 per node). wrt selects (device outputs):
+ *   SPARLAB_VJP_COORDS        grad_coords (3 nn) = v^T d f_int / d X (reference coordinates)
+ *   SPARLAB_VJP_DISPLACEMENT  grad_disp (3 nn)   = v^T d f_int / d u (= v^T K_t)
+ *   SPARLAB_VJP_PARAMETERS    grad_params (11)   = v^T d f_int / d (r0, r45, r90,
+ *                             C_1, gamma_1, ..., C_4, gamma_4): Hill48 r-values (r-value
+ *                             calibration) and the Chaboche backstresses; zero for
+ *                             parameters the material does not have.
+ * u must be the evaluation's displacement; call before commit and before new
+ * coordinates, material or history. Asynchronous on stream. */
+#define SPARLAB_VJP_COORDS 1
+#define SPARLAB_VJP_DISPLACEMENT 2
+#define SPARLAB_VJP_PARAMETERS 4
+#define SPARLAB_VJP_NUM_PARAMETERS 11
+int sparlab_system_vjp_device(sparlab_system* system, const double* u, const double* v, int wrt,
+                              double* grad_coords, double* grad_disp, double* grad_params, void* stream);
+
 /* The pattern of a Hex8 mesh on the host, without a device (the reference's
  * Assembler::build_pattern, src/fem/Assembler.cpp:86-173): returns nnz (or a
  * negative code); with row_ptr (3 num_nodes + 1) and col_idx (nnz) non-null,

@@ -1,0 +1,472 @@
+// The adjoint of the element evaluation: vector-Jacobian products of the
+// condensed internal force with respect to the nodal reference coordinates,
+// the nodal displacements and the Hill48/Chaboche parameters, at the
+// converged modes and the fixed committed history of the last evaluation.
+//
+// At the converged modes r_a(X, u, alpha, theta) = 0 defines alpha, and
+// f* = r_u, so (implicit function theorem)
+//   v^T df*/dy = v^T dr_u/dy - lambda^T dr_a/dy = w^T dr/dy,
+//   lambda = K_aa^-T K_ua^T v,  w = [v; -lambda]  (33 entries),
+// with r = sum_q w_q B_NL^T S the element's full residual
+// (src/fem/IncompatibleModes.cpp:252-275) and K_aa, K_ua its blocks
+// (:279-306). At fixed u, alpha and committed history, S depends on y only
+// through the Green-Lagrange strain E, with dS/dE = the point's tangent (the
+// "material" of :227-247), so
+//   w^T dr/dy = sum_q [ d(w_q B_q w)/dy . S_q + w_q (B_q w) . (mat_q dE_q/dy) ].
+// The coordinate and displacement derivatives of w_q, B_q and E_q are
+// forward-mode dual numbers through the set-up and the kinematics
+// (IncompatibleModes.cpp:134-222), one direction per element DOF. The
+// parameters enter through the stress alone: dS = P_log^T dT, and the general
+// return's stress (src/material/Plasticity.cpp:337-471) moves with its
+// converged (xi, dl) by d(xi, dl) = -J^-1 dR/dtheta, J the closest-point
+// Jacobian, solved by the return's own block solver.
+//
+// One team of 8 lanes per element (lane = integration point for the point
+// work, lane = node for the node rows), 8 elements per wavefront.
+#include "adjoint_kernels.hpp"
+#include "element_pass.hpp"
+#include "team_linalg.hpp"
+#include "workspace_records.hpp"
+
+namespace sbk {
+
+namespace {
+
+constexpr int kTeams = 8;
+
+struct Dual {
+  double v, d;
+};
+__device__ __forceinline__ Dual dual(double v, double d = 0.0) { return Dual{v, d}; }
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b) {
+  const double q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+
+__device__ __forceinline__ Dual det3d(const Dual m[3][3]) {
+  return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+         m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+}
+__device__ __forceinline__ void inverse3d(const Dual m[3][3], Dual r[3][3]) {
+  Dual c[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+      c[i][j] = m[i1][j1] * m[i2][j2] - m[i1][j2] * m[i2][j1];
+    }
+  const Dual det = c[0][0] * m[0][0] + c[1][0] * m[1][0] + c[2][0] * m[2][0];
+  const Dual inv = dual(1.0) / det;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r[i][j] = c[j][i] * inv;
+}
+__device__ __forceinline__ Dual bnld(const Dual F[3][3], const Dual g[3][11], int c, int a, int k) {
+  switch (c) {
+    case 0: return F[k][0] * g[0][a];
+    case 1: return F[k][1] * g[1][a];
+    case 2: return F[k][2] * g[2][a];
+    case 3: return F[k][0] * g[1][a] + F[k][1] * g[0][a];
+    case 4: return F[k][1] * g[2][a] + F[k][2] * g[1][a];
+    default: return F[k][2] * g[0][a] + F[k][0] * g[2][a];
+  }
+}
+
+// One directional derivative of the point's share of w^T r: the direction is
+// DOF j of the element's coordinates (WRT_U false) or displacements (true).
+template <int KIN, bool WRT_U>
+__device__ double point_direction(const double x0[8][3], const double ue[24], const double alpha[9], double xi,
+                                  double eta, double zeta, double wq, int j, const double S[6],
+                                  const double mat[6][6], const double w33[33]) {
+  Dual X[8][3], U[24];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      X[a][k] = dual(x0[a][k], (!WRT_U && 3 * a + k == j) ? 1.0 : 0.0);
+      U[3 * a + k] = dual(ue[3 * a + k], (WRT_U && 3 * a + k == j) ? 1.0 : 0.0);
+    }
+  double dn0[8][3], dn[8][3];
+  shape_gradients(0.0, 0.0, 0.0, dn0);
+  shape_gradients(xi, eta, zeta, dn);
+  Dual j0[3][3], jq[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      Dual s0 = X[0][i] * dn0[0][k], s = X[0][i] * dn[0][k];
+#pragma unroll
+      for (int a = 1; a < 8; ++a) {
+        s0 = s0 + X[a][i] * dn0[a][k];
+        s = s + X[a][i] * dn[a][k];
+      }
+      j0[i][k] = s0;
+      jq[i][k] = s;
+    }
+  const Dual det0 = det3d(j0), det = det3d(jq);
+  Dual ji0[3][3], ji[3][3];
+  inverse3d(j0, ji0);
+  inverse3d(jq, ji);
+  Dual g[3][11];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) g[i][a] = dn[a][0] * ji[0][i] + dn[a][1] * ji[1][i] + dn[a][2] * ji[2][i];
+  const Dual ratio = det0 / det;
+  const double nat[3] = {xi, eta, zeta};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const Dual f = ratio * (-2.0 * nat[k]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) g[i][8 + k] = f * ji0[k][i];
+  }
+  Dual h[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj) {
+      Dual s = U[i] * g[jj][0];
+#pragma unroll
+      for (int a = 1; a < 8; ++a) s = s + U[3 * a + i] * g[jj][a];
+      h[i][jj] = s + (alpha[i] * g[jj][8] + alpha[3 + i] * g[jj][9] + alpha[6 + i] * g[jj][10]);
+    }
+  const Dual w = wq * det;
+  Dual F[3][3], E[6];
+  if constexpr (KIN == kSmall) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) F[i][k] = dual(i == k ? 1.0 : 0.0);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      Dual s = bnld(F, g, c, 0, 0) * U[0];
+#pragma unroll
+      for (int kk = 1; kk < 24; ++kk) s = s + bnld(F, g, c, kk / 3, kk % 3) * U[kk];
+#pragma unroll
+      for (int kk = 0; kk < 9; ++kk) s = s + bnld(F, g, c, 8 + kk / 3, kk % 3) * alpha[kk];
+      E[c] = s;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) F[i][k] = (i == k) ? h[i][k] + dual(1.0) : h[i][k];
+    Dual e3[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const Dual hth = h[0][i] * h[0][k] + h[1][i] * h[1][k] + h[2][i] * h[2][k];
+        e3[i][k] = 0.5 * ((h[i][k] + h[k][i]) + hth);
+      }
+    E[0] = e3[0][0];
+    E[1] = e3[1][1];
+    E[2] = e3[2][2];
+    E[3] = 2.0 * e3[0][1];
+    E[4] = 2.0 * e3[1][2];
+    E[5] = 2.0 * e3[2][0];
+  }
+  double out = 0.0;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    Dual y = bnld(F, g, c, 0, 0) * w33[0];
+#pragma unroll
+    for (int col = 1; col < 33; ++col) y = y + bnld(F, g, c, col / 3, col % 3) * w33[col];
+    const Dual z = w * y;
+    double me = mat[c][0] * E[0].d;
+#pragma unroll
+    for (int d = 1; d < 6; ++d) me = me + mat[c][d] * E[d].d;
+    out = out + z.d * S[c] + z.v * me;
+  }
+  return out;
+}
+
+// d(theta_t, chi_t)/d(gamma_t) of a backstress's decay (Plasticity.cpp:192-212).
+__device__ __forceinline__ void decay_rate_derivative(const DevMaterial& mat, int t, double dl, double& dth,
+                                                      double& dchi) {
+  const double gm = mat.rec[t];
+  if (mat.exponential) {
+    if (gm == 0.0) {
+      dth = -dl;
+      dchi = -0.5 * dl * dl;
+    } else {
+      const double th = exp(-gm * dl);
+      const double chi = -expm1(-gm * dl) / gm;
+      dth = -dl * th;
+      dchi = (dl * th - chi) / gm;
+    }
+  } else {
+    const double th = 1.0 / (1.0 + gm * dl);
+    dth = -dl * th * th;
+    dchi = -dl * dl * th * th;
+  }
+}
+
+// The point's share of w^T dr/dtheta for every parameter (out[kAdjointParams]).
+template <int KIN>
+__device__ void point_parameters(const AdjointArgs& Q, const double x0[8][3], const double ue[24],
+                                 const double alpha[9], int p, const PointState& st, const double w33[33],
+                                 double out[kAdjointParams]) {
+  const DevMaterial& mat = Q.e.mat;
+#pragma unroll
+  for (int k = 0; k < kAdjointParams; ++k) out[k] = 0.0;
+  if (!mat.general || !mat.enabled) return;
+  PointGeom pg;
+  double fv = 0.0;
+  if (setup_point(x0, ue, Q.e.rule.xi[p], Q.e.rule.eta[p], Q.e.rule.zeta[p], Q.e.rule.w[p], pg, fv) != kOk) return;
+  ModeGeom mg;
+  mode_geom<KIN>(pg, ue, mg);
+  double h[3][3], F[3][3];
+  mode_kinematics<KIN>(mg.hu, mg.gm, alpha, h, F);
+  double strain[6];
+  LogData L;
+  if constexpr (KIN == kSmall) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double s2 = bnl_mode(F, mg.gm, c, 0, 0) * alpha[0];
+#pragma unroll
+      for (int kk = 1; kk < 9; ++kk) s2 = s2 + bnl_mode(F, mg.gm, c, kk / 3, kk % 3) * alpha[kk];
+      strain[c] = mg.eps_u[c] + s2;
+    }
+  } else {
+    double e3[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double hth = h[0][i] * h[0][j] + h[1][i] * h[1][j] + h[2][i] * h[2][j];
+        e3[i][j] = 0.5 * ((h[i][j] + h[j][i]) + hth);
+      }
+    double green[6] = {e3[0][0], e3[1][1], e3[2][2], 2.0 * e3[0][1], 2.0 * e3[1][2], 2.0 * e3[2][0]};
+    if constexpr (KIN == kLog) {
+      if (logarithmic_strain(green, L) != 0) return;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) strain[i] = L.strain[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) strain[i] = green[i];
+    }
+  }
+  GenTrial trial;
+  gen_trial(mat, strain, st, trial);
+  if (!(trial.f_tr > 1.0e-12 * trial.sy_n)) return;  // elastic: the stress does not depend on theta
+  double xi[6], dl;
+  GenPoint e;
+  if (!gen_newton(mat, st, trial, xi, dl, e)) return;
+  // z = w (B w) at the converged alpha
+  double zc[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    double y = bnl(F, pg.g, c, 0, 0) * w33[0];
+#pragma unroll
+    for (int col = 1; col < 33; ++col) y = y + bnl(F, pg.g, c, col / 3, col % 3) * w33[col];
+    zc[c] = pg.w * y;
+  }
+  double lp[6][6];
+  if constexpr (KIN == kLog) log_projection(L, lp);
+  JacSolver J;
+  jac_setup(mat, e, dl, st.abar, J);
+  const double g2 = 2.0 * mat.shear;
+  for (int k = 0; k < kAdjointParams; ++k) {
+    const bool hill = k < 3;
+    const int t = (k - 3) / 2;
+    if (hill && !Q.hill) continue;
+    if (!hill && t >= Q.backstresses) continue;
+    double dR[7] = {0, 0, 0, 0, 0, 0, 0}, dpx[6] = {0, 0, 0, 0, 0, 0};
+    double xdx = 0.0;
+    if (hill) {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        double s = Q.dP[k][i * 6] * xi[0];
+#pragma unroll
+        for (int jj = 1; jj < 6; ++jj) s = s + Q.dP[k][i * 6 + jj] * xi[jj];
+        dpx[i] = s;
+      }
+      xdx = dot6(xi, dpx);
+      const double deq = xdx / (2.0 * e.eq);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        const double dm = dpx[i] / e.eq - e.m[i] * deq / e.eq;
+        dR[i] = e.b * (i < 3 ? dm : 0.5 * dm);
+      }
+      dR[6] = deq;
+    } else if ((k - 3) % 2 == 0) {  // C_t
+      double th, chi, dth, dchi;
+      decay(mat, t, dl, th, chi, dth, dchi);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) dR[i] = 2.0 / 3.0 * chi * (i < 3 ? e.m[i] : 0.5 * e.m[i]);
+    } else {  // gamma_t
+      double dthg, dchig;
+      decay_rate_derivative(mat, t, dl, dthg, dchig);
+#pragma unroll
+      for (int i = 0; i < 6; ++i)
+        dR[i] = dthg * st.kin[t][i] + 2.0 / 3.0 * mat.mod[t] * dchig * (i < 3 ? e.m[i] : 0.5 * e.m[i]);
+    }
+    double b1[6], dxi[6], ddl;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) b1[i] = -dR[i];
+    jac_solve(mat, J, b1, -dR[6], dxi, ddl);
+    // dm = (dP xi + P dxi) / eq - m deq / eq,  deq = xi.dP.xi / (2 eq) + m.dxi
+    double pdx[6];
+    p_times(mat, dxi, pdx);
+    const double deq = xdx / (2.0 * e.eq) + dot6(e.m, dxi);
+    double dT[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const double dm = (dpx[i] + pdx[i]) / e.eq - e.m[i] * deq / e.eq;
+      const double mt = i < 3 ? e.m[i] : 0.5 * e.m[i];
+      const double dmt = i < 3 ? dm : 0.5 * dm;
+      dT[i] = -g2 * (ddl * mt + dl * dmt);
+    }
+    double dS[6];
+    if constexpr (KIN == kLog) {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        double s = lp[0][i] * dT[0];
+#pragma unroll
+        for (int r = 1; r < 6; ++r) s = s + lp[r][i] * dT[r];
+        dS[i] = s;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) dS[i] = dT[i];
+    }
+    out[k] = dot6(zc, dS);
+  }
+}
+
+}  // namespace
+
+template <int KIN>
+__global__ __launch_bounds__(64) void adjoint_kernel(AdjointArgs Q) {
+  __shared__ LuSlot slots[kTeams];
+  const ElementArgs& A = Q.e;
+  const int lane = threadIdx.x;
+  const int t = lane >> 3;
+  const int p = lane & 7;
+  LuSlot& T = slots[t];
+  const long long e = static_cast<long long>(blockIdx.x) * kTeams + t;
+  if (e >= A.ne) return;
+  const int n = A.rule.n;
+  const long long q0 = e * n;
+  if (A.status[e] != kOk) {
+    // a failed element contributes nothing
+    for (int k = 0; k < 3; ++k) {
+      if (Q.grad_x) Q.grad_x[e * 24 + 3 * p + k] = 0.0;
+      if (Q.grad_u) Q.grad_u[e * 24 + 3 * p + k] = 0.0;
+    }
+    if (Q.grad_p)
+      for (int k = p; k < kAdjointParams; k += kTeams) Q.grad_p[k * A.ne + e] = 0.0;
+    return;
+  }
+  // K_aa's factors of the final local iteration
+  const double* er = A.elem_ws + e * kElemRecord;
+  for (int i = p; i < 81; i += 8) T.a[i] = er[kErFactors + i];
+  T.rt[p] = static_cast<int>(er[kErRows + p]);
+  T.ct[p] = static_cast<int>(er[kErCols + p]);
+  if (p == 0) {
+    T.rt[8] = static_cast<int>(er[kErRows + 8]);
+    T.ct[8] = static_cast<int>(er[kErCols + 8]);
+  }
+  team_sync();
+  double x0[8][3], ue[24];
+  load_element(A.coords, A.conn, A.u, e, x0, ue);
+  double w33[33];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const long long node = A.conn[e * 8 + a];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) w33[3 * a + k] = Q.v[node * 3 + k];
+  }
+  {
+    // lambda = K_aa^-T K_ua^T v_e (lane a holds K_ua's rows 3a..3a+2)
+    double ru[3], kua[3][9];
+    coupling_rows<KIN>(A, q0, n, p, ru, kua);
+    double g9[9], lam[9];
+#pragma unroll
+    for (int nn = 0; nn < 9; ++nn) {
+      double s = kua[0][nn] * w33[3 * p];
+      s = s + kua[1][nn] * w33[3 * p + 1];
+      s = s + kua[2][nn] * w33[3 * p + 2];
+      g9[nn] = team_sum(s);
+    }
+    lu_solve_transposed(T, g9, lam);
+#pragma unroll
+    for (int nn = 0; nn < 9; ++nn) w33[24 + nn] = -lam[nn];
+  }
+  double alpha[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) alpha[i] = A.alpha_out[e * 9 + i];
+  const bool has_point = p < n;
+  double S[6] = {0, 0, 0, 0, 0, 0}, mat[6][6];
+  {
+    const double* pr = A.point_ws + (q0 + (has_point ? p : 0)) * kPointRecord;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) S[c] = has_point ? pr[kWsStress + c] : 0.0;
+#pragma unroll
+    for (int i = 0; i < 36; ++i) mat[i / 6][i % 6] = has_point ? pr[kWsTangent + i] : 0.0;
+  }
+  const double xi = has_point ? A.rule.xi[p] : 0.0, eta = has_point ? A.rule.eta[p] : 0.0;
+  const double zeta = has_point ? A.rule.zeta[p] : 0.0, wq = has_point ? A.rule.w[p] : 0.0;
+  if (Q.wrt & kWrtCoords) {
+    double mine[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < 24; ++j) {
+      const double c = has_point ? point_direction<KIN, false>(x0, ue, alpha, xi, eta, zeta, wq, j, S, mat, w33) : 0.0;
+      const double tot = team_sum(c);
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if (j == 3 * p + k) mine[k] = tot;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) Q.grad_x[e * 24 + 3 * p + k] = mine[k];
+  }
+  if (Q.wrt & kWrtDisplacement) {
+    double mine[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < 24; ++j) {
+      const double c = has_point ? point_direction<KIN, true>(x0, ue, alpha, xi, eta, zeta, wq, j, S, mat, w33) : 0.0;
+      const double tot = team_sum(c);
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if (j == 3 * p + k) mine[k] = tot;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) Q.grad_u[e * 24 + 3 * p + k] = mine[k];
+  }
+  if (Q.wrt & kWrtParams) {
+    double c[kAdjointParams];
+    if (has_point) {
+      PointState st;
+      load_state(A.mat, A.state_in, A.state_stride, q0 + p, st);
+      point_parameters<KIN>(Q, x0, ue, alpha, p, st, w33, c);
+    } else {
+#pragma unroll
+      for (int k = 0; k < kAdjointParams; ++k) c[k] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < kAdjointParams; ++k) {
+      const double tot = team_sum(c[k]);
+      if (p == (k & 7)) Q.grad_p[k * A.ne + e] = tot;
+    }
+  }
+}
+
+int launch_adjoint(const AdjointArgs& a, hipStream_t stream) {
+  (void)hipGetLastError();
+  const unsigned int blocks = static_cast<unsigned int>((a.e.ne + kTeams - 1) / kTeams);
+  if (blocks == 0) return 0;
+  switch (a.e.kinematics) {
+    case kSmall: hipLaunchKernelGGL(adjoint_kernel<kSmall>, dim3(blocks), dim3(64), 0, stream, a); break;
+    case kFinite: hipLaunchKernelGGL(adjoint_kernel<kFinite>, dim3(blocks), dim3(64), 0, stream, a); break;
+    default: hipLaunchKernelGGL(adjoint_kernel<kLog>, dim3(blocks), dim3(64), 0, stream, a); break;
+  }
+  return static_cast<int>(hipGetLastError());
+}
+
+}  // namespace sbk

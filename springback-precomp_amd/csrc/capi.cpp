@@ -120,6 +120,110 @@ int kinematic_terms(const sparlab_material& m) {
 bool general_law(const sparlab_material& m) { return m.criterion == 1 || m.num_backstresses > 0; }
 int kin_fields(const sparlab_material& m) { return general_law(m) ? kinematic_terms(m) : 1; }
 
+// The material frame's rows (Material.cpp:190-204): the rolling direction
+// made orthogonal to the sheet normal, the transverse direction, the normal.
+bool material_frame(const sparlab_material& m, double q[3][3], std::string& why) {
+  const double* rdi = m.rolling_direction;
+  const double* ndi = m.sheet_normal;
+  const double nn = std::sqrt(ndi[0] * ndi[0] + ndi[1] * ndi[1] + ndi[2] * ndi[2]);
+  const double rn = std::sqrt(rdi[0] * rdi[0] + rdi[1] * rdi[1] + rdi[2] * rdi[2]);
+  if (!(nn > 0.0) || !(rn > 0.0) || !std::isfinite(nn) || !std::isfinite(rn)) {
+    why = "the rolling direction and the sheet normal must be finite non-zero vectors";
+    return false;
+  }
+  double nd[3], rd[3];
+  for (int i = 0; i < 3; ++i) { nd[i] = ndi[i] / nn; rd[i] = rdi[i] / rn; }
+  const double c = rd[0] * nd[0] + rd[1] * nd[1] + rd[2] * nd[2];
+  for (int i = 0; i < 3; ++i) rd[i] -= c * nd[i];
+  const double r2 = std::sqrt(rd[0] * rd[0] + rd[1] * rd[1] + rd[2] * rd[2]);
+  if (!(r2 > 1.0e-6)) {
+    why = "the rolling direction is parallel to the sheet normal";
+    return false;
+  }
+  for (int i = 0; i < 3; ++i) rd[i] /= r2;
+  const double td[3] = {nd[1] * rd[2] - nd[2] * rd[1], nd[2] * rd[0] - nd[0] * rd[2], nd[0] * rd[1] - nd[1] * rd[0]};
+  for (int j = 0; j < 3; ++j) {
+    q[0][j] = rd[j];
+    q[1][j] = td[j];
+    q[2][j] = nd[j];
+  }
+  return true;
+}
+
+// P = T^T P_m T: the Hill48 matrix of (F, G, H, L, M, N) in the material
+// frame, mapped to global axes (Material.cpp:205-216). Linear in the
+// coefficients, so it maps coefficient derivatives as well.
+void hill_global(const double q[3][3], double F, double G, double H, double L, double M, double N, double* out) {
+  double pm[6][6] = {};
+  pm[0][0] = G + H;
+  pm[1][1] = F + H;
+  pm[2][2] = F + G;
+  pm[0][1] = pm[1][0] = -H;
+  pm[1][2] = pm[2][1] = -F;
+  pm[0][2] = pm[2][0] = -G;
+  pm[3][3] = 2.0 * N;
+  pm[4][4] = 2.0 * L;
+  pm[5][5] = 2.0 * M;
+  // Voigt map of sigma' = q sigma q^T, column k = the image of basis tensor k
+  static const int vi[6] = {0, 1, 2, 0, 1, 2};
+  static const int vj[6] = {0, 1, 2, 1, 2, 0};
+  double t[6][6];
+  for (int k = 0; k < 6; ++k) {
+    double e[3][3] = {};
+    e[vi[k]][vj[k]] = 1.0;
+    e[vj[k]][vi[k]] = 1.0;
+    double qe[3][3], r[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) qe[i][j] = q[i][0] * e[0][j] + q[i][1] * e[1][j] + q[i][2] * e[2][j];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) r[i][j] = qe[i][0] * q[j][0] + qe[i][1] * q[j][1] + qe[i][2] * q[j][2];
+    for (int row = 0; row < 6; ++row) t[row][k] = r[vi[row]][vj[row]];
+  }
+  double tp[6][6];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 6; ++k) s += t[k][i] * pm[k][j];
+      tp[i][j] = s;
+    }
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 6; ++k) s += tp[i][k] * t[k][j];
+      out[i * 6 + j] = s;
+    }
+}
+
+}  // namespace
+
+namespace sbk {
+
+// dP/d(r0, r45, r90) of a finalized Hill48 material calibrated by r-values
+// (the derivatives of Material.cpp:140-151 mapped to global axes); false
+// (and zeros) for any other criterion or calibration.
+bool hill_r_derivatives(const sparlab_material& m, double dP[3][36]) {
+  for (int k = 0; k < 3; ++k)
+    for (int i = 0; i < 36; ++i) dP[k][i] = 0.0;
+  if (m.criterion != 1 || m.hill_calibration != 0) return false;
+  double q[3][3];
+  std::string why;
+  if (!material_frame(m, q, why)) return false;
+  const double r0 = m.hill_r[0], r45 = m.hill_r[1], r90 = m.hill_r[2];
+  const double a = 1.0 + r0;
+  // G = 1/(1+r0), H = r0/(1+r0), F = r0/(r90(1+r0)), N = (r0+r90)(1+2r45)/(2 r90 (1+r0))
+  const double dG[3] = {-1.0 / (a * a), 0.0, 0.0};
+  const double dH[3] = {1.0 / (a * a), 0.0, 0.0};
+  const double dF[3] = {1.0 / (r90 * a * a), 0.0, -r0 / (r90 * r90 * a)};
+  const double dN[3] = {(1.0 + 2.0 * r45) * (1.0 - r90) / (2.0 * r90 * a * a), (r0 + r90) / (r90 * a),
+                        -(1.0 + 2.0 * r45) * r0 / (2.0 * r90 * r90 * a)};
+  for (int k = 0; k < 3; ++k) hill_global(q, dF[k], dG[k], dH[k], 0.0, 0.0, dN[k], dP[k]);
+  return true;
+}
+
+}  // namespace sbk
+
+namespace {
+
 // Eigen-decomposition of the symmetric 6 x 6 matrix a by cyclic Jacobi
 // rotations (a is destroyed): a = V diag(l) V^T, V row-major, column k the
 // eigenvector of l[k].
@@ -293,59 +397,10 @@ int sparlab_material_finalize(sparlab_material* m) {
     if (!std::isfinite(v)) return fail(SPARLAB_E_ARGUMENT, "the Hill48 coefficients must be finite");
   if (!(L > 0.0 && M > 0.0 && N > 0.0) || !(F + H > 0.0) || !(F * G + G * H + H * F > 0.0))
     return fail(SPARLAB_E_ARGUMENT, "the Hill48 coefficients do not make a convex yield surface");
-  const double* rdi = m->rolling_direction;
-  const double* ndi = m->sheet_normal;
-  const double nn = std::sqrt(ndi[0] * ndi[0] + ndi[1] * ndi[1] + ndi[2] * ndi[2]);
-  const double rn = std::sqrt(rdi[0] * rdi[0] + rdi[1] * rdi[1] + rdi[2] * rdi[2]);
-  if (!(nn > 0.0) || !(rn > 0.0) || !std::isfinite(nn) || !std::isfinite(rn))
-    return fail(SPARLAB_E_ARGUMENT, "the rolling direction and the sheet normal must be finite non-zero vectors");
-  double nd[3], rd[3];
-  for (int i = 0; i < 3; ++i) { nd[i] = ndi[i] / nn; rd[i] = rdi[i] / rn; }
-  const double c = rd[0] * nd[0] + rd[1] * nd[1] + rd[2] * nd[2];
-  for (int i = 0; i < 3; ++i) rd[i] -= c * nd[i];
-  const double r2 = std::sqrt(rd[0] * rd[0] + rd[1] * rd[1] + rd[2] * rd[2]);
-  if (!(r2 > 1.0e-6)) return fail(SPARLAB_E_ARGUMENT, "the rolling direction is parallel to the sheet normal");
-  for (int i = 0; i < 3; ++i) rd[i] /= r2;
-  const double td[3] = {nd[1] * rd[2] - nd[2] * rd[1], nd[2] * rd[0] - nd[0] * rd[2], nd[0] * rd[1] - nd[1] * rd[0]};
-  const double q[3][3] = {{rd[0], rd[1], rd[2]}, {td[0], td[1], td[2]}, {nd[0], nd[1], nd[2]}};
-  double pm[6][6] = {};
-  pm[0][0] = G + H;
-  pm[1][1] = F + H;
-  pm[2][2] = F + G;
-  pm[0][1] = pm[1][0] = -H;
-  pm[1][2] = pm[2][1] = -F;
-  pm[0][2] = pm[2][0] = -G;
-  pm[3][3] = 2.0 * N;
-  pm[4][4] = 2.0 * L;
-  pm[5][5] = 2.0 * M;
-  // Voigt map of sigma' = q sigma q^T, column k = the image of basis tensor k
-  static const int vi[6] = {0, 1, 2, 0, 1, 2};
-  static const int vj[6] = {0, 1, 2, 1, 2, 0};
-  double t[6][6];
-  for (int k = 0; k < 6; ++k) {
-    double e[3][3] = {};
-    e[vi[k]][vj[k]] = 1.0;
-    e[vj[k]][vi[k]] = 1.0;
-    double qe[3][3], r[3][3];
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) qe[i][j] = q[i][0] * e[0][j] + q[i][1] * e[1][j] + q[i][2] * e[2][j];
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) r[i][j] = qe[i][0] * q[j][0] + qe[i][1] * q[j][1] + qe[i][2] * q[j][2];
-    for (int row = 0; row < 6; ++row) t[row][k] = r[vi[row]][vj[row]];
-  }
-  double tp[6][6];
-  for (int i = 0; i < 6; ++i)
-    for (int j = 0; j < 6; ++j) {
-      double s = 0.0;
-      for (int k = 0; k < 6; ++k) s += t[k][i] * pm[k][j];
-      tp[i][j] = s;
-    }
-  for (int i = 0; i < 6; ++i)
-    for (int j = 0; j < 6; ++j) {
-      double s = 0.0;
-      for (int k = 0; k < 6; ++k) s += tp[i][k] * t[k][j];
-      m->yield_matrix[i * 6 + j] = s;
-    }
+  double q[3][3];
+  std::string why;
+  if (!material_frame(*m, q, why)) return fail(SPARLAB_E_ARGUMENT, why);
+  hill_global(q, F, G, H, L, M, N, m->yield_matrix);
   const double coeffs[6] = {F, G, H, L, M, N};
   std::memcpy(m->hill_coefficients, coeffs, sizeof(coeffs));
   return SPARLAB_OK;

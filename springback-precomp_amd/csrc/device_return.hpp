@@ -394,65 +394,57 @@ __device__ __forceinline__ double kinematic_energy(const DevMaterial& mat, const
   return out;
 }
 
-// return_general_3d; the state is updated in place (want_tangent = true, as
-// IncompatibleModes.cpp:232 asks for it on every pass).
-__device__ int return_general(const DevMaterial& mat, const double strain[6], PointState& st, ReturnOut& out) {
-  const double g = mat.shear, k = mat.bulk;
+// The elastic trial of the general return (Plasticity.cpp:337-364).
+struct GenTrial {
   double el[6], s_tr[6], xi_tr[6];
+  double vol, sy_n, eq_tr, f_tr;
+};
+
+__device__ __forceinline__ void gen_trial(const DevMaterial& mat, const double strain[6], const PointState& st,
+                                          GenTrial& tr) {
+  const double g = mat.shear;
 #pragma unroll
-  for (int i = 0; i < 6; ++i) el[i] = strain[i] - st.ep[i];
-  const double vol = el[0] + el[1] + el[2];
-  const double mean = vol / 3.0;
+  for (int i = 0; i < 6; ++i) tr.el[i] = strain[i] - st.ep[i];
+  tr.vol = tr.el[0] + tr.el[1] + tr.el[2];
+  const double mean = tr.vol / 3.0;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) s_tr[i] = 2.0 * g * (el[i] - mean);
+  for (int i = 0; i < 3; ++i) tr.s_tr[i] = 2.0 * g * (tr.el[i] - mean);
 #pragma unroll
-  for (int i = 3; i < 6; ++i) s_tr[i] = g * el[i];
+  for (int i = 3; i < 6; ++i) tr.s_tr[i] = g * tr.el[i];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) xi_tr[i] = s_tr[i];
+  for (int i = 0; i < 6; ++i) tr.xi_tr[i] = tr.s_tr[i];
 #pragma unroll
   for (int t = 0; t < kMaxTerms; ++t) {
     if (t >= mat.terms) break;
 #pragma unroll
-    for (int i = 0; i < 6; ++i) xi_tr[i] = xi_tr[i] - st.kin[t][i];
+    for (int i = 0; i < 6; ++i) tr.xi_tr[i] = tr.xi_tr[i] - st.kin[t][i];
   }
-  const double an = st.abar;
-  const double sy_n = yield_stress(mat, an);
+  tr.sy_n = yield_stress(mat, st.abar);
   double px[6];
-  p_times(mat, xi_tr, px);
-  double q = xi_tr[0] * px[0];
+  p_times(mat, tr.xi_tr, px);
+  double q = tr.xi_tr[0] * px[0];
 #pragma unroll
-  for (int i = 1; i < 6; ++i) q = q + xi_tr[i] * px[i];
-  const double eq_tr = sqrt(fmax(q, 0.0));
-  const double f_tr = eq_tr - sy_n;
-  if (!isfinite(f_tr)) return kReturnNonFinite;
-  const bool was_loading = st.loading;
-  st.loading = false;
-  if (!mat.enabled || f_tr <= 1.0e-12 * sy_n) {
+  for (int i = 1; i < 6; ++i) q = q + tr.xi_tr[i] * px[i];
+  tr.eq_tr = sqrt(fmax(q, 0.0));
+  tr.f_tr = tr.eq_tr - tr.sy_n;
+}
+
+// The closest-point Newton from the trial (Plasticity.cpp:365-448): xi and
+// dl converge with e the evaluation at the solution; false when the
+// iteration does not converge.
+__device__ __forceinline__ bool gen_newton(const DevMaterial& mat, const PointState& st, const GenTrial& trial,
+                                           double xi[6], double& dl, GenPoint& e) {
 #pragma unroll
-    for (int i = 0; i < 6; ++i) out.stress[i] = s_tr[i] + (i < 3 ? k * vol : 0.0);
-    elastic_C(k, g, out.C);
-    out.energy = 0.5 * k * vol * vol + g * dev_norm2(el) + isotropic_energy(mat, an) + kinematic_energy(mat, st);
-    out.yielding = false;
-    if (mat.enabled && was_loading && f_tr > -1.0e-10 * sy_n && eq_tr > 0.0) {
-      st.loading = true;
-      GenPoint e;
-      if (gen_evaluate(mat, st, s_tr, xi_tr, 0.0, e)) gen_tangent(mat, st, e, 0.0, out.C);
-    }
-    return kReturnOk;
-  }
-  double xi[6];
+  for (int i = 0; i < 6; ++i) xi[i] = trial.xi_tr[i];
+  dl = 0.0;
+  gen_evaluate(mat, st, trial.s_tr, xi, dl, e);
+  double scale = trial.sy_n;
 #pragma unroll
-  for (int i = 0; i < 6; ++i) xi[i] = xi_tr[i];
-  double dl = 0.0;
-  GenPoint e;
-  gen_evaluate(mat, st, s_tr, xi, dl, e);
-  double scale = sy_n;
+  for (int i = 0; i < 6; ++i) scale = fmax(scale, fabs(trial.s_tr[i]));
 #pragma unroll
-  for (int i = 0; i < 6; ++i) scale = fmax(scale, fabs(s_tr[i]));
-#pragma unroll
-  for (int i = 0; i < 6; ++i) scale = fmax(scale, fabs(xi_tr[i]));
+  for (int i = 0; i < 6; ++i) scale = fmax(scale, fabs(trial.xi_tr[i]));
   const double eps_scale = kEps * scale;
-  const double tol = fmax(1.0e-13 * sy_n, 32.0 * eps_scale);
+  const double tol = fmax(1.0e-13 * trial.sy_n, 32.0 * eps_scale);
   bool conv = false;
   for (int it = 0; it < 50; ++it) {
     const double r_inf = maxabs7(e.res);
@@ -478,7 +470,7 @@ __device__ int return_general(const DevMaterial& mat, const double strain[6], Po
       for (int i = 0; i < 6; ++i) xt[i] = xi[i] + t * step[i][0];
       const double dlt = fmax(dl + t * step[6][0], 0.0);
       GenPoint tr;
-      if (gen_evaluate(mat, st, s_tr, xt, dlt, tr) && norm7(tr.res) <= (1.0 - 1.0e-4 * t) * r0) {
+      if (gen_evaluate(mat, st, trial.s_tr, xt, dlt, tr) && norm7(tr.res) <= (1.0 - 1.0e-4 * t) * r0) {
 #pragma unroll
         for (int i = 0; i < 6; ++i) xi[i] = xt[i];
         dl = dlt;
@@ -490,7 +482,7 @@ __device__ int return_general(const DevMaterial& mat, const double strain[6], Po
     }
     if (!accepted) { conv = r_inf <= 4.0 * tol; break; }
   }
-  if (!conv) return kReturnNotConverged;
+  if (!conv) return false;
   if (maxabs7(e.res) > 4.0 * eps_scale) {
     double step[7][1];
     {
@@ -509,18 +501,48 @@ __device__ int return_general(const DevMaterial& mat, const double strain[6], Po
     for (int i = 0; i < 6; ++i) xp[i] = xi[i] + step[i][0];
     const double dlp = fmax(dl + step[6][0], 0.0);
     GenPoint pol;
-    if (gen_evaluate(mat, st, s_tr, xp, dlp, pol) && maxabs7(pol.res) < maxabs7(e.res)) {
+    if (gen_evaluate(mat, st, trial.s_tr, xp, dlp, pol) && maxabs7(pol.res) < maxabs7(e.res)) {
 #pragma unroll
       for (int i = 0; i < 6; ++i) xi[i] = xp[i];
       dl = dlp;
       e = pol;
     }
   }
+  return true;
+}
+
+// return_general_3d; the state is updated in place (want_tangent = true, as
+// IncompatibleModes.cpp:232 asks for it on every pass).
+__device__ int return_general(const DevMaterial& mat, const double strain[6], PointState& st, ReturnOut& out) {
+  const double g = mat.shear, k = mat.bulk;
+  GenTrial trial;
+  gen_trial(mat, strain, st, trial);
+  const double an = st.abar;
+  if (!isfinite(trial.f_tr)) return kReturnNonFinite;
+  const bool was_loading = st.loading;
+  st.loading = false;
+  if (!mat.enabled || trial.f_tr <= 1.0e-12 * trial.sy_n) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) out.stress[i] = trial.s_tr[i] + (i < 3 ? k * trial.vol : 0.0);
+    elastic_C(k, g, out.C);
+    out.energy = 0.5 * k * trial.vol * trial.vol + g * dev_norm2(trial.el) + isotropic_energy(mat, an) +
+                 kinematic_energy(mat, st);
+    out.yielding = false;
+    if (mat.enabled && was_loading && trial.f_tr > -1.0e-10 * trial.sy_n && trial.eq_tr > 0.0) {
+      st.loading = true;
+      GenPoint e;
+      if (gen_evaluate(mat, st, trial.s_tr, trial.xi_tr, 0.0, e)) gen_tangent(mat, st, e, 0.0, out.C);
+    }
+    return kReturnOk;
+  }
+  double xi[6], dl;
+  GenPoint e;
+  if (!gen_newton(mat, st, trial, xi, dl, e)) return kReturnNotConverged;
   out.yielding = true;
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
     const double mt = i < 3 ? e.m[i] : e.m[i] * 0.5;
-    out.stress[i] = s_tr[i] - 2.0 * g * dl * mt + (i < 3 ? k * vol : 0.0);
+    out.stress[i] = trial.s_tr[i] - 2.0 * g * dl * mt + (i < 3 ? k * trial.vol : 0.0);
   }
   // tangent from the committed backstresses (the Jacobian's back_rate uses alpha_n)
   gen_tangent(mat, st, e, dl, out.C);
@@ -539,8 +561,9 @@ __device__ int return_general(const DevMaterial& mat, const double strain[6], Po
   }
   double eln[6];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) eln[i] = el[i] - dl * e.m[i];
-  out.energy = 0.5 * k * vol * vol + g * dev_norm2(eln) + isotropic_energy(mat, st.abar) + kinematic_energy(mat, st);
+  for (int i = 0; i < 6; ++i) eln[i] = trial.el[i] - dl * e.m[i];
+  out.energy = 0.5 * k * trial.vol * trial.vol + g * dev_norm2(eln) + isotropic_energy(mat, st.abar) +
+               kinematic_energy(mat, st);
   return kReturnOk;
 }
 

@@ -25,6 +25,7 @@
 #include "element_kernels.hpp"
 #include "element_pass.hpp"
 #include "team_linalg.hpp"
+#include "workspace_records.hpp"
 
 namespace sbk {
 
@@ -75,18 +76,6 @@ struct LocalSums {
   bool stop;                    // within_tol or r_a = 0: the stopping test (:469)
 };
 
-// A point record of the workspace: S (6) then the tangent (36, row-major).
-constexpr int kWsStress = 0;
-constexpr int kWsTangent = 6;
-// An element record: LU factors (81), r_a (9), row and column transpositions.
-constexpr int kErFactors = 0;
-constexpr int kErResidual = 81;
-constexpr int kErRows = 90;
-constexpr int kErCols = 99;
-// A geometry record: F (9), [G | G~] (3 x 11, row-major), w.
-constexpr int kGrF = 0;
-constexpr int kGrG = 9;
-constexpr int kGrW = 42;
 
 // B_a (6 x 9) of a point: column n is mode n / 3, direction n % 3.
 __device__ __forceinline__ void mode_operator(const double F[3][3], const double gm[3][3], double ba[6][9]) {
@@ -746,57 +735,6 @@ struct CondenseSlot {
   };
 };
 
-// B_NL(F, [G | G~]) entry (c, 3a + k) with F and G read from a geometry record.
-__device__ __forceinline__ double bnl_rec(const double* gr, int c, int a, int k) {
-  const double* F = gr + kGrF;
-  const double* g = gr + kGrG;
-  switch (c) {
-    case 0: return F[k * 3 + 0] * g[0 * 11 + a];
-    case 1: return F[k * 3 + 1] * g[1 * 11 + a];
-    case 2: return F[k * 3 + 2] * g[2 * 11 + a];
-    case 3: return F[k * 3 + 0] * g[1 * 11 + a] + F[k * 3 + 1] * g[0 * 11 + a];
-    case 4: return F[k * 3 + 1] * g[2 * 11 + a] + F[k * 3 + 2] * g[1 * 11 + a];
-    default: return F[k * 3 + 2] * g[0 * 11 + a] + F[k * 3 + 0] * g[2 * 11 + a];
-  }
-}
-
-// z = B_u[:, 3a..3a+2]^T mat of one point (mat row-major in the point record).
-__device__ __forceinline__ void node_times_tangent(const double bu[6][3], const double* pr, double z[3][6]) {
-#pragma unroll
-  for (int d = 0; d < 6; ++d) {
-    double mc[6];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) mc[c] = pr[kWsTangent + c * 6 + d];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      double s = bu[0][k] * mc[0];
-#pragma unroll
-      for (int c = 1; c < 6; ++c) s = s + bu[c][k] * mc[c];
-      z[k][d] = s;
-    }
-  }
-}
-
-__device__ __forceinline__ void node_operator(const double* gr, int a, double bu[6][3]) {
-#pragma unroll
-  for (int c = 0; c < 6; ++c)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) bu[c][k] = bnl_rec(gr, c, a, k);
-}
-
-// (S G)_a: row a of G^T S (3), S tensorial Voigt.
-__device__ __forceinline__ void stress_times_gradient(const double* gr, const double* pr, int a, double gsa[3]) {
-  const double* g = gr + kGrG;
-  const double* S = pr + kWsStress;
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    const double S3j0 = j == 0 ? S[0] : (j == 1 ? S[3] : S[5]);
-    const double S3j1 = j == 0 ? S[3] : (j == 1 ? S[1] : S[4]);
-    const double S3j2 = j == 0 ? S[5] : (j == 1 ? S[4] : S[2]);
-    gsa[j] = g[0 * 11 + a] * S3j0 + g[1 * 11 + a] * S3j1 + g[2 * 11 + a] * S3j2;
-  }
-}
-
 // One team (8 lanes) per element. Lane p first forms the geometry of point p
 // at the converged alpha (F with the pass's own arithmetic) into the
 // workspace; then lane a owns rows 3a..3a+2 of f*, K_ua, K_uu and K* (and
@@ -851,54 +789,7 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
   const int a = p;
   // f_u rows and K_ua of node a
   double ru[3], kua[3][9];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    ru[k] = 0.0;
-#pragma unroll
-    for (int j = 0; j < 9; ++j) kua[k][j] = 0.0;
-  }
-  for (int q = 0; q < n; ++q) {
-    const double* gr = A.geom_ws + (q0 + q) * kGeomRecord;
-    const double* pr = A.point_ws + (q0 + q) * kPointRecord;
-    const double w = gr[kGrW];
-    double bu[6][3];
-    node_operator(gr, a, bu);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      double s = bu[0][k] * pr[kWsStress];
-#pragma unroll
-      for (int c = 1; c < 6; ++c) s = s + bu[c][k] * pr[kWsStress + c];
-      ru[k] = ru[k] + w * s;
-    }
-    double z[3][6];
-    node_times_tangent(bu, pr, z);
-#pragma unroll
-    for (int nn = 0; nn < 9; ++nn) {
-      double bcol[6];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) bcol[c] = bnl_rec(gr, c, 8 + nn / 3, nn % 3);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        double s = z[k][0] * bcol[0];
-#pragma unroll
-        for (int d = 1; d < 6; ++d) s = s + z[k][d] * bcol[d];
-        kua[k][nn] = kua[k][nn] + w * s;
-      }
-    }
-    if constexpr (KIN != kSmall) {
-      // geometric (G_aug^T S G_aug) (x) I: block (a, 8 + bm)
-      const double* g = gr + kGrG;
-      double gsa[3];
-      stress_times_gradient(gr, pr, a, gsa);
-#pragma unroll
-      for (int bm = 0; bm < 3; ++bm) {
-        const double geo = gsa[0] * g[0 * 11 + 8 + bm] + gsa[1] * g[1 * 11 + 8 + bm] + gsa[2] * g[2 * 11 + 8 + bm];
-        const double v = w * geo;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) kua[k][3 * bm + k] = kua[k][3 * bm + k] + v;
-      }
-    }
-  }
+  coupling_rows<KIN>(A, q0, n, a, ru, kua);
   // f* = f_u - K_ua K_aa^-1 r_a
   {
     double ra[9], corr[9];

@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "adjoint_kernels.hpp"
 #include "assembly_kernels.hpp"
 #include "system_impl.hpp"
 
@@ -45,6 +46,7 @@ void release(sparlab_system* s) {
   dfree(s->block_ptr); dfree(s->block_contrib); dfree(s->block_pos); dfree(s->block_row_len);
   dfree(s->node_ptr); dfree(s->node_incidence); dfree(s->diag_block);
   dfree(s->ws); dfree(s->partial); dfree(s->scalars); dfree(s->summary);
+  dfree(s->grad_x); dfree(s->grad_u); dfree(s->grad_p);
 }
 
 }  // namespace
@@ -88,6 +90,7 @@ int system_enqueue(sparlab_system* s, const double* u, int want_tangent, int col
     return capi_fail(SPARLAB_E_DEVICE, "reduction launch failed");
   s->evaluated = true;
   s->last_tangent = want_tangent != 0;
+  s->adjoint_ready = true;
   return SPARLAB_OK;
 }
 
@@ -266,6 +269,7 @@ int sparlab_system_commit(sparlab_system* s) {
   if (!s || !s->evaluated) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "commit needs an evaluation");
   s->cur = 1 - s->cur;
   s->evaluated = false;
+  s->adjoint_ready = false;
   return SPARLAB_OK;
 }
 
@@ -286,6 +290,83 @@ int sparlab_system_set_history(sparlab_system* s, const double* states, const do
   if ((states && hipMemcpy(s->state[s->cur], states, hist * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) ||
       (alpha && hipMemcpy(s->alpha[s->cur], alpha, 9 * s->ne * sizeof(double), hipMemcpyHostToDevice) != hipSuccess))
     return sbk::capi_fail(SPARLAB_E_DEVICE, "upload failed");
+  s->adjoint_ready = false;
+  return SPARLAB_OK;
+}
+
+int sparlab_system_set_coords(sparlab_system* s, const double* coords) {
+  if (!s || !coords) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (hipMemcpy(s->coords, coords, 3 * static_cast<std::size_t>(s->nn) * sizeof(double), hipMemcpyHostToDevice) !=
+      hipSuccess)
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "upload failed");
+  s->adjoint_ready = false;
+  return SPARLAB_OK;
+}
+
+int sparlab_system_set_material(sparlab_system* s, const sparlab_material* material) {
+  if (!s || !material) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  sbk::ElementArgs a = s->args;
+  const int st = sbk::capi_prepare_elements(material, &s->rule, s->kinematics, a);
+  if (st != SPARLAB_OK) return st;
+  if (sbk::state_fields(a.mat.kin_fields) != s->fields)
+    return sbk::capi_fail(SPARLAB_E_ARGUMENT, "the new material needs another history layout (kinematic terms)");
+  a.chunk = s->args.chunk;
+  s->args = a;
+  s->material = *material;
+  s->adjoint_ready = false;
+  return SPARLAB_OK;
+}
+
+int sparlab_system_vjp_device(sparlab_system* s, const double* u, const double* v, int wrt, double* grad_coords,
+                              double* grad_disp, double* grad_params, void* stream) {
+  if (!s || !u || !v) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (!s->adjoint_ready)
+    return sbk::capi_fail(SPARLAB_E_ARGUMENT, "the adjoint needs the last evaluation (before commit or new inputs)");
+  if (((wrt & SPARLAB_VJP_COORDS) && !grad_coords) || ((wrt & SPARLAB_VJP_DISPLACEMENT) && !grad_disp) ||
+      ((wrt & SPARLAB_VJP_PARAMETERS) && !grad_params))
+    return sbk::capi_fail(SPARLAB_E_ARGUMENT, "a requested gradient has no buffer");
+  const std::size_t ne = static_cast<std::size_t>(s->ne);
+  if ((!s->grad_x && !dalloc(s->grad_x, 24 * ne)) || (!s->grad_u && !dalloc(s->grad_u, 24 * ne)) ||
+      (!s->grad_p && !dalloc(s->grad_p, static_cast<std::size_t>(sbk::kAdjointParams) * ne)))
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "device allocation for the adjoint failed");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  sbk::AdjointArgs q{};
+  sbk::ElementArgs& a = q.e;
+  a = s->args;
+  a.coords = s->coords;
+  a.conn = s->conn;
+  a.u = u;
+  a.ne = s->ne;
+  a.state_in = s->state[s->cur];
+  a.state_stride = s->ne * s->points;
+  a.alpha_in = s->alpha[s->cur];
+  a.alpha_out = s->alpha[1 - s->cur];
+  a.state_out = s->state[1 - s->cur];
+  a.status = s->status;
+  sbk::set_workspace(a, s->ws);
+  q.v = v;
+  q.wrt = wrt;
+  q.hill = sbk::hill_r_derivatives(s->material, q.dP) ? 1 : 0;
+  q.backstresses = s->material.num_backstresses;
+  q.grad_x = s->grad_x;
+  q.grad_u = s->grad_u;
+  q.grad_p = s->grad_p;
+  if (sbk::launch_adjoint(q, st) != 0) return sbk::capi_fail(SPARLAB_E_DEVICE, "adjoint launch failed");
+  if (wrt & SPARLAB_VJP_COORDS) {
+    sbk::ForceArgs f{s->grad_x, s->node_ptr, s->node_incidence, static_cast<long long>(s->nn), grad_coords, s->gross};
+    if (sbk::launch_gather_force(f, st) != 0) return sbk::capi_fail(SPARLAB_E_DEVICE, "gather launch failed");
+  }
+  if (wrt & SPARLAB_VJP_DISPLACEMENT) {
+    sbk::ForceArgs f{s->grad_u, s->node_ptr, s->node_incidence, static_cast<long long>(s->nn), grad_disp, s->gross};
+    if (sbk::launch_gather_force(f, st) != 0) return sbk::capi_fail(SPARLAB_E_DEVICE, "gather launch failed");
+  }
+  if (wrt & SPARLAB_VJP_PARAMETERS) {
+    const int parts = static_cast<int>(std::min<int64_t>(1024, s->ne / 4096 + 1));
+    for (int k = 0; k < sbk::kAdjointParams; ++k)
+      if (sbk::launch_reduce(s->grad_p + static_cast<std::size_t>(k) * ne, s->ne, 0, s->partial, parts,
+                             grad_params + k, st) != 0)
+        return sbk::capi_fail(SPARLAB_E_DEVICE, "reduction launch failed");
+  }
   return SPARLAB_OK;
 }
 

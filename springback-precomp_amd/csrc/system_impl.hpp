@@ -43,9 +43,17 @@ struct sparlab_system {
   long long* summary = nullptr; // failed count, lowest failed element, yielding points
   bool evaluated = false;
   bool last_tangent = false;
+  // the adjoint: per-element products and whether the workspace still holds
+  // the last evaluation (cleared by commit and by new coordinates, material
+  // or history)
+  double *grad_x = nullptr, *grad_u = nullptr, *grad_p = nullptr;
+  bool adjoint_ready = false;
 };
 
 namespace sbk {
+// dP/d(r0, r45, r90) of a Hill48 material calibrated by r-values (false and
+// zeros otherwise).
+bool hill_r_derivatives(const sparlab_material& m, double dP[3][36]);
 // The launches of one evaluation of the system on `stream` (what
 // sparlab_system_evaluate_device queues).
 int system_enqueue(sparlab_system* s, const double* u, int want_tangent, int column_major, double* fint,

@@ -135,6 +135,45 @@ __device__ inline void lu_solve(const LuSlot& s, const double b[9], double x[9])
   for (int i = 0; i < 9; ++i) x[i] = c[i];
 }
 
+// Solve A^T x = b with the factors of A (P A Q = L U, the row
+// transpositions rt applied in order, the column ones ct): x = P^T L^-T U^-T
+// Q^T b (every lane, redundantly).
+__device__ inline void lu_solve_transposed(const LuSlot& s, const double b[9], double x[9]) {
+  double c[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) c[i] = b[i];
+  // Q^T b: the column transpositions in order
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int q = s.ct[k];
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i)
+      if (i == q) { const double t = c[k]; c[k] = c[i]; c[i] = t; }
+  }
+  // U^T y = c (forward)
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+#pragma unroll
+    for (int k = 0; k < i; ++k) c[i] = c[i] - s.a[k * 9 + i] * c[k];
+    c[i] = c[i] / s.a[i * 9 + i];
+  }
+  // L^T z = y (backward, unit diagonal)
+#pragma unroll
+  for (int i = 8; i >= 0; --i)
+#pragma unroll
+    for (int k = i + 1; k < 9; ++k) c[i] = c[i] - s.a[k * 9 + i] * c[k];
+  // P^T z: the row transpositions in reverse order
+#pragma unroll
+  for (int k = 8; k >= 0; --k) {
+    const int r = s.rt[k];
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i)
+      if (i == r) { const double t = c[k]; c[k] = c[i]; c[i] = t; }
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) x[i] = c[i];
+}
+
 // Eigen's unblocked LLT on the lower triangle of w (in place, redundant);
 // false at the first non-positive pivot (the rest of w left as it was).
 __device__ inline bool llt9(double* w) {

@@ -506,6 +506,46 @@ class NonlinearSystem:
         al = None if alpha is None else np.ascontiguousarray(alpha, dtype=np.float64)
         _check(lib().sparlab_system_set_history(self.h, _np_ptr(st), _np_ptr(al)))
 
+    def set_coords(self, coords):
+        """New nodal reference coordinates (host array, num_nodes x 3)."""
+        c = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 3)
+        if c.shape[0] != self.num_nodes:
+            raise ValueError("one coordinate triple per node")
+        lib().sparlab_system_set_coords.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _check(lib().sparlab_system_set_coords(self.h, _np_ptr(c)))
+        self.coords = c
+
+    def set_material(self, material):
+        lib().sparlab_system_set_material.argtypes = [ctypes.c_void_p, ctypes.POINTER(CMaterial)]
+        _check(lib().sparlab_system_set_material(self.h, ctypes.byref(material.c)))
+        self.material = material
+
+    def vjp_device(self, u, v, coords=True, displacement=False, parameters=False, stream=None):
+        """Vector-Jacobian products of the last evaluation's internal force at
+        displacement u for the cotangent v (device tensors, 3 per node):
+        dict with 'coords' (3 nn), 'displacement' (3 nn, = K^T v), 'parameters'
+        (11: r0, r45, r90, C_1, gamma_1, ..., C_4, gamma_4) as requested."""
+        import torch
+        L = lib()
+        L.sparlab_system_vjp_device.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int] + [ctypes.c_void_p] * 4
+        s = stream if stream is not None else torch.cuda.current_stream(u.device)
+        out = {}
+        wrt = 0
+        if coords:
+            out["coords"] = torch.zeros(3 * self.num_nodes, dtype=torch.float64, device=u.device)
+            wrt |= 1
+        if displacement:
+            out["displacement"] = torch.zeros(3 * self.num_nodes, dtype=torch.float64, device=u.device)
+            wrt |= 2
+        if parameters:
+            out["parameters"] = torch.zeros(VJP_PARAMETERS, dtype=torch.float64, device=u.device)
+            wrt |= 4
+        p = lambda name: ctypes.c_void_p(out[name].data_ptr()) if name in out else None
+        _check(L.sparlab_system_vjp_device(self.h, ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(v.data_ptr()), wrt,
+                                           p("coords"), p("displacement"), p("parameters"),
+                                           ctypes.c_void_p(s.cuda_stream)))
+        return out
+
     def element_outputs(self):
         fe = np.zeros((self.num_elements, 24))
         ke = np.zeros((self.num_elements, 24, 24))
@@ -606,6 +646,75 @@ class BatchedSystem:
 
     def commit(self):
         self.system.commit()
+
+
+VJP_PARAMETERS = 11
+PARAMETER_NAMES = ("r0", "r45", "r90", "C1", "gamma1", "C2", "gamma2", "C3", "gamma3", "C4", "gamma4")
+
+
+def material_parameters(material):
+    """The differentiable parameters of a material (PARAMETER_NAMES order)."""
+    c = material.c
+    out = np.zeros(VJP_PARAMETERS)
+    out[0:3] = list(c.hill_r)
+    for i in range(c.num_backstresses):
+        out[3 + 2 * i] = c.backstress_modulus[i]
+        out[4 + 2 * i] = c.backstress_recovery[i]
+    return out
+
+
+def with_parameters(material, params):
+    """A copy of `material` with the differentiable parameters replaced."""
+    m = Material.__new__(Material)
+    m.c = CMaterial()
+    ctypes.pointer(m.c)[0] = material.c
+    for i in range(3):
+        m.c.hill_r[i] = float(params[i])
+    for i in range(m.c.num_backstresses):
+        m.c.backstress_modulus[i] = float(params[3 + 2 * i])
+        m.c.backstress_recovery[i] = float(params[4 + 2 * i])
+    _check(lib().sparlab_material_finalize(ctypes.byref(m.c)))
+    return m
+
+
+class InternalForce:
+    """f_int(X, theta) of a NonlinearSystem at a fixed displacement and
+    committed history as a torch.autograd.Function: forward evaluates the
+    assembled internal force for the nodal reference coordinates X
+    (num_nodes x 3) and the material parameters theta (11, PARAMETER_NAMES);
+    backward returns v^T df/dX and v^T df/dtheta from the adjoint kernel.
+
+    Usage: f = InternalForce.apply(X, theta, system, u) with X, theta float64
+    tensors (requires_grad as needed) and u the displacement (device tensor).
+    """
+
+    @staticmethod
+    def _function():
+        import torch
+
+        class _F(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, X, theta, system, u):
+                system.set_coords(X.detach().cpu().numpy())
+                system.set_material(with_parameters(system.material, theta.detach().cpu().numpy()))
+                fint = torch.zeros(3 * system.num_nodes, dtype=torch.float64, device=u.device)
+                system.evaluate_device(u, fint, None, want_tangent=False)
+                system.result()
+                ctx.system, ctx.u = system, u
+                ctx.devices = (X.device, theta.device)
+                return fint
+
+            @staticmethod
+            def backward(ctx, grad):
+                g = ctx.system.vjp_device(ctx.u, grad.contiguous(), coords=True, parameters=True)
+                gx = g["coords"].reshape(-1, 3).to(ctx.devices[0])
+                return gx, g["parameters"].to(ctx.devices[1]), None, None
+
+        return _F
+
+    @classmethod
+    def apply(cls, X, theta, system, u):
+        return cls._function().apply(X, theta, system, u)
 
 
 def make_tool(shape="sphere", centre=(0, 0, 0), centre0=None, radius=0.0, normal=(0, 0, 1), axis=(0, 0, 1),
