@@ -126,6 +126,54 @@ def batched_leg(G, W, torch, dev, steps):
             "yielding_points": int(sc["yielding"].sum())}
 
 
+def differentiation_leg(G, W, torch, dev):
+    """The differentiation form on the 200 x 250 x 4 sheet mid-way through a
+    spiral incremental-forming path: the forward evaluation and the adjoint
+    (v^T df_int/dX and the Hill48/Chaboche parameter products) for a seeded
+    N(0, 1) cotangent, with a central-difference check of the coordinate
+    product along a random direction (h = 1e-6 of the element size)."""
+    coords, conn, u1, u2 = W.differentiation_workload()
+    m = W.microbench_material()
+    s = G.NonlinearSystem(coords, conn, m, G.Rule(2, 0, 2))
+    ua = torch.as_tensor(u1, device=dev)
+    u = torch.as_tensor(u2, device=dev)
+    f = torch.zeros_like(u)
+    s.evaluate_device(ua, f, None, want_tangent=False)
+    s.result()
+    s.commit()
+    rng = np.random.default_rng(2024)
+    v = torch.as_tensor(rng.normal(size=u.numel()), device=dev)
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    s.evaluate_device(u, f, None, want_tangent=False)
+    ev = s.result()
+    e0.record()
+    s.evaluate_device(u, f, None, want_tangent=False)
+    e1.record()
+    g = s.vjp_device(u, v, coords=True, parameters=True)
+    e2.record()
+    torch.cuda.synchronize()
+    ms_fwd, ms_vjp = e0.elapsed_time(e1), e1.elapsed_time(e2)
+    gx = g["coords"].cpu().numpy()
+    d = rng.normal(size=coords.shape) * (0.020 / 200)
+    h = 1e-6
+    vn = v.cpu().numpy()
+
+    def f_at(c):
+        s.set_coords(c)
+        s.evaluate_device(u, f, None, want_tangent=False)
+        s.result()
+        return f.cpu().numpy()
+
+    fd = vn @ (f_at(coords + h * d) - f_at(coords - h * d)) / (2.0 * h)
+    an = gx @ d.reshape(-1)
+    s.set_coords(coords)
+    return {"workload": "200 x 250 x 4 Hex8 sheet (20 x 25 x 1 mm), mid-way through a 3-turn spiral incremental "
+                        "forming path (committed increment, then the next), Hill48 + 2 Chaboche + Voce, log strain",
+            "elements": int(conn.shape[0]), "yielding_points": int(ev.yielding_points),
+            "forward_ms": ms_fwd, "vjp_ms": ms_vjp,
+            "vjp": "coordinates + 11 material parameters", "fd_direction_rel_error": abs(fd - an) / abs(an)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +183,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=0, help="elements of the CPU leg (default: all)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
@@ -229,6 +278,14 @@ def main():
               f"({batched['speedup']:.2f}x, {batched['elements_per_wavefront']} elements per wavefront)",
               file=sys.stderr, flush=True)
 
+    adjoint = None
+    if not args.no_adjoint:
+        torch.cuda.empty_cache()
+        adjoint = differentiation_leg(G, W, torch, dev)
+        print(f"# differentiation: {adjoint['elements']} elements; forward {adjoint['forward_ms']:.2f} ms, "
+              f"VJP {adjoint['vjp_ms']:.2f} ms; FD check {adjoint['fd_direction_rel_error']:.2e}",
+              file=sys.stderr, flush=True)
+
     line = {
         "metric": METRIC, "value": value, "unit": "element evals/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -237,7 +294,7 @@ def main():
                                "log strain, Hill48 + 2 Chaboche + Voce, f_int + K_t + trial history",
                    "elements": ne, "integration_points": npts, "plastic_fraction": plastic,
                    "mean_local_iterations": mean_iters, "failures": failures},
-        "roofline": roof, "cpu_baseline": cpu, "batched": batched,
+        "roofline": roof, "cpu_baseline": cpu, "batched": batched, "differentiation": adjoint,
     }
     print(json.dumps(line))
 

@@ -172,3 +172,33 @@ def batched_problems(count=64, seed=20241203, min_elements=2000, max_elements=40
         u[:, 2] = step * w
         out.append(dict(coords=coords, conn=conn, u=u, family=family, footprint=footprint, depth=depth))
     return out
+
+
+def differentiation_workload(nx=200, ny=250, nz=4, lx=0.020, ly=0.025, thickness=0.001, depth=0.003,
+                             turns=3.0, progress=0.5, step=0.004):
+    """The differentiation workload: an nx x ny x nz Hex8 sheet (200 x 250 x 4
+    = 200 000 elements over 20 x 25 x 1 mm) mid-way through a single-point
+    incremental forming path, a spiral of `turns` turns from the rim towards
+    the centre. The committed displacement u1 is the shape at `progress` of
+    the path (a cone-like pocket of `depth`, deepening along the spiral, plus
+    the tool's local indentation); u2 is the next increment (`step` further).
+    Returns coords, conn, u1, u2 (node-major, SI units)."""
+    coords, conn = structured_hex_mesh(nx, ny, nz, lx, ly, thickness, -lx / 2, -ly / 2, -thickness)
+    x, y = coords[:, 0], coords[:, 1]
+    radius = 0.45 * min(lx, ly)
+
+    def shape(t):
+        # tool at radius r(t) (rim -> centre), angle 2 pi turns t; pocket depth grows with t
+        rt = radius * (1.0 - 0.8 * t)
+        ang = 2.0 * np.pi * turns * t
+        tx, ty = rt * np.cos(ang), rt * np.sin(ang)
+        r = np.hypot(x, y)
+        pocket = -depth * t * np.clip(1.0 - r / radius, 0.0, 1.0)
+        dent = -0.05 * depth * np.exp(-((x - tx) ** 2 + (y - ty) ** 2) / (0.1 * radius) ** 2)
+        u = np.zeros_like(coords)
+        u[:, 2] = pocket + dent
+        return u
+
+    u1 = shape(progress)
+    u2 = shape(progress + step)
+    return coords, conn, u1.reshape(-1), u2.reshape(-1)

@@ -138,3 +138,36 @@ def test_autograd_function(device):
     ref = s.vjp_device(u, v, coords=True, parameters=True)
     assert np.array_equal(X.grad.numpy().reshape(-1), ref["coords"].cpu().numpy())
     assert np.array_equal(theta.grad.numpy(), ref["parameters"].cpu().numpy())
+
+
+def test_spiral_forming_state_matches_central_differences(device):
+    """The differentiation workload's state (spiral incremental forming, mid
+    path) at 20 x 25 x 4: coordinate and parameter products against central
+    differences, h = 1e-6 (of the element size / of each parameter)."""
+    import torch
+    m = W.microbench_material()
+    coords, conn, u1, u2 = W.differentiation_workload(nx=20, ny=25, nz=4)
+    s = G.NonlinearSystem(coords, conn, m, G.Rule(2, 0, 2))
+    s.evaluate(u1)
+    s.commit()
+    ev = s.evaluate(u2)
+    assert ev.yielding_points > 0
+    u = torch.as_tensor(u2, device=device)
+    rng = np.random.default_rng(12)
+    v = rng.normal(size=3 * s.num_nodes)
+    g = s.vjp_device(u, torch.as_tensor(v, device=device), coords=True, parameters=True)
+    gx, gp = g["coords"].cpu().numpy(), g["parameters"].cpu().numpy()
+    d = rng.normal(size=coords.shape) * (0.020 / 20)
+    h = 1e-6
+    fd = v @ (fint_at(s, u, coords + h * d) - fint_at(s, u, coords - h * d)) / (2.0 * h)
+    s.set_coords(coords)
+    assert abs(fd - gx @ d.reshape(-1)) <= 1e-6 * abs(gx @ d.reshape(-1))
+    theta = G.material_parameters(m)
+    dt = rng.normal(size=7) * theta[:7]
+    tp, tm = theta.copy(), theta.copy()
+    tp[:7] += h * dt
+    tm[:7] -= h * dt
+    fd = v @ (fint_at(s, u, material=G.with_parameters(m, tp)) - fint_at(s, u, material=G.with_parameters(m, tm))) / (2.0 * h)
+    s.set_material(m)
+    an = gp[:7] @ dt
+    assert abs(fd - an) <= 1e-6 * abs(an)
