@@ -174,6 +174,92 @@ def differentiation_leg(G, W, torch, dev):
             "vjp": "coordinates + 11 material parameters", "fd_direction_rel_error": abs(fd - an) / abs(an)}
 
 
+def assembly_leg(G, W, torch, dev, reps):
+    """The assembly + contact workload: a 100 x 100 x 4 Hex8 plate (51 005
+    nodes, 153 015 DOFs, nnz 10 600 317) with random FP64 element forces and
+    non-symmetric 24 x 24 tangents (seeded), assembled into the reference's
+    column-major values; 10 000 top-surface slave nodes against a sphere
+    R = 5 mm at a seeded offset pressed 0.2 mm into the plate, Coulomb
+    friction 0.1 with the exact tangent, kappa A = 1e7 N/m per node; the
+    active nodes' 3 x 3 blocks added into the assembled values.
+
+    The CSR assembly is HBM-bound. Its algorithmic bytes: every element
+    matrix read once (4 608 B), one 4-byte contribution code per element
+    block (64 per element, the reference's block_offset table), every value
+    written once (8 B x nnz)."""
+    w = W.assembly_workload()
+    coords, conn, fe, ke = w["coords"], w["conn"], w["fe"], w["ke"]
+    nn, ne = coords.shape[0], conn.shape[0]
+    sysm = G.NonlinearSystem(coords, conn, G.Material(70e9, 0.33), G.Rule(), G.FINITE_LOGARITHMIC)
+    t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    d_fe, d_ke = t(fe), t(ke)
+    del fe, ke
+    values = torch.zeros(sysm.nnz, dtype=torch.float64, device=dev)
+    fint = torch.zeros(3 * nn, dtype=torch.float64, device=dev)
+    gross = torch.zeros(1, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    rng = np.random.default_rng(77)
+    radius = 0.005
+    cx, cy = rng.uniform(-0.03, 0.03, 2)
+    tool = G.make_tool("sphere", (cx, cy, radius - 2e-4), (cx - 1e-4, cy, radius - 2e-4), radius=radius,
+                       friction=0.1)
+    nodes = w["contact_nodes"]
+    d_nodes = t(nodes, torch.int32)
+    d_coords, d_u, d_start = t(coords.ravel()), torch.zeros(3 * nn, dtype=torch.float64, device=dev), \
+        torch.zeros(3 * nn, dtype=torch.float64, device=dev)
+    d_kappa = torch.full((nodes.size,), 1e7, dtype=torch.float64, device=dev)
+    bufs = G.contact_buffers(nodes.size, dev)
+    residual = torch.zeros(3 * nn, dtype=torch.float64, device=dev)
+    first = t(sysm.diagonal_blocks()[nodes], torch.int32)
+    row_ptr, _ = sysm.pattern()
+    row_len = t((row_ptr[3 * nodes + 1] - row_ptr[3 * nodes]).astype(np.int32), torch.int32)
+
+    def contact():
+        return G.contact_evaluate_device(tool, d_coords, d_u, d_start, d_nodes, d_kappa, residual=residual,
+                                         stream=stream, buffers=bufs, read_scalars=False)
+
+    def blocks():
+        G.add_node_blocks_device(bufs["blocks"], bufs["active"], first, row_len, values, column_major=True,
+                                 stream=stream)
+
+    stages = (("values", lambda: sysm.assemble_device(ke=d_ke, values=values, column_major=True, stream=stream)),
+              ("force", lambda: sysm.assemble_device(fe=d_fe, fint=fint, gross=gross, stream=stream)),
+              ("contact", contact), ("node_blocks", blocks))
+    us = {}
+    for name, fn in stages:
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        us[name] = e0.elapsed_time(e1) * 1e3 / reps
+    n_active = int(bufs["active"].sum().item())
+    alg = ne * (4608 + 64 * 4) + 8 * sysm.nnz
+    gbs = alg / (us["values"] * 1e-6) / 1e9
+    return {"workload": "100 x 100 x 4 Hex8 plate, random non-symmetric element matrices, column-major values; "
+                        "10 000 slave nodes vs a sphere R = 5 mm, friction 0.1, exact tangent",
+            "elements": int(ne), "nodes": int(nn), "nnz": int(sysm.nnz), "reps": reps,
+            "values_us": us["values"], "values_bytes": alg, "values_GBps": gbs,
+            "values_hbm_frac": gbs / HBM_PEAK_GBS, "force_us": us["force"], "contact_us": us["contact"],
+            "node_blocks_us": us["node_blocks"], "contact_active_nodes": n_active}
+
+
+def copy_probe(torch, mib):
+    """A plain device copy of `mib` MiB, five times: the known-size kernel
+    whose FETCH_SIZE / WRITE_SIZE readings scale the counters of the others."""
+    dev = torch.device("cuda:0")
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"# copy probe: {mib} MiB read + {mib} MiB written per copy, 5 copies", file=sys.stderr)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +270,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation leg")
+    ap.add_argument("--no-assembly", action="store_true", help="skip the assembly + contact leg")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain MIB-MiB device copy (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
@@ -191,6 +280,9 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_probe > 0:
+        copy_probe(torch, args.copy_probe)
+        return
     import sparlab_gpu as G
     from sparlab_gpu import workloads as W
 
@@ -286,6 +378,15 @@ def main():
               f"VJP {adjoint['vjp_ms']:.2f} ms; FD check {adjoint['fd_direction_rel_error']:.2e}",
               file=sys.stderr, flush=True)
 
+    assembly = None
+    if not args.no_assembly:
+        torch.cuda.empty_cache()
+        assembly = assembly_leg(G, W, torch, dev, 20)
+        print(f"# assembly: {assembly['nnz']} values in {assembly['values_us']:.1f} us "
+              f"({assembly['values_GBps']:.0f} GB/s algorithmic), force gather {assembly['force_us']:.1f} us, "
+              f"contact {assembly['contact_us']:.1f} us ({assembly['contact_active_nodes']} active), node blocks "
+              f"{assembly['node_blocks_us']:.1f} us", file=sys.stderr, flush=True)
+
     line = {
         "metric": METRIC, "value": value, "unit": "element evals/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -295,6 +396,7 @@ def main():
                    "elements": ne, "integration_points": npts, "plastic_fraction": plastic,
                    "mean_local_iterations": mean_iters, "failures": failures},
         "roofline": roof, "cpu_baseline": cpu, "batched": batched, "differentiation": adjoint,
+        "assembly": assembly,
     }
     print(json.dumps(line))
 

@@ -369,6 +369,137 @@ __device__ __forceinline__ double log_divided2(const LogData& L, int a, int b, i
   return sum / (y * y);
 }
 
+// The stress conjugate to E (S = P^T T, logarithmic_stress) and the tangent
+// between Green-Lagrange strains, mat = P^T C P + T:L (IncompatibleModes.cpp:
+// 238-243, LogarithmicStrain.cpp:144-197), formed in the principal frame: with
+// M the principal rotation of tensorial Voigt components, D = diag(sd) the
+// first divided differences (engineering) and Ms = M diag(1,1,1,1/2,1/2,1/2),
+// P = M^T D Ms and T:L = Ms^T H Ms, so
+//   mat = Ms^T [D (M C M^T) D + H] Ms,
+// with H the principal-frame curvature. Four 6 x 6 products instead of seven,
+// and at most three 6 x 6 arrays live at a time.
+__device__ __forceinline__ void log_material(const LogData& L, const double t[6], const double C[6][6], double S[6],
+                                             double mat[6][6]) {
+  double tt[3][3];
+  tt[0][0] = t[0]; tt[1][1] = t[1]; tt[2][2] = t[2];
+  tt[0][1] = tt[1][0] = t[3];
+  tt[1][2] = tt[2][1] = t[4];
+  tt[2][0] = tt[0][2] = t[5];
+  double qt[3][3], tp[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) qt[i][j] = L.q[0][i] * tt[0][j] + L.q[1][i] * tt[1][j] + L.q[2][i] * tt[2][j];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) tp[i][j] = qt[i][0] * L.q[0][j] + qt[i][1] * L.q[1][j] + qt[i][2] * L.q[2][j];
+  {
+    // S = Q (tp o first) Q^T
+    double pr[3][3], qp[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) pr[i][j] = tp[i][j] * L.first[i][j];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) qp[i][j] = L.q[i][0] * pr[0][j] + L.q[i][1] * pr[1][j] + L.q[i][2] * pr[2][j];
+    const int si[6] = {0, 1, 2, 0, 1, 2}, sj[6] = {0, 1, 2, 1, 2, 0};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int i = si[k], j = sj[k];
+      S[k] = qp[i][0] * L.q[j][0] + qp[i][1] * L.q[j][1] + qp[i][2] * L.q[j][2];
+    }
+  }
+  double m[6][6];
+  principal_rotation(L.q, m);
+  // X = M C M^T
+  double x[6][6];
+  {
+    double y[6][6];
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int l = 0; l < 6; ++l) {
+        double s = m[a][0] * C[0][l];
+#pragma unroll
+        for (int k = 1; k < 6; ++k) s = s + m[a][k] * C[k][l];
+        y[a][l] = s;
+      }
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int b = 0; b < 6; ++b) {
+        double s = y[a][0] * m[b][0];
+#pragma unroll
+        for (int l = 1; l < 6; ++l) s = s + y[a][l] * m[b][l];
+        x[a][b] = s;
+      }
+  }
+  // X <- D X D + H; H(k, l) = 2 (half(k, l) + half(l, k)), half(x, y) = the sum
+  // over the principal unit tensors, sum_{(a,c) in X, (c,b) in Y} tp(a,b) l[a,b,c]
+  // with l the second divided differences (ten distinct values, a <= b <= c)
+  double l2[3][3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = a; b < 3; ++b)
+#pragma unroll
+      for (int c = b; c < 3; ++c) {
+        const double v = log_divided2(L, a, b, c);
+        l2[a][b][c] = l2[a][c][b] = l2[b][a][c] = l2[b][c][a] = l2[c][a][b] = l2[c][b][a] = v;
+      }
+  double sd[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) sd[k] = (k < 3 ? 1.0 : 2.0) * L.first[vrow(k)][vcol(k)];
+#pragma unroll
+  for (int kx = 0; kx < 6; ++kx)
+#pragma unroll
+    for (int ky = 0; ky < 6; ++ky) {
+      auto half = [&](int u, int v) {
+        const int xr[2] = {vrow(u), vcol(u)}, xc[2] = {vcol(u), vrow(u)};
+        const int yr[2] = {vrow(v), vcol(v)}, yc[2] = {vcol(v), vrow(v)};
+        const int nx = u < 3 ? 1 : 2, ny = v < 3 ? 1 : 2;
+        double sum = 0.0;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          if (s >= nx) continue;
+#pragma unroll
+          for (int r = 0; r < 2; ++r) {
+            if (r >= ny) continue;
+            if (yr[r] != xc[s]) continue;
+            sum = sum + tp[xr[s]][yc[r]] * l2[xr[s]][yc[r]][xc[s]];
+          }
+        }
+        return sum;
+      };
+      const double h = 2.0 * (half(kx, ky) + half(ky, kx));
+      x[kx][ky] = sd[kx] * x[kx][ky] * sd[ky] + h;
+    }
+  // mat = Ms^T X Ms
+  double z[6][6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+      double s = m[0][i] * x[0][b];
+#pragma unroll
+      for (int a = 1; a < 6; ++a) s = s + m[a][i] * x[a][b];
+      z[i][b] = s;
+    }
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double s = z[i][0] * m[0][j];
+#pragma unroll
+      for (int b = 1; b < 6; ++b) s = s + z[i][b] * m[b][j];
+      const double f = (i < 3 ? 1.0 : 0.5) * (j < 3 ? 1.0 : 0.5);
+      mat[i][j] = f * s;
+    }
+}
+
 // S = P^T T (logarithmic_stress) and T:L (logarithmic_curvature) in one go.
 __device__ __forceinline__ void log_push(const LogData& L, const double t[6], double S[6], double curv[6][6]) {
   double tt[3][3];

@@ -13,13 +13,16 @@
 
 namespace sbk {
 
-// History of one point, in registers.
-struct PointState {
+// History of one point, in registers; NT 6-vectors of kinematic history
+// (kernels are instantiated for the smallest NT that holds the material's).
+template <int NT>
+struct PointStateT {
   double ep[6];
-  double kin[kMaxTerms][6];  // radial return: kin[0] = beta; general: alpha_i
+  double kin[NT][6];  // radial return: kin[0] = beta; general: alpha_i
   double abar;
   bool loading;
 };
+using PointState = PointStateT<kMaxTerms>;
 
 struct ReturnOut {
   double stress[6];
@@ -67,7 +70,8 @@ __device__ __forceinline__ void elastic_C(double k, double g, double C[6][6]) {
 // ---------------------------------------------------------------------------
 // J2 radial return (Plasticity.cpp:66-171); the state is updated in place.
 // ---------------------------------------------------------------------------
-__device__ int return_radial(const DevMaterial& m, const double strain[6], PointState& st, ReturnOut& out) {
+template <int NT>
+__device__ int return_radial(const DevMaterial& m, const double strain[6], PointStateT<NT>& st, ReturnOut& out) {
   const double g = m.shear, k = m.bulk, hk = m.hkin;
   const double sqrt23 = 0.816496580927726;  // sqrt(2/3), rounded as std::sqrt(2.0 / 3.0)
   double el[6], s_tr[6], xi[6];
@@ -203,7 +207,8 @@ __device__ __forceinline__ void p_times(const DevMaterial& m, const double x[6],
   }
 }
 
-__device__ __forceinline__ bool gen_evaluate(const DevMaterial& mat, const PointState& c, const double s_tr[6],
+template <int NT>
+__device__ __forceinline__ bool gen_evaluate(const DevMaterial& mat, const PointStateT<NT>& c, const double s_tr[6],
                                              const double xi[6], double dl, GenPoint& e) {
   double px[6];
   p_times(mat, xi, px);
@@ -220,7 +225,7 @@ __device__ __forceinline__ bool gen_evaluate(const DevMaterial& mat, const Point
   e.b = 2.0 * mat.shear * dl;
   e.db = 2.0 * mat.shear;
 #pragma unroll
-  for (int t = 0; t < kMaxTerms; ++t) {
+  for (int t = 0; t < NT; ++t) {
     if (t >= mat.terms) break;
     double th, chi, dth, dchi;
     decay(mat, t, dl, th, chi, dth, dchi);
@@ -340,7 +345,8 @@ __device__ __forceinline__ double maxabs7(const double r[7]) {
 
 // C_alg = C^e - 2G [m_t a^T + (dl / sigma_bar) S^-1 (P - m m^T) A],
 // [A; a^T] = J^-1 [D_dev; 0] (Plasticity.cpp:302-311), one column at a time.
-__device__ __forceinline__ void gen_tangent(const DevMaterial& mat, const PointState& c, const GenPoint& e,
+template <int NT>
+__device__ __forceinline__ void gen_tangent(const DevMaterial& mat, const PointStateT<NT>& c, const GenPoint& e,
                                             double dl, double C[6][6]) {
   const double g = mat.shear;
   JacSolver J;
@@ -383,10 +389,11 @@ __device__ __forceinline__ void gen_tangent(const DevMaterial& mat, const PointS
   }
 }
 
-__device__ __forceinline__ double kinematic_energy(const DevMaterial& mat, const PointState& st) {
+template <int NT>
+__device__ __forceinline__ double kinematic_energy(const DevMaterial& mat, const PointStateT<NT>& st) {
   double out = 0.0;
 #pragma unroll
-  for (int t = 0; t < kMaxTerms; ++t) {
+  for (int t = 0; t < NT; ++t) {
     if (t >= mat.terms) break;
     const double nrm = tensor_norm(st.kin[t]);
     out = out + 0.75 * nrm * nrm / mat.mod[t];
@@ -400,7 +407,8 @@ struct GenTrial {
   double vol, sy_n, eq_tr, f_tr;
 };
 
-__device__ __forceinline__ void gen_trial(const DevMaterial& mat, const double strain[6], const PointState& st,
+template <int NT>
+__device__ __forceinline__ void gen_trial(const DevMaterial& mat, const double strain[6], const PointStateT<NT>& st,
                                           GenTrial& tr) {
   const double g = mat.shear;
 #pragma unroll
@@ -414,7 +422,7 @@ __device__ __forceinline__ void gen_trial(const DevMaterial& mat, const double s
 #pragma unroll
   for (int i = 0; i < 6; ++i) tr.xi_tr[i] = tr.s_tr[i];
 #pragma unroll
-  for (int t = 0; t < kMaxTerms; ++t) {
+  for (int t = 0; t < NT; ++t) {
     if (t >= mat.terms) break;
 #pragma unroll
     for (int i = 0; i < 6; ++i) tr.xi_tr[i] = tr.xi_tr[i] - st.kin[t][i];
@@ -432,7 +440,8 @@ __device__ __forceinline__ void gen_trial(const DevMaterial& mat, const double s
 // The closest-point Newton from the trial (Plasticity.cpp:365-448): xi and
 // dl converge with e the evaluation at the solution; false when the
 // iteration does not converge.
-__device__ __forceinline__ bool gen_newton(const DevMaterial& mat, const PointState& st, const GenTrial& trial,
+template <int NT>
+__device__ __forceinline__ bool gen_newton(const DevMaterial& mat, const PointStateT<NT>& st, const GenTrial& trial,
                                            double xi[6], double& dl, GenPoint& e) {
 #pragma unroll
   for (int i = 0; i < 6; ++i) xi[i] = trial.xi_tr[i];
@@ -513,7 +522,8 @@ __device__ __forceinline__ bool gen_newton(const DevMaterial& mat, const PointSt
 
 // return_general_3d; the state is updated in place (want_tangent = true, as
 // IncompatibleModes.cpp:232 asks for it on every pass).
-__device__ int return_general(const DevMaterial& mat, const double strain[6], PointState& st, ReturnOut& out) {
+template <int NT>
+__device__ int return_general(const DevMaterial& mat, const double strain[6], PointStateT<NT>& st, ReturnOut& out) {
   const double g = mat.shear, k = mat.bulk;
   GenTrial trial;
   gen_trial(mat, strain, st, trial);
@@ -551,7 +561,7 @@ __device__ int return_general(const DevMaterial& mat, const double strain[6], Po
 #pragma unroll
   for (int i = 0; i < 6; ++i) st.ep[i] = st.ep[i] + dl * e.m[i];
 #pragma unroll
-  for (int t = 0; t < kMaxTerms; ++t) {
+  for (int t = 0; t < NT; ++t) {
     if (t >= mat.terms) break;
     double th, chi, dth, dchi;
     decay(mat, t, dl, th, chi, dth, dchi);

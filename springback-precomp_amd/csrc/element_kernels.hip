@@ -88,16 +88,17 @@ __device__ __forceinline__ void mode_operator(const double F[3][3], const double
 // The team's sums of a pass from each lane's point response (w = 0 and zero
 // response on lanes without a point); the K_aa totals go to kaa (row-major).
 template <int KIN>
-__device__ __forceinline__ void team_sums(const PointResponse& r, const double gm[3][3], double w, int p,
-                                          double* kaa, LocalSums& out, bool want_kaa_only) {
+__device__ __forceinline__ void team_sums(const double F[3][3], const double h[3][3], const PointResponse& r,
+                                          const double gm[3][3], double w, int p, double* kaa, LocalSums& out,
+                                          bool want_kaa_only) {
   double ba[6][9];
-  mode_operator(r.F, gm, ba);
+  mode_operator(F, gm, ba);
   if (!want_kaa_only) {
     double hmax = 0.0, mmax = 0.0;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) hmax = fmax(hmax, fabs(r.h[i][j]));
+      for (int j = 0; j < 3; ++j) hmax = fmax(hmax, fabs(h[i][j]));
 #pragma unroll
     for (int i = 0; i < 6; ++i)
 #pragma unroll
@@ -131,17 +132,8 @@ __device__ __forceinline__ void team_sums(const PointResponse& r, const double g
       out.ra[n] = team_sum(w * sr);
     }
   }
-  // K_aa = sum_q w B_a^T mat B_a (+ the geometric alpha-alpha block)
-  double y[6][9];
-#pragma unroll
-  for (int c = 0; c < 6; ++c)
-#pragma unroll
-    for (int n = 0; n < 9; ++n) {
-      double s = r.mat[c][0] * ba[0][n];
-#pragma unroll
-      for (int d = 1; d < 6; ++d) s = s + r.mat[c][d] * ba[d][n];
-      y[c][n] = s;
-    }
+  // K_aa = sum_q w B_a^T mat B_a (+ the geometric alpha-alpha block), a row
+  // at a time: t = B_a[:, mm]^T mat, then K_aa[mm][n] = t B_a[:, n]
   double geo[3][3];
   if constexpr (KIN != kSmall) {
     // gt^T S gt with gt = G~ (3 x 3)
@@ -157,12 +149,20 @@ __device__ __forceinline__ void team_sums(const PointResponse& r, const double g
       for (int b = 0; b < 3; ++b) geo[a][b] = gs[a][0] * gm[0][b] + gs[a][1] * gm[1][b] + gs[a][2] * gm[2][b];
   }
 #pragma unroll
-  for (int mm = 0; mm < 9; ++mm)
+  for (int mm = 0; mm < 9; ++mm) {
+    double t[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      double s = ba[0][mm] * r.mat[0][d];
+#pragma unroll
+      for (int c = 1; c < 6; ++c) s = s + ba[c][mm] * r.mat[c][d];
+      t[d] = s;
+    }
 #pragma unroll
     for (int n = 0; n < 9; ++n) {
-      double s = ba[0][mm] * y[0][n];
+      double s = t[0] * ba[0][n];
 #pragma unroll
-      for (int c = 1; c < 6; ++c) s = s + ba[c][mm] * y[c][n];
+      for (int d = 1; d < 6; ++d) s = s + t[d] * ba[d][n];
       double v = w * s;
       if constexpr (KIN != kSmall) {
         if (mm % 3 == n % 3) v = v + w * geo[mm / 3][n / 3];
@@ -170,6 +170,7 @@ __device__ __forceinline__ void team_sums(const PointResponse& r, const double g
       const double tot = team_sum(v);
       if (((mm * 9 + n) & 7) == p) kaa[mm * 9 + n] = tot;
     }
+  }
   team_sync();
 }
 
@@ -180,10 +181,6 @@ __device__ __forceinline__ void zero_response(PointResponse& r) {
 #pragma unroll
     for (int d = 0; d < 6; ++d) r.mat[c][d] = 0.0;
   }
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) { r.h[i][j] = 0.0; r.F[i][j] = 0.0; }
   r.energy = 0.0;
   r.yielding = false;
 }
@@ -192,29 +189,49 @@ __device__ __forceinline__ void zero_response(PointResponse& r) {
 // before any return failure, as the reference evaluates every point's
 // kinematics first). Writes the trial states of a successful pass, and the
 // workspace, energy and yielding of a pass that meets the stopping test.
-template <int KIN>
-__device__ __forceinline__ int team_local_pass(const ElementArgs& A, long long e, int p, const ModeGeom& mg,
-                                               const double alpha[9], double* kaa, LocalSums& out) {
+// trial(alpha) fills the pass's mode parameters (from the team's LDS machine);
+// it is called again after the return rather than holding them across it.
+template <int KIN, int NT, class Trial>
+__device__ __forceinline__ int team_local_pass(const ElementArgs& A, long long e, int p, LaneLds geom, LaneLds logs,
+                                               Trial trial, double* kaa, LocalSums& out) {
   const DevMaterial& m = A.mat;
   const bool has_point = p < A.rule.n;
   const long long q = e * A.rule.n + p;
   PointResponse r;
   int fail = 0;
   if (has_point) {
-    PointState st;
+    PointStateT<NT> st;
     load_state(m, A.state_in, A.state_stride, q, st);
-    fail = point_response<KIN>(m, mg, alpha, st, r);
-    if (fail == 0) store_state(m, A.state_out, A.state_stride, q, st);
+    double alpha[9];
+    trial(alpha);
+    fail = point_response<KIN>(m, geom, alpha, st, r, A.state_out, A.state_stride, q, logs);
   } else {
     zero_response(r);
   }
+  reload_barrier();
   const int stage = (fail == kReturnFailed || fail == kNonFiniteTrial) ? 1 : 0;
   const int first = team_imin(fail != 0 ? (stage << 12) | (p << 8) | fail : 1 << 20);
   if (first != (1 << 20)) return first & 0xFF;
-  const double w = has_point ? mg.w : 0.0;
+  // F and h again from the LDS data and the machine's alpha, rather than held
+  // in registers across the return
+  double F[3][3], h[3][3], gm[3][3];
+  double w = 0.0;
+  if (has_point) {
+    double alpha[9], hu[3][3];
+    trial(alpha);
+    get_hu(geom, hu);
+    get_gm(geom, gm);
+    mode_kinematics<KIN>(hu, gm, alpha, h, F);
+    w = geom.get(18);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) { F[i][j] = 0.0; h[i][j] = 0.0; gm[i][j] = 0.0; }
+  }
   const double energy = team_sum(w * r.energy);
   const int yielding = team_isum(r.yielding ? 1 : 0);
-  team_sums<KIN>(r, mg.gm, w, p, kaa, out, false);
+  team_sums<KIN>(F, h, r, gm, w, p, kaa, out, false);
   if (out.stop) {
     if (has_point) {
       double* rec = A.point_ws + q * kPointRecord;
@@ -615,10 +632,14 @@ __device__ int machine_step(Machine& M, TeamSlot& T, bool ok, const LocalSums& c
   return kFailed;
 }
 
-template <int KIN>
+template <int KIN, int NT>
 __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
   __shared__ TeamSlot slots[kWaveTeams];
+  __shared__ double geom_lds[mode_geom_fields<KIN>()][64];
+  __shared__ double log_lds[KIN == kLog ? kLogStash : 1][64];
   const int lane = threadIdx.x;
+  const LaneLds geom{&geom_lds[0][lane]};
+  const LaneLds logs{&log_lds[0][lane]};
   const int t = lane >> 3;
   const int p = lane & 7;
   TeamSlot& T = slots[t];
@@ -630,7 +651,6 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
   long long e = -1;
   if (p == 0) M.phase = kIdle;
   team_sync();
-  ModeGeom mg;
   for (;;) {
     // Hand the chunk's next elements to the idle teams, in team order.
     const bool idle = M.phase == kIdle;
@@ -657,7 +677,9 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
       PointGeom pg;
       if (p < A.rule.n) {
         st = setup_point(x0, ue, A.rule.xi[p], A.rule.eta[p], A.rule.zeta[p], A.rule.w[p], pg, fv);
+        ModeGeom mg;
         mode_geom<KIN>(pg, ue, mg);
+        put_mode_geom<KIN>(mg, geom);
       }
       // the first failing point in point order, as the reference's set-up loop
       const int code = team_imin(st != kOk ? (p << 8) | st : 1 << 20);
@@ -675,16 +697,17 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
       team_sync();
     }
     if (!idle || fresh) {
-      double trial[9];
-      if (M.phase == kFirst) {
+      auto trial = [&M](double a[9]) {
+        if (M.phase == kFirst) {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) trial[i] = M.alpha[i];
-      } else {
+          for (int i = 0; i < 9; ++i) a[i] = M.alpha[i];
+        } else {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) trial[i] = M.alpha[i] + M.s * M.d[i];
-      }
+          for (int i = 0; i < 9; ++i) a[i] = M.alpha[i] + M.s * M.d[i];
+        }
+      };
       LocalSums cur;
-      const int fail = team_local_pass<KIN>(A, e, p, mg, trial, T.lu.a, cur);
+      const int fail = team_local_pass<KIN, NT>(A, e, p, geom, logs, trial, T.lu.a, cur);
       const bool ok = fail == 0;
       int status = ok ? kOk : fail;
       double value = 0.0;
@@ -949,6 +972,14 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+template <int KIN>
+void launch_local_solve_kin(const ElementArgs& a, bool two_terms, unsigned int blocks, hipStream_t stream) {
+  if (two_terms)
+    hipLaunchKernelGGL((local_solve_kernel<KIN, 2>), dim3(blocks), dim3(64), 0, stream, a);
+  else
+    hipLaunchKernelGGL((local_solve_kernel<KIN, kMaxTerms>), dim3(blocks), dim3(64), 0, stream, a);
+}
+
 int launch_local_solve(const ElementArgs& args, hipStream_t stream) {
   (void)hipGetLastError();
   ElementArgs a = args;
@@ -956,10 +987,14 @@ int launch_local_solve(const ElementArgs& args, hipStream_t stream) {
   if (a.chunk % kWaveTeams != 0) return static_cast<int>(hipErrorInvalidValue);
   const unsigned int blocks = static_cast<unsigned int>((a.ne + a.chunk - 1) / a.chunk);
   if (blocks == 0) return 0;
+  // the smallest instantiation that holds the material's kinematic history
+  const int fields = a.mat.terms > a.mat.kin_fields ? a.mat.terms : a.mat.kin_fields;
+  if (fields > kMaxTerms) return static_cast<int>(hipErrorInvalidValue);
+  const bool two = fields <= 2;
   switch (a.kinematics) {
-    case kSmall: hipLaunchKernelGGL(local_solve_kernel<kSmall>, dim3(blocks), dim3(64), 0, stream, a); break;
-    case kFinite: hipLaunchKernelGGL(local_solve_kernel<kFinite>, dim3(blocks), dim3(64), 0, stream, a); break;
-    default: hipLaunchKernelGGL(local_solve_kernel<kLog>, dim3(blocks), dim3(64), 0, stream, a); break;
+    case kSmall: launch_local_solve_kin<kSmall>(a, two, blocks, stream); break;
+    case kFinite: launch_local_solve_kin<kFinite>(a, two, blocks, stream); break;
+    default: launch_local_solve_kin<kLog>(a, two, blocks, stream); break;
   }
   return static_cast<int>(hipGetLastError());
 }

@@ -113,12 +113,13 @@ __device__ __forceinline__ int setup_point(const double x0[8][3], const double u
   return kOk;
 }
 
+template <int NT>
 __device__ __forceinline__ void load_state(const DevMaterial& m, const double* __restrict__ state, long long stride,
-                                           long long q, PointState& st) {
+                                           long long q, PointStateT<NT>& st) {
 #pragma unroll
   for (int i = 0; i < 6; ++i) st.ep[i] = state[i * stride + q];
 #pragma unroll
-  for (int t = 0; t < kMaxTerms; ++t) {
+  for (int t = 0; t < NT; ++t) {
 #pragma unroll
     for (int i = 0; i < 6; ++i) st.kin[t][i] = (t < m.kin_fields) ? state[(6 + 6 * t + i) * stride + q] : 0.0;
   }
@@ -126,12 +127,13 @@ __device__ __forceinline__ void load_state(const DevMaterial& m, const double* _
   st.loading = state[(7 + 6 * m.kin_fields) * stride + q] != 0.0;
 }
 
+template <int NT>
 __device__ __forceinline__ void store_state(const DevMaterial& m, double* __restrict__ state, long long stride,
-                                            long long q, const PointState& st) {
+                                            long long q, const PointStateT<NT>& st) {
 #pragma unroll
   for (int i = 0; i < 6; ++i) state[i * stride + q] = st.ep[i];
 #pragma unroll
-  for (int t = 0; t < kMaxTerms; ++t) {
+  for (int t = 0; t < NT; ++t) {
     if (t < m.kin_fields) {
 #pragma unroll
       for (int i = 0; i < 6; ++i) state[(6 + 6 * t + i) * stride + q] = st.kin[t][i];
@@ -194,9 +196,46 @@ __device__ __forceinline__ void mode_geom(const PointGeom& pg, const double ue[2
   }
 }
 
+// A lane's slice of a per-workgroup LDS array laid out [field][64 lanes], so
+// that the lanes of a wavefront touch consecutive doubles (no bank conflicts).
+// The kernels keep long-lived per-point data here instead of in registers.
+struct LaneLds {
+  double* p;  // &array[0][lane]
+  __device__ __forceinline__ double get(int k) const { return p[k * 64]; }
+  __device__ __forceinline__ void set(int k, double v) const { p[k * 64] = v; }
+};
+
+// Forces the values kept in LDS to be read again after this point instead of
+// being held in registers across it (an empty asm with a memory clobber; no
+// instruction is emitted).
+__device__ __forceinline__ void reload_barrier() { __asm__ volatile("" ::: "memory"); }
+
+// ModeGeom in LDS: gm (9), hu (9), w, and for small strain eps_u (6).
+template <int KIN>
+constexpr int mode_geom_fields() { return KIN == kSmall ? 25 : 19; }
+
+template <int KIN>
+__device__ __forceinline__ void put_mode_geom(const ModeGeom& mg, LaneLds s) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) { s.set(i, mg.gm[i / 3][i % 3]); s.set(9 + i, mg.hu[i / 3][i % 3]); }
+  s.set(18, mg.w);
+  if constexpr (KIN == kSmall) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s.set(19 + c, mg.eps_u[c]);
+  }
+}
+__device__ __forceinline__ void get_gm(LaneLds s, double gm[3][3]) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) gm[i / 3][i % 3] = s.get(i);
+}
+__device__ __forceinline__ void get_hu(LaneLds s, double hu[3][3]) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) hu[i / 3][i % 3] = s.get(9 + i);
+}
+
 // H = hu + A G~^T (IncompatibleModes.cpp:195-205); F = I + H except for small
-// strain (F = I). Every kernel forms F with this function, so a pass and the
-// condensation at the same alpha see the same bits.
+// strain (F = I). Every kernel forms F with this function (the compiler may
+// fuse its products differently in each, so copies agree to rounding).
 template <int KIN>
 __device__ __forceinline__ void mode_kinematics(const double hu[3][3], const double gm[3][3], const double alpha[9],
                                                 double h[3][3], double F[3][3]) {
@@ -214,86 +253,96 @@ __device__ __forceinline__ void mode_kinematics(const double hu[3][3], const dou
     }
 }
 
-// The response of the lane's point at alpha: F (I for small strain), the
-// stress conjugate to B_NL, the tangent between B_NL's, the point's new state,
-// energy density and yielding. Returns 0 or a failure code.
+// The response of the lane's point at alpha: the stress conjugate to B_NL,
+// the tangent between B_NL's, energy density and yielding; the point's new
+// state is written to state_out. Returns 0 or a failure code. The point's
+// ModeGeom is read from LDS (geom); for logarithmic strain the principal axes
+// and divided differences wait in LDS (logs) while the return runs.
 struct PointResponse {
-  double F[3][3];
-  double h[3][3];
   double S[6];
   double mat[6][6];
   double energy;
   bool yielding;
 };
 
-template <int KIN>
-__device__ __forceinline__ int point_response(const DevMaterial& m, const ModeGeom& mg, const double alpha[9],
-                                              PointState& st, PointResponse& r) {
-  mode_kinematics<KIN>(mg.hu, mg.gm, alpha, r.h, r.F);
+constexpr int kLogStash = 18;  // q (9), mu (3), first divided differences (6)
+
+__device__ __forceinline__ void put_log(const LogData& L, LaneLds s) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) s.set(i, L.q[i / 3][i % 3]);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) s.set(9 + a, L.mu[a]);
+  s.set(12, L.first[0][0]); s.set(13, L.first[1][1]); s.set(14, L.first[2][2]);
+  s.set(15, L.first[0][1]); s.set(16, L.first[1][2]); s.set(17, L.first[0][2]);
+}
+__device__ __forceinline__ void get_log(LaneLds s, LogData& L) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) L.q[i / 3][i % 3] = s.get(i);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) L.mu[a] = s.get(9 + a);
+  L.first[0][0] = s.get(12); L.first[1][1] = s.get(13); L.first[2][2] = s.get(14);
+  L.first[0][1] = L.first[1][0] = s.get(15);
+  L.first[1][2] = L.first[2][1] = s.get(16);
+  L.first[0][2] = L.first[2][0] = s.get(17);
+}
+
+template <int KIN, int NT>
+__device__ __forceinline__ int point_response(const DevMaterial& m, LaneLds geom, const double alpha[9],
+                                              PointStateT<NT>& st, PointResponse& r, double* __restrict__ state_out,
+                                              long long stride, long long q, LaneLds logs) {
   double strain[6];
-  LogData L;
-  if constexpr (KIN == kSmall) {
+  {
+    double hu[3][3], gm[3][3], h[3][3], F[3][3];
+    get_hu(geom, hu);
+    get_gm(geom, gm);
+    mode_kinematics<KIN>(hu, gm, alpha, h, F);
+    if constexpr (KIN == kSmall) {
 #pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      double s2 = bnl_mode(r.F, mg.gm, c, 0, 0) * alpha[0];
+      for (int c = 0; c < 6; ++c) {
+        double s2 = bnl_mode(F, gm, c, 0, 0) * alpha[0];
 #pragma unroll
-      for (int kk = 1; kk < 9; ++kk) s2 = s2 + bnl_mode(r.F, mg.gm, c, kk / 3, kk % 3) * alpha[kk];
-      strain[c] = mg.eps_u[c] + s2;
-    }
-  } else {
-    if constexpr (KIN == kLog) {
-      if (!(det3(r.F) > 0.0)) return kInverted;
-    }
-    double e3[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const double hth = r.h[0][i] * r.h[0][j] + r.h[1][i] * r.h[1][j] + r.h[2][i] * r.h[2][j];
-        e3[i][j] = 0.5 * ((r.h[i][j] + r.h[j][i]) + hth);
+        for (int kk = 1; kk < 9; ++kk) s2 = s2 + bnl_mode(F, gm, c, kk / 3, kk % 3) * alpha[kk];
+        strain[c] = geom.get(19 + c) + s2;
       }
-    double green[6] = {e3[0][0], e3[1][1], e3[2][2], 2.0 * e3[0][1], 2.0 * e3[1][2], 2.0 * e3[2][0]};
-    if constexpr (KIN == kLog) {
-      const int lf = logarithmic_strain(green, L);
-      if (lf != 0) return kDegenerate;
-#pragma unroll
-      for (int i = 0; i < 6; ++i) strain[i] = L.strain[i];
     } else {
+      if constexpr (KIN == kLog) {
+        if (!(det3(F) > 0.0)) return kInverted;
+      }
+      double e3[3][3];
 #pragma unroll
-      for (int i = 0; i < 6; ++i) strain[i] = green[i];
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const double hth = h[0][i] * h[0][j] + h[1][i] * h[1][j] + h[2][i] * h[2][j];
+          e3[i][j] = 0.5 * ((h[i][j] + h[j][i]) + hth);
+        }
+      double green[6] = {e3[0][0], e3[1][1], e3[2][2], 2.0 * e3[0][1], 2.0 * e3[1][2], 2.0 * e3[2][0]};
+      if constexpr (KIN == kLog) {
+        LogData L;
+        const int lf = logarithmic_strain(green, L);
+        if (lf != 0) return kDegenerate;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) strain[i] = L.strain[i];
+        put_log(L, logs);
+      } else {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) strain[i] = green[i];
+      }
     }
   }
+  reload_barrier();
   ReturnOut ro;
   const int rf = m.general ? return_general(m, strain, st, ro) : return_radial(m, strain, st, ro);
   if (rf == kReturnNotConverged) return kReturnFailed;
   if (rf == kReturnNonFinite) return kNonFiniteTrial;
   r.energy = ro.energy;
   r.yielding = ro.yielding;
+  // nothing after the return can fail: the trial state is final here
+  store_state(m, state_out, stride, q, st);
   if constexpr (KIN == kLog) {
-    double curv[6][6];
-    log_push(L, ro.stress, r.S, curv);
-    // P^T C P + T:L
-    double lp[6][6];
-    log_projection(L, lp);
-    double pc[6][6];
-#pragma unroll
-    for (int i = 0; i < 6; ++i)
-#pragma unroll
-      for (int j = 0; j < 6; ++j) {
-        double s = lp[0][i] * ro.C[0][j];
-#pragma unroll
-        for (int k = 1; k < 6; ++k) s = s + lp[k][i] * ro.C[k][j];
-        pc[i][j] = s;
-      }
-#pragma unroll
-    for (int i = 0; i < 6; ++i)
-#pragma unroll
-      for (int j = 0; j < 6; ++j) {
-        double s = pc[i][0] * lp[0][j];
-#pragma unroll
-        for (int k = 1; k < 6; ++k) s = s + pc[i][k] * lp[k][j];
-        r.mat[i][j] = s + curv[i][j];
-      }
+    LogData L;
+    get_log(logs, L);
+    log_material(L, ro.stress, ro.C, r.S, r.mat);
   } else {
 #pragma unroll
     for (int i = 0; i < 6; ++i) r.S[i] = ro.stress[i];

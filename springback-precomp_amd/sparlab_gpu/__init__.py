@@ -728,24 +728,36 @@ def make_tool(shape="sphere", centre=(0, 0, 0), centre0=None, radius=0.0, normal
     return t
 
 
+def contact_buffers(num_slaves, device):
+    """Device outputs and work space of one contact evaluation of
+    `num_slaves` slave nodes (reusable across evaluations)."""
+    import torch
+    f64 = dict(dtype=torch.float64, device=device)
+    i32 = dict(dtype=torch.int32, device=device)
+    ns = num_slaves
+    return dict(blocks=torch.zeros((ns, 3, 3), **f64), active=torch.zeros(ns, **i32),
+                trial_present=torch.zeros(ns, **i32), trial_force=torch.zeros((ns, 3), **f64),
+                trial_relative=torch.zeros((ns, 3), **f64), work=torch.zeros((max(ns, 1), 8), **f64),
+                work_status=torch.zeros(max(ns, 1), **i32), scalars=torch.zeros(10, **f64))
+
+
 def contact_evaluate_device(tool, coords, u, start, nodes, kappa_area, history=None, residual=None,
-                            want_tangent=True, friction_symmetric=False, stream=None):
+                            want_tangent=True, friction_symmetric=False, stream=None, buffers=None,
+                            read_scalars=True):
     """ToolContact::evaluate for one tool on device tensors. Returns a dict of
-    device tensors (residual, blocks, active, trial history) and host scalars."""
+    device tensors (residual, blocks, active, trial history) and, with
+    `read_scalars`, the host scalars (which synchronises the stream).
+    `buffers` (contact_buffers) reuses the outputs of an earlier call."""
     import torch
     dev = coords.device
     ns = nodes.shape[0]
     nn = coords.shape[0] if coords.dim() == 2 else coords.shape[0] // 3
-    f64 = dict(dtype=torch.float64, device=dev)
-    i32 = dict(dtype=torch.int32, device=dev)
     if residual is None:
-        residual = torch.zeros(3 * nn, **f64)
-    out = dict(residual=residual, blocks=torch.zeros((ns, 3, 3), **f64), active=torch.zeros(ns, **i32),
-               trial_present=torch.zeros(ns, **i32), trial_force=torch.zeros((ns, 3), **f64),
-               trial_relative=torch.zeros((ns, 3), **f64))
-    work = torch.zeros((max(ns, 1), 8), **f64)
-    work_status = torch.zeros(max(ns, 1), **i32)
-    scalars = torch.zeros(10, **f64)
+        residual = torch.zeros(3 * nn, dtype=torch.float64, device=dev)
+    bufs = buffers if buffers is not None else contact_buffers(ns, dev)
+    out = dict(residual=residual, **{k: bufs[k] for k in ("blocks", "active", "trial_present", "trial_force",
+                                                          "trial_relative")})
+    work, work_status, scalars = bufs["work"], bufs["work_status"], bufs["scalars"]
     hp, hf, hr = (None, None, None) if history is None else history
     s = stream if stream is not None else torch.cuda.current_stream(dev)
     L = lib()
@@ -758,6 +770,8 @@ def contact_evaluate_device(tool, coords, u, start, nodes, kappa_area, history=N
                                              p(out["trial_present"]), p(out["trial_force"]),
                                              p(out["trial_relative"]), p(work), p(work_status), p(scalars),
                                              ctypes.c_void_p(s.cuda_stream)))
+    if not read_scalars:
+        return out
     sc = scalars.cpu().numpy()
     out.update(gross=sc[0], round_off=sc[1], resultant=sc[2:5], max_penetration=sc[5], n_slipping=int(sc[6]),
                symmetric=sc[7] == 0, n_active=int(sc[8]), through=int(sc[9]))
