@@ -192,9 +192,9 @@ __device__ __forceinline__ void zero_response(PointResponse& r) {
 // trial(alpha) fills the pass's mode parameters (from the team's LDS machine);
 // it is called again after the return rather than holding them across it.
 template <int KIN, int NT, class Trial>
-__device__ __forceinline__ int team_local_pass(const ElementArgs& A, long long e, int p, LaneLds geom, LaneLds logs,
-                                               Trial trial, double* kaa, LocalSums& out) {
-  const DevMaterial& m = A.mat;
+__device__ __forceinline__ int team_local_pass(const ElementArgs& A, const DevMaterial& m, long long e, int p,
+                                               LaneLds geom, LaneLds logs, Trial trial, double* kaa,
+                                               LocalSums& out) {
   const bool has_point = p < A.rule.n;
   const long long q = e * A.rule.n + p;
   PointResponse r;
@@ -707,7 +707,7 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
         }
       };
       LocalSums cur;
-      const int fail = team_local_pass<KIN, NT>(A, e, p, geom, logs, trial, T.lu.a, cur);
+      const int fail = team_local_pass<KIN, NT>(A, A.mat, e, p, geom, logs, trial, T.lu.a, cur);
       const bool ok = fail == 0;
       int status = ok ? kOk : fail;
       double value = 0.0;
@@ -749,13 +749,14 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
 // ---------------------------------------------------------------------------
 // The condensation (IncompatibleModes.cpp:279-306, 549-561)
 // ---------------------------------------------------------------------------
-// A team's LDS: K_aa's factors, then the coupling K_aa^-1 K_au (9 x 24), then
-// the rows of K* for the symmetrisation.
+// A team's LDS: K_aa's factors and their composed permutations; K_au's 24
+// columns and r_a as a 25th (row-major, 9 x 25), solved in place into the
+// coupling K_aa^-1 K_au and K_aa^-1 r_a.
+constexpr int kRhs = 25;
 struct CondenseSlot {
-  union {
-    LuSlot lu;
-    double region[216];
-  };
+  LuSlot lu;
+  LuPerm perm;
+  double region[9 * kRhs];
 };
 
 // One team (8 lanes) per element. Lane p first forms the geometry of point p
@@ -808,93 +809,95 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
     T.lu.rt[8] = static_cast<int>(er[kErRows + 8]);
     T.lu.ct[8] = static_cast<int>(er[kErCols + 8]);
   }
+  // r_a: column 24 of the right-hand sides
+  T.region[p * kRhs + 24] = er[kErResidual + p];
+  if (p == 0) T.region[8 * kRhs + 24] = er[kErResidual + 8];
   team_sync_global();  // the geometry records (global) and the factors (LDS)
+  if (p == 0) lu_perm(T.lu, T.perm);
   const int a = p;
   // f_u rows and K_ua of node a
   double ru[3], kua[3][9];
   coupling_rows<KIN>(A, q0, n, a, ru, kua);
-  // f* = f_u - K_ua K_aa^-1 r_a
-  {
-    double ra[9], corr[9];
+  if constexpr (TANGENT) {
+    // the coupling columns 3a..3a+2: K_ua^T, or K_au with
+    // K_au[:, 3a + k] = sum_q w B_a^T mat B_u[:, 3a + k] (+ geometric)
+    double col[3][9];
+    if constexpr (SYM) {
 #pragma unroll
-    for (int i = 0; i < 9; ++i) ra[i] = er[kErResidual + i];
-    lu_solve(T.lu, ra, corr);
+      for (int k = 0; k < 3; ++k)
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      double s = kua[k][0] * corr[0];
+        for (int nn = 0; nn < 9; ++nn) col[k][nn] = kua[k][nn];
+    } else {
 #pragma unroll
-      for (int nn = 1; nn < 9; ++nn) s = s + kua[k][nn] * corr[nn];
-      A.force[e * 24 + 3 * a + k] = ru[k] - s;
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int nn = 0; nn < 9; ++nn) col[k][nn] = 0.0;
+      for (int q = 0; q < n; ++q) {
+        const double* gr = A.geom_ws + (q0 + q) * kGeomRecord;
+        const double* pr = A.point_ws + (q0 + q) * kPointRecord;
+        const double w = gr[kGrW];
+        double bu[6][3];
+        node_operator(gr, a, bu);
+        // y = mat B_u[:, 3a + k]
+        double y[3][6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            double s = pr[kWsTangent + c * 6] * bu[0][k];
+#pragma unroll
+            for (int d = 1; d < 6; ++d) s = s + pr[kWsTangent + c * 6 + d] * bu[d][k];
+            y[k][c] = s;
+          }
+#pragma unroll
+        for (int nn = 0; nn < 9; ++nn) {
+          double bcol[6];
+#pragma unroll
+          for (int c = 0; c < 6; ++c) bcol[c] = bnl_rec(gr, c, 8 + nn / 3, nn % 3);
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            double s = bcol[0] * y[k][0];
+#pragma unroll
+            for (int c = 1; c < 6; ++c) s = s + bcol[c] * y[k][c];
+            col[k][nn] = col[k][nn] + w * s;
+          }
+        }
+        if constexpr (KIN != kSmall) {
+          const double* g = gr + kGrG;
+          double gsa[3];
+          stress_times_gradient(gr, pr, a, gsa);
+#pragma unroll
+          for (int bm = 0; bm < 3; ++bm) {
+            const double geo = gsa[0] * g[0 * 11 + 8 + bm] + gsa[1] * g[1 * 11 + 8 + bm] + gsa[2] * g[2 * 11 + 8 + bm];
+            const double v = w * geo;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) col[k][3 * bm + k] = col[k][3 * bm + k] + v;
+          }
+        }
+      }
     }
+#pragma unroll
+    for (int nn = 0; nn < 9; ++nn)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) T.region[nn * kRhs + 3 * a + k] = col[k][nn];
+  }
+  team_sync();  // the permutations and the right-hand sides
+  // K_aa^-1 [K_au | r_a] in place: lane a its three columns, lane 0 also r_a
+  if (p == 0) lu_solve_column(T.lu, T.perm, T.region, kRhs, 24);
+  if constexpr (TANGENT) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) lu_solve_column(T.lu, T.perm, T.region, kRhs, 3 * a + k);
+  }
+  team_sync();
+  // f* = f_u - K_ua K_aa^-1 r_a
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double s = kua[k][0] * T.region[24];
+#pragma unroll
+    for (int nn = 1; nn < 9; ++nn) s = s + kua[k][nn] * T.region[nn * kRhs + 24];
+    A.force[e * 24 + 3 * a + k] = ru[k] - s;
   }
   if constexpr (!TANGENT) return;
-  // the coupling columns 3a..3a+2: K_aa^-1 K_ua^T, or K_aa^-1 K_au with
-  // K_au[:, 3a + k] = sum_q w B_a^T mat B_u[:, 3a + k] (+ geometric)
-  double col[3][9];
-  if constexpr (SYM) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-      for (int nn = 0; nn < 9; ++nn) col[k][nn] = kua[k][nn];
-  } else {
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-      for (int nn = 0; nn < 9; ++nn) col[k][nn] = 0.0;
-    for (int q = 0; q < n; ++q) {
-      const double* gr = A.geom_ws + (q0 + q) * kGeomRecord;
-      const double* pr = A.point_ws + (q0 + q) * kPointRecord;
-      const double w = gr[kGrW];
-      double bu[6][3];
-      node_operator(gr, a, bu);
-      // y = mat B_u[:, 3a + k]
-      double y[3][6];
-#pragma unroll
-      for (int c = 0; c < 6; ++c)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          double s = pr[kWsTangent + c * 6] * bu[0][k];
-#pragma unroll
-          for (int d = 1; d < 6; ++d) s = s + pr[kWsTangent + c * 6 + d] * bu[d][k];
-          y[k][c] = s;
-        }
-#pragma unroll
-      for (int nn = 0; nn < 9; ++nn) {
-        double bcol[6];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) bcol[c] = bnl_rec(gr, c, 8 + nn / 3, nn % 3);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          double s = bcol[0] * y[k][0];
-#pragma unroll
-          for (int c = 1; c < 6; ++c) s = s + bcol[c] * y[k][c];
-          col[k][nn] = col[k][nn] + w * s;
-        }
-      }
-      if constexpr (KIN != kSmall) {
-        const double* g = gr + kGrG;
-        double gsa[3];
-        stress_times_gradient(gr, pr, a, gsa);
-#pragma unroll
-        for (int bm = 0; bm < 3; ++bm) {
-          const double geo = gsa[0] * g[0 * 11 + 8 + bm] + gsa[1] * g[1 * 11 + 8 + bm] + gsa[2] * g[2 * 11 + 8 + bm];
-          const double v = w * geo;
-#pragma unroll
-          for (int k = 0; k < 3; ++k) col[k][3 * bm + k] = col[k][3 * bm + k] + v;
-        }
-      }
-    }
-  }
-  double cpl[3][9];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) lu_solve(T.lu, col[k], cpl[k]);
-  // publish the coupling (9 x 24)
-  team_sync();
-#pragma unroll
-  for (int nn = 0; nn < 9; ++nn)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) T.region[nn * 24 + 3 * a + k] = cpl[k][nn];
-  team_sync();
   double* out = A.tangent + e * 576 + (3 * a) * 24;
   // K_uu rows of node a in two column halves, K* = K_uu - K_ua coupling
 #pragma unroll
@@ -948,7 +951,7 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
         const int j = 12 * half + jj;
         double s = kua[k][0] * T.region[j];
 #pragma unroll
-        for (int nn = 1; nn < 9; ++nn) s = s + kua[k][nn] * T.region[nn * 24 + j];
+        for (int nn = 1; nn < 9; ++nn) s = s + kua[k][nn] * T.region[nn * kRhs + j];
         out[k * 24 + j] = kuu[k][jj] - s;
       }
   }

@@ -135,6 +135,60 @@ __device__ inline void lu_solve(const LuSlot& s, const double b[9], double x[9])
   for (int i = 0; i < 9; ++i) x[i] = c[i];
 }
 
+// The composed permutations of full-pivot factors, for solves whose right-hand
+// sides stay in memory: row i of P b is row prow[i] of b, and entry j of the
+// substituted solution is entry qcol[j] of x. Indexing memory by them replaces
+// the transposition swaps of lu_solve, which on registers become select chains.
+struct LuPerm {
+  int prow[9];
+  int qcol[9];
+};
+
+// One lane forms the permutations from the transpositions of s.
+__device__ inline void lu_perm(const LuSlot& s, LuPerm& P) {
+  for (int i = 0; i < 9; ++i) P.prow[i] = i;
+  for (int k = 0; k < 9; ++k) {
+    const int r = s.rt[k];
+    const int t = P.prow[k];
+    P.prow[k] = P.prow[r];
+    P.prow[r] = t;
+  }
+  // the column transpositions in reverse order, as lu_solve applies them,
+  // composed on an index list kept in P.qcol's storage first
+  for (int i = 0; i < 9; ++i) P.qcol[i] = i;
+  for (int k = 8; k >= 0; --k) {
+    const int q = s.ct[k];
+    const int t = P.qcol[k];
+    P.qcol[k] = P.qcol[q];
+    P.qcol[q] = t;
+  }
+  // x[i] = c[idx[i]] -> c[j] goes to x[inverse(idx)[j]]
+  int jdx[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) jdx[i] = P.qcol[i];
+  for (int i = 0; i < 9; ++i) P.qcol[jdx[i]] = i;
+}
+
+// x = A^-1 b in place, for b in column `col` of the row-major matrix m with
+// `ld` columns (LDS or global): lu_solve's arithmetic, in the same order.
+__device__ inline void lu_solve_column(const LuSlot& s, const LuPerm& P, double* m, int ld, int col) {
+  double c[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) c[i] = m[P.prow[i] * ld + col];
+#pragma unroll
+  for (int i = 1; i < 9; ++i)
+#pragma unroll
+    for (int k = 0; k < i; ++k) c[i] = c[i] - s.a[i * 9 + k] * c[k];
+#pragma unroll
+  for (int i = 8; i >= 0; --i) {
+#pragma unroll
+    for (int k = i + 1; k < 9; ++k) c[i] = c[i] - s.a[i * 9 + k] * c[k];
+    c[i] = c[i] / s.a[i * 9 + i];
+  }
+#pragma unroll
+  for (int j = 0; j < 9; ++j) m[P.qcol[j] * ld + col] = c[j];
+}
+
 // Solve A^T x = b with the factors of A (P A Q = L U, the row
 // transpositions rt applied in order, the column ones ct): x = P^T L^-T U^-T
 // Q^T b (every lane, redundantly).
