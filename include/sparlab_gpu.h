@@ -154,8 +154,9 @@ typedef struct sparlab_element_batch {
   double* workspace;
 } sparlab_element_batch;
 
-/* Doubles of workspace an element batch needs (86 per integration point and
- * 108 per element), or a negative code for a rule the kernels do not take. */
+/* Doubles of workspace an element batch needs (86 per integration point, 109
+ * per element and 17 more), or a negative code for a rule the kernels do not
+ * take. */
 int64_t sparlab_element_workspace_doubles(int64_t num_elements, const sparlab_rule* rule);
 
 /* Evaluates every element of the batch (replaces the calls of
@@ -170,7 +171,12 @@ int sparlab_elements_evaluate_device(const sparlab_material* material, const spa
 /* The same evaluation in its two stages (stages: 1 the local iteration of
  * the modes, src/fem/IncompatibleModes.cpp:320-546; 2 the coupling blocks and
  * the condensation, :549-561, which needs stage 1 of the same batch to have
- * run; 3 both), so that each stage can be timed or overlapped on its own. */
+ * run; 3 both), so that each stage can be timed or overlapped on its own.
+ * The local iteration takes the elements in the order of their expected work
+ * (a trial of every point at the committed modes, most work first), which
+ * changes no result; adding SPARLAB_STAGE_INDEX_ORDER to `stages` takes them
+ * in index order instead. */
+#define SPARLAB_STAGE_INDEX_ORDER 4
 int sparlab_elements_evaluate_stage_device(const sparlab_material* material, const sparlab_rule* rule,
                                            int kinematics, int want_tangent,
                                            const sparlab_element_batch* batch, int stages, void* stream);

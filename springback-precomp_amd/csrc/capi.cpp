@@ -316,6 +316,7 @@ int capi_prepare_elements(const sparlab_material* material, const sparlab_rule* 
   if (rs == -2) return fail(SPARLAB_E_UNSUPPORTED, why);
   a.kinematics = kinematics;
   a.mat = device_material(*material);
+  a.ordered = 1;
   return SPARLAB_OK;
 }
 
@@ -436,7 +437,8 @@ int sparlab_elements_evaluate_stage_device(const sparlab_material* material, con
                                            int stages, void* stream) {
   if (material == nullptr || rule == nullptr || b == nullptr) return fail(SPARLAB_E_ARGUMENT, "null argument");
   if (kinematics < 0 || kinematics > 2) return fail(SPARLAB_E_ARGUMENT, "kinematics must be 0, 1 or 2");
-  if (stages < 1 || stages > 3) return fail(SPARLAB_E_ARGUMENT, "stages must be 1, 2 or 3");
+  if ((stages & ~(3 | SPARLAB_STAGE_INDEX_ORDER)) != 0 || (stages & 3) == 0)
+    return fail(SPARLAB_E_ARGUMENT, "stages must be 1, 2 or 3, optionally with SPARLAB_STAGE_INDEX_ORDER");
   if (b->num_elements < 0 || b->num_nodes < 0) return fail(SPARLAB_E_ARGUMENT, "negative sizes");
   if (b->num_elements == 0) return SPARLAB_OK;
   if (b->coords == nullptr || b->conn == nullptr || b->u == nullptr || b->state_in == nullptr ||
@@ -465,6 +467,7 @@ int sparlab_elements_evaluate_stage_device(const sparlab_material* material, con
   a.status = b->status;
   a.fail_value = b->fail_value;
   sbk::set_workspace(a, b->workspace);
+  if (stages & SPARLAB_STAGE_INDEX_ORDER) a.ordered = 0;
   // The local solve writes every element's status; the condensation reads it.
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (stages & 1) {

@@ -520,6 +520,38 @@ __device__ __forceinline__ bool gen_newton(const DevMaterial& mat, const PointSt
   return true;
 }
 
+// What the return of a point will cost at this strain, from its elastic
+// trial alone (the tests that open return_3d and return_general_3d,
+// Plasticity.cpp:87-108, 352-364): 2 for a plastic trial (the Newton
+// iteration and the consistent tangent), 1 for an elastic trial that takes the
+// continuum tangent of a loading point, 0 otherwise. Used only to order the
+// elements of a launch by their expected work.
+template <int NT>
+__device__ __forceinline__ int return_cost(const DevMaterial& mat, const double strain[6], const PointStateT<NT>& st) {
+  if (!mat.enabled) return 0;
+  double f_tr, sy, eq;
+  if (mat.general) {
+    GenTrial trial;
+    gen_trial(mat, strain, st, trial);
+    f_tr = trial.f_tr;
+    sy = trial.sy_n;
+    eq = trial.eq_tr;
+  } else {
+    const double sqrt23 = 0.816496580927726;
+    double el[6], xi[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) el[i] = strain[i] - st.ep[i];
+    const double mean = (el[0] + el[1] + el[2]) / 3.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) xi[i] = (i < 3 ? 2.0 * mat.shear * (el[i] - mean) : mat.shear * el[i]) - st.kin[0][i];
+    eq = tensor_norm(xi);
+    sy = sqrt23 * yield_stress(mat, st.abar);
+    f_tr = eq - sy;
+  }
+  if (!(f_tr <= 1.0e-12 * sy)) return 2;
+  return (st.loading && f_tr > -1.0e-10 * sy && eq > 0.0) ? 1 : 0;
+}
+
 // return_general_3d; the state is updated in place (want_tangent = true, as
 // IncompatibleModes.cpp:232 asks for it on every pass).
 template <int NT>

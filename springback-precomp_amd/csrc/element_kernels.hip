@@ -22,6 +22,8 @@
 //                       full-pivot LU, the coupling blocks K_ua, K_au, K_uu
 //                       and the static condensation
 //                       f* = f_u - K_ua K_aa^-1 r_a, K* = K_uu - K_ua K_aa^-1 K_au.
+#include <algorithm>
+
 #include "element_kernels.hpp"
 #include "element_pass.hpp"
 #include "team_linalg.hpp"
@@ -661,7 +663,7 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
     if (idle) {
       const long long cand = next + before;
       if (cand < c1) {
-        e = cand;
+        e = A.ordered ? A.order[cand] : cand;
         fresh = true;
       } else {
         e = -1;
@@ -743,6 +745,140 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
       }
       team_sync();
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The schedule of the local solve
+// ---------------------------------------------------------------------------
+// A wavefront's pass costs what its most expensive point costs, and most of a
+// plastic point's cost is its return; the local solve therefore takes the
+// elements grouped by their expected work, most work first (so that the cheap
+// elements fill the end of the launch). The expected work of an element is
+// the sum over its points of return_cost at the committed modes (the elastic
+// trial of the first pass). The order changes no result: every element's
+// evaluation is independent of the others.
+
+// The strain of a point at alpha (the head of point_response); false for an
+// inverted or degenerate point, which the local solve itself reports.
+template <int KIN>
+__device__ __forceinline__ bool trial_strain(const ModeGeom& mg, const double alpha[9], double strain[6]) {
+  double h[3][3], F[3][3];
+  mode_kinematics<KIN>(mg.hu, mg.gm, alpha, h, F);
+  if constexpr (KIN == kSmall) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double s2 = bnl_mode(F, mg.gm, c, 0, 0) * alpha[0];
+#pragma unroll
+      for (int kk = 1; kk < 9; ++kk) s2 = s2 + bnl_mode(F, mg.gm, c, kk / 3, kk % 3) * alpha[kk];
+      strain[c] = mg.eps_u[c] + s2;
+    }
+    return true;
+  } else {
+    if constexpr (KIN == kLog) {
+      if (!(det3(F) > 0.0)) return false;
+    }
+    double e3[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double hth = h[0][i] * h[0][j] + h[1][i] * h[1][j] + h[2][i] * h[2][j];
+        e3[i][j] = 0.5 * ((h[i][j] + h[j][i]) + hth);
+      }
+    double green[6] = {e3[0][0], e3[1][1], e3[2][2], 2.0 * e3[0][1], 2.0 * e3[1][2], 2.0 * e3[2][0]};
+    if constexpr (KIN == kLog) {
+      LogData L;
+      if (logarithmic_strain(green, L) != 0) return false;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) strain[i] = L.strain[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) strain[i] = green[i];
+    }
+    return true;
+  }
+}
+
+// One team per element, lane p = point p: cost[e] = the element's expected
+// work (0 .. 2 points). Block 0 also clears the class counts and cursors.
+template <int KIN, int NT>
+__global__ __launch_bounds__(64) void cost_key_kernel(ElementArgs A) {
+  const int lane = threadIdx.x;
+  const int t = lane >> 3;
+  const int p = lane & 7;
+  if (blockIdx.x == 0 && lane < 2 * kCostClasses) A.counts[lane] = 0;
+  const long long e = static_cast<long long>(blockIdx.x) * kWaveTeams + t;
+  int c = 0;
+  if (e < A.ne && p < A.rule.n) {
+    double x0[8][3], ue[24];
+    load_element(A.coords, A.conn, A.u, e, x0, ue);
+    PointGeom pg;
+    double fv = 0.0;
+    if (setup_point(x0, ue, A.rule.xi[p], A.rule.eta[p], A.rule.zeta[p], A.rule.w[p], pg, fv) == kOk) {
+      ModeGeom mg;
+      mode_geom<KIN>(pg, ue, mg);
+      double alpha[9], strain[6];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) alpha[i] = A.alpha_in ? A.alpha_in[e * 9 + i] : 0.0;
+      if (trial_strain<KIN>(mg, alpha, strain)) {
+        PointStateT<NT> st;
+        load_state(A.mat, A.state_in, A.state_stride, e * A.rule.n + p, st);
+        c = return_cost(A.mat, strain, st);
+      }
+    }
+  }
+  const int key = team_isum(c);
+  if (e < A.ne && p == 0) A.cost[e] = key;
+}
+
+// Elements per block of the counting sort: each block owns a contiguous range.
+__device__ __forceinline__ void block_range(long long ne, long long& b0, long long& b1) {
+  const long long per = (ne + gridDim.x - 1) / gridDim.x;
+  b0 = static_cast<long long>(blockIdx.x) * per;
+  b1 = b0 + per < ne ? b0 + per : ne;
+}
+
+__global__ __launch_bounds__(256) void cost_count_kernel(const int* __restrict__ cost, long long ne, int* counts) {
+  __shared__ int h[kCostClasses];
+  if (threadIdx.x < kCostClasses) h[threadIdx.x] = 0;
+  __syncthreads();
+  long long b0, b1;
+  block_range(ne, b0, b1);
+  for (long long i = b0 + threadIdx.x; i < b1; i += blockDim.x) atomicAdd(&h[cost[i]], 1);
+  __syncthreads();
+  if (threadIdx.x < kCostClasses && h[threadIdx.x] != 0) atomicAdd(&counts[threadIdx.x], h[threadIdx.x]);
+}
+
+// Cursors: the classes in descending cost, each starting where the costlier
+// ones end.
+__global__ void cost_scan_kernel(int* counts) {
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int c = kCostClasses - 1; c >= 0; --c) {
+      counts[kCostClasses + c] = run;
+      run += counts[c];
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void cost_scatter_kernel(const int* __restrict__ cost, long long ne, int* counts,
+                                                           int* __restrict__ order) {
+  __shared__ int h[kCostClasses], base[kCostClasses];
+  if (threadIdx.x < kCostClasses) h[threadIdx.x] = 0;
+  __syncthreads();
+  long long b0, b1;
+  block_range(ne, b0, b1);
+  for (long long i = b0 + threadIdx.x; i < b1; i += blockDim.x) atomicAdd(&h[cost[i]], 1);
+  __syncthreads();
+  if (threadIdx.x < kCostClasses) {
+    base[threadIdx.x] = h[threadIdx.x] != 0 ? atomicAdd(&counts[kCostClasses + threadIdx.x], h[threadIdx.x]) : 0;
+    h[threadIdx.x] = 0;
+  }
+  __syncthreads();
+  for (long long i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+    const int c = cost[i];
+    order[base[c] + atomicAdd(&h[c], 1)] = static_cast<int>(i);
   }
 }
 
@@ -975,12 +1111,27 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+template <int KIN, int NT>
+void launch_local_solve_nt(const ElementArgs& a, unsigned int blocks, hipStream_t stream) {
+  if (a.ordered) {
+    const unsigned int teams = static_cast<unsigned int>((a.ne + kWaveTeams - 1) / kWaveTeams);
+    const long long span = 2048;  // elements per block of the counting sort
+    const unsigned int sort_blocks = static_cast<unsigned int>(std::min<long long>(1024, (a.ne + span - 1) / span));
+    hipLaunchKernelGGL((cost_key_kernel<KIN, NT>), dim3(teams), dim3(64), 0, stream, a);
+    hipLaunchKernelGGL(cost_count_kernel, dim3(sort_blocks), dim3(256), 0, stream, a.cost, a.ne, a.counts);
+    hipLaunchKernelGGL(cost_scan_kernel, dim3(1), dim3(64), 0, stream, a.counts);
+    hipLaunchKernelGGL(cost_scatter_kernel, dim3(sort_blocks), dim3(256), 0, stream, a.cost, a.ne, a.counts,
+                       a.order);
+  }
+  hipLaunchKernelGGL((local_solve_kernel<KIN, NT>), dim3(blocks), dim3(64), 0, stream, a);
+}
+
 template <int KIN>
 void launch_local_solve_kin(const ElementArgs& a, bool two_terms, unsigned int blocks, hipStream_t stream) {
   if (two_terms)
-    hipLaunchKernelGGL((local_solve_kernel<KIN, 2>), dim3(blocks), dim3(64), 0, stream, a);
+    launch_local_solve_nt<KIN, 2>(a, blocks, stream);
   else
-    hipLaunchKernelGGL((local_solve_kernel<KIN, kMaxTerms>), dim3(blocks), dim3(64), 0, stream, a);
+    launch_local_solve_nt<KIN, kMaxTerms>(a, blocks, stream);
 }
 
 int launch_local_solve(const ElementArgs& args, hipStream_t stream) {

@@ -48,23 +48,40 @@ struct ElementArgs {
   double* point_ws;
   double* geom_ws;
   double* elem_ws;
+  // and the schedule of the local solve, as int32:
+  //   cost      [ne]: the expected work of each element (cost_key_kernel)
+  //   order     [ne]: the elements in the order the local solve takes them
+  //   counts    [2 * kCostClasses]: per-class counts, then cursors
+  int* cost;
+  int* order;
+  int* counts;
   // elements per wavefront of the local solve (a multiple of 8; 0: 64)
   int chunk;
+  // 1: the local solve takes the elements in `order` (most work first);
+  // 0: in index order
+  int ordered;
 };
 
 constexpr int kPointRecord = 42;
 constexpr int kGeomRecord = 44;
 constexpr int kElemRecord = 108;
+// Element cost classes: sum over the points of return_cost (0, 1 or 2).
+constexpr int kCostClasses = 2 * kMaxPoints + 1;
 // Doubles of workspace for ne elements of `points` points each.
 inline long long workspace_doubles(long long ne, int points) {
-  return ne * points * (kPointRecord + kGeomRecord) + ne * kElemRecord;
+  const long long ints = 2 * ne + 2 * kCostClasses;
+  return ne * points * (kPointRecord + kGeomRecord) + ne * kElemRecord + (ints + 1) / 2;
 }
-// Points the three regions of ElementArgs into one workspace.
+// Points the regions of ElementArgs into one workspace.
 inline void set_workspace(ElementArgs& a, double* ws) {
   const long long npts = a.ne * a.rule.n;
   a.point_ws = ws;
   a.geom_ws = ws + npts * kPointRecord;
   a.elem_ws = ws + npts * (kPointRecord + kGeomRecord);
+  int* sched = reinterpret_cast<int*>(a.elem_ws + a.ne * kElemRecord);
+  a.cost = sched;
+  a.order = sched + a.ne;
+  a.counts = sched + 2 * a.ne;
 }
 
 int launch_local_solve(const ElementArgs& a, hipStream_t stream);

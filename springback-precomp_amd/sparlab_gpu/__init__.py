@@ -91,7 +91,7 @@ _lib = None
 
 def workspace_doubles(num_elements, points):
     """Doubles of device scratch an element batch needs (sparlab_element_workspace_doubles)."""
-    return num_elements * points * (42 + 44) + num_elements * 108
+    return num_elements * points * (42 + 44) + num_elements * 108 + (2 * num_elements + 2 * 17 + 1) // 2
 
 
 def lib():
@@ -304,6 +304,7 @@ class ElementBatch:
 
 LOCAL_SOLVE = 1
 CONDENSATION = 2
+INDEX_ORDER = 4  # the local solve takes the elements in index order (no cost ordering)
 
 
 def evaluate_elements(material, rule, batch, kinematics=FINITE_LOGARITHMIC, want_tangent=True, stream=None,

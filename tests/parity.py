@@ -21,8 +21,10 @@ def oracle_material(m):
 
 
 def run_gpu(material, rule, coords, conn, u, states_ref, alpha=None, kinematics=2, want_tangent=True,
-            device="cuda:0"):
-    """Evaluates a batch on the GPU; returns numpy outputs (states as 39-records)."""
+            device="cuda:0", index_order=False):
+    """Evaluates a batch on the GPU; returns numpy outputs (states as 39-records).
+    index_order: the local solve takes the elements in index order instead of
+    by expected work."""
     import torch
     ne = conn.shape[0]
     npts = rule.points
@@ -30,7 +32,8 @@ def run_gpu(material, rule, coords, conn, u, states_ref, alpha=None, kinematics=
     t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
     b = G.ElementBatch(t(coords), t(conn, torch.int32), t(u), t(st),
                        None if alpha is None else t(alpha), npts, want_tangent)
-    G.evaluate_elements(material, rule, b, kinematics, want_tangent)
+    stages = G.LOCAL_SOLVE | G.CONDENSATION | (G.INDEX_ORDER if index_order else 0)
+    G.evaluate_elements(material, rule, b, kinematics, want_tangent, stages=stages)
     torch.cuda.synchronize()
     return dict(force=b.force.cpu().numpy(), tangent=None if b.tangent is None else b.tangent.cpu().numpy(),
                 alpha=b.alpha_out.cpu().numpy(),

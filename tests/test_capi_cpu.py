@@ -147,3 +147,14 @@ def test_union_pattern_is_block_diagonal():
         v0, v1 = row_ptr[r0], row_ptr[r1]
         assert np.array_equal(row_ptr[r0:r1 + 1] - v0, rp)
         assert np.array_equal(col_idx[v0:v1] - 3 * node_off[i], ci)
+
+
+def test_workspace_size_mirrors_the_library():
+    """sparlab_gpu.workspace_doubles (used to size device scratch from Python)
+    equals the library's sparlab_element_workspace_doubles."""
+    import ctypes
+    lib = G.lib()
+    for points_rule in (G.Rule(2, 0, 2), G.Rule(1, 0, 2), G.Rule(2, 1, 2)):
+        for ne in (0, 1, 7, 1000, 1_000_003):
+            got = lib.sparlab_element_workspace_doubles(ne, ctypes.byref(points_rule.c))
+            assert got == G.workspace_doubles(ne, points_rule.points), (ne, points_rule.points)

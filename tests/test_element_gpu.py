@@ -135,6 +135,22 @@ def test_microbench_sample_parity():
     assert np.array_equal(gpu_r["force"], gpu["force"]), "residual-only force differs from the tangent pass"
 
 
+def test_cost_order_changes_no_result():
+    """The local solve's schedule (elements by expected work, or in index
+    order) changes no bit of any output, with a mix of yielding, elastic and
+    refused elements."""
+    m = W.microbench_material()
+    w = W.microbench(3000, seed=11)
+    coords, u = w["coords"].copy(), w["u"].copy()
+    # element 5: a displacement that turns it inside out (refused in its first pass)
+    u[w["conn"][5, 4:]] += np.array([0.0, 0.0, -3.0])
+    by_cost = run_gpu(m, G.Rule(), coords, w["conn"], u, w["states"], kinematics=2)
+    by_index = run_gpu(m, G.Rule(), coords, w["conn"], u, w["states"], kinematics=2, index_order=True)
+    assert by_cost["status"][5] != 0 and (by_cost["yielding"] > 0).any() and (by_cost["yielding"] == 0).any()
+    for k in by_cost:
+        assert np.array_equal(by_cost[k], by_index[k]), k
+
+
 def test_refused_elements_report_status():
     """An inverted element is refused at set-up (MeshError), a step that
     inverts a point fails its first pass (SolverError); the others evaluate."""
