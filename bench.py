@@ -223,6 +223,7 @@ def assembly_leg(G, W, torch, dev, reps):
                                  stream=stream)
 
     stages = (("values", lambda: sysm.assemble_device(ke=d_ke, values=values, column_major=True, stream=stream)),
+              ("values_row", lambda: sysm.assemble_device(ke=d_ke, values=values, column_major=False, stream=stream)),
               ("force", lambda: sysm.assemble_device(fe=d_fe, fint=fint, gross=gross, stream=stream)),
               ("contact", contact), ("node_blocks", blocks))
     us = {}
@@ -243,7 +244,8 @@ def assembly_leg(G, W, torch, dev, reps):
                         "10 000 slave nodes vs a sphere R = 5 mm, friction 0.1, exact tangent",
             "elements": int(ne), "nodes": int(nn), "nnz": int(sysm.nnz), "reps": reps,
             "values_us": us["values"], "values_bytes": alg, "values_GBps": gbs,
-            "values_hbm_frac": gbs / HBM_PEAK_GBS, "force_us": us["force"], "contact_us": us["contact"],
+            "values_hbm_frac": gbs / HBM_PEAK_GBS, "values_row_major_us": us["values_row"],
+            "force_us": us["force"], "contact_us": us["contact"],
             "node_blocks_us": us["node_blocks"], "contact_active_nodes": n_active}
 
 

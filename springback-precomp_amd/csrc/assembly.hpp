@@ -33,6 +33,10 @@ struct HostPattern {
   // node incidences for f_int: [node_ptr[n], node_ptr[n+1]) of e * 8 + a_local, ascending e
   std::vector<int32_t> node_ptr;
   std::vector<int32_t> node_incidence;
+  // for every incidence (e, n): byte m of inc_pos is the position of element
+  // node m within n's sorted neighbourhood (the node-wise assembly's map)
+  std::vector<uint64_t> inc_pos;
+  int32_t max_degree = 0;                // the largest neighbourhood
   // the diagonal block of every node (contact blocks)
   std::vector<int32_t> diag_block;       // node -> block index of (node, node)
   std::vector<int32_t> block_offset;     // the reference's per-element (a, b) offsets, 64 per element

@@ -2,48 +2,99 @@
 // element-order scatter (src/fem/Assembler.cpp:183-215) and force gather
 // (src/fem/NonlinearSystem.hpp:240-262).
 //
-// assemble_values_kernel: one thread per (3 x 3 block, block row). It sums the
-//   block's contributions in their list order (ascending element), so every
-//   value is the reference's sum, added in the reference's order, bit for bit.
-//   Three consecutive doubles per thread; consecutive threads write
-//   consecutive blocks of a row, so the stores coalesce. HBM-bound: each
-//   element matrix entry is read once and each value written once.
+// assemble_nodes_kernel: one wavefront per node n. Because the pattern is
+//   blocked and structurally symmetric, n's three rows of the row-major
+//   values (or n's three columns of the reference's column-major ones) are
+//   9 deg(n) contiguous values from row_ptr[3n]. The wavefront adds each
+//   incident element's 72 values for n (three rows, or three columns, of K_e)
+//   into LDS accumulators that start at 0, element after element in ascending
+//   element order, so every value is the reference's sum added in the
+//   reference's order, bit for bit; then it writes the node's values with
+//   contiguous stores. HBM-bound: each element matrix entry is read once and
+//   each value written once, and a wavefront issues the loads of up to eight
+//   incidences before it adds them.
 // gather_force_kernel: one thread per DOF: f_int and the gross |f_e| sum.
 // sum_squares_kernel: a fixed-shape tree for the norms (deterministic).
 #include "assembly_kernels.hpp"
 
 namespace sbk {
 
-__global__ __launch_bounds__(256) void assemble_values_kernel(AssemblyArgs A) {
-  const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= 3ll * A.num_blocks) return;
-  const int blk = static_cast<int>(t / 3);
-  const int j = static_cast<int>(t - 3ll * blk);
-  const int c0 = A.block_ptr[blk], c1 = A.block_ptr[blk + 1];
-  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-  for (int c = c0; c < c1; ++c) {
-    const int code = A.block_contrib[c];
-    const long long e = code >> 6;
-    const int al = (code >> 3) & 7, bl = code & 7;
-    const double* k = A.ke + e * 576;
-    if (!A.column_major) {
-      // row 3 al + j, columns 3 bl + 0..2 (contiguous)
-      const double* r = k + (3 * al + j) * 24 + 3 * bl;
-      v0 = v0 + r[0];
-      v1 = v1 + r[1];
-      v2 = v2 + r[2];
-    } else {
-      // the reference's column-major values: column 3 al + j, rows 3 bl + 0..2
-      const double* r = k + (3 * bl) * 24 + 3 * al + j;
-      v0 = v0 + r[0];
-      v1 = v1 + r[24];
-      v2 = v2 + r[48];
+constexpr int kNodeWaves = 4;  // nodes (wavefronts) per block
+constexpr int kBatch = 8;      // incidences loaded ahead
+constexpr int kXcds = 8;       // MI355X: workgroups go round-robin to 8 XCDs, each with its own L2
+
+__global__ __launch_bounds__(256) void assemble_nodes_kernel(AssemblyArgs A) {
+  extern __shared__ double acc_all[];
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  // Column-major values read three columns of K_e (24-byte runs 192 bytes
+  // apart), so the nodes of an element share its cache lines. There the
+  // mapping is XCD-aware: XCD x takes the x-th contiguous range of node blocks,
+  // so the nodes of an element (close in number) mostly meet on one L2. Row-
+  // major values read contiguous rows and keep the plain order.
+  const long long per = gridDim.x / kXcds;  // the grid is a multiple of kXcds
+  const long long blk = A.column_major ? static_cast<long long>(blockIdx.x % kXcds) * per + blockIdx.x / kXcds
+                                       : static_cast<long long>(blockIdx.x);
+  const long long n = blk * kNodeWaves + w;
+  if (n >= A.num_nodes) return;  // the whole wavefront
+  double* acc = acc_all + w * 9 * A.max_degree;
+  const int base = A.row_ptr[3 * n];
+  const int len = A.row_ptr[3 * n + 3] - base;  // 9 deg(n)
+  const int deg3 = len / 3;                     // the length of one row
+  for (int v = lane; v < len; v += 64) acc[v] = 0.0;
+  // input j of an incidence (two per lane: j = lane, and j = 64 + lane for
+  // lanes 0-7): row k of n's three, element column block m, column k'
+  // (row-major), or column k, element row block m, row k' (column-major)
+  int k[2], m[2], kp[2];
+  for (int r = 0; r < 2; ++r) {
+    const int j = lane + 64 * r;
+    if (!A.column_major) { k[r] = j / 24; m[r] = (j % 24) / 3; kp[r] = j % 3; }
+    else { m[r] = j / 9; kp[r] = (j / 3) % 3; k[r] = j % 3; }
+  }
+  const bool second = lane < 8;
+  const int i0 = A.node_ptr[n], i1 = A.node_ptr[n + 1];
+  for (int ib = i0; ib < i1; ib += kBatch) {
+    double val[kBatch][2];
+    unsigned long long pos[kBatch];
+#pragma unroll
+    for (int b = 0; b < kBatch; ++b) {
+      const int i = ib + b;
+      if (i < i1) {
+        const int code = A.node_incidence[i];
+        const long long e = code >> 3;
+        const int nl = code & 7;
+        const double* ke = A.ke + e * 576;
+        pos[b] = A.inc_pos[i];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          if (r == 0 || second) {
+            // row-major: K_e(3 nl + k, 3 m + k'); column-major: K_e(3 m + k', 3 nl + k)
+            const int off = !A.column_major ? (3 * nl + k[r]) * 24 + 3 * m[r] + kp[r]
+                                            : (3 * m[r] + kp[r]) * 24 + 3 * nl + k[r];
+            val[b][r] = ke[off];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < kBatch; ++b) {
+      if (ib + b < i1) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          if (r == 0 || second) {
+            const int pm = static_cast<int>((pos[b] >> (8 * m[r])) & 0xffull);
+            const int v = k[r] * deg3 + 3 * pm + kp[r];
+            acc[v] = acc[v] + val[b][r];
+          }
+        }
+        // the next incidence adds into values this one's other lanes wrote
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
     }
   }
-  double* out = A.values + A.block_pos[blk] + static_cast<long long>(j) * A.block_row_len[blk];
-  out[0] = v0;
-  out[1] = v1;
-  out[2] = v2;
+  for (int v = lane; v < len; v += 64) A.values[base + v] = acc[v];
 }
 
 __global__ __launch_bounds__(256) void gather_force_kernel(ForceArgs A) {
@@ -120,10 +171,11 @@ __global__ __launch_bounds__(256) void status_summary_kernel(const int* __restri
 
 int launch_assemble_values(const AssemblyArgs& a, hipStream_t s) {
   (void)hipGetLastError();
-  const long long threads = 3ll * a.num_blocks;
-  if (threads == 0) return 0;
-  const unsigned int grid = static_cast<unsigned int>((threads + 255) / 256);
-  hipLaunchKernelGGL(assemble_values_kernel, dim3(grid), dim3(256), 0, s, a);
+  if (a.num_nodes == 0) return 0;
+  const long long blocks = (a.num_nodes + kNodeWaves - 1) / kNodeWaves;
+  const unsigned int grid = static_cast<unsigned int>((blocks + kXcds - 1) / kXcds * kXcds);
+  const std::size_t lds = sizeof(double) * kNodeWaves * 9 * static_cast<std::size_t>(a.max_degree);
+  hipLaunchKernelGGL(assemble_nodes_kernel, dim3(grid), dim3(64 * kNodeWaves), lds, s, a);
   return static_cast<int>(hipGetLastError());
 }
 

@@ -6,14 +6,15 @@
 namespace sbk {
 
 struct AssemblyArgs {
-  const double* ke;            // 576 per element, row-major
-  const int* block_ptr;        // num_blocks + 1
-  const int* block_contrib;    // e * 64 + a_local * 8 + b_local
-  const int* block_pos;        // first value of the block's first row
-  const int* block_row_len;    // stride between the block's rows
-  long long num_blocks;
-  int column_major;            // 0: row-major values of K; 1: the reference's column-major values
-  double* values;              // nnz
+  const double* ke;             // 576 per element, row-major
+  const int* row_ptr;           // 3 num_nodes + 1: node n's values are [row_ptr[3n], row_ptr[3n + 3])
+  const int* node_ptr;          // num_nodes + 1
+  const int* node_incidence;    // e * 8 + a_local, ascending e
+  const unsigned long long* inc_pos;  // per incidence: byte m = position of element node m in n's neighbourhood
+  long long num_nodes;
+  int max_degree;               // the largest neighbourhood (sizes the LDS accumulators)
+  int column_major;             // 0: row-major values of K; 1: the reference's column-major values
+  double* values;               // nnz
 };
 
 struct ForceArgs {

@@ -111,6 +111,25 @@ bool build_pattern(int64_t nn, int64_t ne, const int32_t* conn, HostPattern& p, 
       for (int ab = 0; ab < 64; ++ab)
         p.block_contrib[fill[eb[static_cast<std::size_t>(e * 64 + ab)]]++] = static_cast<int32_t>(e * 64 + ab);
   }
+  // node-wise assembly map (positions fit a byte: neighbourhoods of at most
+  // 255 nodes, checked)
+  p.max_degree = 0;
+  for (int64_t n = 0; n < nn; ++n) p.max_degree = std::max(p.max_degree, nbr_ptr[n + 1] - nbr_ptr[n]);
+  if (p.max_degree > 255) {
+    if (why) *why = "a node has more than 255 neighbours";
+    return false;
+  }
+  p.inc_pos.resize(static_cast<std::size_t>(ne * npe));
+  for (int64_t n = 0; n < nn; ++n)
+    for (int32_t i = p.node_ptr[n]; i < p.node_ptr[n + 1]; ++i) {
+      const int32_t* nodes = conn + (p.node_incidence[i] / npe) * npe;
+      uint64_t packed = 0;
+      for (int m = 0; m < npe; ++m) {
+        const uint64_t pos = static_cast<uint64_t>(block_of(static_cast<int32_t>(n), nodes[m]) - nbr_ptr[n]);
+        packed |= pos << (8 * m);
+      }
+      p.inc_pos[static_cast<std::size_t>(i)] = packed;
+    }
   p.diag_block.resize(static_cast<std::size_t>(nn));
   for (int64_t n = 0; n < nn; ++n) p.diag_block[n] = block_of(static_cast<int32_t>(n), static_cast<int32_t>(n));
   // the reference's block offsets: position of a within the neighbourhood of b, times 3

@@ -43,7 +43,7 @@ void release(sparlab_system* s) {
   for (int i = 0; i < 2; ++i) { dfree(s->state[i]); dfree(s->alpha[i]); }
   dfree(s->fe); dfree(s->ke); dfree(s->energy); dfree(s->fail_value); dfree(s->gross);
   dfree(s->yielding); dfree(s->iterations); dfree(s->status);
-  dfree(s->block_ptr); dfree(s->block_contrib); dfree(s->block_pos); dfree(s->block_row_len);
+  dfree(s->row_ptr); dfree(s->inc_pos);
   dfree(s->node_ptr); dfree(s->node_incidence); dfree(s->diag_block);
   dfree(s->ws); dfree(s->partial); dfree(s->scalars); dfree(s->summary);
   dfree(s->grad_x); dfree(s->grad_u); dfree(s->grad_p);
@@ -52,6 +52,20 @@ void release(sparlab_system* s) {
 }  // namespace
 
 namespace sbk {
+
+AssemblyArgs assembly_args(const sparlab_system* s, const double* ke, int column_major, double* values) {
+  AssemblyArgs g{};
+  g.ke = ke;
+  g.row_ptr = s->row_ptr;
+  g.node_ptr = s->node_ptr;
+  g.node_incidence = s->node_incidence;
+  g.inc_pos = reinterpret_cast<const unsigned long long*>(s->inc_pos);
+  g.num_nodes = s->nn;
+  g.max_degree = s->pattern.max_degree;
+  g.column_major = column_major ? 1 : 0;
+  g.values = values;
+  return g;
+}
 
 int system_enqueue(sparlab_system* s, const double* u, int want_tangent, int column_major, double* fint,
                    double* values, hipStream_t st) {
@@ -78,9 +92,8 @@ int system_enqueue(sparlab_system* s, const double* u, int want_tangent, int col
   ForceArgs f{s->fe, s->node_ptr, s->node_incidence, static_cast<long long>(s->nn), fint, s->gross};
   if (launch_gather_force(f, st) != 0) return capi_fail(SPARLAB_E_DEVICE, "force gather launch failed");
   if (want_tangent) {
-    AssemblyArgs g{s->ke, s->block_ptr, s->block_contrib, s->block_pos, s->block_row_len,
-                        static_cast<long long>(s->pattern.block_pos.size()), column_major ? 1 : 0, values};
-    if (launch_assemble_values(g, st) != 0) return capi_fail(SPARLAB_E_DEVICE, "assembly launch failed");
+    if (launch_assemble_values(assembly_args(s, s->ke, column_major, values), st) != 0)
+      return capi_fail(SPARLAB_E_DEVICE, "assembly launch failed");
   }
   const int parts = static_cast<int>(std::min<int64_t>(1024, (3 * s->nn + 4095) / 4096 + 1));
   if (launch_reduce(s->gross, 3 * s->nn, 1, s->partial, parts, s->scalars + 1, st) != 0 ||
@@ -127,9 +140,8 @@ int sparlab_system_create(const double* coords, int64_t num_nodes, const int32_t
             dalloc(s->iterations, ne) && dalloc(s->status, ne) &&
             dalloc(s->ws, static_cast<std::size_t>(sbk::workspace_doubles(num_elements, s->points))) &&
             dalloc(s->partial, 1024) &&
-            dalloc(s->scalars, 2) && dalloc(s->summary, 3) && upload(s->block_ptr, s->pattern.block_ptr) &&
-            upload(s->block_contrib, s->pattern.block_contrib) && upload(s->block_pos, s->pattern.block_pos) &&
-            upload(s->block_row_len, s->pattern.block_row_len) && upload(s->node_ptr, s->pattern.node_ptr) &&
+            dalloc(s->scalars, 2) && dalloc(s->summary, 3) && upload(s->row_ptr, s->pattern.row_ptr) &&
+            upload(s->inc_pos, s->pattern.inc_pos) && upload(s->node_ptr, s->pattern.node_ptr) &&
             upload(s->node_incidence, s->pattern.node_incidence) && upload(s->diag_block, s->pattern.diag_block);
   ok = ok && hipMemcpy(s->coords, coords, 3 * nn * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(s->conn, conn, 8 * ne * sizeof(int32_t), hipMemcpyHostToDevice) == hipSuccess &&
@@ -199,9 +211,8 @@ int sparlab_system_assemble_device(sparlab_system* s, const double* fe, const do
       return sbk::capi_fail(SPARLAB_E_DEVICE, "reduction launch failed");
   }
   if (values) {
-    sbk::AssemblyArgs g{ke, s->block_ptr, s->block_contrib, s->block_pos, s->block_row_len,
-                        static_cast<long long>(s->pattern.block_pos.size()), column_major ? 1 : 0, values};
-    if (sbk::launch_assemble_values(g, st) != 0) return sbk::capi_fail(SPARLAB_E_DEVICE, "assembly launch failed");
+    if (sbk::launch_assemble_values(sbk::assembly_args(s, ke, column_major, values), st) != 0)
+      return sbk::capi_fail(SPARLAB_E_DEVICE, "assembly launch failed");
   }
   return SPARLAB_OK;
 }

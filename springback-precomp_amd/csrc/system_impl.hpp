@@ -10,6 +10,7 @@
 #include <string>
 
 #include "assembly.hpp"
+#include "assembly_kernels.hpp"
 #include "element_kernels.hpp"
 
 namespace sbk {
@@ -35,7 +36,8 @@ struct sparlab_system {
   int cur = 0;
   double *fe = nullptr, *ke = nullptr, *energy = nullptr, *fail_value = nullptr, *gross = nullptr;
   int32_t *yielding = nullptr, *iterations = nullptr, *status = nullptr;
-  int32_t *block_ptr = nullptr, *block_contrib = nullptr, *block_pos = nullptr, *block_row_len = nullptr;
+  int32_t* row_ptr = nullptr;
+  uint64_t* inc_pos = nullptr;
   int32_t *node_ptr = nullptr, *node_incidence = nullptr, *diag_block = nullptr;
   double* ws = nullptr;         // the element kernels' workspace
   double* partial = nullptr;    // 1024 partial sums
@@ -58,4 +60,6 @@ bool hill_r_derivatives(const sparlab_material& m, double dP[3][36]);
 // sparlab_system_evaluate_device queues).
 int system_enqueue(sparlab_system* s, const double* u, int want_tangent, int column_major, double* fint,
                    double* values, hipStream_t stream);
+// The node-wise assembly of element matrices ke into the system's pattern.
+AssemblyArgs assembly_args(const sparlab_system* s, const double* ke, int column_major, double* values);
 }  // namespace sbk
