@@ -250,6 +250,13 @@ __device__ __forceinline__ int team_local_pass(const ElementArgs& A, const DevMa
   return 0;
 }
 
+// Chunk c of a balanced partition of ne positions into w chunks starts at
+// chunk_start(ne, w, c): the first ne % w chunks hold one position more.
+__device__ __forceinline__ long long chunk_start(long long ne, long long w, long long c) {
+  const long long q = ne / w, r = ne % w;
+  return c * q + (c < r ? c : r);
+}
+
 // ---------------------------------------------------------------------------
 // The local Newton iteration as a state machine (IncompatibleModes.cpp:320-546)
 // ---------------------------------------------------------------------------
@@ -646,9 +653,10 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
   const int p = lane & 7;
   TeamSlot& T = slots[t];
   Machine& M = T.m;
-  const long long chunk = A.chunk;
-  const long long c0 = static_cast<long long>(blockIdx.x) * chunk;
-  const long long c1 = c0 + chunk < A.ne ? c0 + chunk : A.ne;
+  // the wavefront's positions: a balanced partition of [0, ne) into gridDim.x
+  // chunks of at most A.chunk elements (chunk_start)
+  const long long c0 = chunk_start(A.ne, gridDim.x, blockIdx.x);
+  const long long c1 = chunk_start(A.ne, gridDim.x, blockIdx.x + 1);
   long long next = c0;
   long long e = -1;
   if (p == 0) M.phase = kIdle;
@@ -752,12 +760,18 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
 // The schedule of the local solve
 // ---------------------------------------------------------------------------
 // A wavefront's pass costs what its most expensive point costs, and most of a
-// plastic point's cost is its return; the local solve therefore takes the
-// elements grouped by their expected work, most work first (so that the cheap
-// elements fill the end of the launch). The expected work of an element is
-// the sum over its points of return_cost at the committed modes (the elastic
-// trial of the first pass). The order changes no result: every element's
-// evaluation is independent of the others.
+// plastic point's cost is its return; the local solve therefore has each
+// wavefront take its elements in descending expected work, so that its 8 teams
+// work on elements of like cost at the same time. A launch of many wavefronts
+// per SIMD takes the elements in one descending run (each chunk then holds
+// elements of one cost, and the cheap chunks fill the end of the launch); a
+// launch of few wavefronts, which all run at once, deals them to the chunks
+// round-robin in descending order (rank r to chunk r mod w, at slot r / w), so
+// that every chunk holds the same mix of work and no wavefront outlasts the
+// others. The expected work of an element is the sum over its points of
+// return_cost at the committed modes (the elastic trial of the first pass).
+// The order changes no result: every element's evaluation is independent of
+// the others.
 
 // The strain of a point at alpha (the head of point_response); false for an
 // inverted or degenerate point, which the local solve itself reports.
@@ -862,8 +876,11 @@ __global__ void cost_scan_kernel(int* counts) {
   }
 }
 
+// order[r] (deal = 0) or order[chunk_start(ne, chunks, r % chunks) + r / chunks]
+// (deal = 1) = the element of rank r in descending cost (ties in no
+// particular order).
 __global__ __launch_bounds__(256) void cost_scatter_kernel(const int* __restrict__ cost, long long ne, int* counts,
-                                                           int* __restrict__ order) {
+                                                           long long chunks, int deal, int* __restrict__ order) {
   __shared__ int h[kCostClasses], base[kCostClasses];
   if (threadIdx.x < kCostClasses) h[threadIdx.x] = 0;
   __syncthreads();
@@ -878,7 +895,8 @@ __global__ __launch_bounds__(256) void cost_scatter_kernel(const int* __restrict
   __syncthreads();
   for (long long i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
     const int c = cost[i];
-    order[base[c] + atomicAdd(&h[c], 1)] = static_cast<int>(i);
+    const long long r = base[c] + atomicAdd(&h[c], 1);
+    order[deal ? chunk_start(ne, chunks, r % chunks) + r / chunks : r] = static_cast<int>(i);
   }
 }
 
@@ -949,7 +967,7 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
   T.region[p * kRhs + 24] = er[kErResidual + p];
   if (p == 0) T.region[8 * kRhs + 24] = er[kErResidual + 8];
   team_sync_global();  // the geometry records (global) and the factors (LDS)
-  if (p == 0) lu_perm(T.lu, T.perm);
+  if (p == 0) lu_perm(T.lu.rt, T.lu.ct, T.perm);
   const int a = p;
   // f_u rows and K_ua of node a
   double ru[3], kua[3][9];
@@ -1019,10 +1037,10 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
   }
   team_sync();  // the permutations and the right-hand sides
   // K_aa^-1 [K_au | r_a] in place: lane a its three columns, lane 0 also r_a
-  if (p == 0) lu_solve_column(T.lu, T.perm, T.region, kRhs, 24);
+  if (p == 0) lu_solve_column(T.lu.a, T.perm, T.region, kRhs, 24);
   if constexpr (TANGENT) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) lu_solve_column(T.lu, T.perm, T.region, kRhs, 3 * a + k);
+    for (int k = 0; k < 3; ++k) lu_solve_column(T.lu.a, T.perm, T.region, kRhs, 3 * a + k);
   }
   team_sync();
   // f* = f_u - K_ua K_aa^-1 r_a
@@ -1111,9 +1129,24 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+// Wavefront slots of the current device at one wavefront per SIMD (the local
+// solve's occupancy): 4 SIMDs per CU.
+int resident_waves() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1024;
+  if (cached[dev] == 0) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    cached[dev] = 4 * cus;
+  }
+  return cached[dev];
+}
+
 template <int KIN, int NT>
 void launch_local_solve_nt(const ElementArgs& a, unsigned int blocks, hipStream_t stream) {
   if (a.ordered) {
+    const int deal = static_cast<long long>(blocks) < 4ll * resident_waves() ? 1 : 0;
     const unsigned int teams = static_cast<unsigned int>((a.ne + kWaveTeams - 1) / kWaveTeams);
     const long long span = 2048;  // elements per block of the counting sort
     const unsigned int sort_blocks = static_cast<unsigned int>(std::min<long long>(1024, (a.ne + span - 1) / span));
@@ -1121,7 +1154,7 @@ void launch_local_solve_nt(const ElementArgs& a, unsigned int blocks, hipStream_
     hipLaunchKernelGGL(cost_count_kernel, dim3(sort_blocks), dim3(256), 0, stream, a.cost, a.ne, a.counts);
     hipLaunchKernelGGL(cost_scan_kernel, dim3(1), dim3(64), 0, stream, a.counts);
     hipLaunchKernelGGL(cost_scatter_kernel, dim3(sort_blocks), dim3(256), 0, stream, a.cost, a.ne, a.counts,
-                       a.order);
+                       static_cast<long long>(blocks), deal, a.order);
   }
   hipLaunchKernelGGL((local_solve_kernel<KIN, NT>), dim3(blocks), dim3(64), 0, stream, a);
 }

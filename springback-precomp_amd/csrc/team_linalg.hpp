@@ -144,11 +144,12 @@ struct LuPerm {
   int qcol[9];
 };
 
-// One lane forms the permutations from the transpositions of s.
-__device__ inline void lu_perm(const LuSlot& s, LuPerm& P) {
+// One lane forms the permutations from the row and column transpositions
+// (rt, ct: 9 each, as LuSlot holds them).
+__device__ inline void lu_perm(const int rt[9], const int ct[9], LuPerm& P) {
   for (int i = 0; i < 9; ++i) P.prow[i] = i;
   for (int k = 0; k < 9; ++k) {
-    const int r = s.rt[k];
+    const int r = rt[k];
     const int t = P.prow[k];
     P.prow[k] = P.prow[r];
     P.prow[r] = t;
@@ -157,7 +158,7 @@ __device__ inline void lu_perm(const LuSlot& s, LuPerm& P) {
   // composed on an index list kept in P.qcol's storage first
   for (int i = 0; i < 9; ++i) P.qcol[i] = i;
   for (int k = 8; k >= 0; --k) {
-    const int q = s.ct[k];
+    const int q = ct[k];
     const int t = P.qcol[k];
     P.qcol[k] = P.qcol[q];
     P.qcol[q] = t;
@@ -170,20 +171,21 @@ __device__ inline void lu_perm(const LuSlot& s, LuPerm& P) {
 }
 
 // x = A^-1 b in place, for b in column `col` of the row-major matrix m with
-// `ld` columns (LDS or global): lu_solve's arithmetic, in the same order.
-__device__ inline void lu_solve_column(const LuSlot& s, const LuPerm& P, double* m, int ld, int col) {
+// `ld` columns (LDS or global), with the factors f (LuSlot::a's layout):
+// lu_solve's arithmetic, in the same order.
+__device__ inline void lu_solve_column(const double* f, const LuPerm& P, double* m, int ld, int col) {
   double c[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) c[i] = m[P.prow[i] * ld + col];
 #pragma unroll
   for (int i = 1; i < 9; ++i)
 #pragma unroll
-    for (int k = 0; k < i; ++k) c[i] = c[i] - s.a[i * 9 + k] * c[k];
+    for (int k = 0; k < i; ++k) c[i] = c[i] - f[i * 9 + k] * c[k];
 #pragma unroll
   for (int i = 8; i >= 0; --i) {
 #pragma unroll
-    for (int k = i + 1; k < 9; ++k) c[i] = c[i] - s.a[i * 9 + k] * c[k];
-    c[i] = c[i] / s.a[i * 9 + i];
+    for (int k = i + 1; k < 9; ++k) c[i] = c[i] - f[i * 9 + k] * c[k];
+    c[i] = c[i] / f[i * 9 + i];
   }
 #pragma unroll
   for (int j = 0; j < 9; ++j) m[P.qcol[j] * ld + col] = c[j];

@@ -151,6 +151,29 @@ def test_cost_order_changes_no_result():
         assert np.array_equal(by_cost[k], by_index[k]), k
 
 
+def test_cost_order_of_a_large_launch_changes_no_result():
+    """A launch of many wavefronts per SIMD takes the elements in one
+    descending run of expected work (a small one deals them round-robin): the
+    outputs again equal the index-order ones bit for bit (compared on the
+    device)."""
+    import torch
+    m = W.microbench_material()
+    ne = 262144
+    w = W.microbench(ne, seed=13)
+    st = G.states_from_reference(w["states"], m)
+    t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device="cuda:0")
+    out = []
+    for stages in (G.LOCAL_SOLVE | G.CONDENSATION, G.LOCAL_SOLVE | G.CONDENSATION | G.INDEX_ORDER):
+        b = G.ElementBatch(t(w["coords"]), t(w["conn"], torch.int32), t(w["u"]), t(st), None, 8, True)
+        G.evaluate_elements(m, G.Rule(), b, G.FINITE_LOGARITHMIC, True, stages=stages)
+        torch.cuda.synchronize()
+        out.append(b)
+    a, b = out
+    assert (a.yielding > 0).any()
+    for name in ("force", "tangent", "state_out", "alpha_out", "energy", "yielding", "iterations", "status"):
+        assert torch.equal(getattr(a, name), getattr(b, name)), name
+
+
 def test_refused_elements_report_status():
     """An inverted element is refused at set-up (MeshError), a step that
     inverts a point fails its first pass (SolverError); the others evaluate."""
