@@ -321,13 +321,7 @@ __device__ int machine_top(Machine& M, TeamSlot& T, const LocalSums& cur, int p,
 #pragma unroll
   for (int i = 0; i < 9; ++i) M.newton[i] = -nt[i];
   // sym(K_aa) positive definite?
-  for (int i = p; i < 81; i += 8) {
-    const int r = i / 9, c = i % 9;
-    T.w()[i] = 0.5 * (T.k0[r * 9 + c] + T.k0[c * 9 + r]);
-  }
-  team_sync();
-  const bool spd = llt9(T.w());
-  team_sync();
+  const bool spd = spd9(T.k0);
   if (spd) {
     // line_search(newton, r0), SPD context
     M.ls_ctx = kLsSpd;

@@ -24,20 +24,34 @@ struct LuSlot {
 // Eigen's FullPivLU pivoting: the largest |a_ij| of the remaining corner, the
 // first in column-major order on ties; invertible when every pivot exceeds
 // 9 eps times the largest pivot.
-__device__ inline bool coop_full_lu(LuSlot& s, int p) {
+// The factorisation is a function of its own (not inlined): its unrolled
+// code would otherwise change the register allocation of the kernel around
+// it. The slot is passed as an LDS pointer so that its accesses stay LDS
+// instructions across the call.
+using LdsLuSlot = __attribute__((address_space(3))) LuSlot;
+__device__ __noinline__ bool coop_full_lu_lds(LdsLuSlot* sp, int p) {
+  LdsLuSlot& s = *sp;
+  // Every loop is unrolled (the steps k included), so that all indices but
+  // the pivot's are constants and each phase issues its LDS loads together
+  // instead of waiting out one LDS round trip per element.
   double maxpivot = 0.0;
   int nonzero = 9;
+#pragma unroll
   for (int k = 0; k < 9; ++k) {
     // candidates: lane p scans column k + p, lane 0 also column k + 8 (k = 0)
     double best = -1.0;
     int bidx = 1 << 30;
+#pragma unroll
     for (int jj = 0; jj < 2; ++jj) {
       const int j = k + p + 8 * jj;
       if (j < 9 && (jj == 0 || p == 0)) {
+        double v[9];
+#pragma unroll
+        for (int i = k; i < 9; ++i) v[i] = fabs(s.a[i * 9 + j]);
+#pragma unroll
         for (int i = k; i < 9; ++i) {
-          const double v = fabs(s.a[i * 9 + j]);
           const int idx = j * 9 + i;
-          if (v > best || (v == best && idx < bidx)) { best = v; bidx = idx; }
+          if (v[i] > best || (v[i] == best && idx < bidx)) { best = v[i]; bidx = idx; }
         }
       }
     }
@@ -54,6 +68,7 @@ __device__ inline bool coop_full_lu(LuSlot& s, int p) {
     if (best == 0.0) {
       nonzero = k;
       if (p == 0)
+#pragma unroll
         for (int i = k; i < 9; ++i) { s.rt[i] = i; s.ct[i] = i; }
       break;
     }
@@ -62,6 +77,7 @@ __device__ inline bool coop_full_lu(LuSlot& s, int p) {
     if (p == 0) { s.rt[k] = br; s.ct[k] = bc; }
     // row swap k <-> br: lane p swaps column p (lane 0 also column 8)
     if (br != k) {
+#pragma unroll
       for (int jj = 0; jj < 2; ++jj) {
         const int j = p + 8 * jj;
         if (j < 9 && (jj == 0 || p == 0)) {
@@ -73,6 +89,7 @@ __device__ inline bool coop_full_lu(LuSlot& s, int p) {
     }
     team_sync();
     if (bc != k) {
+#pragma unroll
       for (int ii = 0; ii < 2; ++ii) {
         const int i = p + 8 * ii;
         if (i < 9 && (ii == 0 || p == 0)) {
@@ -86,9 +103,13 @@ __device__ inline bool coop_full_lu(LuSlot& s, int p) {
     if (k < 8) {
       const int i = k + 1 + p;
       if (i < 9) {
-        const double l = s.a[i * 9 + k] / s.a[k * 9 + k];
+        double rowk[9], rowi[9];
+#pragma unroll
+        for (int j = k; j < 9; ++j) { rowk[j] = s.a[k * 9 + j]; rowi[j] = s.a[i * 9 + j]; }
+        const double l = rowi[k] / rowk[k];
         s.a[i * 9 + k] = l;
-        for (int j = k + 1; j < 9; ++j) s.a[i * 9 + j] = s.a[i * 9 + j] - l * s.a[k * 9 + j];
+#pragma unroll
+        for (int j = k + 1; j < 9; ++j) s.a[i * 9 + j] = rowi[j] - l * rowk[j];
       }
     }
     team_sync();
@@ -97,9 +118,17 @@ __device__ inline bool coop_full_lu(LuSlot& s, int p) {
   team_sync();
   const double thr = maxpivot * (kEps * 9.0);
   int rank = 0;
-  for (int i = 0; i < nonzero; ++i)
-    if (fabs(s.a[i * 9 + i]) > thr) ++rank;
+  double d[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) d[i] = s.a[i * 9 + i];
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+    if (i < nonzero && fabs(d[i]) > thr) ++rank;
   return rank == 9;
+}
+
+__device__ __forceinline__ bool coop_full_lu(LuSlot& s, int p) {
+  return coop_full_lu_lds((LdsLuSlot*)(&s), p);
 }
 
 // Solve A x = b with the factors (every lane, redundantly).
@@ -253,6 +282,44 @@ __device__ inline bool llt9(double* w) {
   }
   return true;
 }
+
+// Whether sym(K) = (K + K^T) / 2 of the row-major K in LDS is positive
+// definite: llt9's operations on it, in the same order, in registers (every
+// lane; only the verdict is kept). Not inlined, for the reason coop_full_lu
+// is not.
+using LdsDouble = __attribute__((address_space(3))) double;
+__device__ __noinline__ bool spd9_lds(const LdsDouble* k) {
+  double w[9][9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) w[i][j] = 0.5 * (k[i * 9 + j] + k[j * 9 + i]);
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    double x = w[c][c];
+    if (c > 0) {
+      double s = w[c][0] * w[c][0];
+#pragma unroll
+      for (int j = 1; j < c; ++j) s = s + w[c][j] * w[c][j];
+      x = x - s;
+    }
+    if (x <= 0.0) return false;
+    x = sqrt(x);
+    w[c][c] = x;
+    if (c > 0)
+#pragma unroll
+      for (int i = c + 1; i < 9; ++i) {
+        double s = w[i][0] * w[c][0];
+#pragma unroll
+        for (int j = 1; j < c; ++j) s = s + w[i][j] * w[c][j];
+        w[i][c] = w[i][c] - s;
+      }
+#pragma unroll
+    for (int i = c + 1; i < 9; ++i) w[i][c] = w[i][c] / x;
+  }
+  return true;
+}
+__device__ __forceinline__ bool spd9(const double* k) { return spd9_lds((const LdsDouble*)k); }
 
 __device__ inline void llt9_solve(const double* w, const double b[9], double x[9]) {
   double y[9];
