@@ -36,6 +36,26 @@ FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_1M_microbench_and_assembly.md")
+
+
+def measured_traffic(kernel, elements):
+    """HBM bytes per launch of `kernel` on the `elements`-element workload,
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by
+    the copy calibration recorded there), or None when it holds no figure for
+    this workload."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 4 and parts[0] == "TRAFFIC" and parts[1] == kernel and \
+                        parts[2] == f"elements={elements}" and parts[3].startswith("bytes="):
+                    return float(parts[3][len("bytes="):])
+    except OSError:
+        return None
+    return None
+
+
 def cpu_baseline_leg(workload, sample, threads):
     """The oracle on the host: evals/s on `threads` cores over the first
     `sample` elements, and the algorithmic FLOPs per element (op-counting
@@ -350,7 +370,9 @@ def main():
         achieved = fl * ne / (ms_local * 1e-3) / 1e12
         roof = {"bound": "mfma", "pipe": "fp64 (vector ALU; MI355X FP64 vector peak = FP64 matrix peak)",
                 "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic("local_solve_kernel", ne),
+                "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated; "
+                                "profiles/hbm_traffic_1M_microbench_and_assembly.md)",
                 "kernel": "local_solve_kernel (the dominant kernel)",
                 "flops_per_elem": fl, "flops_sample": cb["flops_sample"],
                 "kernel_ms": ms_local,
