@@ -1164,7 +1164,14 @@ void launch_local_solve_kin(const ElementArgs& a, bool two_terms, unsigned int b
 int launch_local_solve(const ElementArgs& args, hipStream_t stream) {
   (void)hipGetLastError();
   ElementArgs a = args;
-  if (a.chunk <= 0) a.chunk = kDefaultChunk;
+  if (a.chunk <= 0) {
+    // Elements per wavefront when the caller sets none: 64, or fewer for a
+    // launch too small to give every SIMD's wavefront slot two chunks (a small
+    // mesh would otherwise run on a few wavefronts, each working through 64
+    // elements in turn); a multiple of 8, at least 8.
+    const long long per = a.ne / (2ll * resident_waves());
+    a.chunk = per >= kDefaultChunk ? kDefaultChunk : (per < kWaveTeams ? kWaveTeams : static_cast<int>(per / kWaveTeams) * kWaveTeams);
+  }
   if (a.chunk % kWaveTeams != 0) return static_cast<int>(hipErrorInvalidValue);
   const unsigned int blocks = static_cast<unsigned int>((a.ne + a.chunk - 1) / a.chunk);
   if (blocks == 0) return 0;

@@ -55,7 +55,8 @@ struct ElementArgs {
   int* cost;
   int* order;
   int* counts;
-  // elements per wavefront of the local solve (a multiple of 8; 0: 64)
+  // elements per wavefront of the local solve (a multiple of 8; 0: 64, or
+  // fewer for a launch too small to fill the device)
   int chunk;
   // 1: the local solve takes the elements in `order` (most work first);
   // 0: in index order
