@@ -1,0 +1,224 @@
+"""The plain CPU-runnable forming case (the reference's C1 configuration)
+through the device path inside a Newton loop, against the oracle through the
+same loop.
+
+Blank 20 x 20 x 1 mm, 20 x 20 x 2 incompatible-mode Hex8 (800 elements,
+1 323 nodes), edges clamped, E = 70 GPa, nu = 0.33, J2 with Voce hardening
+sigma_y = 95 + 150 (1 - exp(-10 eps_p)) MPa, logarithmic strain; a rigid
+sphere R = 5 mm pushed 1 mm at the centre in 10 increments against every
+top-surface node (penalty kappa A = 1e7 N/m per node, i.e. k = 1e4 N/mm over
+1 mm^2), Coulomb friction 0.1 with the exact tangent; then the release: the
+tool removed and its final force on the blank taken off as a load in 10 steps,
+with the clamp held.
+
+The Newton driver is test code (the reference's driver is src/fem/
+Forming.cpp:891-1140 and stays out of the device path): residual = system
+residual + contact residual, tangent = assembled K + the contact node blocks
+(Forming.cpp:747, 773, 988), solved on the free DOFs with scipy, with a
+backtracking step on the residual norm. The device
+side is NonlinearSystem.evaluate (element kernels, the reference's
+column-major CSC values) + contact + add_node_blocks; the oracle side is its
+element loop, scatter, force gather, contact and add_node_block. Bars: the
+residual norms of every iteration agree to 1e-6 relative while they are above
+1e-7 of the increment's first residual (below it both sides are at round-off),
+the iteration counts agree, and the top surface's heights after the release
+agree within 1e-6 mm. (The reference halves an increment whose evaluation
+fails; this driver's steps are small enough that none does.)
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle_lib as O
+import sparlab_gpu as G
+from parity import oracle_material
+from sparlab_gpu import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+R = 0.005
+STEP = 1.0e-4          # 1 mm in 10 increments
+KAPPA_A = 1.0e7        # N/m per slave node
+FRICTION = 0.1
+
+
+def c1_model():
+    coords, conn = W.structured_hex_mesh(20, 20, 2, 0.02, 0.02, 0.001, -0.01, -0.01, -0.001)
+    m = G.Material(70.0e9, 0.33, yield_stress=95.0e6, saturation_stress=150.0e6, saturation_rate=10.0)
+    top = np.nonzero(np.isclose(coords[:, 2], 0.0))[0].astype(np.int32)
+    edge = np.isclose(np.abs(coords[:, 0]), 0.01) | np.isclose(np.abs(coords[:, 1]), 0.01)
+    free = np.ones(3 * coords.shape[0], dtype=bool)
+    for k in range(3):
+        free[3 * np.nonzero(edge)[0] + k] = False
+    return coords, conn, m, top, free
+
+
+class ElementFailure(RuntimeError):
+    """An element evaluation failed (the reference's SolverError)."""
+
+
+class DeviceSide:
+    def __init__(self, coords, conn, m, top):
+        import torch
+        self.torch = torch
+        self.dev = torch.device("cuda:0")
+        self.coords, self.conn, self.top = coords, conn, top
+        self.sys = G.NonlinearSystem(coords, conn, m)
+        t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=self.dev)
+        self.t = t
+        self.d_coords = t(coords.ravel())
+        self.d_nodes = t(top, torch.int32)
+        self.d_kappa = t(np.full(top.size, KAPPA_A))
+        self.hist = (t(np.zeros(top.size), torch.int32), t(np.zeros(3 * top.size)), t(np.zeros(3 * top.size)))
+        row_ptr, _ = self.sys.pattern()
+        self.first = t(self.sys.diagonal_blocks()[top], torch.int32)
+        self.row_len = t((row_ptr[3 * top + 1] - row_ptr[3 * top]).astype(np.int32), torch.int32)
+        self.trial_hist = None
+
+    def evaluate(self, u, centre, centre0, start, tool_on=True):
+        torch = self.torch
+        try:
+            ev = self.sys.evaluate(u.reshape(-1, 3), want_tangent=True, column_major=True)
+        except G.SolverError as err:
+            raise ElementFailure(err.element) from err
+        r = ev.internal.copy()
+        values = ev.tangent_values.copy()
+        if tool_on:
+            tool = G.make_tool("sphere", centre, centre0, radius=R, friction=FRICTION)
+            out = G.contact_evaluate_device(tool, self.d_coords, self.t(u.ravel()), self.t(start.ravel()), self.d_nodes,
+                                            self.d_kappa, history=self.hist)
+            d_values = self.t(values)
+            G.add_node_blocks_device(out["blocks"], out["active"], self.first, self.row_len, d_values, column_major=True)
+            torch.cuda.synchronize()
+            values = d_values.cpu().numpy()
+            self.contact_residual = out["residual"].cpu().numpy()
+            r = r + self.contact_residual
+            self.trial_hist = (out["trial_present"].clone(), out["trial_force"].reshape(-1).clone(),
+                               out["trial_relative"].reshape(-1).clone())
+        return r, values
+
+    def commit(self, tool_on=True):
+        self.sys.commit()
+        if tool_on and self.trial_hist is not None:
+            self.hist = self.trial_hist
+
+
+class OracleSide:
+    def __init__(self, coords, conn, m, top):
+        self.coords, self.conn, self.top = coords, conn, top
+        self.om = oracle_material(m)
+        self.nn, self.ne = coords.shape[0], conn.shape[0]
+        self.states = np.zeros((self.ne, 8, 39))
+        self.alpha = np.zeros((self.ne, 9))
+        self.hist = (np.zeros(top.size, np.int32), np.zeros((top.size, 3)), np.zeros((top.size, 3)))
+        self.tool = None
+        self.trial = None
+
+    def evaluate(self, u, centre, centre0, start, tool_on=True):
+        x0 = self.coords[self.conn]
+        ue = u.reshape(-1, 3)[self.conn].reshape(self.ne, 24)
+        ref = O.element_batch(x0, ue, self.om, self.states, alpha=self.alpha, kinematics=2, threads=8)
+        if (ref["status"] != 0).any():
+            raise ElementFailure(int(np.nonzero(ref["status"])[0][0]))
+        r, _ = O.gather_force(self.nn, self.conn, ref["force"])
+        values = O.scatter(self.nn, self.conn, ref["tangent"].reshape(self.ne, 576))
+        hist = self.hist
+        if tool_on:
+            tool = O.tool_record(1, centre, centre0, radius=R, friction=FRICTION)
+            c = O.contact(tool, self.coords, u.ravel(), start.ravel(), self.top, np.full(self.top.size, KAPPA_A),
+                          history=hist)
+            act = c["active"] == 1
+            values = O.add_node_blocks(self.nn, self.conn, values, self.top[act], c["blocks"][act].reshape(-1, 9))
+            self.contact_residual = c["residual"]
+            r = r + c["residual"]
+            hist = (c["t_present"], c["t_force"], c["t_relative"])
+        self.trial = (ref["states"], ref["alpha"], hist)
+        return r, values
+
+    def commit(self, tool_on=True):
+        self.states, self.alpha, hist = self.trial
+        if tool_on:
+            self.hist = hist
+
+
+def newton(side, u0, free, outer, inner, centre, centre0, tool_on=True, max_it=25, load=None):
+    """One increment: Newton on the free DOFs until the residual falls below
+    1e-9 of the increment's first residual (or to its round-off floor); a
+    step whose evaluation fails or whose residual is not smaller is halved
+    (up to 16 times). Returns the converged u, the residual norms and the
+    step lengths taken."""
+    n = u0.size
+    u = u0.copy()
+    def evaluate(x):
+        r, values = side.evaluate(x, centre, centre0, u0, tool_on)
+        return (r if load is None else r + load), values
+
+    r, values = evaluate(u)
+    norms = [float(np.linalg.norm(r[free]))]
+    steps = []
+    for it in range(max_it):
+        rn = norms[-1]
+        if it > 0 and (rn <= 1e-9 * norms[0] or (it > 2 and rn >= 0.5 * norms[-2] and rn <= 1e-6 * norms[0])):
+            break
+        K = sp.csc_matrix((values, inner, outer), shape=(n, n))
+        du = spla.spsolve(K[free][:, free].tocsc(), -r[free])
+        step = 1.0
+        for _ in range(16):
+            trial = u.copy()
+            trial[free] += step * du
+            try:
+                rt, vt = evaluate(trial)
+            except ElementFailure:
+                step *= 0.5
+                continue
+            if np.linalg.norm(rt[free]) < rn:
+                u, r, values = trial, rt, vt
+                break
+            step *= 0.5
+        else:
+            raise ElementFailure(-1)
+        steps.append(step)
+        norms.append(float(np.linalg.norm(r[free])))
+    side.commit(tool_on)
+    return u, norms, steps
+
+
+def test_c1_newton_histories_and_release_heights():
+    coords, conn, m, top, free = c1_model()
+    nn = coords.shape[0]
+    outer, inner, _ = O.pattern(nn, conn)
+    dev = DeviceSide(coords, conn, m, top)
+    ora = OracleSide(coords, conn, m, top)
+    row_ptr, col_idx = dev.sys.pattern()
+    assert np.array_equal(row_ptr, outer) and np.array_equal(col_idx, inner)
+    u_d = np.zeros(3 * nn)
+    u_o = np.zeros(3 * nn)
+    centre0 = (0.0, 0.0, R)
+    for inc in range(1, 11):
+        centre = (0.0, 0.0, R - STEP * inc)
+        u_d, h_d, s_d = newton(dev, u_d, free, outer, inner, centre, centre0)
+        u_o, h_o, s_o = newton(ora, u_o, free, outer, inner, centre, centre0)
+        assert len(h_d) == len(h_o) and s_d == s_o, (inc, h_d, h_o, s_d, s_o)
+        for a, b in zip(h_d, h_o):
+            if b > 1e-7 * h_o[0]:
+                assert abs(a - b) <= 1e-6 * b, (inc, h_d, h_o)
+        centre0 = centre
+    # yielded, and the tool pressed the blank
+    assert ora.states[..., 36].max() > 1e-3  # equivalent plastic strain
+    assert u_o[3 * top + 2].min() < -5e-4
+    # release: the tool removed, its final force on the blank kept as a load
+    # and taken off in 10 steps, the clamp held
+    load_d, load_o = dev.contact_residual.copy(), ora.contact_residual.copy()
+    assert np.abs(load_d - load_o).max() <= 1e-9 * np.abs(load_o).max()
+    for step in range(9, -1, -1):
+        lam = step / 10.0
+        u_d, h_d, s_d = newton(dev, u_d, free, outer, inner, centre0, centre0, tool_on=False, load=lam * load_d)
+        u_o, h_o, s_o = newton(ora, u_o, free, outer, inner, centre0, centre0, tool_on=False, load=lam * load_o)
+        assert len(h_d) == len(h_o) and s_d == s_o, ("release", step, h_d, h_o, s_d, s_o)
+        for a, b in zip(h_d, h_o):
+            if b > 1e-7 * h_o[0]:
+                assert abs(a - b) <= 1e-6 * b, ("release", step, h_d, h_o)
+    heights_d = coords[top, 2] + u_d[3 * top + 2]
+    heights_o = coords[top, 2] + u_o[3 * top + 2]
+    assert np.abs(heights_d - heights_o).max() <= 1e-9  # 1e-6 mm
