@@ -128,6 +128,9 @@ int sparlab_state_fields(const sparlab_material* material);
  *   a residual-only evaluation); energy: per element; yielding, iterations,
  *   status: int32 per element; fail_value: per element (the refused Jacobian
  *   determinant for SPARLAB_ELEMENT_*_JACOBIAN).
+ *   A failed element (status != 0) outputs zero force, tangent, energy and
+ *   yielding, alpha_in as alpha_out and state_in as state_out (its trial is
+ *   rejected, as after the reference's SolverError).
  * workspace: sparlab_element_workspace_doubles(num_elements, rule) doubles
  *   of device scratch (each point's stress and tangent of its final local
  *   pass and each element's K_aa factors, passed from the local solve to the

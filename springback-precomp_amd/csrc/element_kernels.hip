@@ -250,6 +250,24 @@ __device__ __forceinline__ int team_local_pass(const ElementArgs& A, const DevMa
   return 0;
 }
 
+// The outputs of a failed element: no energy, no yielding points, the
+// committed modes and history (the reference rejects the trial after a
+// SolverError); the condensation zeroes its force and tangent. Every output
+// of a failed element is thereby fixed, whatever its team did before.
+__device__ __forceinline__ void failed_element_outputs(const ElementArgs& A, long long e, int p) {
+  A.alpha_out[e * 9 + p] = A.alpha_in ? A.alpha_in[e * 9 + p] : 0.0;
+  if (p == 0) {
+    A.alpha_out[e * 9 + 8] = A.alpha_in ? A.alpha_in[e * 9 + 8] : 0.0;
+    A.energy[e] = 0.0;
+    A.yielding[e] = 0;
+  }
+  if (p < A.rule.n) {
+    const long long q = e * A.rule.n + p;
+    const int fields = state_fields(A.mat.kin_fields);
+    for (int f = 0; f < fields; ++f) A.state_out[f * A.state_stride + q] = A.state_in[f * A.state_stride + q];
+  }
+}
+
 // Chunk c of a balanced partition of ne positions into w chunks starts at
 // chunk_start(ne, w, c): the first ne % w chunks hold one position more.
 __device__ __forceinline__ long long chunk_start(long long ne, long long w, long long c) {
@@ -689,6 +707,7 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
       const int code = team_imin(st != kOk ? (p << 8) | st : 1 << 20);
       if (code != (1 << 20)) {
         if (p == (code >> 8)) { A.status[e] = code & 0xFF; A.fail_value[e] = fv; }
+        failed_element_outputs(A, e, p);
         continue;  // the team stays idle and takes another element
       }
       if (p == 0) {
@@ -742,6 +761,7 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
           A.fail_value[e] = value;
           A.iterations[e] = M.iterations;
         }
+        failed_element_outputs(A, e, p);
         team_sync();
         if (p == 0) M.phase = kIdle;
       }
@@ -763,12 +783,15 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
 // round-robin in descending order (rank r to chunk r mod w, at slot r / w), so
 // that every chunk holds the same mix of work and no wavefront outlasts the
 // others. The expected work of an element is the sum over its points of
-// return_cost at the committed modes (the elastic trial of the first pass).
+// return_cost at the committed modes (the elastic trial of the first pass,
+// with the Green-Lagrange strain standing in for the logarithmic one).
 // The order changes no result: every element's evaluation is independent of
 // the others.
 
-// The strain of a point at alpha (the head of point_response); false for an
-// inverted or degenerate point, which the local solve itself reports.
+// The strain of a point at alpha, for the schedule only: the head of
+// point_response, except that logarithmic kinematics take the Green-Lagrange
+// strain (E_log's first-order approximation) instead of its eigen-decomposition.
+// False for an inverted point, which the local solve itself reports.
 template <int KIN>
 __device__ __forceinline__ bool trial_strain(const ModeGeom& mg, const double alpha[9], double strain[6]) {
   double h[3][3], F[3][3];
@@ -794,16 +817,12 @@ __device__ __forceinline__ bool trial_strain(const ModeGeom& mg, const double al
         const double hth = h[0][i] * h[0][j] + h[1][i] * h[1][j] + h[2][i] * h[2][j];
         e3[i][j] = 0.5 * ((h[i][j] + h[j][i]) + hth);
       }
-    double green[6] = {e3[0][0], e3[1][1], e3[2][2], 2.0 * e3[0][1], 2.0 * e3[1][2], 2.0 * e3[2][0]};
-    if constexpr (KIN == kLog) {
-      LogData L;
-      if (logarithmic_strain(green, L) != 0) return false;
-#pragma unroll
-      for (int i = 0; i < 6; ++i) strain[i] = L.strain[i];
-    } else {
-#pragma unroll
-      for (int i = 0; i < 6; ++i) strain[i] = green[i];
-    }
+    strain[0] = e3[0][0];
+    strain[1] = e3[1][1];
+    strain[2] = e3[2][2];
+    strain[3] = 2.0 * e3[0][1];
+    strain[4] = 2.0 * e3[1][2];
+    strain[5] = 2.0 * e3[2][0];
     return true;
   }
 }
@@ -923,7 +942,13 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
   CondenseSlot& T = slots[t];
   const long long e = static_cast<long long>(blockIdx.x) * kWaveTeams + t;
   if (e >= A.ne) return;
-  if (A.status[e] != kOk) return;
+  if (A.status[e] != kOk) {
+    // a failed element: zero force and tangent (lane a: rows 3a..3a+2)
+    for (int k = 0; k < 3; ++k) A.force[e * 24 + 3 * p + k] = 0.0;
+    if constexpr (TANGENT)
+      for (int i = 0; i < 72; ++i) A.tangent[e * 576 + 72 * p + i] = 0.0;
+    return;
+  }
   const int n = A.rule.n;
   const long long q0 = e * n;
   // geometry of the lane's point

@@ -192,3 +192,40 @@ def test_refused_elements_report_status():
     assert gpu["status"][0] == 0
     assert gpu["status"][1] in (1, 2)
     assert gpu["status"][2] == 3
+
+
+@pytest.mark.parametrize("material", ["J2", "Chaboche"])
+def test_refused_elements_output_the_committed_state(material):
+    """A refused element (at set-up, or in a pass) outputs zero force,
+    tangent, energy and yielding, its committed modes and its committed
+    history, whatever its output buffers held before (filled with NaN here)."""
+    import torch
+    m = materials()[material]
+    coords, conn, u = single_hex_batch([0.01, 0.01, 0.01])
+    coords = coords.copy()
+    top = coords[8 + 4:8 + 8].copy()
+    coords[8 + 4:8 + 8] = coords[8 + 0:8 + 4]
+    coords[8 + 0:8 + 4] = top
+    u = u.copy()
+    u[16:24, 2] = -3.0 * DISTORTED[:, 2]
+    rng = np.random.default_rng(5)
+    st = np.zeros((m.state_fields, 3 * 8))
+    st[:, 8:] = rng.uniform(0.0, 1e-3, (m.state_fields, 16))
+    alpha = np.zeros((3, 9))
+    alpha[1:] = rng.uniform(-1e-4, 1e-4, (2, 9))
+    t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device="cuda:0")
+    b = G.ElementBatch(t(coords), t(conn, torch.int32), t(u), t(st), t(alpha), 8, True)
+    for name in ("state_out", "alpha_out", "force", "tangent", "energy"):
+        getattr(b, name).fill_(float("nan"))
+    b.yielding.fill_(-1)
+    G.evaluate_elements(m, G.Rule(), b, G.FINITE_LOGARITHMIC, True)
+    torch.cuda.synchronize()
+    status = b.status.cpu().numpy()
+    assert status[0] == 0 and status[1] in (1, 2) and status[2] == 3
+    state_out = b.state_out.cpu().numpy()
+    for e in (1, 2):
+        assert (b.force[e] == 0).all() and (b.tangent[e] == 0).all()
+        assert b.energy[e].item() == 0.0 and b.yielding[e].item() == 0
+        assert np.array_equal(b.alpha_out[e].cpu().numpy(), alpha[e])
+        assert np.array_equal(state_out[:, 8 * e:8 * e + 8], st[:, 8 * e:8 * e + 8])
+    assert np.isfinite(b.force[0].cpu().numpy()).all() and np.isfinite(state_out[:, :8]).all()
