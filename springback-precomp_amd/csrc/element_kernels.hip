@@ -931,8 +931,9 @@ struct CondenseSlot {
 // workspace; then lane a owns rows 3a..3a+2 of f*, K_ua, K_uu and K* (and
 // columns 3a..3a+2 of K_au), each accumulated point by point in point order
 // from the geometry records and the point records of the final pass. The
-// accumulations run as separate point loops (K_ua; K_au; K_uu in two column
-// halves) so that few accumulators are live at a time.
+// accumulations run as three point loops (f_u, K_ua and K_au together; K_uu
+// in two column halves after the solve): each loop re-reads the records, and
+// fewer loops would hold more accumulators than the registers take.
 template <int KIN, bool TANGENT, bool SYM>
 __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
   __shared__ CondenseSlot slots[kWaveTeams];
@@ -988,12 +989,15 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
   team_sync_global();  // the geometry records (global) and the factors (LDS)
   if (p == 0) lu_perm(T.lu.rt, T.lu.ct, T.perm);
   const int a = p;
-  // f_u rows and K_ua of node a
   double ru[3], kua[3][9];
-  coupling_rows<KIN>(A, q0, n, a, ru, kua);
+  if constexpr (!TANGENT || SYM) {
+    // f_u rows and K_ua of node a
+    coupling_rows<KIN>(A, q0, n, a, ru, kua);
+  }
   if constexpr (TANGENT) {
     // the coupling columns 3a..3a+2: K_ua^T, or K_au with
-    // K_au[:, 3a + k] = sum_q w B_a^T mat B_u[:, 3a + k] (+ geometric)
+    // K_au[:, 3a + k] = sum_q w B_a^T mat B_u[:, 3a + k] (+ geometric),
+    // accumulated in the same point loop as f_u and K_ua
     double col[3][9];
     if constexpr (SYM) {
 #pragma unroll
@@ -1002,17 +1006,27 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
         for (int nn = 0; nn < 9; ++nn) col[k][nn] = kua[k][nn];
     } else {
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
+      for (int k = 0; k < 3; ++k) {
+        ru[k] = 0.0;
 #pragma unroll
-        for (int nn = 0; nn < 9; ++nn) col[k][nn] = 0.0;
+        for (int nn = 0; nn < 9; ++nn) { kua[k][nn] = 0.0; col[k][nn] = 0.0; }
+      }
       for (int q = 0; q < n; ++q) {
         const double* gr = A.geom_ws + (q0 + q) * kGeomRecord;
         const double* pr = A.point_ws + (q0 + q) * kPointRecord;
         const double w = gr[kGrW];
         double bu[6][3];
         node_operator(gr, a, bu);
-        // y = mat B_u[:, 3a + k]
-        double y[3][6];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          double s = bu[0][k] * pr[kWsStress];
+#pragma unroll
+          for (int c = 1; c < 6; ++c) s = s + bu[c][k] * pr[kWsStress + c];
+          ru[k] = ru[k] + w * s;
+        }
+        // z = B_u[:, 3a + k]^T mat, y = mat B_u[:, 3a + k]
+        double z[3][6], y[3][6];
+        node_times_tangent(bu, pr, z);
 #pragma unroll
         for (int c = 0; c < 6; ++c)
 #pragma unroll
@@ -1029,10 +1043,14 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
           for (int c = 0; c < 6; ++c) bcol[c] = bnl_rec(gr, c, 8 + nn / 3, nn % 3);
 #pragma unroll
           for (int k = 0; k < 3; ++k) {
-            double s = bcol[0] * y[k][0];
+            double s = z[k][0] * bcol[0];
 #pragma unroll
-            for (int c = 1; c < 6; ++c) s = s + bcol[c] * y[k][c];
-            col[k][nn] = col[k][nn] + w * s;
+            for (int d = 1; d < 6; ++d) s = s + z[k][d] * bcol[d];
+            kua[k][nn] = kua[k][nn] + w * s;
+            double t = bcol[0] * y[k][0];
+#pragma unroll
+            for (int c = 1; c < 6; ++c) t = t + bcol[c] * y[k][c];
+            col[k][nn] = col[k][nn] + w * t;
           }
         }
         if constexpr (KIN != kSmall) {
@@ -1044,7 +1062,10 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
             const double geo = gsa[0] * g[0 * 11 + 8 + bm] + gsa[1] * g[1 * 11 + 8 + bm] + gsa[2] * g[2 * 11 + 8 + bm];
             const double v = w * geo;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) col[k][3 * bm + k] = col[k][3 * bm + k] + v;
+            for (int k = 0; k < 3; ++k) {
+              kua[k][3 * bm + k] = kua[k][3 * bm + k] + v;
+              col[k][3 * bm + k] = col[k][3 * bm + k] + v;
+            }
           }
         }
       }
