@@ -29,11 +29,28 @@ struct LuSlot {
 // it. The slot is passed as an LDS pointer so that its accesses stay LDS
 // instructions across the call.
 using LdsLuSlot = __attribute__((address_space(3))) LuSlot;
+
+// A permutation of 0..8 packed 4 bits per entry (entry i in bits 4i..4i+3).
+constexpr unsigned long long kIdentityPerm = 0x876543210ull;
+__device__ __forceinline__ int perm_at(unsigned long long P, int i) {
+  return static_cast<int>((P >> (4 * i)) & 15ull);
+}
+__device__ __forceinline__ unsigned long long perm_swap(unsigned long long P, int i, int j) {
+  const unsigned long long vi = (P >> (4 * i)) & 15ull, vj = (P >> (4 * j)) & 15ull;
+  P &= ~((15ull << (4 * i)) | (15ull << (4 * j)));
+  return P | (vi << (4 * j)) | (vj << (4 * i));
+}
+
 __device__ __noinline__ bool coop_full_lu_lds(LdsLuSlot* sp, int p) {
   LdsLuSlot& s = *sp;
   // Every loop is unrolled (the steps k included), so that all indices but
   // the pivot's are constants and each phase issues its LDS loads together
-  // instead of waiting out one LDS round trip per element.
+  // instead of waiting out one LDS round trip per element. Rows and columns
+  // are not swapped in memory: entry (i, j) of the partly factored matrix
+  // stays at (R[i], C[j]) of the slot, R and C in registers (the same on the
+  // team's lanes), and one gather at the end stores the factors in place;
+  // the arithmetic is that of the swapped matrix, operation for operation.
+  unsigned long long R = kIdentityPerm, C = kIdentityPerm;
   double maxpivot = 0.0;
   int nonzero = 9;
 #pragma unroll
@@ -45,9 +62,10 @@ __device__ __noinline__ bool coop_full_lu_lds(LdsLuSlot* sp, int p) {
     for (int jj = 0; jj < 2; ++jj) {
       const int j = k + p + 8 * jj;
       if (j < 9 && (jj == 0 || p == 0)) {
+        const int cj = perm_at(C, j);
         double v[9];
 #pragma unroll
-        for (int i = k; i < 9; ++i) v[i] = fabs(s.a[i * 9 + j]);
+        for (int i = k; i < 9; ++i) v[i] = fabs(s.a[perm_at(R, i) * 9 + cj]);
 #pragma unroll
         for (int i = k; i < 9; ++i) {
           const int idx = j * 9 + i;
@@ -65,7 +83,8 @@ __device__ __noinline__ bool coop_full_lu_lds(LdsLuSlot* sp, int p) {
       else { ov = dpp_f64<kHalfMirror>(best); oi = dpp_i32<kHalfMirror>(bidx); }
       if (ov > best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
     }
-    if (best == 0.0) {
+    if (!(best > 0.0)) {
+      // a zero corner (or no candidate at all: a NaN matrix)
       nonzero = k;
       if (p == 0)
 #pragma unroll
@@ -75,44 +94,45 @@ __device__ __noinline__ bool coop_full_lu_lds(LdsLuSlot* sp, int p) {
     if (best > maxpivot) maxpivot = best;
     const int br = bidx % 9, bc = bidx / 9;
     if (p == 0) { s.rt[k] = br; s.ct[k] = bc; }
-    // row swap k <-> br: lane p swaps column p (lane 0 also column 8)
-    if (br != k) {
-#pragma unroll
-      for (int jj = 0; jj < 2; ++jj) {
-        const int j = p + 8 * jj;
-        if (j < 9 && (jj == 0 || p == 0)) {
-          const double t = s.a[k * 9 + j];
-          s.a[k * 9 + j] = s.a[br * 9 + j];
-          s.a[br * 9 + j] = t;
-        }
-      }
-    }
-    team_sync();
-    if (bc != k) {
-#pragma unroll
-      for (int ii = 0; ii < 2; ++ii) {
-        const int i = p + 8 * ii;
-        if (i < 9 && (ii == 0 || p == 0)) {
-          const double t = s.a[i * 9 + k];
-          s.a[i * 9 + k] = s.a[i * 9 + bc];
-          s.a[i * 9 + bc] = t;
-        }
-      }
-    }
-    team_sync();
+    R = perm_swap(R, k, br);
+    C = perm_swap(C, k, bc);
     if (k < 8) {
       const int i = k + 1 + p;
       if (i < 9) {
+        const int rk = perm_at(R, k), ri = perm_at(R, i);
         double rowk[9], rowi[9];
 #pragma unroll
-        for (int j = k; j < 9; ++j) { rowk[j] = s.a[k * 9 + j]; rowi[j] = s.a[i * 9 + j]; }
+        for (int j = k; j < 9; ++j) {
+          const int cj = perm_at(C, j);
+          rowk[j] = s.a[rk * 9 + cj];
+          rowi[j] = s.a[ri * 9 + cj];
+        }
         const double l = rowi[k] / rowk[k];
-        s.a[i * 9 + k] = l;
+        s.a[ri * 9 + perm_at(C, k)] = l;
 #pragma unroll
-        for (int j = k + 1; j < 9; ++j) s.a[i * 9 + j] = rowi[j] - l * rowk[j];
+        for (int j = k + 1; j < 9; ++j) s.a[ri * 9 + perm_at(C, j)] = rowi[j] - l * rowk[j];
       }
     }
     team_sync();
+  }
+  // the factors in place: lane p row p (lane 0 also row 8)
+  double rows[2][9];
+#pragma unroll
+  for (int ii = 0; ii < 2; ++ii) {
+    const int i = p + 8 * ii;
+    if (i < 9 && (ii == 0 || p == 0)) {
+      const int ri = perm_at(R, i);
+#pragma unroll
+      for (int j = 0; j < 9; ++j) rows[ii][j] = s.a[ri * 9 + perm_at(C, j)];
+    }
+  }
+  team_sync();
+#pragma unroll
+  for (int ii = 0; ii < 2; ++ii) {
+    const int i = p + 8 * ii;
+    if (i < 9 && (ii == 0 || p == 0))
+#pragma unroll
+      for (int j = 0; j < 9; ++j) s.a[i * 9 + j] = rows[ii][j];
   }
   if (p == 0) { s.nonzero = nonzero; s.maxpivot = maxpivot; }
   team_sync();
