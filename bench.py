@@ -82,6 +82,41 @@ def cpu_baseline_leg(workload, sample, threads):
                 plastic_fraction=float((r["scalars"][:, 1] > 0).mean()))
 
 
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_single_core_leg(workload, sample):
+    """The oracle's element loop on one pinned host core over the first
+    `sample` elements: microseconds per evaluation, the figure comparable
+    with a single-threaded element loop."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from parity import oracle_material
+    from sparlab_gpu import workloads as W
+
+    om = oracle_material(W.microbench_material())
+    x0 = workload["coords"][workload["conn"][:sample]]
+    ue = workload["u"][workload["conn"][:sample]].reshape(sample, 24)
+    st = workload["states"][:sample]
+    cpus = sorted(os.sched_getaffinity(0))
+    os.sched_setaffinity(0, {cpus[0]})
+    try:
+        t0 = time.perf_counter()
+        O.element_batch(x0, ue, om, st, kinematics=2, want_tangent=True, threads=1)
+        dt = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, set(cpus))
+    return dict(evals_per_s=sample / dt, us_per_eval=dt / sample * 1e6, seconds=dt, core=cpus[0])
+
+
 def batched_leg(G, W, torch, dev, steps):
     """The batched form beside the plain drop-in on the 64-problem forming
     workload (workloads.batched_problems: 4 part families, 2 000-40 000 Hex8
@@ -365,7 +400,13 @@ def main():
         cb = cpu_baseline_leg(w, sample, threads)
         cpu = {"value": cb["evals_per_s"], "unit": "element evals/s", "cores": threads, "kind": "port",
                "sample": f"the first {sample} elements of the same workload (all of it by default), the oracle's "
-                         f"element loop with OpenMP schedule(static) on {threads} threads, {cb['seconds']:.1f} s wall"}
+                         f"element loop with OpenMP schedule(static) on {threads} threads, {cb['seconds']:.1f} s wall",
+               "host_cpu": host_cpu_model()}
+        one = cpu_single_core_leg(w, min(ne, 20000))
+        cpu["single_core"] = {"value": one["evals_per_s"], "unit": "element evals/s", "cores": 1,
+                              "us_per_eval": one["us_per_eval"],
+                              "sample": f"the first {min(ne, 20000)} elements, one thread pinned to core {one['core']}, "
+                                        f"{one['seconds']:.1f} s wall"}
         fl, fc = cb["flops_local"], cb["flops_condense"]
         achieved = fl * ne / (ms_local * 1e-3) / 1e12
         roof = {"bound": "mfma", "pipe": "fp64 (vector ALU; MI355X FP64 vector peak = FP64 matrix peak)",
@@ -380,7 +421,8 @@ def main():
                                  "achieved": fc * ne / (ms_cond * 1e-3) / 1e12},
                 "element_evaluation": {"flops_per_elem": cb["flops_per_elem"], "ms": ms,
                                        "achieved": cb["flops_per_elem"] * ne / (ms * 1e-3) / 1e12}}
-        print(f"# cpu oracle: {cb['evals_per_s']:.0f} evals/s on {threads} threads; {cb['flops_per_elem']:.0f} "
+        print(f"# cpu oracle: {cb['evals_per_s']:.0f} evals/s on {threads} threads, {one['us_per_eval']:.1f} us/eval on "
+              f"one core ({cpu['host_cpu']}); {cb['flops_per_elem']:.0f} "
               f"FLOP/elem ({fl:.0f} local + {fc:.0f} condensation); oracle plastic fraction "
               f"{cb['plastic_fraction']:.3f}", file=sys.stderr)
 
