@@ -277,8 +277,12 @@ def assembly_leg(G, W, torch, dev, reps):
         G.add_node_blocks_device(bufs["blocks"], bufs["active"], first, row_len, values, column_major=True,
                                  stream=stream)
 
+    # the same element matrices stored column-major (Eigen's storage of the reference's Ke)
+    d_ke_cm = d_ke.reshape(ne, 24, 24).transpose(1, 2).contiguous().reshape(-1)
     stages = (("values", lambda: sysm.assemble_device(ke=d_ke, values=values, column_major=True, stream=stream)),
               ("values_row", lambda: sysm.assemble_device(ke=d_ke, values=values, column_major=False, stream=stream)),
+              ("values_ke_cm", lambda: sysm.assemble_device(ke=d_ke_cm, values=values, column_major=True,
+                                                            ke_column_major=True, stream=stream)),
               ("force", lambda: sysm.assemble_device(fe=d_fe, fint=fint, gross=gross, stream=stream)),
               ("contact", contact), ("node_blocks", blocks))
     us = {}
@@ -295,11 +299,15 @@ def assembly_leg(G, W, torch, dev, reps):
     n_active = int(bufs["active"].sum().item())
     alg = ne * (4608 + 64 * 4) + 8 * sysm.nnz
     gbs = alg / (us["values"] * 1e-6) / 1e9
+    gbs_cm = alg / (us["values_ke_cm"] * 1e-6) / 1e9
     return {"workload": "100 x 100 x 4 Hex8 plate, random non-symmetric element matrices, column-major values; "
                         "10 000 slave nodes vs a sphere R = 5 mm, friction 0.1, exact tangent",
             "elements": int(ne), "nodes": int(nn), "nnz": int(sysm.nnz), "reps": reps,
             "values_us": us["values"], "values_bytes": alg, "values_GBps": gbs,
             "values_hbm_frac": gbs / HBM_PEAK_GBS, "values_row_major_us": us["values_row"],
+            "values_ke_column_major": {"note": "element matrices stored column-major (Eigen's storage of the "
+                                               "reference's Ke), column-major values",
+                                       "us": us["values_ke_cm"], "GBps": gbs_cm, "hbm_frac": gbs_cm / HBM_PEAK_GBS},
             "force_us": us["force"], "contact_us": us["contact"],
             "node_blocks_us": us["node_blocks"], "contact_active_nodes": n_active}
 
@@ -449,7 +457,9 @@ def main():
         torch.cuda.empty_cache()
         assembly = assembly_leg(G, W, torch, dev, 20)
         print(f"# assembly: {assembly['nnz']} values in {assembly['values_us']:.1f} us "
-              f"({assembly['values_GBps']:.0f} GB/s algorithmic), force gather {assembly['force_us']:.1f} us, "
+              f"({assembly['values_GBps']:.0f} GB/s algorithmic; row-major values {assembly['values_row_major_us']:.1f} us, "
+              f"column-major K_e {assembly['values_ke_column_major']['us']:.1f} us = "
+              f"{assembly['values_ke_column_major']['GBps']:.0f} GB/s), force gather {assembly['force_us']:.1f} us, "
               f"contact {assembly['contact_us']:.1f} us ({assembly['contact_active_nodes']} active), node blocks "
               f"{assembly['node_blocks_us']:.1f} us", file=sys.stderr, flush=True)
 

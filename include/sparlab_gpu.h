@@ -234,13 +234,20 @@ int sparlab_system_diagonal_blocks(const sparlab_system* system, int32_t* first_
  * kept for sparlab_system_commit. */
 int sparlab_system_evaluate_device(sparlab_system* system, const double* u, int want_tangent, int column_major,
                                    double* fint, double* values, void* stream);
-/* Assembler::assemble_elementwise / the force gather of NonlinearSystem for
- * given element arrays (device pointers): fe (24 per element) into fint
- * (3 nn) and, when gross is non-null, gross[0] = |sum_e |f_e|| (device);
- * ke (576 per element, row-major) into values (nnz). Either part may be
- * skipped with null outputs. Every value is the sum of its contributions in
- * ascending element order, the order of the reference's serial scatter. */
-int sparlab_system_assemble_device(sparlab_system* system, const double* fe, const double* ke, int column_major,
+/* Layout flags of sparlab_system_assemble_device. */
+#define SPARLAB_VALUES_COLUMN_MAJOR 1 /* write the reference's column-major values (else row-major) */
+#define SPARLAB_KE_COLUMN_MAJOR 2     /* ke is column-major per element (Eigen's storage of the
+                                         reference's Matrix<double, 24, 24>; else row-major) */
+/* Assembler::assemble_elementwise (src/fem/Assembler.cpp:183-215) / the force
+ * gather of NonlinearSystem for given element arrays (device pointers): fe
+ * (24 per element) into fint (3 nn) and, when gross is non-null, gross[0] =
+ * |sum_e |f_e|| (device); ke (576 per element) into values (nnz), in the
+ * orders `layout` (SPARLAB_VALUES_COLUMN_MAJOR | SPARLAB_KE_COLUMN_MAJOR; 0:
+ * row-major both; 1 is the column_major = 1 of the other calls) selects.
+ * Either part may be skipped with null outputs. Every value is the sum of its
+ * contributions in ascending element order, the order of the reference's
+ * serial scatter. Values in the order of ke's storage read ke contiguously. */
+int sparlab_system_assemble_device(sparlab_system* system, const double* fe, const double* ke, int layout,
                                    double* fint, double* values, double* gross, void* stream);
 /* Waits for the last evaluation and reads its scalars. SPARLAB_E_SOLVER when
  * an element failed (the reference's SolverError after its element loop). */

@@ -11,8 +11,11 @@
 //   element order, so every value is the reference's sum added in the
 //   reference's order, bit for bit; then it writes the node's values with
 //   contiguous stores. HBM-bound: each element matrix entry is read once and
-//   each value written once, and a wavefront issues the loads of up to eight
-//   incidences before it adds them.
+//   each value written once, and a wavefront issues the loads of up to four
+//   incidences before it adds them. When K_e is stored in the order of the
+//   values (row-major K_e for row-major values, column-major K_e, Eigen's
+//   storage, for the reference's column-major values) the node's 72 entries of
+//   K_e are contiguous; otherwise they are three 24-byte runs per row block.
 // gather_force_kernel: one thread per DOF: f_int and the gross |f_e| sum.
 // sum_squares_kernel: a fixed-shape tree for the norms (deterministic).
 #include "assembly_kernels.hpp"
@@ -20,21 +23,23 @@
 namespace sbk {
 
 constexpr int kNodeWaves = 4;  // nodes (wavefronts) per block
-constexpr int kBatch = 8;      // incidences loaded ahead
+constexpr int kBatch = 4;      // incidences loaded ahead (more registers cost more than they hide)
 constexpr int kXcds = 8;       // MI355X: workgroups go round-robin to 8 XCDs, each with its own L2
 
 __global__ __launch_bounds__(256) void assemble_nodes_kernel(AssemblyArgs A) {
   extern __shared__ double acc_all[];
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  // Column-major values read three columns of K_e (24-byte runs 192 bytes
-  // apart), so the nodes of an element share its cache lines. There the
-  // mapping is XCD-aware: XCD x takes the x-th contiguous range of node blocks,
-  // so the nodes of an element (close in number) mostly meet on one L2. Row-
-  // major values read contiguous rows and keep the plain order.
+  // Values in the other order than K_e's storage read 24-byte runs of K_e
+  // 192 bytes apart, so the nodes of an element share its cache lines. There
+  // the mapping is XCD-aware: XCD x takes the x-th contiguous range of node
+  // blocks, so the nodes of an element (close in number) mostly meet on one
+  // L2. Values in K_e's own order read contiguous runs and keep the plain
+  // order.
+  const bool across = A.column_major != A.ke_column_major;
   const long long per = gridDim.x / kXcds;  // the grid is a multiple of kXcds
-  const long long blk = A.column_major ? static_cast<long long>(blockIdx.x % kXcds) * per + blockIdx.x / kXcds
-                                       : static_cast<long long>(blockIdx.x);
+  const long long blk = across ? static_cast<long long>(blockIdx.x % kXcds) * per + blockIdx.x / kXcds
+                               : static_cast<long long>(blockIdx.x);
   const long long n = blk * kNodeWaves + w;
   if (n >= A.num_nodes) return;  // the whole wavefront
   double* acc = acc_all + w * 9 * A.max_degree;
@@ -43,12 +48,14 @@ __global__ __launch_bounds__(256) void assemble_nodes_kernel(AssemblyArgs A) {
   const int deg3 = len / 3;                     // the length of one row
   for (int v = lane; v < len; v += 64) acc[v] = 0.0;
   // input j of an incidence (two per lane: j = lane, and j = 64 + lane for
-  // lanes 0-7): row k of n's three, element column block m, column k'
-  // (row-major), or column k, element row block m, row k' (column-major)
+  // lanes 0-7): line k of n's three (a row of K for row-major values, a
+  // column for column-major ones), element block m across it, entry k' of
+  // the block; in K_e's storage order j runs along the line, otherwise
+  // across the lines
   int k[2], m[2], kp[2];
   for (int r = 0; r < 2; ++r) {
     const int j = lane + 64 * r;
-    if (!A.column_major) { k[r] = j / 24; m[r] = (j % 24) / 3; kp[r] = j % 3; }
+    if (!across) { k[r] = j / 24; m[r] = (j % 24) / 3; kp[r] = j % 3; }
     else { m[r] = j / 9; kp[r] = (j / 3) % 3; k[r] = j % 3; }
   }
   const bool second = lane < 8;
@@ -68,9 +75,10 @@ __global__ __launch_bounds__(256) void assemble_nodes_kernel(AssemblyArgs A) {
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
           if (r == 0 || second) {
-            // row-major: K_e(3 nl + k, 3 m + k'); column-major: K_e(3 m + k', 3 nl + k)
-            const int off = !A.column_major ? (3 * nl + k[r]) * 24 + 3 * m[r] + kp[r]
-                                            : (3 * m[r] + kp[r]) * 24 + 3 * nl + k[r];
+            // row-major values: K_e(3 nl + k, 3 m + k'); column-major values:
+            // K_e(3 m + k', 3 nl + k); at its offset in K_e's storage
+            const int off = !across ? (3 * nl + k[r]) * 24 + 3 * m[r] + kp[r]
+                                    : (3 * m[r] + kp[r]) * 24 + 3 * nl + k[r];
             val[b][r] = ke[off];
           }
         }

@@ -6,7 +6,7 @@
 namespace sbk {
 
 struct AssemblyArgs {
-  const double* ke;             // 576 per element, row-major
+  const double* ke;             // 576 per element, row-major or column-major (ke_column_major)
   const int* row_ptr;           // 3 num_nodes + 1: node n's values are [row_ptr[3n], row_ptr[3n + 3])
   const int* node_ptr;          // num_nodes + 1
   const int* node_incidence;    // e * 8 + a_local, ascending e
@@ -14,6 +14,7 @@ struct AssemblyArgs {
   long long num_nodes;
   int max_degree;               // the largest neighbourhood (sizes the LDS accumulators)
   int column_major;             // 0: row-major values of K; 1: the reference's column-major values
+  int ke_column_major;          // 0: K_e row-major; 1: K_e column-major (Eigen's storage, the reference's Ke)
   double* values;               // nnz
 };
 

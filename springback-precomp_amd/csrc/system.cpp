@@ -53,7 +53,8 @@ void release(sparlab_system* s) {
 
 namespace sbk {
 
-AssemblyArgs assembly_args(const sparlab_system* s, const double* ke, int column_major, double* values) {
+AssemblyArgs assembly_args(const sparlab_system* s, const double* ke, int column_major, double* values,
+                           int ke_column_major) {
   AssemblyArgs g{};
   g.ke = ke;
   g.row_ptr = s->row_ptr;
@@ -63,6 +64,7 @@ AssemblyArgs assembly_args(const sparlab_system* s, const double* ke, int column
   g.num_nodes = s->nn;
   g.max_degree = s->pattern.max_degree;
   g.column_major = column_major ? 1 : 0;
+  g.ke_column_major = ke_column_major ? 1 : 0;
   g.values = values;
   return g;
 }
@@ -92,7 +94,7 @@ int system_enqueue(sparlab_system* s, const double* u, int want_tangent, int col
   ForceArgs f{s->fe, s->node_ptr, s->node_incidence, static_cast<long long>(s->nn), fint, s->gross};
   if (launch_gather_force(f, st) != 0) return capi_fail(SPARLAB_E_DEVICE, "force gather launch failed");
   if (want_tangent) {
-    if (launch_assemble_values(assembly_args(s, s->ke, column_major, values), st) != 0)
+    if (launch_assemble_values(assembly_args(s, s->ke, column_major, values, 0), st) != 0)
       return capi_fail(SPARLAB_E_DEVICE, "assembly launch failed");
   }
   const int parts = static_cast<int>(std::min<int64_t>(1024, (3 * s->nn + 4095) / 4096 + 1));
@@ -199,9 +201,13 @@ int sparlab_system_evaluate_device(sparlab_system* s, const double* u, int want_
   return sbk::system_enqueue(s, u, want_tangent, column_major, fint, values, static_cast<hipStream_t>(stream));
 }
 
-int sparlab_system_assemble_device(sparlab_system* s, const double* fe, const double* ke, int column_major,
+int sparlab_system_assemble_device(sparlab_system* s, const double* fe, const double* ke, int layout,
                                    double* fint, double* values, double* gross, void* stream) {
   if (!s || (!fe && fint) || (!ke && values)) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (layout & ~(SPARLAB_VALUES_COLUMN_MAJOR | SPARLAB_KE_COLUMN_MAJOR))
+    return sbk::capi_fail(SPARLAB_E_ARGUMENT, "unknown layout flags");
+  const int column_major = (layout & SPARLAB_VALUES_COLUMN_MAJOR) != 0;
+  const int ke_column_major = (layout & SPARLAB_KE_COLUMN_MAJOR) != 0;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (fint) {
     sbk::ForceArgs f{fe, s->node_ptr, s->node_incidence, static_cast<long long>(s->nn), fint, s->gross};
@@ -211,7 +217,7 @@ int sparlab_system_assemble_device(sparlab_system* s, const double* fe, const do
       return sbk::capi_fail(SPARLAB_E_DEVICE, "reduction launch failed");
   }
   if (values) {
-    if (sbk::launch_assemble_values(sbk::assembly_args(s, ke, column_major, values), st) != 0)
+    if (sbk::launch_assemble_values(sbk::assembly_args(s, ke, column_major, values, ke_column_major), st) != 0)
       return sbk::capi_fail(SPARLAB_E_DEVICE, "assembly launch failed");
   }
   return SPARLAB_OK;

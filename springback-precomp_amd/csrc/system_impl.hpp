@@ -61,5 +61,6 @@ bool hill_r_derivatives(const sparlab_material& m, double dP[3][36]);
 int system_enqueue(sparlab_system* s, const double* u, int want_tangent, int column_major, double* fint,
                    double* values, hipStream_t stream);
 // The node-wise assembly of element matrices ke into the system's pattern.
-AssemblyArgs assembly_args(const sparlab_system* s, const double* ke, int column_major, double* values);
+AssemblyArgs assembly_args(const sparlab_system* s, const double* ke, int column_major, double* values,
+                           int ke_column_major);
 }  // namespace sbk

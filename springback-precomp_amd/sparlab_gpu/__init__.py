@@ -305,6 +305,9 @@ class ElementBatch:
 LOCAL_SOLVE = 1
 CONDENSATION = 2
 INDEX_ORDER = 4  # the local solve takes the elements in index order (no cost ordering)
+# layout flags of the assembly of given element matrices
+VALUES_COLUMN_MAJOR = 1
+KE_COLUMN_MAJOR = 2
 
 
 def evaluate_elements(material, rule, batch, kinematics=FINITE_LOGARITHMIC, want_tangent=True, stream=None,
@@ -468,15 +471,18 @@ class NonlinearSystem:
                                                     ctypes.c_void_p(s.cuda_stream)))
 
     def assemble_device(self, fe=None, ke=None, fint=None, values=None, gross=None, column_major=False,
-                        stream=None):
-        """Gathers given element arrays (device tensors) into fint / values."""
+                        stream=None, ke_column_major=False):
+        """Gathers given element arrays (device tensors) into fint / values.
+        column_major: the reference's column-major values; ke_column_major:
+        every element matrix stored column-major (Eigen's storage)."""
         import torch
         L = lib()
         L.sparlab_system_assemble_device.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int] + [ctypes.c_void_p] * 4
         ref = next(t for t in (fe, ke) if t is not None)
         s = stream if stream is not None else torch.cuda.current_stream(ref.device)
         p = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
-        _check(L.sparlab_system_assemble_device(self.h, p(fe), p(ke), int(column_major), p(fint), p(values), p(gross),
+        layout = (VALUES_COLUMN_MAJOR if column_major else 0) | (KE_COLUMN_MAJOR if ke_column_major else 0)
+        _check(L.sparlab_system_assemble_device(self.h, p(fe), p(ke), layout, p(fint), p(values), p(gross),
                                                 ctypes.c_void_p(s.cuda_stream)))
 
     def result(self, stream=None):

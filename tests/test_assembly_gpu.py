@@ -55,7 +55,22 @@ def test_assembly_workload_bit_exact(device):
     # row-major values of K
     sysm.assemble_device(ke=d_ke, values=values, column_major=False)
     torch.cuda.synchronize()
-    assert np.array_equal(values.cpu().numpy(), csc_to_csr_of_k(o_outer, o_inner, ref_csc))
+    ref_csr = csc_to_csr_of_k(o_outer, o_inner, ref_csc)
+    assert np.array_equal(values.cpu().numpy(), ref_csr)
+    # element matrices stored column-major (Eigen's storage of the reference's
+    # Ke): the same values bit for bit in either order
+    d_ke_cm = t(ke.reshape(-1, 24, 24).transpose(0, 2, 1))
+    del d_ke
+    for column_major, ref in ((True, ref_csc), (False, ref_csr)):
+        values.fill_(float("nan"))
+        sysm.assemble_device(ke=d_ke_cm, values=values, column_major=column_major, ke_column_major=True)
+        torch.cuda.synchronize()
+        assert np.array_equal(values.cpu().numpy(), ref), column_major
+    # an unknown layout flag is refused before any work is queued
+    import ctypes
+    rc = G.lib().sparlab_system_assemble_device(sysm.h, None, ctypes.c_void_p(d_ke_cm.data_ptr()), 4, None,
+                                                ctypes.c_void_p(values.data_ptr()), None, None)
+    assert rc == -1
 
 
 def jittered_mesh(nx, ny, nz, lx, ly, lz, seed):
