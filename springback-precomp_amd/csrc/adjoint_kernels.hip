@@ -13,9 +13,10 @@
 // through the Green-Lagrange strain E, with dS/dE = the point's tangent (the
 // "material" of :227-247), so
 //   w^T dr/dy = sum_q [ d(w_q B_q w)/dy . S_q + w_q (B_q w) . (mat_q dE_q/dy) ].
-// The coordinate and displacement derivatives of w_q, B_q and E_q are
-// forward-mode dual numbers through the set-up and the kinematics
-// (IncompatibleModes.cpp:134-222), one direction per element DOF. The
+// The coordinate and displacement derivatives of w_q, B_q and E_q are the
+// tangent-linear forms of the set-up and the kinematics
+// (IncompatibleModes.cpp:134-222) for each element DOF, formed on the point's
+// values (point_products). The
 // parameters enter through the stress alone: dS = P_log^T dT, and the general
 // return's stress (src/material/Plasticity.cpp:337-471) moves with its
 // converged (xi, dl) by d(xi, dl) = -J^-1 dR/dtheta, J the closest-point
@@ -34,158 +35,169 @@ namespace {
 
 constexpr int kTeams = 8;
 
-struct Dual {
-  double v, d;
-};
-__device__ __forceinline__ Dual dual(double v, double d = 0.0) { return Dual{v, d}; }
-__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
-__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
-__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
-__device__ __forceinline__ Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
-__device__ __forceinline__ Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
-__device__ __forceinline__ Dual operator/(Dual a, Dual b) {
-  const double q = a.v / b.v;
-  return {q, (a.d - q * b.d) / b.v};
-}
-
-__device__ __forceinline__ Dual det3d(const Dual m[3][3]) {
-  return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
-         m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
-}
-__device__ __forceinline__ void inverse3d(const Dual m[3][3], Dual r[3][3]) {
-  Dual c[3][3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
-      c[i][j] = m[i1][j1] * m[i2][j2] - m[i1][j2] * m[i2][j1];
-    }
-  const Dual det = c[0][0] * m[0][0] + c[1][0] * m[1][0] + c[2][0] * m[2][0];
-  const Dual inv = dual(1.0) / det;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) r[i][j] = c[j][i] * inv;
-}
-__device__ __forceinline__ Dual bnld(const Dual F[3][3], const Dual g[3][11], int c, int a, int k) {
-  switch (c) {
-    case 0: return F[k][0] * g[0][a];
-    case 1: return F[k][1] * g[1][a];
-    case 2: return F[k][2] * g[2][a];
-    case 3: return F[k][0] * g[1][a] + F[k][1] * g[0][a];
-    case 4: return F[k][1] * g[2][a] + F[k][2] * g[1][a];
-    default: return F[k][2] * g[0][a] + F[k][0] * g[2][a];
-  }
-}
-
-// One directional derivative of the point's share of w^T r: the direction is
-// DOF j of the element's coordinates (WRT_U false) or displacements (true).
+// The point's share of w^T dr/dy for all 24 directions y of the element's
+// coordinates (WRT_U false) or displacements (true), out[3 b + l] for node b,
+// axis l. With the point's values formed once (the set-up and kinematics of
+// IncompatibleModes.cpp:134-222: Jacobians, gradients g = [G | G~], h, F, E,
+// w = w_q det J), each direction is linear algebra on them:
+// * coordinates: the Jacobians change in row l only (dJ = e_l dn_b^T), so
+//   d(ln det J) = t = g[l][b], d(ln det J0) = t0 = g0[l][b] (g0: the centre's
+//   gradients), dJ^-1 = -J^-1 e_l g_b^T, and
+//     dg[i][a] = -g[l][a] g[i][b]                       (a < 8),
+//     dG~[i][k] = (t0 - t) G~[i][k] - G~[l][k] g0[i][b]  (G~ ~ det J0 / det J J0^-T),
+//     dh = -hu[:, l] g_b^T + (t0 - t) ha - ha[:, l] g0_b^T, dw = w t,
+//   with h = hu + ha (displacement and mode parts), hu[i][m] = sum_a u_a,i g[m][a];
+// * displacements: only h moves, dh = e_l g_b^T.
+// The residual side uses y = B_NL(F, g) w33 = voigt(F^T M), M[k][m] =
+// sum_a w33[3 a + k] g[m][a] (so dM has the same form as dg), and
+//   d(w_q B w33 . S) + w_q (B w33) . (mat dE) = (dw y + w dy) . S + (mat^T z) . dE,
+// z = w y, dE from dh (Green-Lagrange: dE = sym(dh) + sym(h^T dh)).
 template <int KIN, bool WRT_U>
-__device__ double point_direction(const double x0[8][3], const double ue[24], const double alpha[9], double xi,
-                                  double eta, double zeta, double wq, int j, const double S[6],
-                                  const double mat[6][6], const double w33[33]) {
-  Dual X[8][3], U[24];
-#pragma unroll
-  for (int a = 0; a < 8; ++a)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      X[a][k] = dual(x0[a][k], (!WRT_U && 3 * a + k == j) ? 1.0 : 0.0);
-      U[3 * a + k] = dual(ue[3 * a + k], (WRT_U && 3 * a + k == j) ? 1.0 : 0.0);
-    }
+__device__ void point_products(const double x0[8][3], const double ue[24], const double alpha[9], double xi,
+                               double eta, double zeta, double wq, const double S[6], const double mat[6][6],
+                               const double w33[33], double out[24]) {
   double dn0[8][3], dn[8][3];
   shape_gradients(0.0, 0.0, 0.0, dn0);
   shape_gradients(xi, eta, zeta, dn);
-  Dual j0[3][3], jq[3][3];
+  double j0[3][3], jq[3][3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      Dual s0 = X[0][i] * dn0[0][k], s = X[0][i] * dn[0][k];
+      double s0 = x0[0][i] * dn0[0][k], s = x0[0][i] * dn[0][k];
 #pragma unroll
       for (int a = 1; a < 8; ++a) {
-        s0 = s0 + X[a][i] * dn0[a][k];
-        s = s + X[a][i] * dn[a][k];
+        s0 = s0 + x0[a][i] * dn0[a][k];
+        s = s + x0[a][i] * dn[a][k];
       }
       j0[i][k] = s0;
       jq[i][k] = s;
     }
-  const Dual det0 = det3d(j0), det = det3d(jq);
-  Dual ji0[3][3], ji[3][3];
-  inverse3d(j0, ji0);
-  inverse3d(jq, ji);
-  Dual g[3][11];
+  const double det0 = det3(j0), det = det3(jq);
+  double ji0[3][3], ji[3][3];
+  inverse3(j0, ji0);
+  inverse3(jq, ji);
+  double g[3][8], g0[3][8], gm[3][3];
 #pragma unroll
   for (int a = 0; a < 8; ++a)
 #pragma unroll
-    for (int i = 0; i < 3; ++i) g[i][a] = dn[a][0] * ji[0][i] + dn[a][1] * ji[1][i] + dn[a][2] * ji[2][i];
-  const Dual ratio = det0 / det;
+    for (int i = 0; i < 3; ++i) {
+      g[i][a] = dn[a][0] * ji[0][i] + dn[a][1] * ji[1][i] + dn[a][2] * ji[2][i];
+      g0[i][a] = dn0[a][0] * ji0[0][i] + dn0[a][1] * ji0[1][i] + dn0[a][2] * ji0[2][i];
+    }
+  const double ratio = det0 / det;
   const double nat[3] = {xi, eta, zeta};
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const Dual f = ratio * (-2.0 * nat[k]);
+    const double f = ratio * (-2.0 * nat[k]);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) g[i][8 + k] = f * ji0[k][i];
+    for (int i = 0; i < 3; ++i) gm[i][k] = f * ji0[k][i];
   }
-  Dual h[3][3];
+  // h = hu + ha; M = Mu + Mm (the w33-weighted gradients, u and mode columns)
+  double hu[3][3], ha[3][3], h[3][3], mu[3][3], mm[3][3], M[3][3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int jj = 0; jj < 3; ++jj) {
-      Dual s = U[i] * g[jj][0];
+    for (int m = 0; m < 3; ++m) {
+      double su = ue[i] * g[m][0], sw = w33[i] * g[m][0];
 #pragma unroll
-      for (int a = 1; a < 8; ++a) s = s + U[3 * a + i] * g[jj][a];
-      h[i][jj] = s + (alpha[i] * g[jj][8] + alpha[3 + i] * g[jj][9] + alpha[6 + i] * g[jj][10]);
-    }
-  const Dual w = wq * det;
-  Dual F[3][3], E[6];
-  if constexpr (KIN == kSmall) {
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) F[i][k] = dual(i == k ? 1.0 : 0.0);
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      Dual s = bnld(F, g, c, 0, 0) * U[0];
-#pragma unroll
-      for (int kk = 1; kk < 24; ++kk) s = s + bnld(F, g, c, kk / 3, kk % 3) * U[kk];
-#pragma unroll
-      for (int kk = 0; kk < 9; ++kk) s = s + bnld(F, g, c, 8 + kk / 3, kk % 3) * alpha[kk];
-      E[c] = s;
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) F[i][k] = (i == k) ? h[i][k] + dual(1.0) : h[i][k];
-    Dual e3[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const Dual hth = h[0][i] * h[0][k] + h[1][i] * h[1][k] + h[2][i] * h[2][k];
-        e3[i][k] = 0.5 * ((h[i][k] + h[k][i]) + hth);
+      for (int a = 1; a < 8; ++a) {
+        su = su + ue[3 * a + i] * g[m][a];
+        sw = sw + w33[3 * a + i] * g[m][a];
       }
-    E[0] = e3[0][0];
-    E[1] = e3[1][1];
-    E[2] = e3[2][2];
-    E[3] = 2.0 * e3[0][1];
-    E[4] = 2.0 * e3[1][2];
-    E[5] = 2.0 * e3[2][0];
+      hu[i][m] = su;
+      ha[i][m] = alpha[i] * gm[m][0] + alpha[3 + i] * gm[m][1] + alpha[6 + i] * gm[m][2];
+      h[i][m] = su + ha[i][m];
+      mu[i][m] = sw;
+      mm[i][m] = w33[24 + i] * gm[m][0] + w33[27 + i] * gm[m][1] + w33[30 + i] * gm[m][2];
+      M[i][m] = sw + mm[i][m];
+    }
+  const double w = wq * det;
+  double F[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) F[i][k] = (KIN == kSmall ? 0.0 : h[i][k]) + (i == k ? 1.0 : 0.0);
+  // y = voigt(F^T M), z = w y, q = mat^T z
+  double P[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) P[i][j] = F[0][i] * M[0][j] + F[1][i] * M[1][j] + F[2][i] * M[2][j];
+  const double y[6] = {P[0][0], P[1][1], P[2][2], P[0][1] + P[1][0], P[1][2] + P[2][1], P[2][0] + P[0][2]};
+  double q[6];
+#pragma unroll
+  for (int d = 0; d < 6; ++d) {
+    double s = w * y[0] * mat[0][d];
+#pragma unroll
+    for (int c = 1; c < 6; ++c) s = s + w * y[c] * mat[c][d];
+    q[d] = s;
   }
-  double out = 0.0;
 #pragma unroll
-  for (int c = 0; c < 6; ++c) {
-    Dual y = bnld(F, g, c, 0, 0) * w33[0];
+  for (int b = 0; b < 8; ++b)
 #pragma unroll
-    for (int col = 1; col < 33; ++col) y = y + bnld(F, g, c, col / 3, col % 3) * w33[col];
-    const Dual z = w * y;
-    double me = mat[c][0] * E[0].d;
+    for (int l = 0; l < 3; ++l) {
+      double dh[3][3], dM[3][3], dw;
+      if constexpr (WRT_U) {
 #pragma unroll
-    for (int d = 1; d < 6; ++d) me = me + mat[c][d] * E[d].d;
-    out = out + z.d * S[c] + z.v * me;
-  }
-  return out;
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int m = 0; m < 3; ++m) { dh[i][m] = i == l ? g[m][b] : 0.0; dM[i][m] = 0.0; }
+        dw = 0.0;
+      } else {
+        const double t = g[l][b], dlt = g0[l][b] - t;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            dh[i][m] = -g[m][b] * hu[i][l] + dlt * ha[i][m] - g0[m][b] * ha[i][l];
+            dM[i][m] = -g[m][b] * mu[i][l] + dlt * mm[i][m] - g0[m][b] * mm[i][l];
+          }
+        dw = w * t;
+      }
+      double dE[6];
+      if constexpr (KIN == kSmall) {
+        dE[0] = dh[0][0];
+        dE[1] = dh[1][1];
+        dE[2] = dh[2][2];
+        dE[3] = dh[0][1] + dh[1][0];
+        dE[4] = dh[1][2] + dh[2][1];
+        dE[5] = dh[2][0] + dh[0][2];
+      } else {
+        double de[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            double s = dh[0][i] * h[0][k] + h[0][i] * dh[0][k];
+#pragma unroll
+            for (int m = 1; m < 3; ++m) s = s + dh[m][i] * h[m][k] + h[m][i] * dh[m][k];
+            de[i][k] = 0.5 * ((dh[i][k] + dh[k][i]) + s);
+          }
+        dE[0] = de[0][0];
+        dE[1] = de[1][1];
+        dE[2] = de[2][2];
+        dE[3] = 2.0 * de[0][1];
+        dE[4] = 2.0 * de[1][2];
+        dE[5] = 2.0 * de[2][0];
+      }
+      // dP = dF^T M + F^T dM (dF = dh; 0 for small strain)
+      double dP[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double s = F[0][i] * dM[0][j] + F[1][i] * dM[1][j] + F[2][i] * dM[2][j];
+          if constexpr (KIN != kSmall) s = s + dh[0][i] * M[0][j] + dh[1][i] * M[1][j] + dh[2][i] * M[2][j];
+          dP[i][j] = s;
+        }
+      const double dy[6] = {dP[0][0], dP[1][1], dP[2][2], dP[0][1] + dP[1][0], dP[1][2] + dP[2][1],
+                            dP[2][0] + dP[0][2]};
+      double o = 0.0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) o = o + (dw * y[c] + w * dy[c]) * S[c] + q[c] * dE[c];
+      out[3 * b + l] = o;
+    }
 }
 
 // d(theta_t, chi_t)/d(gamma_t) of a backstress's decay (Plasticity.cpp:192-212).
@@ -415,26 +427,37 @@ __global__ __launch_bounds__(64) void adjoint_kernel(AdjointArgs Q) {
   }
   const double xi = has_point ? A.rule.xi[p] : 0.0, eta = has_point ? A.rule.eta[p] : 0.0;
   const double zeta = has_point ? A.rule.zeta[p] : 0.0, wq = has_point ? A.rule.w[p] : 0.0;
+  // the team's sums of the 24 directions: lane b keeps node b's three
   if (Q.wrt & kWrtCoords) {
-    double mine[3] = {0.0, 0.0, 0.0};
-    for (int j = 0; j < 24; ++j) {
-      const double c = has_point ? point_direction<KIN, false>(x0, ue, alpha, xi, eta, zeta, wq, j, S, mat, w33) : 0.0;
-      const double tot = team_sum(c);
+    double c[24];
+    if (has_point) {
+      point_products<KIN, false>(x0, ue, alpha, xi, eta, zeta, wq, S, mat, w33, c);
+    } else {
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
-        if (j == 3 * p + k) mine[k] = tot;
+      for (int j = 0; j < 24; ++j) c[j] = 0.0;
+    }
+    double mine[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 24; ++j) {
+      const double tot = team_sum(c[j]);
+      if (j / 3 == p) mine[j % 3] = tot;
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) Q.grad_x[e * 24 + 3 * p + k] = mine[k];
   }
   if (Q.wrt & kWrtDisplacement) {
-    double mine[3] = {0.0, 0.0, 0.0};
-    for (int j = 0; j < 24; ++j) {
-      const double c = has_point ? point_direction<KIN, true>(x0, ue, alpha, xi, eta, zeta, wq, j, S, mat, w33) : 0.0;
-      const double tot = team_sum(c);
+    double c[24];
+    if (has_point) {
+      point_products<KIN, true>(x0, ue, alpha, xi, eta, zeta, wq, S, mat, w33, c);
+    } else {
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
-        if (j == 3 * p + k) mine[k] = tot;
+      for (int j = 0; j < 24; ++j) c[j] = 0.0;
+    }
+    double mine[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 24; ++j) {
+      const double tot = team_sum(c[j]);
+      if (j / 3 == p) mine[j % 3] = tot;
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) Q.grad_u[e * 24 + 3 * p + k] = mine[k];

@@ -424,6 +424,8 @@ class NonlinearSystem:
         else:
             h = _handle
         self.h = h
+        # kept for __del__, which may run after the module's globals are gone
+        self._destroy = L.sparlab_system_destroy
         self.num_nodes = self.coords.shape[0]
         self.num_elements = self.conn.shape[0]
         self.points = self.rule.points
@@ -431,8 +433,9 @@ class NonlinearSystem:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h and getattr(self, "owned", True):
-            lib().sparlab_system_destroy(h)
+        destroy = getattr(self, "_destroy", None)
+        if h and getattr(self, "owned", True) and destroy is not None:
+            destroy(h)
         self.h = None
 
     def pattern(self):
@@ -599,6 +602,7 @@ class BatchedSystem:
                                       _np_ptr(ucoords), _np_ptr(lconn), ctypes.byref(material.c),
                                       ctypes.byref(rule.c), kinematics, ctypes.byref(h)))
         self.h = h
+        self._destroy = L.sparlab_batch_destroy
         uconn = lconn + np.repeat(self.node_offset[:-1], [k.shape[0] for k in conns])[:, None].astype(np.int32)
         self.system = NonlinearSystem(ucoords, uconn, material, rule, kinematics,
                                       _handle=ctypes.c_void_p(L.sparlab_batch_system(h)))
@@ -607,11 +611,12 @@ class BatchedSystem:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h:
+        destroy = getattr(self, "_destroy", None)
+        if h and destroy is not None:
             sys_ = getattr(self, "system", None)
             if sys_ is not None:
                 sys_.h = None
-            lib().sparlab_batch_destroy(h)
+            destroy(h)
             self.h = None
 
     def set_graphs(self, enabled):
