@@ -1,7 +1,8 @@
 """The differentiation form: vector-Jacobian products of the assembled
 internal force (sparlab_system_vjp_device) at the converged modes and a fixed
 committed history, on a plastically deforming Hill48 + Chaboche sheet with
-logarithmic strain.
+logarithmic strain (the displacement and coordinate checks also with small
+strain and the Green-Lagrange finite strain).
 
 Checks (the reference has no adjoint; these pin it by mathematics):
   * the same adjoint machinery in the displacements reproduces v^T K_t of the
@@ -33,12 +34,15 @@ def sheet(nx=4, ny=4, nz=2):
     return coords, conn, u1.reshape(-1), u2.reshape(-1)
 
 
-def prepared_system(device, material=None):
+KINEMATICS = [G.SMALL_STRAIN, G.FINITE, G.FINITE_LOGARITHMIC]
+
+
+def prepared_system(device, material=None, kinematics=G.FINITE_LOGARITHMIC):
     """A system with a plastic committed history, evaluated at the next increment."""
     import torch
     coords, conn, u1, u2 = sheet()
     m = material or W.microbench_material()
-    s = G.NonlinearSystem(coords, conn, m, G.Rule(2, 0, 2))
+    s = G.NonlinearSystem(coords, conn, m, G.Rule(2, 0, 2), kinematics)
     s.evaluate(u1)
     s.commit()
     ev = s.evaluate(u2)
@@ -58,9 +62,10 @@ def fint_at(s, u_dev, coords=None, material=None):
     return f.cpu().numpy()
 
 
-def test_displacement_vjp_is_tangent_transpose(device):
+@pytest.mark.parametrize("kinematics", KINEMATICS)
+def test_displacement_vjp_is_tangent_transpose(device, kinematics):
     import torch
-    s, coords, conn, u, ev = prepared_system(device)
+    s, coords, conn, u, ev = prepared_system(device, kinematics=kinematics)
     row_ptr, col_idx = s.pattern()
     n = 3 * s.num_nodes
     vals = ev.tangent_values
@@ -75,9 +80,10 @@ def test_displacement_vjp_is_tangent_transpose(device):
     assert np.abs(got - ktv).max() <= 1e-10 * np.abs(ktv).max()
 
 
-def test_coordinate_vjp_matches_central_differences(device):
+@pytest.mark.parametrize("kinematics", KINEMATICS)
+def test_coordinate_vjp_matches_central_differences(device, kinematics):
     import torch
-    s, coords, conn, u, _ = prepared_system(device)
+    s, coords, conn, u, _ = prepared_system(device, kinematics=kinematics)
     rng = np.random.default_rng(6)
     v = rng.normal(size=3 * s.num_nodes)
     g = s.vjp_device(u, torch.as_tensor(v, device=device), coords=True)["coords"].cpu().numpy()
@@ -95,9 +101,10 @@ def test_coordinate_vjp_matches_central_differences(device):
     assert worst <= 1e-6, worst
 
 
-def test_coordinate_vjp_translation_invariance(device):
+@pytest.mark.parametrize("kinematics", KINEMATICS)
+def test_coordinate_vjp_translation_invariance(device, kinematics):
     import torch
-    s, coords, conn, u, _ = prepared_system(device)
+    s, coords, conn, u, _ = prepared_system(device, kinematics=kinematics)
     v = np.random.default_rng(7).normal(size=3 * s.num_nodes)
     g = s.vjp_device(u, torch.as_tensor(v, device=device), coords=True)["coords"].cpu().numpy().reshape(-1, 3)
     assert np.abs(g.sum(axis=0)).max() <= 1e-9 * np.abs(g).sum(axis=0).max()
