@@ -222,11 +222,24 @@ def differentiation_leg(G, W, torch, dev):
     fd = vn @ (f_at(coords + h * d) - f_at(coords - h * d)) / (2.0 * h)
     an = gx @ d.reshape(-1)
     s.set_coords(coords)
+    # end-of-step stress recovery of every element, after the step's commit
+    s.evaluate_device(u, f, None, want_tangent=False)
+    s.result()
+    s.commit()
+    stress, status = s.stress_device(u)
+    e0.record()
+    for _ in range(5):
+        s.stress_device(u, stress, status)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_stress = e0.elapsed_time(e1) / 5
+    stress_failures = int((status != 0).sum().item())
     return {"workload": "200 x 250 x 4 Hex8 sheet (20 x 25 x 1 mm), mid-way through a 3-turn spiral incremental "
                         "forming path (committed increment, then the next), Hill48 + 2 Chaboche + Voce, log strain",
             "elements": int(conn.shape[0]), "yielding_points": int(ev.yielding_points),
             "forward_ms": ms_fwd, "vjp_ms": ms_vjp,
-            "vjp": "coordinates + 11 material parameters", "fd_direction_rel_error": abs(fd - an) / abs(an)}
+            "vjp": "coordinates + 11 material parameters", "fd_direction_rel_error": abs(fd - an) / abs(an),
+            "stress_recovery_ms": ms_stress, "stress_recovery_failures": stress_failures}
 
 
 def assembly_leg(G, W, torch, dev, reps):
@@ -449,8 +462,8 @@ def main():
         torch.cuda.empty_cache()
         adjoint = differentiation_leg(G, W, torch, dev)
         print(f"# differentiation: {adjoint['elements']} elements; forward {adjoint['forward_ms']:.2f} ms, "
-              f"VJP {adjoint['vjp_ms']:.2f} ms; FD check {adjoint['fd_direction_rel_error']:.2e}",
-              file=sys.stderr, flush=True)
+              f"VJP {adjoint['vjp_ms']:.2f} ms; FD check {adjoint['fd_direction_rel_error']:.2e}; stress recovery "
+              f"{adjoint['stress_recovery_ms']:.2f} ms", file=sys.stderr, flush=True)
 
     assembly = None
     if not args.no_assembly:

@@ -249,6 +249,28 @@ int sparlab_system_evaluate_device(sparlab_system* system, const double* u, int 
  * serial scatter. Values in the order of ke's storage read ke contiguously. */
 int sparlab_system_assemble_device(sparlab_system* system, const double* fe, const double* ke, int layout,
                                    double* fint, double* values, double* gross, void* stream);
+/* Per-element stress measures (the reference's ElastoplasticStress,
+ * include/sparlab/fem/Elastoplastic.hpp:206-221), SPARLAB_STRESS_FIELDS
+ * doubles per element at these offsets (stresses and strains in tensorial
+ * Voigt order xx, yy, zz, xy, yz, zx). */
+#define SPARLAB_STRESS_FIELDS 28
+#define SPARLAB_STRESS_CAUCHY 0                          /* 6, averaged over the points [Pa] */
+#define SPARLAB_STRESS_PIOLA_KIRCHHOFF 6                 /* 6, second Piola-Kirchhoff [Pa] */
+#define SPARLAB_STRESS_KIRCHHOFF 12                      /* 6, logarithmic kinematics only [Pa] */
+#define SPARLAB_STRESS_LOGARITHMIC_STRAIN 18             /* 6, logarithmic kinematics only [-] */
+#define SPARLAB_STRESS_VON_MISES 24                      /* of the averaged Cauchy stress */
+#define SPARLAB_STRESS_MAX_EQUIVALENT_PLASTIC_STRAIN 25
+#define SPARLAB_STRESS_MAX_STRAIN 26                     /* largest tensor strain component */
+#define SPARLAB_STRESS_MIN_JACOBIAN 27                   /* smallest det F (1 for small strain) */
+/* elastoplastic_stress (src/fem/Elastoplastic.cpp:342-350 ->
+ * src/fem/IncompatibleModes.cpp:602-664) of every element at displacement u
+ * (device, 3 nn) from the system's committed history and modes, as the
+ * end-of-step output of src/fem/Forming.cpp:1263-1281 calls it: stress
+ * (SPARLAB_STRESS_FIELDS per element, device); status (int32 per element,
+ * SPARLAB_ELEMENT_* codes; may be null). A failed element (the reference's
+ * MeshError / SolverError) gets zeros and its status. Asynchronous on stream. */
+int sparlab_system_stress_device(sparlab_system* system, const double* u, double* stress, int32_t* status,
+                                 void* stream);
 /* Waits for the last evaluation and reads its scalars. SPARLAB_E_SOLVER when
  * an element failed (the reference's SolverError after its element loop). */
 int sparlab_system_result(sparlab_system* system, sparlab_evaluation* out, void* stream);

@@ -16,6 +16,7 @@
 #include "material.hpp"
 
 #include <functional>
+#include <limits>
 #include <vector>
 
 namespace oracle {
@@ -197,6 +198,24 @@ template <class T> struct ElementResult {
   int internal_iterations = 0;
   int passes = 0;  // pass() calls, successful or not (diagnostic only)
 };
+
+// include/sparlab/fem/Elastoplastic.hpp:206-221 (ElastoplasticStress).
+template <class T> struct ElementStress {
+  Vec<T, 6> cauchy = Vec<T, 6>::zero();
+  Vec<T, 6> piola_kirchhoff = Vec<T, 6>::zero();
+  Vec<T, 6> kirchhoff = Vec<T, 6>::zero();
+  Vec<T, 6> logarithmic_strain = Vec<T, 6>::zero();
+  T von_mises = T(0.0);
+  T max_equivalent_plastic_strain = T(0.0);
+  T max_strain = T(0.0);
+  T min_jacobian = T(1.0);
+};
+
+// Plasticity.cpp:541-547
+template <class T> T von_mises_stress(const Vec<T, 6>& s) {
+  const T a = s(0) - s(1), b = s(1) - s(2), c = s(2) - s(0);
+  return sqrt_(T(0.5) * (a * a + b * b + c * c) + T(3.0) * (s(3) * s(3) + s(4) * s(4) + s(5) * s(5)));
+}
 
 template <class T>
 class IncompatibleHex8 {
@@ -694,6 +713,82 @@ class IncompatibleHex8 {
     Vec<T, 24> f;
     for (int i = 0; i < 24; ++i) f(i) = sums.r(i);
     return f;
+  }
+
+  // IncompatibleModes.cpp:602-664 (incompatible_elastoplastic_stress): the
+  // element's stress measures at its committed history and modes, after
+  // set_up. Each point's return runs again from the committed state (an
+  // elastic check, without its tangent); the measures are those of F = I + H.
+  ElementStress<T> stress(const std::vector<PlasticState<T>>& committed, const T* internal) {
+    if (committed.size() != points_.size())
+      throw SolverError("elastoplastic element: the stored internal variables do not match the integration rule");
+    Vec<T, 9> alpha = Vec<T, 9>::zero();
+    if (internal != nullptr)
+      for (int i = 0; i < 9; ++i) alpha(i) = internal[i];
+    kinematics_at(alpha);
+    const bool finite = kin_ == kFinite, logarithmic = kin_ == kFiniteLogarithmic;
+    ElementStress<T> out;
+    out.min_jacobian = T(std::numeric_limits<double>::infinity());
+    for (std::size_t q = 0; q < points_.size(); ++q) {
+      const PointWork<T>& p = points_[q];
+      const PlasticResponse<T> r = plastic_return(m_, p.strain, committed[q], false);
+      Vec<T, 6> cauchy = r.stress, pk = r.stress;
+      Mat<T, 3, 3> sym, f;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          sym(i, j) = T(0.5) * (p.h(i, j) + p.h(j, i));
+          f(i, j) = (i == j ? T(1.0) : T(0.0)) + p.h(i, j);
+        }
+      if (logarithmic) {
+        pk = logarithmic_stress(p.log, r.stress);
+        const Vec<T, 6> tau = voigt_of(mul(mul(f, tensor_of(pk, 1.0)), transpose(f)), 1.0);
+        Vec<T, 6> strain = p.strain;
+        for (int i = 3; i < 6; ++i) strain(i) = strain(i) * T(0.5);  // tensor components
+        const T j = exp_(strain(0) + strain(1) + strain(2));
+        for (int i = 0; i < 6; ++i) {
+          cauchy(i) = tau(i) / j;
+          out.kirchhoff(i) = out.kirchhoff(i) + tau(i);
+          out.logarithmic_strain(i) = out.logarithmic_strain(i) + strain(i);
+        }
+        T m = abs_(strain(0));
+        for (int i = 1; i < 6; ++i) m = max_(m, abs_(strain(i)));
+        out.max_strain = max_(out.max_strain, m);
+        out.min_jacobian = min_(out.min_jacobian, det3(f));
+      } else if (finite) {
+        const T j = det3(f);
+        const Mat<T, 3, 3> sf = mul(mul(f, tensor_of(r.stress, 1.0)), transpose(f));
+        Mat<T, 3, 3> sj;
+        for (int i = 0; i < 9; ++i) sj.a[i] = sf.a[i] / j;
+        cauchy = voigt_of(sj, 1.0);
+        T m = T(0.0);
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) {
+            const T hth = p.h(0, a) * p.h(0, b) + p.h(1, a) * p.h(1, b) + p.h(2, a) * p.h(2, b);
+            m = max_(m, abs_(sym(a, b) + T(0.5) * hth));
+          }
+        out.max_strain = max_(out.max_strain, m);
+        out.min_jacobian = min_(out.min_jacobian, j);
+      } else {
+        T m = T(0.0);
+        for (int i = 0; i < 9; ++i) m = max_(m, abs_(sym.a[i]));
+        out.max_strain = max_(out.max_strain, m);
+      }
+      for (int i = 0; i < 6; ++i) {
+        out.cauchy(i) = out.cauchy(i) + cauchy(i);
+        out.piola_kirchhoff(i) = out.piola_kirchhoff(i) + pk(i);
+      }
+      out.max_equivalent_plastic_strain = max_(out.max_equivalent_plastic_strain, r.state.abar);
+    }
+    const T count = T(static_cast<double>(points_.size()));
+    for (int i = 0; i < 6; ++i) {
+      out.cauchy(i) = out.cauchy(i) / count;
+      out.piola_kirchhoff(i) = out.piola_kirchhoff(i) / count;
+      out.kirchhoff(i) = out.kirchhoff(i) / count;
+      out.logarithmic_strain(i) = out.logarithmic_strain(i) / count;
+    }
+    out.von_mises = von_mises_stress(out.cauchy);
+    if (!finite && !logarithmic) out.min_jacobian = T(1.0);
+    return out;
   }
 
  private:

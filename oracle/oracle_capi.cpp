@@ -280,6 +280,49 @@ int oracle_element_batch(long long ne, const double* x0, const double* ue, const
   return failures;
 }
 
+// incompatible_elastoplastic_stress (IncompatibleModes.cpp:602-664) for a
+// batch of elements at their committed states and modes (alpha_in may be
+// null: zero), OpenMP over elements. out[e*28]: cauchy 6, piola_kirchhoff 6,
+// kirchhoff 6, logarithmic_strain 6 (tensorial Voigt), von_mises,
+// max_equivalent_plastic_strain, max_strain, min_jacobian; status[e] as
+// oracle_element. Returns the failures.
+int oracle_element_stress_batch(long long ne, const double* x0, const double* ue, const double* states_in,
+                                const double* alpha_in, const double* mat, int in_plane, int thickness, int axis,
+                                int kinematics, int threads, double* out, int* status) {
+  const MaterialData m = read_material(mat);
+  const Rule rule = make_rule(in_plane, thickness, axis);
+  const long long per = static_cast<long long>(rule.n) * kStateRecord;
+  int failures = 0;
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(static) reduction(+ : failures)
+  for (long long e = 0; e < ne; ++e) {
+    double* o = out + e * 28;
+    const int st = guarded(nullptr, 0, [&] {
+      IncompatibleHex8<double> el(m, rule, kinematics);
+      el.set_up(x0 + e * 24, ue + e * 24, e);
+      std::vector<PlasticState<double>> committed(static_cast<std::size_t>(rule.n));
+      for (int q = 0; q < rule.n; ++q)
+        committed[static_cast<std::size_t>(q)] = read_state<double>(states_in + e * per + q * kStateRecord);
+      const ElementStress<double> r = el.stress(committed, alpha_in ? alpha_in + e * 9 : nullptr);
+      for (int i = 0; i < 6; ++i) {
+        o[i] = r.cauchy(i);
+        o[6 + i] = r.piola_kirchhoff(i);
+        o[12 + i] = r.kirchhoff(i);
+        o[18 + i] = r.logarithmic_strain(i);
+      }
+      o[24] = r.von_mises;
+      o[25] = r.max_equivalent_plastic_strain;
+      o[26] = r.max_strain;
+      o[27] = r.min_jacobian;
+    });
+    if (st != 0)
+      for (int i = 0; i < 28; ++i) o[i] = 0.0;
+    status[e] = st;
+    if (st != 0) ++failures;
+  }
+  return failures;
+}
+
 // The standard Hex8 force at the same kinematics (modes held at zero).
 int oracle_standard_force(const double* x0, const double* ue, const double* states_in, const double* mat,
                           int in_plane, int thickness, int axis, int kinematics, double* f, char* msg,

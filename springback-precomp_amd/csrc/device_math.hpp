@@ -95,6 +95,18 @@ __device__ __forceinline__ int team_or(int v) {
   v = v | dpp_i32<kHalfMirror>(v);
   return v;
 }
+__device__ __forceinline__ double team_max(double v) {
+  v = fmax(v, dpp_f64<kXor1>(v));
+  v = fmax(v, dpp_f64<kXor2>(v));
+  v = fmax(v, dpp_f64<kHalfMirror>(v));
+  return v;
+}
+__device__ __forceinline__ double team_min(double v) {
+  v = fmin(v, dpp_f64<kXor1>(v));
+  v = fmin(v, dpp_f64<kXor2>(v));
+  v = fmin(v, dpp_f64<kHalfMirror>(v));
+  return v;
+}
 __device__ __forceinline__ int team_isum(int v) {
   v = v + dpp_i32<kXor1>(v);
   v = v + dpp_i32<kXor2>(v);
@@ -500,14 +512,14 @@ __device__ __forceinline__ void log_material(const LogData& L, const double t[6]
     }
 }
 
-// S = P^T T (logarithmic_stress) and T:L (logarithmic_curvature) in one go.
-__device__ __forceinline__ void log_push(const LogData& L, const double t[6], double S[6], double curv[6][6]) {
+// T' = Q^T T Q: a tensorial-Voigt stress in the principal axes.
+__device__ __forceinline__ void principal_stress(const LogData& L, const double t[6], double tp[3][3]) {
   double tt[3][3];
   tt[0][0] = t[0]; tt[1][1] = t[1]; tt[2][2] = t[2];
   tt[0][1] = tt[1][0] = t[3];
   tt[1][2] = tt[2][1] = t[4];
   tt[2][0] = tt[0][2] = t[5];
-  double qt[3][3], tp[3][3];
+  double qt[3][3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -516,7 +528,11 @@ __device__ __forceinline__ void log_push(const LogData& L, const double t[6], do
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) tp[i][j] = qt[i][0] * L.q[0][j] + qt[i][1] * L.q[1][j] + qt[i][2] * L.q[2][j];
-  // S = Q (tp o first) Q^T
+}
+
+// S = Q (T' o first) Q^T from the principal-axes stress T' (logarithmic_stress,
+// LogarithmicStrain.cpp:144-149).
+__device__ __forceinline__ void log_stress_from_principal(const LogData& L, const double tp[3][3], double S[6]) {
   double pr[3][3], qp[3][3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -532,6 +548,20 @@ __device__ __forceinline__ void log_push(const LogData& L, const double t[6], do
 #pragma unroll
     for (int j = 0; j < 3; ++j) sm[i][j] = qp[i][0] * L.q[j][0] + qp[i][1] * L.q[j][1] + qp[i][2] * L.q[j][2];
   S[0] = sm[0][0]; S[1] = sm[1][1]; S[2] = sm[2][2]; S[3] = sm[0][1]; S[4] = sm[1][2]; S[5] = sm[2][0];
+}
+
+// S = P^T T (logarithmic_stress).
+__device__ __forceinline__ void log_stress(const LogData& L, const double t[6], double S[6]) {
+  double tp[3][3];
+  principal_stress(L, t, tp);
+  log_stress_from_principal(L, tp, S);
+}
+
+// S = P^T T (logarithmic_stress) and T:L (logarithmic_curvature) in one go.
+__device__ __forceinline__ void log_push(const LogData& L, const double t[6], double S[6], double curv[6][6]) {
+  double tp[3][3];
+  principal_stress(L, t, tp);
+  log_stress_from_principal(L, tp, S);
   // second divided differences: ten distinct values l[a,b,c], a <= b <= c
   double l2[3][3][3];
 #pragma unroll

@@ -1167,6 +1167,181 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
 }
 
 // ---------------------------------------------------------------------------
+// Stress recovery (IncompatibleModes.cpp:602-664)
+// ---------------------------------------------------------------------------
+// One team per element, lane p = point p, at the committed history and modes
+// (state_in, alpha_in): the point's return runs again from its committed
+// state (an elastic check that reproduces it), then the stress measures of
+// F = I + H: Cauchy, the second Piola-Kirchhoff stress and, for logarithmic
+// kinematics, the Kirchhoff stress and the log strain, averaged over the
+// points, with the extremes of the strain, det F and the equivalent plastic
+// strain. out: kStressFields per element in the order of the reference's
+// ElastoplasticStress; a failing element (set-up, then kinematics, then the
+// returns, each in point order, as the reference's loops) gets zeros and its
+// status.
+template <int KIN, int NT>
+__global__ __launch_bounds__(64) void stress_kernel(ElementArgs A, double* __restrict__ out) {
+  const int lane = threadIdx.x;
+  const int t = lane >> 3;
+  const int p = lane & 7;
+  const long long e = static_cast<long long>(blockIdx.x) * kWaveTeams + t;
+  if (e >= A.ne) return;  // the whole team
+  const int n = A.rule.n;
+  // cauchy 6, piola_kirchhoff 6, kirchhoff 6, log strain 6 of the lane's point
+  double v[24];
+#pragma unroll
+  for (int i = 0; i < 24; ++i) v[i] = 0.0;
+  double max_strain = 0.0, min_jacobian = __builtin_inf(), max_abar = 0.0, fv = 0.0;
+  int st = kOk, stage = 0;
+  if (p < n) {
+    double x0[8][3], ue[24];
+    load_element(A.coords, A.conn, A.u, e, x0, ue);
+    PointGeom pg;
+    st = setup_point(x0, ue, A.rule.xi[p], A.rule.eta[p], A.rule.zeta[p], A.rule.w[p], pg, fv);
+    if (st == kOk) {
+      stage = 1;
+      ModeGeom mg;
+      mode_geom<KIN>(pg, ue, mg);
+      double alpha[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) alpha[i] = A.alpha_in ? A.alpha_in[e * 9 + i] : 0.0;
+      double h[3][3], F[3][3];
+      mode_kinematics<KIN>(mg.hu, mg.gm, alpha, h, F);
+      double strain[6];
+      LogData L;
+      if constexpr (KIN == kSmall) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          double s2 = bnl_mode(F, mg.gm, c, 0, 0) * alpha[0];
+#pragma unroll
+          for (int kk = 1; kk < 9; ++kk) s2 = s2 + bnl_mode(F, mg.gm, c, kk / 3, kk % 3) * alpha[kk];
+          strain[c] = mg.eps_u[c] + s2;
+        }
+      } else {
+        double e3[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const double hth = h[0][i] * h[0][j] + h[1][i] * h[1][j] + h[2][i] * h[2][j];
+            e3[i][j] = 0.5 * ((h[i][j] + h[j][i]) + hth);
+          }
+        const double green[6] = {e3[0][0], e3[1][1], e3[2][2], 2.0 * e3[0][1], 2.0 * e3[1][2], 2.0 * e3[2][0]};
+        if constexpr (KIN == kLog) {
+          if (!(det3(F) > 0.0)) st = kInverted;
+          else if (logarithmic_strain(green, L) != 0) st = kDegenerate;
+#pragma unroll
+          for (int i = 0; i < 6; ++i) strain[i] = L.strain[i];
+        } else {
+#pragma unroll
+          for (int i = 0; i < 6; ++i) strain[i] = green[i];
+        }
+      }
+      if (st == kOk) {
+        stage = 2;
+        PointStateT<NT> ps;
+        load_state(A.mat, A.state_in, A.state_stride, e * n + p, ps);
+        ReturnOut ro;
+        const int rf = A.mat.general ? return_general(A.mat, strain, ps, ro) : return_radial(A.mat, strain, ps, ro);
+        if (rf == kReturnNotConverged) {
+          st = kReturnFailed;
+        } else if (rf == kReturnNonFinite) {
+          st = kNonFiniteTrial;
+        } else {
+          double f[3][3], sym[3][3];
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+              f[i][j] = (i == j ? 1.0 : 0.0) + h[i][j];
+              sym[i][j] = 0.5 * (h[i][j] + h[j][i]);
+            }
+          double pk[6];
+#pragma unroll
+          for (int i = 0; i < 6; ++i) pk[i] = ro.stress[i];
+          if constexpr (KIN == kSmall) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) v[i] = ro.stress[i];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+              for (int j = 0; j < 3; ++j) max_strain = fmax(max_strain, fabs(sym[i][j]));
+          } else {
+            if constexpr (KIN == kLog) log_stress(L, ro.stress, pk);
+            // F S F^T (tensorial Voigt)
+            const double s3[3][3] = {{pk[0], pk[3], pk[5]}, {pk[3], pk[1], pk[4]}, {pk[5], pk[4], pk[2]}};
+            double fs[3][3], fsf[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+              for (int j = 0; j < 3; ++j) fs[i][j] = f[i][0] * s3[0][j] + f[i][1] * s3[1][j] + f[i][2] * s3[2][j];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+              for (int j = 0; j < 3; ++j) fsf[i][j] = fs[i][0] * f[j][0] + fs[i][1] * f[j][1] + fs[i][2] * f[j][2];
+            const double tau[6] = {fsf[0][0], fsf[1][1], fsf[2][2], fsf[0][1], fsf[1][2], fsf[2][0]};
+            if constexpr (KIN == kLog) {
+              double e6[6];
+#pragma unroll
+              for (int i = 0; i < 6; ++i) e6[i] = i < 3 ? strain[i] : strain[i] * 0.5;  // tensor components
+              const double j = exp(e6[0] + e6[1] + e6[2]);
+#pragma unroll
+              for (int i = 0; i < 6; ++i) {
+                v[i] = tau[i] / j;
+                v[12 + i] = tau[i];
+                v[18 + i] = e6[i];
+                max_strain = fmax(max_strain, fabs(e6[i]));
+              }
+              min_jacobian = det3(f);
+            } else {
+              const double j = det3(f);
+#pragma unroll
+              for (int i = 0; i < 6; ++i) v[i] = tau[i] / j;
+#pragma unroll
+              for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                  const double hth = h[0][a] * h[0][b] + h[1][a] * h[1][b] + h[2][a] * h[2][b];
+                  max_strain = fmax(max_strain, fabs(sym[a][b] + 0.5 * hth));
+                }
+              min_jacobian = j;
+            }
+          }
+#pragma unroll
+          for (int i = 0; i < 6; ++i) v[6 + i] = pk[i];
+          max_abar = ps.abar;
+        }
+      }
+    }
+  }
+  double* o = out + e * kStressFields;
+  const int code = team_imin(st != kOk ? (stage << 12) | (p << 8) | st : 1 << 20);
+  if (code != (1 << 20)) {
+    for (int i = p; i < kStressFields; i += kWaveTeams) o[i] = 0.0;
+    if (p == 0 && A.status) A.status[e] = code & 0xFF;
+    if (p == ((code >> 8) & 0xF) && A.fail_value) A.fail_value[e] = fv;
+    return;
+  }
+  const double count = static_cast<double>(n);
+  double avg[24];
+#pragma unroll
+  for (int i = 0; i < 24; ++i) avg[i] = team_sum(v[i]) / count;
+  max_strain = team_max(max_strain);
+  min_jacobian = KIN == kSmall ? 1.0 : team_min(min_jacobian);
+  max_abar = team_max(max_abar);
+  const double a = avg[0] - avg[1], b = avg[1] - avg[2], c = avg[2] - avg[0];
+  const double vm = sqrt(0.5 * (a * a + b * b + c * c) + 3.0 * (avg[3] * avg[3] + avg[4] * avg[4] + avg[5] * avg[5]));
+  const double tail[4] = {vm, max_abar, max_strain, min_jacobian};
+#pragma unroll
+  for (int i = 0; i < 24; ++i)
+    if ((i & 7) == p) o[i] = avg[i];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (((24 + i) & 7) == p) o[24 + i] = tail[i];
+  if (p == 0 && A.status) A.status[e] = kOk;
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 // Wavefront slots of the current device at one wavefront per SIMD (the local
@@ -1241,6 +1416,29 @@ void launch_condense_kin(const ElementArgs& a, bool tangent, unsigned int blocks
     hipLaunchKernelGGL((condense_kernel<KIN, true, true>), dim3(blocks), dim3(64), 0, stream, a);
   else
     hipLaunchKernelGGL((condense_kernel<KIN, true, false>), dim3(blocks), dim3(64), 0, stream, a);
+}
+
+template <int KIN>
+void launch_stress_kin(const ElementArgs& a, bool two_terms, double* out, unsigned int blocks, hipStream_t stream) {
+  if (two_terms)
+    hipLaunchKernelGGL((stress_kernel<KIN, 2>), dim3(blocks), dim3(64), 0, stream, a, out);
+  else
+    hipLaunchKernelGGL((stress_kernel<KIN, kMaxTerms>), dim3(blocks), dim3(64), 0, stream, a, out);
+}
+
+int launch_element_stress(const ElementArgs& a, double* out, hipStream_t stream) {
+  (void)hipGetLastError();
+  const unsigned int blocks = static_cast<unsigned int>((a.ne + kWaveTeams - 1) / kWaveTeams);
+  if (blocks == 0) return 0;
+  const int fields = a.mat.terms > a.mat.kin_fields ? a.mat.terms : a.mat.kin_fields;
+  if (fields > kMaxTerms || a.rule.n > kWaveTeams) return static_cast<int>(hipErrorInvalidValue);
+  const bool two = fields <= 2;
+  switch (a.kinematics) {
+    case kSmall: launch_stress_kin<kSmall>(a, two, out, blocks, stream); break;
+    case kFinite: launch_stress_kin<kFinite>(a, two, out, blocks, stream); break;
+    default: launch_stress_kin<kLog>(a, two, out, blocks, stream); break;
+  }
+  return static_cast<int>(hipGetLastError());
 }
 
 int launch_condense(const ElementArgs& a, bool tangent, hipStream_t stream) {

@@ -87,5 +87,9 @@ inline void set_workspace(ElementArgs& a, double* ws) {
 
 int launch_local_solve(const ElementArgs& a, hipStream_t stream);
 int launch_condense(const ElementArgs& a, bool tangent, hipStream_t stream);
+// incompatible_elastoplastic_stress of every element at state_in / alpha_in:
+// kStressFields per element into out; status (if set) per element.
+constexpr int kStressFields = 28;
+int launch_element_stress(const ElementArgs& a, double* out, hipStream_t stream);
 
 }  // namespace sbk

@@ -223,6 +223,24 @@ int sparlab_system_assemble_device(sparlab_system* s, const double* fe, const do
   return SPARLAB_OK;
 }
 
+int sparlab_system_stress_device(sparlab_system* s, const double* u, double* stress, int32_t* status,
+                                 void* stream) {
+  if (!s || !u || !stress) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  sbk::ElementArgs a = s->args;
+  a.coords = s->coords;
+  a.conn = s->conn;
+  a.u = u;
+  a.ne = s->ne;
+  a.state_in = s->state[s->cur];
+  a.state_stride = s->ne * s->points;
+  a.alpha_in = s->alpha[s->cur];
+  a.status = status;
+  a.fail_value = nullptr;
+  if (sbk::launch_element_stress(a, stress, static_cast<hipStream_t>(stream)) != 0)
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "stress kernel launch failed");
+  return SPARLAB_OK;
+}
+
 int sparlab_system_result(sparlab_system* s, sparlab_evaluation* out, void* stream) {
   if (!s || !out) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
   if (!s->evaluated) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "no evaluation to read");

@@ -305,6 +305,11 @@ class ElementBatch:
 LOCAL_SOLVE = 1
 CONDENSATION = 2
 INDEX_ORDER = 4  # the local solve takes the elements in index order (no cost ordering)
+# the per-element stress measures of NonlinearSystem.stress (name, offset, width)
+STRESS_FIELDS = 28
+STRESS_LAYOUT = (("cauchy", 0, 6), ("piola_kirchhoff", 6, 6), ("kirchhoff", 12, 6), ("logarithmic_strain", 18, 6),
+                 ("von_mises", 24, 1), ("max_equivalent_plastic_strain", 25, 1), ("max_strain", 26, 1),
+                 ("min_jacobian", 27, 1))
 # layout flags of the assembly of given element matrices
 VALUES_COLUMN_MAJOR = 1
 KE_COLUMN_MAJOR = 2
@@ -555,6 +560,37 @@ class NonlinearSystem:
                                            p("coords"), p("displacement"), p("parameters"),
                                            ctypes.c_void_p(s.cuda_stream)))
         return out
+
+    def stress_device(self, u, stress=None, status=None, stream=None):
+        """elastoplastic_stress of every element at displacement u (device
+        tensor, 3 per node) from the committed history and modes: (stress
+        (elements, STRESS_FIELDS), status (elements,) int32) device tensors."""
+        import torch
+        L = lib()
+        L.sparlab_system_stress_device.argtypes = [ctypes.c_void_p] * 5
+        if stress is None:
+            stress = torch.empty((self.num_elements, STRESS_FIELDS), dtype=torch.float64, device=u.device)
+        if status is None:
+            status = torch.empty((self.num_elements,), dtype=torch.int32, device=u.device)
+        s = stream if stream is not None else torch.cuda.current_stream(u.device)
+        _check(L.sparlab_system_stress_device(self.h, ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(stress.data_ptr()),
+                                              ctypes.c_void_p(status.data_ptr()), ctypes.c_void_p(s.cuda_stream)))
+        return stress, status
+
+    def stress(self, u):
+        """The same from a host array; raises SolverError for the lowest
+        failing element, as the reference's call throws."""
+        import torch
+        d_u = torch.as_tensor(np.ascontiguousarray(u, dtype=np.float64).reshape(-1), device="cuda")
+        stress, status = self.stress_device(d_u)
+        torch.cuda.synchronize()
+        status = status.cpu().numpy()
+        bad = np.nonzero(status)[0]
+        if bad.size:
+            e = int(bad[0])
+            raise SolverError(f"Hex8 element {e}: {ELEMENT_STATUS.get(int(status[e]), 'unknown')}", e, int(status[e]))
+        out = stress.cpu().numpy()
+        return {name: out[:, off:off + width] if width > 1 else out[:, off] for name, off, width in STRESS_LAYOUT}
 
     def element_outputs(self):
         fe = np.zeros((self.num_elements, 24))

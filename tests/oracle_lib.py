@@ -201,6 +201,25 @@ def element_batch(x0, ue, mat, states, alpha=None, rule=(2, 0, 2), kinematics=2,
     return dict(force=f, tangent=k, alpha=a_out, states=s_out, scalars=sc, status=status)
 
 
+STRESS_FIELDS = 28  # cauchy 6, piola_kirchhoff 6, kirchhoff 6, logarithmic_strain 6, von_mises,
+#                    max_equivalent_plastic_strain, max_strain, min_jacobian
+
+
+def element_stress_batch(x0, ue, mat, states, alpha=None, rule=(2, 0, 2), kinematics=2, threads=0):
+    """incompatible_elastoplastic_stress for a batch at committed states
+    (E, P, 39) and modes (E, 9) or None: (E, 28) measures and statuses."""
+    ne = x0.shape[0]
+    x0 = np.ascontiguousarray(x0, dtype=np.float64)
+    ue = np.ascontiguousarray(ue, dtype=np.float64)
+    states = np.ascontiguousarray(states, dtype=np.float64)
+    a_in = None if alpha is None else np.ascontiguousarray(alpha, dtype=np.float64)
+    out = np.zeros((ne, STRESS_FIELDS))
+    status = np.zeros(ne, dtype=np.int32)
+    lib().oracle_element_stress_batch(ctypes.c_longlong(ne), dp(x0), dp(ue), dp(states), dp(a_in), dp(mat),
+                                      rule[0], rule[1], rule[2], kinematics, threads, dp(out), ip(status))
+    return dict(stress=out, status=status)
+
+
 def standard_force(x0, ue, mat, states=None, rule=(2, 0, 2), kinematics=2):
     x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(24)
     ue = np.ascontiguousarray(ue, dtype=np.float64).reshape(24)
