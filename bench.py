@@ -290,6 +290,22 @@ def assembly_leg(G, W, torch, dev, reps):
         G.add_node_blocks_device(bufs["blocks"], bufs["active"], first, row_len, values, column_major=True,
                                  stream=stream)
 
+    # the increment set-up and the anticipation over the same slaves (a
+    # correction that pushes every node up by 0.3 mm)
+    du = torch.zeros(3 * nn, dtype=torch.float64, device=dev)
+    du[2::3] = 3e-4
+
+    cand_bufs = G.contact_candidates_device(tool, d_coords, d_u, d_nodes, 2e-4, stream=stream)
+    cand_bufs = (*cand_bufs, torch.zeros(nodes.size + nodes.size // 256 + 1, dtype=torch.int32, device=dev))
+    ant_bufs = G.contact_anticipate_device(tool, d_coords, d_u, du, d_nodes, d_kappa, residual, stream=stream)
+
+    def candidates():
+        G.contact_candidates_device(tool, d_coords, d_u, d_nodes, 2e-4, stream=stream, buffers=cand_bufs)
+
+    def anticipate():
+        G.contact_anticipate_device(tool, d_coords, d_u, du, d_nodes, d_kappa, residual, stream=stream,
+                                    buffers=ant_bufs)
+
     # the same element matrices stored column-major (Eigen's storage of the reference's Ke)
     d_ke_cm = d_ke.reshape(ne, 24, 24).transpose(1, 2).contiguous().reshape(-1)
     stages = (("values", lambda: sysm.assemble_device(ke=d_ke, values=values, column_major=True, stream=stream)),
@@ -297,7 +313,7 @@ def assembly_leg(G, W, torch, dev, reps):
               ("values_ke_cm", lambda: sysm.assemble_device(ke=d_ke_cm, values=values, column_major=True,
                                                             ke_column_major=True, stream=stream)),
               ("force", lambda: sysm.assemble_device(fe=d_fe, fint=fint, gross=gross, stream=stream)),
-              ("contact", contact), ("node_blocks", blocks))
+              ("contact", contact), ("node_blocks", blocks), ("candidates", candidates), ("anticipate", anticipate))
     us = {}
     for name, fn in stages:
         fn()
@@ -322,7 +338,8 @@ def assembly_leg(G, W, torch, dev, reps):
                                                "reference's Ke), column-major values",
                                        "us": us["values_ke_cm"], "GBps": gbs_cm, "hbm_frac": gbs_cm / HBM_PEAK_GBS},
             "force_us": us["force"], "contact_us": us["contact"],
-            "node_blocks_us": us["node_blocks"], "contact_active_nodes": n_active}
+            "node_blocks_us": us["node_blocks"], "contact_active_nodes": n_active,
+            "contact_candidates_us": us["candidates"], "contact_anticipate_us": us["anticipate"]}
 
 
 def copy_probe(torch, mib):
@@ -474,7 +491,8 @@ def main():
               f"column-major K_e {assembly['values_ke_column_major']['us']:.1f} us = "
               f"{assembly['values_ke_column_major']['GBps']:.0f} GB/s), force gather {assembly['force_us']:.1f} us, "
               f"contact {assembly['contact_us']:.1f} us ({assembly['contact_active_nodes']} active), node blocks "
-              f"{assembly['node_blocks_us']:.1f} us", file=sys.stderr, flush=True)
+              f"{assembly['node_blocks_us']:.1f} us, candidates {assembly['contact_candidates_us']:.1f} us, "
+              f"anticipation {assembly['contact_anticipate_us']:.1f} us", file=sys.stderr, flush=True)
 
     line = {
         "metric": METRIC, "value": value, "unit": "element evals/s", "n_gpus": 1, "steps": args.steps,

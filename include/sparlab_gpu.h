@@ -402,6 +402,31 @@ int sparlab_contact_evaluate_device(const sparlab_tool* tool, int friction_symme
                                     double* trial_relative, double* work, int32_t* work_status,
                                     double* scalars, void* stream);
 
+/* ToolContact::begin_increment (src/fem/RigidTool.cpp:345-393) for one tool:
+ * the slaves whose node at u0 (3 per node) lies within radius + margin of the
+ * tool's path from tool->centre0 = c(t0) to tool->centre = c(t1) (a cylinder:
+ * in the plane normal to its axis; a plane: within margin of either
+ * position), in slave order: candidates (int32 slave indices, num_slaves
+ * capacity) and count (int32[1]). work: num_slaves + num_slaves / 256 + 1
+ * int32 of scratch. Device pointers; asynchronous on stream. */
+int sparlab_contact_candidates_device(const sparlab_tool* tool, const double* coords, const double* u0,
+                                      int64_t num_slaves, const int32_t* nodes, double margin, int32_t* candidates,
+                                      int32_t* count, int32_t* work, void* stream);
+
+/* ToolContact::anticipate (src/fem/RigidTool.cpp:557-603) for one tool at
+ * tool->centre over the listed slaves (nodes and kappa_area per listed slave,
+ * e.g. the candidates): a slave whose node lies outside the tool at u and
+ * that the Newton correction du (3 per node) drives into it (or through it)
+ * gets residual += kappa_area g n (residual: 3 per node) and the block
+ * kappa_area n n^T (blocks: 9 per listed slave, row-major, zero for the
+ * others) with active = 1; added (int32[1]) counts them. The blocks go into
+ * the matrix with sparlab_add_node_blocks_device. Device pointers;
+ * asynchronous on stream. */
+int sparlab_contact_anticipate_device(const sparlab_tool* tool, const double* coords, const double* u,
+                                      const double* du, int64_t num_slaves, const int32_t* nodes,
+                                      const double* kappa_area, double* residual, double* blocks, int32_t* active,
+                                      int32_t* added, void* stream);
+
 /* add_node_block (src/fem/Forming.cpp:280-297) for every active slave: its
  * block into the assembled values at the node's diagonal block
  * (first_value, row_len per slave: sparlab_system_diagonal_blocks and 3 times

@@ -11,6 +11,7 @@
 #include "om.hpp"
 
 #include <algorithm>
+#include <limits>
 #include <vector>
 
 namespace oracle {
@@ -157,6 +158,103 @@ inline double gap(const Tool& t, const double x[3], double n[3], double dn[3][3]
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) dn[i][j] = (base[i][j] - n[i] * n[j]) / d;
   return d - t.radius;
+}
+
+// RigidTool.cpp:44-50: the distance from p to the segment [a, b].
+inline double segment_distance(const double p[3], const double a[3], const double b[3]) {
+  const double s[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  const double len2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+  double t = 0.0;
+  if (len2 > 0.0) t = std::clamp(((p[0] - a[0]) * s[0] + (p[1] - a[1]) * s[1] + (p[2] - a[2]) * s[2]) / len2, 0.0, 1.0);
+  const double r[3] = {p[0] - a[0] - t * s[0], p[1] - a[1] - t * s[1], p[2] - a[2] - t * s[2]};
+  return std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+}
+
+// ToolContact::begin_increment (RigidTool.cpp:345-393) for one tool, three
+// dimensions: the slaves (in slave order) whose node at u0 lies within
+// radius + margin of the tool's path from c0 = centre0 to c1 = centre (a
+// cylinder: in the plane normal to its axis; a plane: within margin of
+// either position).
+inline std::vector<int> contact_candidates(const Tool& t, const double* coords, const double* u0, int ns,
+                                           const int* node, double margin) {
+  std::vector<int> out;
+  const double* c0 = t.centre0;
+  const double* c1 = t.centre;
+  for (int i = 0; i < ns; ++i) {
+    const long n = node[i];
+    double x[3];
+    for (int k = 0; k < 3; ++k) x[k] = coords[3 * n + k] + u0[3 * n + k];
+    bool candidate = false;
+    if (t.shape == 1) {
+      candidate = segment_distance(x, c0, c1) <= t.radius + margin;
+    } else if (t.shape == 2) {
+      const double an = std::sqrt(t.axis[0] * t.axis[0] + t.axis[1] * t.axis[1] + t.axis[2] * t.axis[2]);
+      const double a[3] = {t.axis[0] / an, t.axis[1] / an, t.axis[2] / an};
+      double px[3], pc[3];
+      const double d0[3] = {x[0] - c0[0], x[1] - c0[1], x[2] - c0[2]};
+      const double d1[3] = {c1[0] - c0[0], c1[1] - c0[1], c1[2] - c0[2]};
+      for (int r = 0; r < 3; ++r) {
+        px[r] = 0.0;
+        pc[r] = 0.0;
+        for (int k = 0; k < 3; ++k) {
+          const double pk = (r == k ? 1.0 : 0.0) - a[r] * a[k];
+          px[r] += pk * d0[k];
+          pc[r] += pk * d1[k];
+        }
+      }
+      const double zero[3] = {0.0, 0.0, 0.0};
+      candidate = segment_distance(px, zero, pc) <= t.radius + margin;
+    } else {
+      const double nn = std::sqrt(t.normal[0] * t.normal[0] + t.normal[1] * t.normal[1] + t.normal[2] * t.normal[2]);
+      const double nv[3] = {t.normal[0] / nn, t.normal[1] / nn, t.normal[2] / nn};
+      const double g0 = nv[0] * (x[0] - c0[0]) + nv[1] * (x[1] - c0[1]) + nv[2] * (x[2] - c0[2]);
+      const double g1 = nv[0] * (x[0] - c1[0]) + nv[1] * (x[1] - c1[1]) + nv[2] * (x[2] - c1[2]);
+      candidate = std::min(g0, g1) <= margin;
+    }
+    if (candidate) out.push_back(i);
+  }
+  return out;
+}
+
+// ToolContact::anticipate (RigidTool.cpp:557-603) for one tool at its position
+// centre, over the listed (candidate) slaves: a slave outside the tool at u
+// whose node the correction du drives into it (or through it) gets
+// residual += kappa A g n and the block kappa A n n^T; returns their count.
+inline int contact_anticipate(const Tool& t, const double* coords, const double* u, const double* du, int ns,
+                              const int* node, const double* kappa_a, double* residual, double* blocks,
+                              int* active) {
+  int added = 0;
+  for (int i = 0; i < ns; ++i) {
+    active[i] = 0;
+    const long n = node[i];
+    double x[3], xn[3];
+    for (int k = 0; k < 3; ++k) {
+      x[k] = coords[3 * n + k] + u[3 * n + k];
+      xn[k] = x[k] + du[3 * n + k];
+    }
+    double nrm[3], dn[3][3], g = 0.0;
+    try {
+      g = gap(t, x, nrm, dn);
+    } catch (const SolverError&) {
+      continue;  // deep in the tool
+    }
+    if (!(g > 0.0)) continue;
+    double g_next = 0.0;
+    try {
+      double n2[3];
+      g_next = gap(t, xn, n2, dn);
+    } catch (const SolverError&) {
+      g_next = -std::numeric_limits<double>::infinity();
+    }
+    if (!(g_next < 0.0)) continue;
+    const double ka = kappa_a[i];
+    for (int k = 0; k < 3; ++k) residual[3 * n + k] += ka * g * nrm[k];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) blocks[9 * i + 3 * a + b] = ka * (nrm[a] * nrm[b]);
+    active[i] = 1;
+    ++added;
+  }
+  return added;
 }
 
 struct ContactOut {

@@ -384,6 +384,39 @@ void oracle_add_node_blocks(int nn, int ne, const int* conn, double* values, int
 // tool: shape, centre[3], centre0[3], radius, normal[3], axis[3], friction,
 // tangential_ratio (16 doubles). scalars_out: gross, round_off, force_norm,
 // resultant[3], active, slipping, max_penetration, symmetric (10).
+// The tool record of the contact entries: shape, centre 3, centre0 3, radius,
+// normal 3, axis 3, friction, tangential ratio.
+static Tool tool_from_record(const double* tool) {
+  Tool t;
+  t.shape = static_cast<int>(tool[0]);
+  for (int i = 0; i < 3; ++i) {
+    t.centre[i] = tool[1 + i];
+    t.centre0[i] = tool[4 + i];
+    t.normal[i] = tool[8 + i];
+    t.axis[i] = tool[11 + i];
+  }
+  t.radius = tool[7];
+  t.friction = tool[14];
+  t.tangential_ratio = tool[15];
+  return t;
+}
+
+// ToolContact::begin_increment for one tool; candidates (ns capacity) gets
+// the candidate slave indices in order; returns their count.
+int oracle_contact_candidates(const double* tool, const double* coords, const double* u0, int ns, const int* node,
+                              double margin, int* candidates) {
+  const std::vector<int> c = contact_candidates(tool_from_record(tool), coords, u0, ns, node, margin);
+  for (std::size_t i = 0; i < c.size(); ++i) candidates[i] = c[i];
+  return static_cast<int>(c.size());
+}
+
+// ToolContact::anticipate for one tool over the listed slaves; returns the
+// number added.
+int oracle_contact_anticipate(const double* tool, const double* coords, const double* u, const double* du, int ns,
+                              const int* node, const double* kappa_a, double* residual, double* blocks, int* active) {
+  return contact_anticipate(tool_from_record(tool), coords, u, du, ns, node, kappa_a, residual, blocks, active);
+}
+
 int oracle_contact(const double* tool, int friction_symmetric, int nn, const double* coords, const double* u,
                    const double* start, int ns, const int* node, const double* kappa_a, const int* h_present,
                    const double* h_force, const double* h_relative, int want_tangent, double* residual,

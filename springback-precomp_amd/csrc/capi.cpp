@@ -23,6 +23,29 @@ int fail(int code, const std::string& message) {
 
 bool finite_positive(double v) { return std::isfinite(v) && v > 0.0; }
 
+// The kernels' copy of a tool record.
+sbk::ContactTool contact_tool(const sparlab_tool& tool) {
+  sbk::ContactTool t{};
+  t.shape = tool.shape;
+  for (int k = 0; k < 3; ++k) {
+    t.centre[k] = tool.centre[k];
+    t.centre0[k] = tool.centre0[k];
+    t.normal[k] = tool.normal[k];
+    t.axis[k] = tool.axis[k];
+  }
+  t.radius = tool.radius;
+  t.friction = tool.friction;
+  t.tangential_ratio = tool.tangential_ratio;
+  return t;
+}
+
+// What is wrong with a tool record, or null.
+const char* tool_problem(const sparlab_tool& tool) {
+  if (tool.shape < 0 || tool.shape > 2) return "tool shape must be 0, 1 or 2";
+  if (tool.shape != 0 && !(tool.radius > 0.0)) return "a sphere or cylinder needs a radius";
+  return nullptr;
+}
+
 // Gauss-Legendre abscissae and weights on [-1, 1]: the closed forms for 1..4
 // points and Newton's iteration on P_n for 5..7
 // (src/elements/Quadrature.cpp:16-48, 100-139).
@@ -490,6 +513,36 @@ int sparlab_elements_evaluate_device(const sparlab_material* material, const spa
   return sparlab_elements_evaluate_stage_device(material, rule, kinematics, want_tangent, b, 3, stream);
 }
 
+int sparlab_contact_candidates_device(const sparlab_tool* tool, const double* coords, const double* u0,
+                                      int64_t num_slaves, const int32_t* nodes, double margin, int32_t* candidates,
+                                      int32_t* count, int32_t* work, void* stream) {
+  if (tool == nullptr || coords == nullptr || u0 == nullptr || count == nullptr ||
+      (num_slaves > 0 && (nodes == nullptr || candidates == nullptr || work == nullptr)))
+    return fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (num_slaves < 0 || num_slaves > (int64_t{1} << 31) - 257) return fail(SPARLAB_E_ARGUMENT, "bad slave count");
+  if (const char* why = tool_problem(*tool)) return fail(SPARLAB_E_ARGUMENT, why);
+  if (sbk::launch_contact_candidates(contact_tool(*tool), coords, u0, num_slaves, nodes, margin, candidates, count, work,
+                                     static_cast<hipStream_t>(stream)) != 0)
+    return fail(SPARLAB_E_DEVICE, "candidate launch failed");
+  return SPARLAB_OK;
+}
+
+int sparlab_contact_anticipate_device(const sparlab_tool* tool, const double* coords, const double* u,
+                                      const double* du, int64_t num_slaves, const int32_t* nodes,
+                                      const double* kappa_area, double* residual, double* blocks, int32_t* active,
+                                      int32_t* added, void* stream) {
+  if (tool == nullptr || coords == nullptr || u == nullptr || du == nullptr || residual == nullptr ||
+      added == nullptr || (num_slaves > 0 && (nodes == nullptr || kappa_area == nullptr || blocks == nullptr ||
+                                              active == nullptr)))
+    return fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (num_slaves < 0) return fail(SPARLAB_E_ARGUMENT, "bad slave count");
+  if (const char* why = tool_problem(*tool)) return fail(SPARLAB_E_ARGUMENT, why);
+  if (sbk::launch_contact_anticipate(contact_tool(*tool), coords, u, du, num_slaves, nodes, kappa_area, residual,
+                                     blocks, active, added, static_cast<hipStream_t>(stream)) != 0)
+    return fail(SPARLAB_E_DEVICE, "anticipation launch failed");
+  return SPARLAB_OK;
+}
+
 int sparlab_contact_evaluate_device(const sparlab_tool* tool, int friction_symmetric, int want_tangent,
                                     const double* coords, const double* u, const double* start,
                                     int64_t num_slaves, const int32_t* nodes, const double* kappa_area,
@@ -503,21 +556,11 @@ int sparlab_contact_evaluate_device(const sparlab_tool* tool, int friction_symme
       work == nullptr || work_status == nullptr || scalars == nullptr || (num_slaves > 0 && (nodes == nullptr || kappa_area == nullptr)))
     return fail(SPARLAB_E_ARGUMENT, "null argument");
   if (want_tangent && blocks == nullptr) return fail(SPARLAB_E_ARGUMENT, "want_tangent needs a blocks buffer");
-  if (tool->shape < 0 || tool->shape > 2) return fail(SPARLAB_E_ARGUMENT, "tool shape must be 0, 1 or 2");
-  if (tool->shape != 0 && !(tool->radius > 0.0)) return fail(SPARLAB_E_ARGUMENT, "a sphere or cylinder needs a radius");
+  if (const char* why = tool_problem(*tool)) return fail(SPARLAB_E_ARGUMENT, why);
   if (hist_present != nullptr && (hist_force == nullptr || hist_relative == nullptr))
     return fail(SPARLAB_E_ARGUMENT, "a friction history needs its forces and positions");
   sbk::ContactArgs a{};
-  a.tool.shape = tool->shape;
-  for (int k = 0; k < 3; ++k) {
-    a.tool.centre[k] = tool->centre[k];
-    a.tool.centre0[k] = tool->centre0[k];
-    a.tool.normal[k] = tool->normal[k];
-    a.tool.axis[k] = tool->axis[k];
-  }
-  a.tool.radius = tool->radius;
-  a.tool.friction = tool->friction;
-  a.tool.tangential_ratio = tool->tangential_ratio;
+  a.tool = contact_tool(*tool);
   a.friction_symmetric = friction_symmetric;
   a.want_tangent = want_tangent;
   a.coords = coords;

@@ -14,6 +14,9 @@
 //   norms, the resultant, counts, the largest penetration, symmetry.
 // add_node_blocks_kernel: each active node's 3 x 3 block into its diagonal
 //   block of the assembled values (row- or column-major).
+// candidate_*_kernel: the increment's candidate slaves in slave order (flags,
+//   block offsets, an in-block scan); anticipate_kernel: the penalty of the
+//   nodes a Newton correction would drive into the tool.
 #include "contact_kernels.hpp"
 
 namespace sbk {
@@ -22,6 +25,48 @@ __device__ __forceinline__ double dot3(const double a[3], const double b[3]) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 __device__ __forceinline__ double norm3(const double a[3]) { return sqrt(dot3(a, a)); }
+
+// RigidTool::gap (RigidTool.cpp:237-270) at x for the tool at c: the gap g,
+// the normal n and its derivative dn; false when the node lies less than half
+// the radius from a sphere's centre or a cylinder's axis (the tool is through
+// the surface: the reference's SolverError).
+__device__ __forceinline__ bool tool_gap(const ContactTool& T, const double x[3], const double c[3], double n[3],
+                                         double dn[3][3], double& g) {
+  if (T.shape == 0) {
+    const double nn = norm3(T.normal);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) n[k] = T.normal[k] / nn;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) dn[a][b] = 0.0;
+    g = n[0] * (x[0] - c[0]) + n[1] * (x[1] - c[1]) + n[2] * (x[2] - c[2]);
+    return true;
+  }
+  double r[3] = {x[0] - c[0], x[1] - c[1], x[2] - c[2]};
+  double base[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  if (T.shape == 2) {
+    const double an = norm3(T.axis);
+    const double ax[3] = {T.axis[0] / an, T.axis[1] / an, T.axis[2] / an};
+    const double ar = dot3(ax, r);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r[k] = r[k] - ar * ax[k];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) base[a][b] = base[a][b] - ax[a] * ax[b];
+  }
+  const double d = norm3(r);
+  if (!(d > 0.5 * T.radius)) return false;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) n[k] = r[k] / d;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) dn[a][b] = (base[a][b] - n[a] * n[b]) / d;
+  g = d - T.radius;
+  return true;
+}
 
 __global__ __launch_bounds__(256) void contact_node_kernel(ContactArgs A) {
   const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -42,45 +87,12 @@ __global__ __launch_bounds__(256) void contact_node_kernel(ContactArgs A) {
   A.node_terms[i * 8 + 6] = 0.0;  // slipping
   A.node_terms[i * 8 + 7] = 0.0;  // asymmetric block
   A.status[i] = 0;
-  // gap (RigidTool.cpp:237-270)
   double n[3], dn[3][3];
   double g;
   const double* c = T.centre;
-  if (T.shape == 0) {
-    const double nn = norm3(T.normal);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) n[k] = T.normal[k] / nn;
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b) dn[a][b] = 0.0;
-    g = n[0] * (x[0] - c[0]) + n[1] * (x[1] - c[1]) + n[2] * (x[2] - c[2]);
-  } else {
-    double r[3] = {x[0] - c[0], x[1] - c[1], x[2] - c[2]};
-    double base[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-    if (T.shape == 2) {
-      const double an = norm3(T.axis);
-      const double ax[3] = {T.axis[0] / an, T.axis[1] / an, T.axis[2] / an};
-      const double ar = dot3(ax, r);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) r[k] = r[k] - ar * ax[k];
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) base[a][b] = base[a][b] - ax[a] * ax[b];
-    }
-    const double d = norm3(r);
-    if (!(d > 0.5 * T.radius)) {
-      A.status[i] = 1;  // through the tool: SolverError
-      return;
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) n[k] = r[k] / d;
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b) dn[a][b] = (base[a][b] - n[a] * n[b]) / d;
-    g = d - T.radius;
+  if (!tool_gap(T, x, c, n, dn, g)) {
+    A.status[i] = 1;  // through the tool: SolverError
+    return;
   }
   if (!(g < 0.0)) return;
   const double ka = A.kappa_area[i];
@@ -292,6 +304,209 @@ __global__ __launch_bounds__(256) void add_node_blocks_kernel(const double* __re
       const long long pos = column_major ? f + b * rl + a : f + a * rl + b;
       values[pos] = values[pos] + blocks[i * 9 + a * 3 + b];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Increment set-up (ToolContact::begin_increment, RigidTool.cpp:345-393) and
+// the semismooth anticipation (ToolContact::anticipate, :557-603)
+// ---------------------------------------------------------------------------
+// RigidTool.cpp:44-50: the distance from p to the segment [a, b].
+__device__ __forceinline__ double segment_distance(const double p[3], const double a[3], const double b[3]) {
+  const double s[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  const double len2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+  double t = 0.0;
+  if (len2 > 0.0)
+    t = fmin(fmax(((p[0] - a[0]) * s[0] + (p[1] - a[1]) * s[1] + (p[2] - a[2]) * s[2]) / len2, 0.0), 1.0);
+  const double r[3] = {p[0] - a[0] - t * s[0], p[1] - a[1] - t * s[1], p[2] - a[2] - t * s[2]};
+  return sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+}
+
+// A node at x is a candidate of the increment whose tool moves from centre0
+// to centre: within radius + margin of that path (a cylinder: in the plane
+// normal to its axis), or for a plane within margin of either position.
+__device__ __forceinline__ bool is_candidate(const ContactTool& T, const double x[3], double margin) {
+  const double* c0 = T.centre0;
+  const double* c1 = T.centre;
+  if (T.shape == 1) return segment_distance(x, c0, c1) <= T.radius + margin;
+  if (T.shape == 2) {
+    const double an = norm3(T.axis);
+    const double a[3] = {T.axis[0] / an, T.axis[1] / an, T.axis[2] / an};
+    const double d0[3] = {x[0] - c0[0], x[1] - c0[1], x[2] - c0[2]};
+    const double d1[3] = {c1[0] - c0[0], c1[1] - c0[1], c1[2] - c0[2]};
+    double px[3], pc[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      px[r] = 0.0;
+      pc[r] = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double pk = (r == k ? 1.0 : 0.0) - a[r] * a[k];
+        px[r] = px[r] + pk * d0[k];
+        pc[r] = pc[r] + pk * d1[k];
+      }
+    }
+    const double zero[3] = {0.0, 0.0, 0.0};
+    return segment_distance(px, zero, pc) <= T.radius + margin;
+  }
+  const double nn = norm3(T.normal);
+  const double nv[3] = {T.normal[0] / nn, T.normal[1] / nn, T.normal[2] / nn};
+  const double g0 = nv[0] * (x[0] - c0[0]) + nv[1] * (x[1] - c0[1]) + nv[2] * (x[2] - c0[2]);
+  const double g1 = nv[0] * (x[0] - c1[0]) + nv[1] * (x[1] - c1[1]) + nv[2] * (x[2] - c1[2]);
+  return fmin(g0, g1) <= margin;
+}
+
+// Candidate flags and each block's count (blocks of 256 slaves).
+__global__ __launch_bounds__(256) void candidate_flags_kernel(ContactTool T, const double* __restrict__ coords,
+                                                              const double* __restrict__ u0, long long ns,
+                                                              const int* __restrict__ nodes, double margin,
+                                                              int* __restrict__ flags, int* __restrict__ block_counts) {
+  __shared__ int cnt[256];
+  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  int f = 0;
+  if (i < ns) {
+    const long long node = nodes[i];
+    double x[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) x[k] = coords[node * 3 + k] + u0[node * 3 + k];
+    f = is_candidate(T, x, margin) ? 1 : 0;
+    flags[i] = f;
+  }
+  cnt[threadIdx.x] = f;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) cnt[threadIdx.x] += cnt[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) block_counts[blockIdx.x] = cnt[0];
+}
+
+// Exclusive offsets of the block counts in place, and the total (one block:
+// each thread scans a contiguous run of the counts).
+__global__ __launch_bounds__(256) void candidate_offsets_kernel(int* __restrict__ block_counts, int nb,
+                                                                int* __restrict__ count) {
+  __shared__ int part[256];
+  const int per = (nb + 255) / 256;
+  const int b0 = static_cast<int>(threadIdx.x) * per;
+  const int b1 = b0 + per < nb ? b0 + per : nb;
+  int run = 0;
+  for (int b = b0; b < b1; ++b) run += block_counts[b];
+  part[threadIdx.x] = run;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int k = 0; k < 256; ++k) {
+      const int v = part[k];
+      part[k] = acc;
+      acc += v;
+    }
+    count[0] = acc;
+  }
+  __syncthreads();
+  run = part[threadIdx.x];
+  for (int b = b0; b < b1; ++b) {
+    const int v = block_counts[b];
+    block_counts[b] = run;
+    run += v;
+  }
+}
+
+// The candidates in slave order: each block places its flagged slaves after
+// the earlier blocks' (an in-block exclusive scan of the flags).
+__global__ __launch_bounds__(256) void candidate_scatter_kernel(const int* __restrict__ flags, long long ns,
+                                                                const int* __restrict__ block_offsets,
+                                                                int* __restrict__ candidates) {
+  __shared__ int scan[256];
+  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  const int f = i < ns ? flags[i] : 0;
+  scan[threadIdx.x] = f;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int add = static_cast<int>(threadIdx.x) >= off ? scan[threadIdx.x - off] : 0;
+    __syncthreads();
+    scan[threadIdx.x] += add;
+    __syncthreads();
+  }
+  if (f) candidates[block_offsets[blockIdx.x] + scan[threadIdx.x] - 1] = static_cast<int>(i);
+}
+
+// One thread per listed slave: a node outside the tool at u that the
+// correction du drives into it (or through it) gets residual += kappa A g n
+// and the block kappa A n n^T; the others get active = 0 and a zero block.
+__global__ __launch_bounds__(256) void anticipate_kernel(ContactTool T, const double* __restrict__ coords,
+                                                         const double* __restrict__ u, const double* __restrict__ du,
+                                                         long long ns, const int* __restrict__ nodes,
+                                                         const double* __restrict__ kappa_area,
+                                                         double* __restrict__ residual, double* __restrict__ blocks,
+                                                         int* __restrict__ active) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  active[i] = 0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) blocks[i * 9 + k] = 0.0;
+  const long long node = nodes[i];
+  double x[3], xn[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    x[k] = coords[node * 3 + k] + u[node * 3 + k];
+    xn[k] = x[k] + du[node * 3 + k];
+  }
+  double n[3], dn[3][3], g;
+  if (!tool_gap(T, x, T.centre, n, dn, g)) return;  // deep in the tool
+  if (!(g > 0.0)) return;                           // already active: in the tangent
+  double n2[3], g_next;
+  if (!tool_gap(T, xn, T.centre, n2, dn, g_next)) g_next = -__builtin_inf();  // through the tool
+  if (!(g_next < 0.0)) return;
+  const double ka = kappa_area[i];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) residual[node * 3 + k] = residual[node * 3 + k] + ka * g * n[k];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) blocks[i * 9 + 3 * a + b] = ka * (n[a] * n[b]);
+  active[i] = 1;
+}
+
+// count[0] = sum of flags[0, n) (one block, fixed order).
+__global__ __launch_bounds__(256) void count_flags_kernel(const int* __restrict__ flags, long long n,
+                                                          int* __restrict__ count) {
+  __shared__ int s[256];
+  int acc = 0;
+  for (long long i = threadIdx.x; i < n; i += 256) acc += flags[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) count[0] = s[0];
+}
+
+int launch_contact_candidates(const ContactTool& t, const double* coords, const double* u0, long long ns,
+                              const int* nodes, double margin, int* candidates, int* count, int* work,
+                              hipStream_t s) {
+  (void)hipGetLastError();
+  const long long nb = (ns + 255) / 256;
+  int* flags = work;
+  int* block_counts = work + ns;
+  if (nb > 0)
+    hipLaunchKernelGGL(candidate_flags_kernel, dim3(static_cast<unsigned int>(nb)), dim3(256), 0, s, t, coords, u0,
+                       ns, nodes, margin, flags, block_counts);
+  hipLaunchKernelGGL(candidate_offsets_kernel, dim3(1), dim3(256), 0, s, block_counts, static_cast<int>(nb), count);
+  if (nb > 0)
+    hipLaunchKernelGGL(candidate_scatter_kernel, dim3(static_cast<unsigned int>(nb)), dim3(256), 0, s, flags, ns,
+                       block_counts, candidates);
+  return static_cast<int>(hipGetLastError());
+}
+
+int launch_contact_anticipate(const ContactTool& t, const double* coords, const double* u, const double* du,
+                              long long ns, const int* nodes, const double* kappa_area, double* residual,
+                              double* blocks, int* active, int* added, hipStream_t s) {
+  (void)hipGetLastError();
+  if (ns > 0)
+    hipLaunchKernelGGL(anticipate_kernel, dim3(static_cast<unsigned int>((ns + 255) / 256)), dim3(256), 0, s, t,
+                       coords, u, du, ns, nodes, kappa_area, residual, blocks, active);
+  hipLaunchKernelGGL(count_flags_kernel, dim3(1), dim3(256), 0, s, active, ns, added);
+  return static_cast<int>(hipGetLastError());
 }
 
 int launch_contact(const ContactArgs& a, double* scalars, hipStream_t s) {

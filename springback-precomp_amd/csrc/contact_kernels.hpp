@@ -39,5 +39,12 @@ struct ContactArgs {
 int launch_contact(const ContactArgs& a, double* scalars, hipStream_t s);
 int launch_add_node_blocks(const double* blocks, const int* active, const int* first_value, const int* row_len,
                            long long ns, int column_major, double* values, hipStream_t s);
+// work: ns + ceil(ns / 256) int32
+int launch_contact_candidates(const ContactTool& t, const double* coords, const double* u0, long long ns,
+                              const int* nodes, double margin, int* candidates, int* count, int* work,
+                              hipStream_t s);
+int launch_contact_anticipate(const ContactTool& t, const double* coords, const double* u, const double* du,
+                              long long ns, const int* nodes, const double* kappa_area, double* residual,
+                              double* blocks, int* active, int* added, hipStream_t s);
 
 }  // namespace sbk

@@ -829,6 +829,49 @@ def contact_evaluate_device(tool, coords, u, start, nodes, kappa_area, history=N
     return out
 
 
+def contact_candidates_device(tool, coords, u0, nodes, margin, stream=None, buffers=None):
+    """ToolContact::begin_increment for one tool (centre0 = c(t0), centre =
+    c(t1)) on device tensors: (candidates int32 (num_slaves capacity), count
+    int32[1]); the first count entries are the candidate slave indices in
+    order. `buffers` (candidates, count, work) reuses an earlier call's."""
+    import torch
+    dev = coords.device
+    ns = nodes.shape[0]
+    if buffers is None:
+        buffers = (torch.zeros(ns, dtype=torch.int32, device=dev), torch.zeros(1, dtype=torch.int32, device=dev),
+                   torch.zeros(ns + ns // 256 + 1, dtype=torch.int32, device=dev))
+    candidates, count, work = buffers
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    L = lib()
+    L.sparlab_contact_candidates_device.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int64, ctypes.c_void_p,
+                                                                            ctypes.c_double] + [ctypes.c_void_p] * 4
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    _check(L.sparlab_contact_candidates_device(ctypes.byref(tool), p(coords), p(u0), ns, p(nodes), float(margin),
+                                               p(candidates), p(count), p(work), ctypes.c_void_p(s.cuda_stream)))
+    return candidates, count
+
+
+def contact_anticipate_device(tool, coords, u, du, nodes, kappa_area, residual, stream=None, buffers=None):
+    """ToolContact::anticipate for one tool over the listed slaves on device
+    tensors: adds into `residual` (3 per node) and returns (blocks (n, 3, 3),
+    active int32, added int32[1]); `buffers` reuses an earlier call's."""
+    import torch
+    dev = coords.device
+    ns = nodes.shape[0]
+    if buffers is None:
+        buffers = (torch.zeros((ns, 3, 3), dtype=torch.float64, device=dev), torch.zeros(ns, dtype=torch.int32, device=dev),
+                   torch.zeros(1, dtype=torch.int32, device=dev))
+    blocks, active, added = buffers
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    L = lib()
+    L.sparlab_contact_anticipate_device.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int64] + [ctypes.c_void_p] * 7
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    _check(L.sparlab_contact_anticipate_device(ctypes.byref(tool), p(coords), p(u), p(du), ns, p(nodes), p(kappa_area),
+                                               p(residual), p(blocks), p(active), p(added),
+                                               ctypes.c_void_p(s.cuda_stream)))
+    return blocks, active, added
+
+
 def add_node_blocks_device(blocks, active, first_value, row_len, values, column_major=False, stream=None):
     import torch
     s = stream if stream is not None else torch.cuda.current_stream(values.device)

@@ -288,6 +288,33 @@ def tool_record(shape=1, centre=(0, 0, 0), centre0=None, radius=0.0, normal=(0, 
                     dtype=np.float64)
 
 
+def contact_candidates(tool, coords, u0, nodes, margin):
+    """ToolContact::begin_increment for one tool: candidate slave indices."""
+    coords = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+    u0 = np.ascontiguousarray(u0, dtype=np.float64).reshape(-1)
+    nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+    out = np.zeros(nodes.shape[0], np.int32)
+    n = lib().oracle_contact_candidates(dp(tool), dp(coords), dp(u0), nodes.shape[0], ip(nodes),
+                                        ctypes.c_double(margin), ip(out))
+    return out[:n].copy()
+
+
+def contact_anticipate(tool, coords, u, du, nodes, kappa_a):
+    """ToolContact::anticipate for one tool over the listed slaves:
+    (residual 3 nn, blocks (ns, 9), active (ns,), added)."""
+    coords = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+    nn = coords.shape[0] // 3
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+    du = np.ascontiguousarray(du, dtype=np.float64).reshape(-1)
+    nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+    kappa_a = np.ascontiguousarray(kappa_a, dtype=np.float64)
+    ns = nodes.shape[0]
+    residual, blocks, active = np.zeros(3 * nn), np.zeros(9 * ns), np.zeros(ns, np.int32)
+    added = lib().oracle_contact_anticipate(dp(tool), dp(coords), dp(u), dp(du), ns, ip(nodes), dp(kappa_a),
+                                            dp(residual), dp(blocks), ip(active))
+    return residual, blocks.reshape(ns, 9), active, added
+
+
 def contact(tool, coords, u, start, nodes, kappa_a, history=None, want_tangent=True, friction_symmetric=False):
     coords = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
     nn = coords.shape[0] // 3

@@ -107,3 +107,71 @@ def test_contact_refuses_a_node_through_the_tool():
     tool = O.tool_record(1, (0.005, 0.005, 0.0), radius=0.005)
     with pytest.raises(O.OracleError):
         O.contact(tool, coords, u, u, top, np.ones(top.size))
+
+
+# ---------------------------------------------------------------------------
+# The increment set-up (ToolContact::begin_increment, RigidTool.cpp:345-393)
+# and the semismooth anticipation (ToolContact::anticipate, :557-603), pinned
+# by case analysis independent of the oracle's projection formula.
+# ---------------------------------------------------------------------------
+def path_distance(px, py, pz, a0, a1, h):
+    """Distance from (px, py, pz) to the segment from (a0, 0, h) to (a1, 0, h),
+    a0 < a1, by cases on where px falls."""
+    ex = np.where(px < a0, px - a0, np.where(px > a1, px - a1, 0.0))
+    return np.sqrt(ex * ex + py * py + (pz - h) ** 2)
+
+
+def candidate_points(seed):
+    rng = np.random.default_rng(seed)
+    xy = np.stack(np.meshgrid(np.linspace(-0.01, 0.01, 21), np.linspace(-0.01, 0.01, 21)), -1).reshape(-1, 2)
+    coords = np.concatenate([xy, rng.uniform(-2e-4, 2e-4, (xy.shape[0], 1))], 1)
+    u0 = rng.uniform(-5e-5, 5e-5, coords.shape)
+    nodes = rng.permutation(coords.shape[0])[:300].astype(np.int32)
+    return coords, u0, nodes
+
+
+@pytest.mark.parametrize("shape", ["sphere", "cylinder", "plane"])
+def test_candidates_match_case_analysis(shape):
+    coords, u0, nodes = candidate_points(3)
+    a0, a1, h, radius = -0.004, 0.003, 0.0052, 0.005
+    margin = 1.2e-4
+    x = (coords + u0)[nodes]
+    if shape == "sphere":
+        tool = O.tool_record(1, (a1, 0.0, h), (a0, 0.0, h), radius=radius)
+        want = path_distance(x[:, 0], x[:, 1], x[:, 2], a0, a1, h) <= radius + margin
+    elif shape == "cylinder":  # axis y: the distance in the x-z plane
+        tool = O.tool_record(2, (a1, 0.0, h), (a0, 0.0, h), radius=radius, axis=(0.0, 1.0, 0.0))
+        want = path_distance(x[:, 0], 0.0, x[:, 2], a0, a1, h) <= radius + margin
+    else:  # normal +z, from z0 down to z1
+        z0, z1 = -1e-4, -2e-4
+        tool = O.tool_record(0, (0.0, 0.0, z1), (0.0, 0.0, z0), normal=(0.0, 0.0, 2.0))
+        want = np.minimum(x[:, 2] - z0, x[:, 2] - z1) <= margin
+    got = O.contact_candidates(tool, coords, u0, nodes, margin)
+    assert np.array_equal(got, np.nonzero(want)[0]) and 0 < got.size < nodes.size
+    # a larger margin keeps every candidate
+    wider = O.contact_candidates(tool, coords, u0, nodes, 2 * margin)
+    assert set(got) <= set(wider)
+
+
+def test_anticipation_cases():
+    """Sphere R = 5 mm at the origin, kappa A = 1e7: (a) outside, driven inside:
+    added with residual kappa A g n and block kappa A n n^T; (b) outside,
+    staying outside; (c) already inside; (d) deep in the tool (skipped);
+    (e) outside, driven through the tool (added)."""
+    radius, ka = 0.005, 1e7
+    tool = O.tool_record(1, (0.0, 0.0, 0.0), radius=radius)
+    dirs = np.array([[0.0, 0.0, 1.0], [0.6, 0.0, 0.8], [0.0, 1.0, 0.0], [1.0, 0.0, 0.0], [0.0, -0.6, 0.8]])
+    dist = np.array([5.2e-3, 5.3e-3, 4.9e-3, 2.0e-3, 5.1e-3])
+    step = np.array([-4.0e-4, -1.0e-4, -2.0e-4, -1.0e-4, -4.0e-3])
+    coords = dirs * dist[:, None]
+    du = dirs * step[:, None]
+    nodes = np.arange(5, dtype=np.int32)
+    res, blocks, active, added = O.contact_anticipate(tool, coords, np.zeros(15), du.reshape(-1), nodes, np.full(5, ka))
+    assert list(active) == [1, 0, 0, 0, 1] and added == 2
+    for i in range(5):
+        if active[i]:
+            g = dist[i] - radius
+            assert np.allclose(res[3 * i:3 * i + 3], ka * g * dirs[i], rtol=1e-14, atol=0.0)
+            assert np.allclose(blocks[i].reshape(3, 3), ka * np.outer(dirs[i], dirs[i]), rtol=1e-14, atol=1e-7)
+        else:
+            assert not res[3 * i:3 * i + 3].any() and not blocks[i].any()
