@@ -299,6 +299,17 @@ def assembly_leg(G, W, torch, dev, reps):
     cand_bufs = (*cand_bufs, torch.zeros(nodes.size + nodes.size // 256 + 1, dtype=torch.int32, device=dev))
     ant_bufs = G.contact_anticipate_device(tool, d_coords, d_u, du, d_nodes, d_kappa, residual, stream=stream)
 
+    # stiffness_gross over every DOF, on the assembled values
+    kx = torch.as_tensor(np.random.default_rng(5).normal(size=3 * nn), device=dev)
+    kfree = torch.arange(3 * nn, dtype=torch.int32, device=dev)
+    kprod = torch.empty(3 * nn, dtype=torch.float64, device=dev)
+
+    def gross_cm():
+        sysm.stiffness_gross_device(values, kx, kfree, column_major=True, product=kprod, stream=stream)
+
+    def gross_rm():
+        sysm.stiffness_gross_device(values, kx, kfree, column_major=False, product=kprod, stream=stream)
+
     def candidates():
         G.contact_candidates_device(tool, d_coords, d_u, d_nodes, 2e-4, stream=stream, buffers=cand_bufs)
 
@@ -313,7 +324,8 @@ def assembly_leg(G, W, torch, dev, reps):
               ("values_ke_cm", lambda: sysm.assemble_device(ke=d_ke_cm, values=values, column_major=True,
                                                             ke_column_major=True, stream=stream)),
               ("force", lambda: sysm.assemble_device(fe=d_fe, fint=fint, gross=gross, stream=stream)),
-              ("contact", contact), ("node_blocks", blocks), ("candidates", candidates), ("anticipate", anticipate))
+              ("contact", contact), ("node_blocks", blocks), ("candidates", candidates), ("anticipate", anticipate),
+              ("gross_cm", gross_cm), ("gross_rm", gross_rm))
     us = {}
     for name, fn in stages:
         fn()
@@ -339,7 +351,9 @@ def assembly_leg(G, W, torch, dev, reps):
                                        "us": us["values_ke_cm"], "GBps": gbs_cm, "hbm_frac": gbs_cm / HBM_PEAK_GBS},
             "force_us": us["force"], "contact_us": us["contact"],
             "node_blocks_us": us["node_blocks"], "contact_active_nodes": n_active,
-            "contact_candidates_us": us["candidates"], "contact_anticipate_us": us["anticipate"]}
+            "contact_candidates_us": us["candidates"], "contact_anticipate_us": us["anticipate"],
+            "stiffness_gross_us": us["gross_cm"], "stiffness_gross_row_major_us": us["gross_rm"],
+            "stiffness_gross_GBps": (8 * sysm.nnz + 3 * 8 * 3 * nn) / (us["gross_cm"] * 1e-6) / 1e9}
 
 
 def copy_probe(torch, mib):
@@ -492,7 +506,9 @@ def main():
               f"{assembly['values_ke_column_major']['GBps']:.0f} GB/s), force gather {assembly['force_us']:.1f} us, "
               f"contact {assembly['contact_us']:.1f} us ({assembly['contact_active_nodes']} active), node blocks "
               f"{assembly['node_blocks_us']:.1f} us, candidates {assembly['contact_candidates_us']:.1f} us, "
-              f"anticipation {assembly['contact_anticipate_us']:.1f} us", file=sys.stderr, flush=True)
+              f"anticipation {assembly['contact_anticipate_us']:.1f} us, stiffness_gross {assembly['stiffness_gross_us']:.1f} "
+              f"us ({assembly['stiffness_gross_GBps']:.0f} GB/s; row-major {assembly['stiffness_gross_row_major_us']:.1f} us)",
+              file=sys.stderr, flush=True)
 
     line = {
         "metric": METRIC, "value": value, "unit": "element evals/s", "n_gpus": 1, "steps": args.steps,

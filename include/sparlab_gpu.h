@@ -249,6 +249,19 @@ int sparlab_system_evaluate_device(sparlab_system* system, const double* u, int 
  * serial scatter. Values in the order of ke's storage read ke contiguously. */
 int sparlab_system_assemble_device(sparlab_system* system, const double* fe, const double* ke, int layout,
                                    double* fint, double* values, double* gross, void* stream);
+/* stiffness_gross (src/fem/NonlinearSystem.hpp:55-67), the round-off scale of
+ * the stiffness times the displacement (src/fem/Forming.cpp:991-993):
+ * out[0] = || (|K| |x|)_free ||_2 for the system's pattern with values (nnz;
+ * column_major as in sparlab_system_evaluate), x (3 nn) and the free DOFs
+ * free_dofs[0, num_free) (int32). product (3 nn, may be null) receives
+ * |K| |x|; each entry is the reference's sum, bit for bit (ascending columns,
+ * products rounded before they are added); the norm's sum is a fixed-order
+ * tree. Device pointers; asynchronous on stream. Calls on one system must not
+ * overlap (they share its scratch). */
+int sparlab_system_stiffness_gross_device(sparlab_system* system, const double* values, int column_major,
+                                         const double* x, const int32_t* free_dofs, int64_t num_free, double* out,
+                                         double* product, void* stream);
+
 /* Per-element stress measures (the reference's ElastoplasticStress,
  * include/sparlab/fem/Elastoplastic.hpp:206-221), SPARLAB_STRESS_FIELDS
  * doubles per element at these offsets (stresses and strains in tensorial

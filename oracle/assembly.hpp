@@ -118,6 +118,23 @@ inline double gather_force(int nn, int ne, const int* conn, const double* fe, do
   return std::sqrt(s);
 }
 
+// NonlinearSystem.hpp:55-67 (stiffness_gross) on the column-major values of
+// the pattern: product = |K| |x| column by column, then the 2-norm over the
+// listed free DOFs in their order.
+inline double stiffness_gross(const Pattern& p, const double* values, const double* x, const int* free_dofs,
+                              int num_free, double* product) {
+  const std::size_t n = p.outer.size() - 1;
+  for (std::size_t i = 0; i < n; ++i) product[i] = 0.0;
+  for (std::size_t col = 0; col < n; ++col) {
+    const double uc = std::fabs(x[col]);
+    if (uc == 0.0) continue;
+    for (int q = p.outer[col]; q < p.outer[col + 1]; ++q) product[p.inner[q]] += std::fabs(values[q]) * uc;
+  }
+  double sum = 0.0;
+  for (int i = 0; i < num_free; ++i) sum += product[free_dofs[i]] * product[free_dofs[i]];
+  return std::sqrt(sum);
+}
+
 // ---------------------------------------------------------------------------
 // Contact (RigidTool.cpp), three-dimensional meshes
 // ---------------------------------------------------------------------------

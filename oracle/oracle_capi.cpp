@@ -368,6 +368,14 @@ void oracle_scatter(int nn, int ne, const int* conn, const double* ke, double* v
   scatter(p, ne, conn, ke, values);
 }
 
+// NonlinearSystem.hpp:55-67 on the column-major values of the pattern of
+// conn; product (3 nn) gets |K| |x|; returns the norm over the free DOFs.
+double oracle_stiffness_gross(int nn, int ne, const int* conn, const double* values, const double* x,
+                              const int* free_dofs, int num_free, double* product) {
+  const Pattern p = build_pattern(nn, ne, conn);
+  return stiffness_gross(p, values, x, free_dofs, num_free, product);
+}
+
 // NonlinearSystem.hpp:240-262; returns the gross norm.
 double oracle_gather_force(int nn, int ne, const int* conn, const double* fe, double* fint) {
   return gather_force(nn, ne, conn, fe, fint);

@@ -37,6 +37,11 @@ struct HostPattern {
   // node m within n's sorted neighbourhood (the node-wise assembly's map)
   std::vector<uint64_t> inc_pos;
   int32_t max_degree = 0;                // the largest neighbourhood
+  // per block (row node a, k-th neighbour b), row-major block order: b, and
+  // the index of a within b's neighbourhood (where block (a, b)'s rows sit in
+  // the reference's column-major values)
+  std::vector<int32_t> block_col_node;
+  std::vector<int32_t> block_mirror;
   // the diagonal block of every node (contact blocks)
   std::vector<int32_t> diag_block;       // node -> block index of (node, node)
   std::vector<int32_t> block_offset;     // the reference's per-element (a, b) offsets, 64 per element

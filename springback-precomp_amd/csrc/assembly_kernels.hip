@@ -121,6 +121,67 @@ __global__ __launch_bounds__(256) void gather_force_kernel(ForceArgs A) {
   A.gross[dof] = g;
 }
 
+// stiffness_gross (src/fem/NonlinearSystem.hpp:55-67): product[r] = the sum
+// over row r's columns c, in ascending order, of |K(r, c)| |x(c)|, each
+// product rounded before it is added (the reference's column loop reaches a
+// row's entries in that order, with no fused multiply-add), columns with
+// x(c) = 0 skipped as there; for row-major values or the reference's
+// column-major ones (block_mirror: where the row sits in the column). One
+// wavefront per node: its lanes read the node's 9 deg values together and
+// keep the rounded products in LDS (exactly 0 for a skipped column, which
+// then adds nothing), and three lanes add their rows in column order.
+__global__ __launch_bounds__(256) void abs_product_kernel(AbsProductArgs A) {
+  extern __shared__ double prod_all[];
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const long long n = static_cast<long long>(blockIdx.x) * kNodeWaves + w;
+  if (n >= A.num_nodes) return;  // the whole wavefront
+  double* prod = prod_all + w * 9 * A.max_degree;
+  const int first = A.row_ptr[3 * n];
+  const int row_len = A.row_ptr[3 * n + 1] - first;  // 3 deg(n)
+  const int len = 3 * row_len;
+  const int blk0 = first / 9;                        // 9 deg values per earlier node
+  for (int j = lane; j < len; j += 64) {
+    // entry (row l, block k, column kk of the block): row-major values run
+    // along the rows (j = l row_len + 3 k + kk), column-major ones along a
+    // column's three rows (j = 9 k + 3 kk + l), so that a wavefront's loads
+    // are contiguous runs either way
+    int l, k, kk;
+    if (A.column_major) {
+      k = j / 9;
+      kk = (j / 3) % 3;
+      l = j % 3;
+    } else {
+      l = j / row_len;
+      const int q = j - l * row_len;
+      k = q / 3;
+      kk = q - 3 * k;
+    }
+    const long long c = 3ll * A.block_col[blk0 + k] + kk;
+    const double xc = fabs(A.x[c]);
+    double p = 0.0;
+    if (xc != 0.0) {
+      const double v = A.column_major ? A.values[A.row_ptr[c] + 3 * A.block_mirror[blk0 + k] + l] : A.values[first + j];
+      p = __dmul_rn(fabs(v), xc);
+    }
+    prod[l * row_len + 3 * k + kk] = p;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane < 3) {
+    double acc = 0.0;
+    for (int q = 0; q < row_len; ++q) acc = __dadd_rn(acc, prod[lane * row_len + q]);
+    A.product[3 * n + lane] = acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void gather_kernel(const double* __restrict__ src, const int* __restrict__ idx,
+                                                     long long n, double* __restrict__ dst) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
 // Sums (of squares, or of values) of x[0, n) in a fixed order: each block of
 // the grid sums a contiguous slice with a fixed in-block tree and writes a
 // partial; one block then sums the partials the same way.
@@ -185,6 +246,22 @@ int launch_assemble_values(const AssemblyArgs& a, hipStream_t s) {
   const std::size_t lds = sizeof(double) * kNodeWaves * 9 * static_cast<std::size_t>(a.max_degree);
   hipLaunchKernelGGL(assemble_nodes_kernel, dim3(grid), dim3(64 * kNodeWaves), lds, s, a);
   return static_cast<int>(hipGetLastError());
+}
+
+int launch_stiffness_gross(const AbsProductArgs& a, hipStream_t s) {
+  (void)hipGetLastError();
+  if (a.num_nodes > 0) {
+    const unsigned int grid = static_cast<unsigned int>((a.num_nodes + kNodeWaves - 1) / kNodeWaves);
+    const std::size_t lds = sizeof(double) * kNodeWaves * 9 * static_cast<std::size_t>(a.max_degree);
+    hipLaunchKernelGGL(abs_product_kernel, dim3(grid), dim3(64 * kNodeWaves), lds, s, a);
+  }
+  if (a.num_free > 0)
+    hipLaunchKernelGGL(gather_kernel, dim3(static_cast<unsigned int>((a.num_free + 255) / 256)), dim3(256), 0, s,
+                       a.product, a.free, a.num_free, a.gathered);
+  const int parts = static_cast<int>(a.num_free / 4096 + 1 < 1024 ? a.num_free / 4096 + 1 : 1024);
+  const int rc = static_cast<int>(hipGetLastError());
+  if (rc != 0) return rc;
+  return launch_reduce(a.gathered, a.num_free, 1, a.partial, parts, a.out, s);
 }
 
 int launch_gather_force(const ForceArgs& a, hipStream_t s) {

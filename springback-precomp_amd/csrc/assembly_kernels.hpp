@@ -27,7 +27,27 @@ struct ForceArgs {
   double* gross;               // 3 num_nodes: sum_e |f_e| per DOF
 };
 
+// stiffness_gross: product = |K| |x| (3 num_nodes), out[0] = the 2-norm of
+// product over the free DOFs (gathered: num_free doubles of scratch).
+struct AbsProductArgs {
+  const int* row_ptr;
+  const int* block_col;         // per block (row-major block order): its column node
+  const int* block_mirror;      // per block (a, b): the index of a in b's neighbourhood
+  long long num_nodes;
+  int max_degree;               // the largest neighbourhood (sizes the LDS products)
+  const double* values;         // nnz, row-major or the reference's column-major
+  int column_major;
+  const double* x;              // 3 num_nodes
+  const int* free;              // num_free DOF indices
+  long long num_free;
+  double* product;              // 3 num_nodes
+  double* gathered;             // num_free
+  double* partial;              // 1024
+  double* out;                  // 1
+};
+
 int launch_assemble_values(const AssemblyArgs& a, hipStream_t s);
+int launch_stiffness_gross(const AbsProductArgs& a, hipStream_t s);
 int launch_gather_force(const ForceArgs& a, hipStream_t s);
 // out[0] = sum x (square = 0) or sqrt(sum x^2) (square = 1), fixed-order tree
 // over nparts contiguous slices (nparts <= 1024 partial sums).

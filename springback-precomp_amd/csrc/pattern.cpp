@@ -130,6 +130,13 @@ bool build_pattern(int64_t nn, int64_t ne, const int32_t* conn, HostPattern& p, 
       }
       p.inc_pos[static_cast<std::size_t>(i)] = packed;
     }
+  p.block_col_node.assign(nbr.begin(), nbr.end());
+  p.block_mirror.resize(nblocks);
+  for (int64_t a = 0; a < nn; ++a)
+    for (int32_t k = nbr_ptr[a]; k < nbr_ptr[a + 1]; ++k) {
+      const int32_t b = nbr[static_cast<std::size_t>(k)];
+      p.block_mirror[static_cast<std::size_t>(k)] = block_of(b, static_cast<int32_t>(a)) - nbr_ptr[b];
+    }
   p.diag_block.resize(static_cast<std::size_t>(nn));
   for (int64_t n = 0; n < nn; ++n) p.diag_block[n] = block_of(static_cast<int32_t>(n), static_cast<int32_t>(n));
   // the reference's block offsets: position of a within the neighbourhood of b, times 3

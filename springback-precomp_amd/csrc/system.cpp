@@ -45,6 +45,7 @@ void release(sparlab_system* s) {
   dfree(s->yielding); dfree(s->iterations); dfree(s->status);
   dfree(s->row_ptr); dfree(s->inc_pos);
   dfree(s->node_ptr); dfree(s->node_incidence); dfree(s->diag_block);
+  dfree(s->block_col); dfree(s->block_mirror); dfree(s->kproduct); dfree(s->kgathered);
   dfree(s->ws); dfree(s->partial); dfree(s->scalars); dfree(s->summary);
   dfree(s->grad_x); dfree(s->grad_u); dfree(s->grad_p);
 }
@@ -144,7 +145,9 @@ int sparlab_system_create(const double* coords, int64_t num_nodes, const int32_t
             dalloc(s->partial, 1024) &&
             dalloc(s->scalars, 2) && dalloc(s->summary, 3) && upload(s->row_ptr, s->pattern.row_ptr) &&
             upload(s->inc_pos, s->pattern.inc_pos) && upload(s->node_ptr, s->pattern.node_ptr) &&
-            upload(s->node_incidence, s->pattern.node_incidence) && upload(s->diag_block, s->pattern.diag_block);
+            upload(s->node_incidence, s->pattern.node_incidence) && upload(s->diag_block, s->pattern.diag_block) &&
+            upload(s->block_col, s->pattern.block_col_node) && upload(s->block_mirror, s->pattern.block_mirror) &&
+            dalloc(s->kproduct, 3 * nn) && dalloc(s->kgathered, 3 * nn);
   ok = ok && hipMemcpy(s->coords, coords, 3 * nn * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(s->conn, conn, 8 * ne * sizeof(int32_t), hipMemcpyHostToDevice) == hipSuccess &&
        hipMemset(s->state[0], 0, hist * sizeof(double)) == hipSuccess &&
@@ -220,6 +223,31 @@ int sparlab_system_assemble_device(sparlab_system* s, const double* fe, const do
     if (sbk::launch_assemble_values(sbk::assembly_args(s, ke, column_major, values, ke_column_major), st) != 0)
       return sbk::capi_fail(SPARLAB_E_DEVICE, "assembly launch failed");
   }
+  return SPARLAB_OK;
+}
+
+int sparlab_system_stiffness_gross_device(sparlab_system* s, const double* values, int column_major, const double* x,
+                                         const int32_t* free_dofs, int64_t num_free, double* out, double* product,
+                                         void* stream) {
+  if (!s || !values || !x || !out || (num_free > 0 && !free_dofs)) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (num_free < 0 || num_free > 3 * s->nn) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "bad free DOF count");
+  sbk::AbsProductArgs a{};
+  a.row_ptr = s->row_ptr;
+  a.block_col = s->block_col;
+  a.block_mirror = s->block_mirror;
+  a.num_nodes = s->nn;
+  a.max_degree = s->pattern.max_degree;
+  a.values = values;
+  a.column_major = column_major ? 1 : 0;
+  a.x = x;
+  a.free = free_dofs;
+  a.num_free = num_free;
+  a.product = product ? product : s->kproduct;
+  a.gathered = s->kgathered;
+  a.partial = s->partial;
+  a.out = out;
+  if (sbk::launch_stiffness_gross(a, static_cast<hipStream_t>(stream)) != 0)
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "stiffness_gross launch failed");
   return SPARLAB_OK;
 }
 

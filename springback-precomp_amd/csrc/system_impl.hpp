@@ -39,6 +39,8 @@ struct sparlab_system {
   int32_t* row_ptr = nullptr;
   uint64_t* inc_pos = nullptr;
   int32_t *node_ptr = nullptr, *node_incidence = nullptr, *diag_block = nullptr;
+  int32_t *block_col = nullptr, *block_mirror = nullptr;  // stiffness_gross maps
+  double *kproduct = nullptr, *kgathered = nullptr;      // stiffness_gross scratch (3 nn each)
   double* ws = nullptr;         // the element kernels' workspace
   double* partial = nullptr;    // 1024 partial sums
   double* scalars = nullptr;    // [0] energy, [1] gross

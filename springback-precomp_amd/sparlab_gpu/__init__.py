@@ -561,6 +561,23 @@ class NonlinearSystem:
                                            ctypes.c_void_p(s.cuda_stream)))
         return out
 
+    def stiffness_gross_device(self, values, x, free, column_major=True, product=None, stream=None):
+        """stiffness_gross on device tensors: values (nnz), x (3 per node),
+        free (int32 DOF indices). Returns (out (1,), product (3 per node) or
+        None); out[0] = ||(|K| |x|)_free||."""
+        import torch
+        L = lib()
+        L.sparlab_system_stiffness_gross_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        out = torch.empty(1, dtype=torch.float64, device=x.device)
+        s = stream if stream is not None else torch.cuda.current_stream(x.device)
+        p = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+        _check(L.sparlab_system_stiffness_gross_device(self.h, p(values), int(column_major), p(x), p(free),
+                                                       free.shape[0], p(out), p(product),
+                                                       ctypes.c_void_p(s.cuda_stream)))
+        return out, product
+
     def stress_device(self, u, stress=None, status=None, stream=None):
         """elastoplastic_stress of every element at displacement u (device
         tensor, 3 per node) from the committed history and modes: (stress

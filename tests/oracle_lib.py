@@ -281,6 +281,19 @@ def add_node_blocks(nn, conn, values, nodes, blocks):
     return values
 
 
+def stiffness_gross(nn, conn, values, x, free):
+    """stiffness_gross on the column-major values: (norm, product (3 nn))."""
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    values = np.ascontiguousarray(values, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    free = np.ascontiguousarray(free, dtype=np.int32)
+    product = np.zeros(3 * nn)
+    L = lib()
+    L.oracle_stiffness_gross.restype = ctypes.c_double
+    norm = L.oracle_stiffness_gross(nn, conn.shape[0], ip(conn), dp(values), dp(x), ip(free), free.size, dp(product))
+    return norm, product
+
+
 def tool_record(shape=1, centre=(0, 0, 0), centre0=None, radius=0.0, normal=(0, 0, 1), axis=(0, 0, 1),
                 friction=0.0, tangential_ratio=1.0):
     centre0 = centre if centre0 is None else centre0

@@ -213,3 +213,31 @@ def test_contact_blocks_into_the_assembled_matrix(device):
     ref = O.scatter(nn, conn, w["ke"].reshape(ne, 576))
     ref = O.add_node_blocks(nn, conn, ref, nodes[active == 1], blocks[active == 1].reshape(-1, 9))
     assert np.array_equal(values.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("column_major", [True, False])
+def test_stiffness_gross_bit_exact(device, column_major):
+    """stiffness_gross (NonlinearSystem.hpp:55-67): |K| |x| equal to the
+    oracle's column loop bit for bit, from the reference's column-major values
+    or the row-major values of the same (non-symmetric) K; the norm over the
+    free DOFs within 1e-14 (its sum is a tree); zero entries of x skipped."""
+    import torch
+    w = W.assembly_workload(nx=24, ny=20, nz=3, contact_nodes=10)
+    coords, conn = w["coords"], w["conn"]
+    nn, ne = coords.shape[0], conn.shape[0]
+    sysm = G.NonlinearSystem(coords, conn, G.Material(70e9, 0.33))
+    outer, inner, _ = O.pattern(nn, conn)
+    csc = O.scatter(nn, conn, w["ke"].reshape(ne, 576))
+    rng = np.random.default_rng(31)
+    x = rng.normal(size=3 * nn)
+    x[rng.choice(3 * nn, nn, replace=False)] = 0.0
+    free = np.sort(rng.choice(3 * nn, 2 * nn, replace=False)).astype(np.int32)
+    norm, product = O.stiffness_gross(nn, conn, csc, x, free)
+    values = csc if column_major else csc_to_csr_of_k(outer, inner, csc)
+    t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
+    d_product = torch.full((3 * nn,), float("nan"), dtype=torch.float64, device=device)
+    out, _ = sysm.stiffness_gross_device(t(values), t(x), t(free, torch.int32), column_major=column_major,
+                                         product=d_product)
+    torch.cuda.synchronize()
+    assert np.array_equal(d_product.cpu().numpy(), product)
+    assert abs(out.item() - norm) <= 1e-14 * norm

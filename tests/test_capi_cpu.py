@@ -158,3 +158,33 @@ def test_workspace_size_mirrors_the_library():
         for ne in (0, 1, 7, 1000, 1_000_003):
             got = lib.sparlab_element_workspace_doubles(ne, ctypes.byref(points_rule.c))
             assert got == G.workspace_doubles(ne, points_rule.points), (ne, points_rule.points)
+
+
+def test_oracle_stiffness_gross_exact_on_integers():
+    """stiffness_gross (NonlinearSystem.hpp:55-67) of the oracle: with small
+    integer values and displacements every product and sum is exact in FP64,
+    so |K| |x| must equal the exact integer product bit for bit (a transposed
+    or misplaced value changes it: K is not symmetric), and the norm over the
+    free DOFs equals the exact norm to rounding; with random real values it
+    matches a dense |K| |x|."""
+    import scipy.sparse as sp
+    coords, conn = W.structured_hex_mesh(4, 3, 2, 1.0, 1.0, 1.0)
+    nn = coords.shape[0]
+    outer, inner, _ = O.pattern(nn, conn)
+    nnz = inner.size
+    rng = np.random.default_rng(12)
+    values = rng.integers(-8, 9, nnz).astype(np.float64)
+    x = rng.integers(-5, 6, 3 * nn).astype(np.float64)
+    free = np.sort(rng.choice(3 * nn, 2 * nn, replace=False)).astype(np.int32)
+    k = sp.csc_matrix((values, inner, outer), shape=(3 * nn, 3 * nn))
+    exact = np.abs(k).astype(np.int64) @ np.abs(x).astype(np.int64)
+    norm, product = O.stiffness_gross(nn, conn, values, x, free)
+    assert np.array_equal(product, exact.astype(np.float64))
+    assert not np.array_equal(product, (np.abs(k).T @ np.abs(x)))
+    assert abs(norm - np.sqrt(float(np.sum(exact[free].astype(np.float64) ** 2)))) <= 1e-15 * norm
+    values = rng.normal(size=nnz)
+    x = rng.normal(size=3 * nn)
+    k = sp.csc_matrix((values, inner, outer), shape=(3 * nn, 3 * nn))
+    norm, product = O.stiffness_gross(nn, conn, values, x, free)
+    dense = np.abs(k.toarray()) @ np.abs(x)
+    assert np.abs(product - dense).max() <= 1e-14 * np.abs(dense).max()
