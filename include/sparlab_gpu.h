@@ -253,7 +253,8 @@ int sparlab_system_assemble_device(sparlab_system* system, const double* fe, con
  * the stiffness times the displacement (src/fem/Forming.cpp:991-993):
  * out[0] = || (|K| |x|)_free ||_2 for the system's pattern with values (nnz;
  * column_major as in sparlab_system_evaluate), x (3 nn) and the free DOFs
- * free_dofs[0, num_free) (int32). product (3 nn, may be null) receives
+ * free_dofs[0, num_free) (int32, each in [0, 3 nn); one outside gives a NaN
+ * norm). product (3 nn, may be null) receives
  * |K| |x|; each entry is the reference's sum, bit for bit (ascending columns,
  * products rounded before they are added); the norm's sum is a fixed-order
  * tree. Device pointers; asynchronous on stream. Calls on one system must not

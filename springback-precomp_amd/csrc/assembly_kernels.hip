@@ -176,10 +176,15 @@ __global__ __launch_bounds__(256) void abs_product_kernel(AbsProductArgs A) {
   }
 }
 
-__global__ __launch_bounds__(256) void gather_kernel(const double* __restrict__ src, const int* __restrict__ idx,
-                                                     long long n, double* __restrict__ dst) {
+// dst[i] = src[idx[i]]; an index outside [0, limit) gives NaN (which the
+// norm then shows) instead of a read outside src.
+__global__ __launch_bounds__(256) void gather_kernel(const double* __restrict__ src, long long limit,
+                                                     const int* __restrict__ idx, long long n,
+                                                     double* __restrict__ dst) {
   const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[idx[i]];
+  if (i >= n) return;
+  const int k = idx[i];
+  dst[i] = (k >= 0 && k < limit) ? src[k] : __builtin_nan("");
 }
 
 // Sums (of squares, or of values) of x[0, n) in a fixed order: each block of
@@ -257,7 +262,7 @@ int launch_stiffness_gross(const AbsProductArgs& a, hipStream_t s) {
   }
   if (a.num_free > 0)
     hipLaunchKernelGGL(gather_kernel, dim3(static_cast<unsigned int>((a.num_free + 255) / 256)), dim3(256), 0, s,
-                       a.product, a.free, a.num_free, a.gathered);
+                       a.product, 3 * a.num_nodes, a.free, a.num_free, a.gathered);
   const int parts = static_cast<int>(a.num_free / 4096 + 1 < 1024 ? a.num_free / 4096 + 1 : 1024);
   const int rc = static_cast<int>(hipGetLastError());
   if (rc != 0) return rc;

@@ -1228,10 +1228,14 @@ __global__ __launch_bounds__(64) void stress_kernel(ElementArgs A, double* __res
           }
         const double green[6] = {e3[0][0], e3[1][1], e3[2][2], 2.0 * e3[0][1], 2.0 * e3[1][2], 2.0 * e3[2][0]};
         if constexpr (KIN == kLog) {
-          if (!(det3(F) > 0.0)) st = kInverted;
-          else if (logarithmic_strain(green, L) != 0) st = kDegenerate;
+          if (!(det3(F) > 0.0)) {
+            st = kInverted;
+          } else if (logarithmic_strain(green, L) != 0) {
+            st = kDegenerate;
+          } else {
 #pragma unroll
-          for (int i = 0; i < 6; ++i) strain[i] = L.strain[i];
+            for (int i = 0; i < 6; ++i) strain[i] = L.strain[i];
+          }
         } else {
 #pragma unroll
           for (int i = 0; i < 6; ++i) strain[i] = green[i];

@@ -241,3 +241,9 @@ def test_stiffness_gross_bit_exact(device, column_major):
     torch.cuda.synchronize()
     assert np.array_equal(d_product.cpu().numpy(), product)
     assert abs(out.item() - norm) <= 1e-14 * norm
+    # a free DOF outside [0, 3 nn) is caught by the kernel's bounds check: a NaN norm
+    bad = free.copy()
+    bad[-1] = 3 * nn
+    out, _ = sysm.stiffness_gross_device(t(values), t(x), t(bad, torch.int32), column_major=column_major)
+    torch.cuda.synchronize()
+    assert np.isnan(out.item())
