@@ -46,6 +46,10 @@ def materials():
                              hill=dict(r0=1.9, r45=1.5, r90=2.3, rolling_direction=(1.0, 0.5, 0.0))),
         "Chaboche": G.Material(200.0e9, 0.3, yield_stress=250.0e6, saturation_stress=100.0e6,
                                saturation_rate=20.0, backstresses=((40.0e9, 400.0), (5.0e9, 20.0))),
+        # backward-Euler integration of the backstresses (the other value of the flag)
+        "ChabocheBE": G.Material(200.0e9, 0.3, yield_stress=250.0e6, saturation_stress=100.0e6,
+                                 saturation_rate=20.0, backstresses=((40.0e9, 400.0), (5.0e9, 20.0)),
+                                 kinematic_integration="backward_euler"),
         # four kinematic terms (three backstresses and Prager's modulus) on Hill48
         "Chaboche4": G.Material(200.0e9, 0.3, yield_stress=250.0e6, saturation_stress=100.0e6,
                                 saturation_rate=20.0, kinematic_hardening_modulus=1.0e9,
@@ -92,7 +96,7 @@ def single_hex_batch(scales, extra_z=0.2):
     return coords, conn, u.reshape(8 * ne, 3)
 
 
-@pytest.mark.parametrize("law", ["elastic", "J2", "Hill48", "Chaboche", "Chaboche4"])
+@pytest.mark.parametrize("law", ["elastic", "J2", "Hill48", "Chaboche", "ChabocheBE", "Chaboche4"])
 @pytest.mark.parametrize("kinematics", [0, 1, 2])
 def test_single_hex_laws(law, kinematics):
     """The laws x kinematics of test_incompatible_modes.cpp:433-508, from a
