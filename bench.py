@@ -310,6 +310,22 @@ def assembly_leg(G, W, torch, dev, reps):
     def gross_rm():
         sysm.stiffness_gross_device(values, kx, kfree, column_major=False, product=kprod, stream=stream)
 
+    # the free-free block of the Newton system: the plate clamped at both
+    # ends in x (every DOF of those nodes fixed)
+    ends = np.isclose(coords[:, 0], coords[:, 0].min()) | np.isclose(coords[:, 0], coords[:, 0].max())
+    free_mask = np.repeat(~ends, 3)
+    d_pfree = t(np.nonzero(free_mask)[0], torch.int32)
+    block_nnz = sysm.partition_device(d_pfree, stream=stream)
+    block_values = torch.empty(max(block_nnz, 1), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        sysm.partition_device(d_pfree, stream=stream)
+    partition_ms = (time.perf_counter() - t0) * 1e3 / 5
+
+    def free_block():
+        sysm.free_block_device(values, out=block_values, stream=stream)
+
     def candidates():
         G.contact_candidates_device(tool, d_coords, d_u, d_nodes, 2e-4, stream=stream, buffers=cand_bufs)
 
@@ -325,7 +341,7 @@ def assembly_leg(G, W, torch, dev, reps):
                                                             ke_column_major=True, stream=stream)),
               ("force", lambda: sysm.assemble_device(fe=d_fe, fint=fint, gross=gross, stream=stream)),
               ("contact", contact), ("node_blocks", blocks), ("candidates", candidates), ("anticipate", anticipate),
-              ("gross_cm", gross_cm), ("gross_rm", gross_rm))
+              ("gross_cm", gross_cm), ("gross_rm", gross_rm), ("free_block", free_block))
     us = {}
     for name, fn in stages:
         fn()
@@ -353,7 +369,15 @@ def assembly_leg(G, W, torch, dev, reps):
             "node_blocks_us": us["node_blocks"], "contact_active_nodes": n_active,
             "contact_candidates_us": us["candidates"], "contact_anticipate_us": us["anticipate"],
             "stiffness_gross_us": us["gross_cm"], "stiffness_gross_row_major_us": us["gross_rm"],
-            "stiffness_gross_GBps": (8 * sysm.nnz + 3 * 8 * 3 * nn) / (us["gross_cm"] * 1e-6) / 1e9}
+            "stiffness_gross_GBps": (8 * sysm.nnz + 3 * 8 * 3 * nn) / (us["gross_cm"] * 1e-6) / 1e9,
+            "free_block": {"note": "PartitionedFactor: the free-free block of the assembled values, the plate "
+                                   "clamped at both x ends; partition = pattern set-up (host clock, one "
+                                   "synchronization), values = the per-iteration copy (20 B per block entry: "
+                                   "source index, value read, value written)",
+                           "free_dofs": int(free_mask.sum()), "block_nnz": int(block_nnz),
+                           "partition_ms": partition_ms, "values_us": us["free_block"],
+                           "values_GBps": 20 * block_nnz / (us["free_block"] * 1e-6) / 1e9,
+                           "values_hbm_frac": 20 * block_nnz / (us["free_block"] * 1e-6) / 1e9 / HBM_PEAK_GBS}}
 
 
 def copy_probe(torch, mib):
@@ -507,7 +531,9 @@ def main():
               f"contact {assembly['contact_us']:.1f} us ({assembly['contact_active_nodes']} active), node blocks "
               f"{assembly['node_blocks_us']:.1f} us, candidates {assembly['contact_candidates_us']:.1f} us, "
               f"anticipation {assembly['contact_anticipate_us']:.1f} us, stiffness_gross {assembly['stiffness_gross_us']:.1f} "
-              f"us ({assembly['stiffness_gross_GBps']:.0f} GB/s; row-major {assembly['stiffness_gross_row_major_us']:.1f} us)",
+              f"us ({assembly['stiffness_gross_GBps']:.0f} GB/s; row-major {assembly['stiffness_gross_row_major_us']:.1f} us), "
+              f"free-free block: partition {assembly['free_block']['partition_ms']:.2f} ms, values "
+              f"{assembly['free_block']['values_us']:.1f} us ({assembly['free_block']['values_GBps']:.0f} GB/s)",
               file=sys.stderr, flush=True)
 
     line = {

@@ -263,6 +263,32 @@ int sparlab_system_stiffness_gross_device(sparlab_system* system, const double* 
                                          const double* x, const int32_t* free_dofs, int64_t num_free, double* out,
                                          double* product, void* stream);
 
+/* The free-free block of the Newton system: PartitionedFactor::reset and
+ * build (src/fem/Forming.cpp:75-85, 208-236) for the free DOFs
+ * free_dofs[0, num_free) (device, int32, strictly ascending in [0, 3 nn)).
+ * The block's pattern is built on the device and kept by the system until the
+ * next partition; *block_nnz (host) receives its nonzero count. The block's
+ * columns are the free columns in order, each keeping its free rows in the
+ * full pattern's order renumbered to free positions, as the reference's
+ * kff_. A list that does not strictly ascend or leaves [0, 3 nn) is refused
+ * (SPARLAB_E_ARGUMENT, no partition). Synchronizes stream (it reads the
+ * count). */
+int sparlab_system_partition_device(sparlab_system* system, const int32_t* free_dofs, int64_t num_free,
+                                    int64_t* block_nnz, void* stream);
+/* The block's pattern (device buffers): block_ptr (num_free + 1) and
+ * block_index (block_nnz), the reference's kff_ outer and inner index arrays.
+ * The system's pattern is symmetric, so they are also the block's row
+ * pointers and column indices. Asynchronous on stream. */
+int sparlab_system_partition_pattern_device(sparlab_system* system, int32_t* block_ptr, int32_t* block_index,
+                                            void* stream);
+/* PartitionedFactor::assemble (src/fem/Forming.cpp:89-98): block_values
+ * (block_nnz, device) from the assembled values (nnz, device), bit for bit.
+ * The block keeps the values' layout: the reference's column-major values
+ * give its column-major kff_, row-major values the block's rows. Asynchronous
+ * on stream. */
+int sparlab_system_free_block_device(sparlab_system* system, const double* values, double* block_values,
+                                     void* stream);
+
 /* Per-element stress measures (the reference's ElastoplasticStress,
  * include/sparlab/fem/Elastoplastic.hpp:206-221), SPARLAB_STRESS_FIELDS
  * doubles per element at these offsets (stresses and strains in tensorial

@@ -135,6 +135,44 @@ inline double stiffness_gross(const Pattern& p, const double* values, const doub
   return std::sqrt(sum);
 }
 
+// PartitionedFactor::reset and build (Forming.cpp:75-85, 208-236): the
+// free-free block of a column-major pattern for ascending free DOFs. Its
+// columns are the free columns in order, each keeping its free rows in the
+// full matrix's (sorted) order, renumbered to free positions; position[q] is
+// where entry q of the full values lands in the block's values, or -1.
+struct FreeBlock {
+  std::vector<int> outer, inner;
+  std::vector<long long> position;
+};
+
+inline FreeBlock free_block(const Pattern& p, const int* free_dofs, int num_free) {
+  const std::size_t n = p.outer.size() - 1;
+  std::vector<int> map(n, -1);
+  for (int i = 0; i < num_free; ++i) map[static_cast<std::size_t>(free_dofs[i])] = i;
+  FreeBlock b;
+  b.outer.assign(static_cast<std::size_t>(num_free) + 1, 0);
+  b.position.assign(p.inner.size(), -1);
+  for (std::size_t col = 0; col < n; ++col) {
+    const int c = map[col];
+    if (c < 0) continue;
+    for (int q = p.outer[col]; q < p.outer[col + 1]; ++q) {
+      const int r = map[static_cast<std::size_t>(p.inner[q])];
+      if (r < 0) continue;
+      b.position[static_cast<std::size_t>(q)] = static_cast<long long>(b.inner.size());
+      b.inner.push_back(r);
+    }
+    b.outer[static_cast<std::size_t>(c) + 1] = static_cast<int>(b.inner.size());
+  }
+  return b;
+}
+
+// PartitionedFactor::assemble (Forming.cpp:89-98): the block's values out of
+// the full values.
+inline void free_block_values(const FreeBlock& b, const double* values, double* out) {
+  for (std::size_t q = 0; q < b.position.size(); ++q)
+    if (b.position[q] >= 0) out[b.position[q]] = values[q];
+}
+
 // ---------------------------------------------------------------------------
 // Contact (RigidTool.cpp), three-dimensional meshes
 // ---------------------------------------------------------------------------

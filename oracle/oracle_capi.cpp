@@ -11,6 +11,7 @@
 #include "assembly.hpp"
 #include "element.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <omp.h>
 
@@ -374,6 +375,25 @@ double oracle_stiffness_gross(int nn, int ne, const int* conn, const double* val
                               const int* free_dofs, int num_free, double* product) {
   const Pattern p = build_pattern(nn, ne, conn);
   return stiffness_gross(p, values, x, free_dofs, num_free, product);
+}
+
+// PartitionedFactor::build + assemble (Forming.cpp:89-98, 208-236) on the
+// pattern of conn for ascending free DOFs: block_outer (num_free + 1),
+// block_inner and block_values (capacity entries each) and position (one
+// per full nonzero); returns the block's nonzero count, or -1 when it
+// exceeds capacity.
+long long oracle_free_block(int nn, int ne, const int* conn, const double* values, const int* free_dofs,
+                            int num_free, long long capacity, int* block_outer, int* block_inner,
+                            double* block_values, long long* position) {
+  const Pattern p = build_pattern(nn, ne, conn);
+  const FreeBlock b = free_block(p, free_dofs, num_free);
+  const long long nnz = static_cast<long long>(b.inner.size());
+  if (nnz > capacity) return -1;
+  std::copy(b.outer.begin(), b.outer.end(), block_outer);
+  std::copy(b.inner.begin(), b.inner.end(), block_inner);
+  std::copy(b.position.begin(), b.position.end(), position);
+  free_block_values(b, values, block_values);
+  return nnz;
 }
 
 // NonlinearSystem.hpp:240-262; returns the gross norm.

@@ -12,12 +12,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "adjoint_kernels.hpp"
 #include "assembly_kernels.hpp"
+#include "partition_kernels.hpp"
 #include "system_impl.hpp"
 
 namespace {
@@ -38,7 +40,16 @@ void dfree(T*& p) {
   p = nullptr;
 }
 
+void release_partition(sparlab_system* s) {
+  dfree(s->part_counts); dfree(s->part_sums); dfree(s->part_bad);
+  dfree(s->part_ptr); dfree(s->part_index); dfree(s->part_source);
+  s->part_free = -1;
+  s->part_nnz = 0;
+}
+
 void release(sparlab_system* s) {
+  release_partition(s);
+  dfree(s->part_map);
   dfree(s->coords); dfree(s->conn);
   for (int i = 0; i < 2; ++i) { dfree(s->state[i]); dfree(s->alpha[i]); }
   dfree(s->fe); dfree(s->ke); dfree(s->energy); dfree(s->fail_value); dfree(s->gross);
@@ -248,6 +259,90 @@ int sparlab_system_stiffness_gross_device(sparlab_system* s, const double* value
   a.out = out;
   if (sbk::launch_stiffness_gross(a, static_cast<hipStream_t>(stream)) != 0)
     return sbk::capi_fail(SPARLAB_E_DEVICE, "stiffness_gross launch failed");
+  return SPARLAB_OK;
+}
+
+int sparlab_system_partition_device(sparlab_system* s, const int32_t* free_dofs, int64_t num_free,
+                                    int64_t* block_nnz, void* stream) {
+  if (!s || !block_nnz || (num_free > 0 && !free_dofs)) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  const int64_t n = 3 * s->nn;
+  if (num_free < 0 || num_free > n) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "bad free DOF count");
+  if (s->pattern.col_idx.size() > static_cast<std::size_t>(INT32_MAX))
+    return sbk::capi_fail(SPARLAB_E_ARGUMENT, "the pattern is too large for 32-bit block sources");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // the previous partition goes first (its buffers are freed with the
+  // stream's work on them finished)
+  if (hipStreamSynchronize(st) != hipSuccess) return sbk::capi_fail(SPARLAB_E_DEVICE, "stream synchronization failed");
+  release_partition(s);
+  *block_nnz = 0;
+  const std::size_t nf = static_cast<std::size_t>(num_free);
+  const std::size_t nb = (nf + 255) / 256;
+  if ((!s->part_map && !dalloc(s->part_map, static_cast<std::size_t>(n))) || !dalloc(s->part_counts, nf) ||
+      !dalloc(s->part_sums, nb + 1) || !dalloc(s->part_bad, 1) || !dalloc(s->part_ptr, nf + 1)) {
+    release_partition(s);
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "device allocation failed");
+  }
+  sbk::PartitionArgs a{};
+  a.row_ptr = s->row_ptr;
+  a.block_col = s->block_col;
+  a.num_nodes = s->nn;
+  a.free_dofs = free_dofs;
+  a.num_free = num_free;
+  a.map = s->part_map;
+  a.counts = s->part_counts;
+  a.block_sums = s->part_sums;
+  a.bad = s->part_bad;
+  a.block_ptr = s->part_ptr;
+  int32_t head[2] = {0, 0};  // the block's nonzero count, the refusal flag
+  if (sbk::launch_partition_count(a, st) != 0 ||
+      hipMemcpyAsync(&head[0], s->part_sums + nb, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(&head[1], s->part_bad, sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    release_partition(s);
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "the partition's count failed");
+  }
+  if (head[1] != 0) {
+    release_partition(s);
+    return sbk::capi_fail(SPARLAB_E_ARGUMENT, "the free DOFs are not strictly ascending in [0, 3 num_nodes)");
+  }
+  const std::size_t nnz = static_cast<std::size_t>(head[0]);
+  if (!dalloc(s->part_index, nnz) || !dalloc(s->part_source, nnz)) {
+    release_partition(s);
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "device allocation failed");
+  }
+  a.block_index = s->part_index;
+  a.source = s->part_source;
+  if (sbk::launch_partition_fill(a, st) != 0) {
+    release_partition(s);
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "the partition's fill failed");
+  }
+  s->part_free = num_free;
+  s->part_nnz = static_cast<int64_t>(nnz);
+  *block_nnz = s->part_nnz;
+  return SPARLAB_OK;
+}
+
+int sparlab_system_partition_pattern_device(sparlab_system* s, int32_t* block_ptr, int32_t* block_index,
+                                            void* stream) {
+  if (!s || !block_ptr || (s->part_nnz > 0 && !block_index))
+    return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (s->part_free < 0) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "no partition");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (hipMemcpyAsync(block_ptr, s->part_ptr, static_cast<std::size_t>(s->part_free + 1) * sizeof(int32_t),
+                     hipMemcpyDeviceToDevice, st) != hipSuccess ||
+      (s->part_nnz > 0 &&
+       hipMemcpyAsync(block_index, s->part_index, static_cast<std::size_t>(s->part_nnz) * sizeof(int32_t),
+                      hipMemcpyDeviceToDevice, st) != hipSuccess))
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "copying the block's pattern failed");
+  return SPARLAB_OK;
+}
+
+int sparlab_system_free_block_device(sparlab_system* s, const double* values, double* block_values, void* stream) {
+  if (!s || !values || (s->part_nnz > 0 && !block_values)) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (s->part_free < 0) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "no partition");
+  if (sbk::launch_free_block_values(values, s->part_source, s->part_nnz, block_values,
+                                    static_cast<hipStream_t>(stream)) != 0)
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "free block launch failed");
   return SPARLAB_OK;
 }
 

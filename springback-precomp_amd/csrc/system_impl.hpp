@@ -41,6 +41,10 @@ struct sparlab_system {
   int32_t *node_ptr = nullptr, *node_incidence = nullptr, *diag_block = nullptr;
   int32_t *block_col = nullptr, *block_mirror = nullptr;  // stiffness_gross maps
   double *kproduct = nullptr, *kgathered = nullptr;      // stiffness_gross scratch (3 nn each)
+  // the free-free block of the last partition (part_free < 0: none)
+  int32_t *part_map = nullptr, *part_counts = nullptr, *part_sums = nullptr, *part_bad = nullptr;
+  int32_t *part_ptr = nullptr, *part_index = nullptr, *part_source = nullptr;
+  int64_t part_free = -1, part_nnz = 0;
   double* ws = nullptr;         // the element kernels' workspace
   double* partial = nullptr;    // 1024 partial sums
   double* scalars = nullptr;    // [0] energy, [1] gross

@@ -578,6 +578,56 @@ class NonlinearSystem:
                                                        ctypes.c_void_p(s.cuda_stream)))
         return out, product
 
+    def partition_device(self, free, stream=None):
+        """PartitionedFactor::reset + build on the device for the free DOFs
+        (int32 device tensor, strictly ascending): the free-free block's
+        pattern, kept by the system. Returns its nonzero count; the library
+        refuses (RuntimeError) a list that does not strictly ascend in range."""
+        import torch
+        L = lib()
+        L.sparlab_system_partition_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                                      ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p]
+        if free.dtype != torch.int32:
+            raise ValueError("free DOFs must be int32")
+        nnz = ctypes.c_int64(0)
+        s = stream if stream is not None else torch.cuda.current_stream(free.device)
+        self._block = (-1, 0)  # a refused list leaves no partition
+        _check(L.sparlab_system_partition_device(self.h, ctypes.c_void_p(free.data_ptr()), free.shape[0],
+                                                 ctypes.byref(nnz), ctypes.c_void_p(s.cuda_stream)))
+        self._block = (free.shape[0], nnz.value)
+        return nnz.value
+
+    def partition_pattern_device(self, device="cuda", stream=None):
+        """The free-free block's pattern: (block_ptr (num_free + 1),
+        block_index (nnz)) int32 device tensors, the reference's kff_ outer
+        and inner index arrays."""
+        import torch
+        L = lib()
+        L.sparlab_system_partition_pattern_device.argtypes = [ctypes.c_void_p] * 4
+        nf, nnz = getattr(self, "_block", (-1, 0))
+        ptr = torch.empty(max(nf + 1, 1), dtype=torch.int32, device=device)
+        index = torch.empty(max(nnz, 1), dtype=torch.int32, device=device)
+        s = stream if stream is not None else torch.cuda.current_stream(ptr.device)
+        _check(L.sparlab_system_partition_pattern_device(self.h, ctypes.c_void_p(ptr.data_ptr()),
+                                                         ctypes.c_void_p(index.data_ptr()),
+                                                         ctypes.c_void_p(s.cuda_stream)))
+        return ptr, index[:nnz]
+
+    def free_block_device(self, values, out=None, stream=None):
+        """PartitionedFactor::assemble: the free-free block's values (device
+        tensor, nnz of the block) out of the assembled values, in their
+        layout."""
+        import torch
+        L = lib()
+        L.sparlab_system_free_block_device.argtypes = [ctypes.c_void_p] * 4
+        nf, nnz = getattr(self, "_block", (-1, 0))  # without a partition the library refuses
+        if out is None:
+            out = torch.empty(max(nnz, 1), dtype=torch.float64, device=values.device)
+        s = stream if stream is not None else torch.cuda.current_stream(values.device)
+        _check(L.sparlab_system_free_block_device(self.h, ctypes.c_void_p(values.data_ptr()),
+                                                  ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(s.cuda_stream)))
+        return out[:nnz]
+
     def stress_device(self, u, stress=None, status=None, stream=None):
         """elastoplastic_stress of every element at displacement u (device
         tensor, 3 per node) from the committed history and modes: (stress

@@ -294,6 +294,26 @@ def stiffness_gross(nn, conn, values, x, free):
     return norm, product
 
 
+def free_block(nn, conn, values, free):
+    """PartitionedFactor::build + assemble: the free-free block of the
+    column-major values for ascending free DOFs, as (outer, inner, values,
+    position)."""
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    values = np.ascontiguousarray(values, dtype=np.float64)
+    free = np.ascontiguousarray(free, dtype=np.int32)
+    cap = values.size
+    outer = np.zeros(free.size + 1, np.int32)
+    inner = np.zeros(cap, np.int32)
+    out = np.zeros(cap)
+    position = np.zeros(cap, np.int64)
+    L = lib()
+    L.oracle_free_block.restype = ctypes.c_longlong
+    nnz = L.oracle_free_block(nn, conn.shape[0], ip(conn), dp(values), ip(free), free.size, ctypes.c_longlong(cap),
+                              ip(outer), ip(inner), dp(out), position.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)))
+    assert nnz >= 0
+    return outer, inner[:nnz].copy(), out[:nnz].copy(), position
+
+
 def tool_record(shape=1, centre=(0, 0, 0), centre0=None, radius=0.0, normal=(0, 0, 1), axis=(0, 0, 1),
                 friction=0.0, tangential_ratio=1.0):
     centre0 = centre if centre0 is None else centre0

@@ -188,3 +188,44 @@ def test_oracle_stiffness_gross_exact_on_integers():
     norm, product = O.stiffness_gross(nn, conn, values, x, free)
     dense = np.abs(k.toarray()) @ np.abs(x)
     assert np.abs(product - dense).max() <= 1e-14 * np.abs(dense).max()
+
+
+@pytest.mark.parametrize("which", ["clamped_edges", "random", "all", "none"])
+def test_oracle_free_block_is_the_submatrix(which):
+    """PartitionedFactor::build + assemble (Forming.cpp:89-98, 208-236) of the
+    oracle against scipy's K[free][:, free] of the column-major matrix, with
+    values that are not symmetric (a transposed block, a dropped or misplaced
+    entry or a renumbering off by one fails): pattern and values equal, and
+    position sends exactly the entries whose row and column are both free."""
+    import scipy.sparse as sp
+    coords, conn = W.structured_hex_mesh(5, 4, 2, 1.0, 1.0, 0.5)
+    nn = coords.shape[0]
+    n = 3 * nn
+    outer, inner, _ = O.pattern(nn, conn)
+    rng = np.random.default_rng(31)
+    values = rng.normal(size=inner.size)
+    if which == "clamped_edges":
+        fixed = np.nonzero(np.isclose(coords[:, 0], 0.0) | np.isclose(coords[:, 0], 1.0))[0]
+        mask = np.ones(n, bool)
+        mask[(3 * fixed[:, None] + np.arange(3)).ravel()] = False
+        mask[2::3] &= ~np.isclose(coords[:, 2], 0.0)  # and the bottom's z
+        free = np.nonzero(mask)[0]
+    elif which == "random":
+        free = np.sort(rng.choice(n, n // 2, replace=False))
+    elif which == "all":
+        free = np.arange(n)
+    else:
+        free = np.zeros(0, np.int64)
+    free = free.astype(np.int32)
+    b_outer, b_inner, b_values, position = O.free_block(nn, conn, values, free)
+    k = sp.csc_matrix((values, inner, outer), shape=(n, n))
+    sub = k[free][:, free].tocsc()
+    sub.sort_indices()
+    assert np.array_equal(b_outer, sub.indptr) and np.array_equal(b_inner, sub.indices)
+    assert np.array_equal(b_values, sub.data)
+    cols = np.repeat(np.arange(n), np.diff(outer))
+    both = np.isin(inner, free) & np.isin(cols, free)
+    assert np.array_equal(position >= 0, both)
+    assert np.array_equal(np.sort(position[both]), np.arange(b_inner.size))
+    if which == "all":
+        assert np.array_equal(b_values, values)
