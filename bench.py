@@ -326,6 +326,23 @@ def assembly_leg(G, W, torch, dev, reps):
     def free_block():
         sysm.free_block_device(values, out=block_values, stream=stream)
 
+    # the block solve on the plate's own elastic tangent (symmetric positive
+    # definite with the ends clamped), a seeded right-hand side
+    k_values = torch.empty(sysm.nnz, dtype=torch.float64, device=dev)
+    sysm.evaluate_device(torch.zeros(3 * nn, dtype=torch.float64, device=dev),
+                         torch.empty(3 * nn, dtype=torch.float64, device=dev), k_values, column_major=True,
+                         stream=stream)
+    k_block = sysm.free_block_device(k_values, stream=stream)
+    rhs = torch.as_tensor(np.random.default_rng(9).normal(size=d_pfree.shape[0]) * 1e3, device=dev)
+    x_sol = torch.empty_like(rhs)
+    sysm.solve_free_block_device(k_block, rhs, x=x_sol, tolerance=1e-10, stream=stream)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, solve_its, solve_rel = sysm.solve_free_block_device(k_block, rhs, x=x_sol, tolerance=1e-10, stream=stream)
+    torch.cuda.synchronize()
+    solve_ms = (time.perf_counter() - t0) * 1e3
+    del k_values, k_block
+
     def candidates():
         G.contact_candidates_device(tool, d_coords, d_u, d_nodes, 2e-4, stream=stream, buffers=cand_bufs)
 
@@ -377,7 +394,15 @@ def assembly_leg(G, W, torch, dev, reps):
                            "free_dofs": int(free_mask.sum()), "block_nnz": int(block_nnz),
                            "partition_ms": partition_ms, "values_us": us["free_block"],
                            "values_GBps": 20 * block_nnz / (us["free_block"] * 1e-6) / 1e9,
-                           "values_hbm_frac": 20 * block_nnz / (us["free_block"] * 1e-6) / 1e9 / HBM_PEAK_GBS}}
+                           "values_hbm_frac": 20 * block_nnz / (us["free_block"] * 1e-6) / 1e9 / HBM_PEAK_GBS},
+            "block_solve": {"note": "K_ff x = b on the clamped plate's elastic tangent by Jacobi-preconditioned "
+                                    "conjugate gradients to ||b - K x|| <= 1e-10 ||b|| (host clock, reads of the "
+                                    "residual every 16 iterations included); per iteration 12 B per block entry "
+                                    "(the product) + 136 B per free DOF (the vectors)",
+                            "iterations": int(solve_its), "relative_residual": solve_rel, "ms": solve_ms,
+                            "us_per_iteration": solve_ms * 1e3 / max(solve_its, 1),
+                            "GBps": (12 * block_nnz + 136 * int(free_mask.sum())) * max(solve_its, 1)
+                                    / (solve_ms * 1e-3) / 1e9}}
 
 
 def copy_probe(torch, mib):
@@ -533,7 +558,9 @@ def main():
               f"anticipation {assembly['contact_anticipate_us']:.1f} us, stiffness_gross {assembly['stiffness_gross_us']:.1f} "
               f"us ({assembly['stiffness_gross_GBps']:.0f} GB/s; row-major {assembly['stiffness_gross_row_major_us']:.1f} us), "
               f"free-free block: partition {assembly['free_block']['partition_ms']:.2f} ms, values "
-              f"{assembly['free_block']['values_us']:.1f} us ({assembly['free_block']['values_GBps']:.0f} GB/s)",
+              f"{assembly['free_block']['values_us']:.1f} us ({assembly['free_block']['values_GBps']:.0f} GB/s); block "
+              f"solve {assembly['block_solve']['iterations']} CG iterations in {assembly['block_solve']['ms']:.1f} ms "
+              f"({assembly['block_solve']['us_per_iteration']:.1f} us each, {assembly['block_solve']['GBps']:.0f} GB/s)",
               file=sys.stderr, flush=True)
 
     line = {

@@ -288,6 +288,24 @@ int sparlab_system_partition_pattern_device(sparlab_system* system, int32_t* blo
  * on stream. */
 int sparlab_system_free_block_device(sparlab_system* system, const double* values, double* block_values,
                                      void* stream);
+/* Solves K_ff x = b on the partition's block (block_values from
+ * sparlab_system_free_block_device; b, x: num_free, device) where the
+ * reference factorises and solves a symmetric tangent
+ * (PartitionedFactor::factorize and solve, src/fem/Forming.cpp:104-172,
+ * called at :1001-1016 with b = -r). The method differs: conjugate
+ * gradients preconditioned with the block's diagonal, from x = 0, until the
+ * iteration's residual satisfies ||b - K x|| <= tolerance ||b|| (checked every
+ * 16 iterations) or max_iterations pass. *iterations (host) receives the
+ * iterations run and *relative_residual (host) the final true residual
+ * ||b - K x|| / ||b||. SPARLAB_E_SOLVER when the block is not positive
+ * definite (a non-positive diagonal or curvature; the reference then falls
+ * back to LDL^T or LU) or the iteration does not converge; x then holds the
+ * last iterate. The block is read by rows (K_ff symmetric: either layout).
+ * Every sum has a fixed order: a solve repeats bit for bit. Synchronizes
+ * stream between runs of iterations. */
+int sparlab_system_solve_free_block_device(sparlab_system* system, const double* block_values, const double* b,
+                                           double* x, double tolerance, int64_t max_iterations,
+                                           int64_t* iterations, double* relative_residual, void* stream);
 
 /* Per-element stress measures (the reference's ElastoplasticStress,
  * include/sparlab/fem/Elastoplastic.hpp:206-221), SPARLAB_STRESS_FIELDS

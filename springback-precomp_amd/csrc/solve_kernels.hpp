@@ -1,0 +1,48 @@
+// Conjugate gradients on the free-free block of a symmetric tangent: the
+// device solve of K_ff x = b beside the reference's direct factorisations
+// (PartitionedFactor::factorize and solve, src/fem/Forming.cpp:104-172).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace sbk {
+
+// Device scalars of the iteration (doubles).
+enum CgScalar : int {
+  kCgRz = 0,      // r . z
+  kCgAlpha = 1,
+  kCgBeta = 2,
+  kCgRr = 3,      // r . r
+  kCgBb = 4,      // b . b
+  kCgBreak = 5,   // 1: p . K p <= 0 (not positive definite) or a non-finite step
+  kCgScalars = 8
+};
+
+struct CgArgs {
+  const int* ptr;       // n + 1 row pointers of the block
+  const int* index;     // the block's column indices
+  const double* values; // the block's values (rows; K_ff symmetric)
+  long long n;
+  int nparts;           // partial sums per dot product (fixed: the sums' order)
+  double* dinv;         // n: 1 / K_ii
+  double* r;            // n
+  double* z;            // n
+  double* p;            // n
+  double* q;            // n: K p
+  double* partial;      // 2 nparts
+  double* scalars;      // kCgScalars
+};
+
+// The diagonal's inverse (a missing or non-positive diagonal sets kCgBreak),
+// x = 0, r = b, z = D^-1 r, p = z and the first r . z, r . r, b . b.
+int launch_cg_start(const CgArgs& a, const double* b, double* x, hipStream_t s);
+// `count` iterations: q = K p, alpha = rz / p.q, x += alpha p, r -= alpha q,
+// z = D^-1 r, beta = r.z / rz, p = z + beta p.
+int launch_cg_iterations(const CgArgs& a, double* x, int count, hipStream_t s);
+// The true residual's r . r (r = b - K x) into scalars[kCgRr].
+int launch_cg_residual(const CgArgs& a, const double* b, const double* x, hipStream_t s);
+// y = K x over the block's rows (one wavefront per row, fixed-order sums).
+int launch_block_spmv(const int* ptr, const int* index, const double* values, long long n, const double* x,
+                      double* y, hipStream_t s);
+
+}  // namespace sbk

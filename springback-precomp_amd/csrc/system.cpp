@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -20,6 +21,7 @@
 #include "adjoint_kernels.hpp"
 #include "assembly_kernels.hpp"
 #include "partition_kernels.hpp"
+#include "solve_kernels.hpp"
 #include "system_impl.hpp"
 
 namespace {
@@ -42,9 +44,16 @@ void dfree(T*& p) {
 
 void release_partition(sparlab_system* s) {
   dfree(s->part_counts); dfree(s->part_sums); dfree(s->part_bad);
-  dfree(s->part_ptr); dfree(s->part_index); dfree(s->part_source);
+  dfree(s->part_ptr); dfree(s->part_index); dfree(s->part_source); dfree(s->part_work);
   s->part_free = -1;
   s->part_nnz = 0;
+}
+
+// The block solve's partial sums per dot product (fixed for a block size).
+int cg_parts(int64_t n) { return static_cast<int>(n / 4096 + 1 < 1024 ? n / 4096 + 1 : 1024); }
+
+std::size_t cg_work_doubles(int64_t n) {
+  return 5 * static_cast<std::size_t>(n) + 2 * static_cast<std::size_t>(cg_parts(n)) + sbk::kCgScalars;
 }
 
 void release(sparlab_system* s) {
@@ -306,7 +315,8 @@ int sparlab_system_partition_device(sparlab_system* s, const int32_t* free_dofs,
     return sbk::capi_fail(SPARLAB_E_ARGUMENT, "the free DOFs are not strictly ascending in [0, 3 num_nodes)");
   }
   const std::size_t nnz = static_cast<std::size_t>(head[0]);
-  if (!dalloc(s->part_index, nnz) || !dalloc(s->part_source, nnz)) {
+  if (!dalloc(s->part_index, nnz) || !dalloc(s->part_source, nnz) ||
+      !dalloc(s->part_work, cg_work_doubles(num_free))) {
     release_partition(s);
     return sbk::capi_fail(SPARLAB_E_DEVICE, "device allocation failed");
   }
@@ -343,6 +353,62 @@ int sparlab_system_free_block_device(sparlab_system* s, const double* values, do
   if (sbk::launch_free_block_values(values, s->part_source, s->part_nnz, block_values,
                                     static_cast<hipStream_t>(stream)) != 0)
     return sbk::capi_fail(SPARLAB_E_DEVICE, "free block launch failed");
+  return SPARLAB_OK;
+}
+
+int sparlab_system_solve_free_block_device(sparlab_system* s, const double* block_values, const double* b,
+                                           double* x, double tolerance, int64_t max_iterations,
+                                           int64_t* iterations, double* relative_residual, void* stream) {
+  if (!s || !iterations || !relative_residual) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (s->part_free < 0) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "no partition");
+  const int64_t n = s->part_free;
+  if ((s->part_nnz > 0 && !block_values) || (n > 0 && (!b || !x)))
+    return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (!(tolerance > 0.0) || !std::isfinite(tolerance) || max_iterations < 0)
+    return sbk::capi_fail(SPARLAB_E_ARGUMENT, "bad tolerance or iteration limit");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  sbk::CgArgs a{};
+  a.ptr = s->part_ptr;
+  a.index = s->part_index;
+  a.values = block_values;
+  a.n = n;
+  a.nparts = cg_parts(n);
+  const std::size_t un = static_cast<std::size_t>(n);
+  a.dinv = s->part_work;
+  a.r = a.dinv + un;
+  a.z = a.r + un;
+  a.p = a.z + un;
+  a.q = a.p + un;
+  a.partial = a.q + un;
+  a.scalars = a.partial + 2 * static_cast<std::size_t>(a.nparts);
+  double sc[sbk::kCgScalars];
+  auto read = [&]() {
+    return hipMemcpyAsync(sc, a.scalars, sizeof(sc), hipMemcpyDeviceToHost, st) == hipSuccess &&
+           hipStreamSynchronize(st) == hipSuccess;
+  };
+  *iterations = 0;
+  *relative_residual = 0.0;
+  if (sbk::launch_cg_start(a, b, x, st) != 0 || !read())
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's start failed");
+  const double goal = tolerance * tolerance * sc[sbk::kCgBb];
+  int64_t done = 0;
+  constexpr int64_t kRun = 16;  // iterations between two reads of the residual
+  bool converged = sc[sbk::kCgRr] <= goal;
+  while (!converged && sc[sbk::kCgBreak] == 0.0 && done < max_iterations) {
+    const int run = static_cast<int>(max_iterations - done < kRun ? max_iterations - done : kRun);
+    if (sbk::launch_cg_iterations(a, x, run, st) != 0 || !read())
+      return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's iterations failed");
+    done += run;
+    converged = sc[sbk::kCgRr] <= goal;
+  }
+  *iterations = done;
+  if (sc[sbk::kCgBreak] != 0.0)
+    return sbk::capi_fail(SPARLAB_E_SOLVER, "the free-free block is not positive definite (conjugate gradients)");
+  const double bb = sc[sbk::kCgBb];
+  if (sbk::launch_cg_residual(a, b, x, st) != 0 || !read())
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's residual failed");
+  *relative_residual = bb > 0.0 ? std::sqrt(sc[sbk::kCgRr] / bb) : std::sqrt(sc[sbk::kCgRr]);
+  if (!converged) return sbk::capi_fail(SPARLAB_E_SOLVER, "the block solve did not converge");
   return SPARLAB_OK;
 }
 

@@ -45,6 +45,7 @@ struct sparlab_system {
   int32_t *part_map = nullptr, *part_counts = nullptr, *part_sums = nullptr, *part_bad = nullptr;
   int32_t *part_ptr = nullptr, *part_index = nullptr, *part_source = nullptr;
   int64_t part_free = -1, part_nnz = 0;
+  double* part_work = nullptr;  // the block solve's vectors, partial sums and scalars
   double* ws = nullptr;         // the element kernels' workspace
   double* partial = nullptr;    // 1024 partial sums
   double* scalars = nullptr;    // [0] energy, [1] gross
