@@ -398,10 +398,12 @@ def assembly_leg(G, W, torch, dev, reps):
             "block_solve": {"note": "K_ff x = b on the clamped plate's elastic tangent by Jacobi-preconditioned "
                                     "conjugate gradients to ||b - K x|| <= 1e-10 ||b|| (host clock, reads of the "
                                     "residual every 16 iterations included); per iteration 12 B per block entry "
-                                    "(the product) + 136 B per free DOF (the vectors)",
+                                    "(the product's values and indices) + 112 B per free DOF (p read twice, q "
+                                    "written and read, x, r read and written, D^-1 read, z written and read, p "
+                                    "written)",
                             "iterations": int(solve_its), "relative_residual": solve_rel, "ms": solve_ms,
                             "us_per_iteration": solve_ms * 1e3 / max(solve_its, 1),
-                            "GBps": (12 * block_nnz + 136 * int(free_mask.sum())) * max(solve_its, 1)
+                            "GBps": (12 * block_nnz + 112 * int(free_mask.sum())) * max(solve_its, 1)
                                     / (solve_ms * 1e-3) / 1e9}}
 
 

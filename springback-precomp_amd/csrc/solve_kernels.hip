@@ -1,11 +1,16 @@
 // Jacobi-preconditioned conjugate gradients on the free-free block of a
 // symmetric tangent (K_ff x = b), HBM-bound: each iteration reads the block
-// once in its product q = K p and streams six vectors.
+// once in its product q = K p and streams the vectors twice.
 //
-// Every sum has a fixed order (a wavefront's butterfly per row of the
-// product, fixed partial ranges and one block's tree for the dot products),
-// so a solve repeats bit for bit. The iteration's scalars stay on the device:
-// the host launches iterations in runs and reads r . r between runs only.
+// An iteration is three launches: the product (which also sums p . q per
+// block), the update of x, r and z (which sums r . z and r . r per block) and
+// the new direction. Each block of the update and of the direction derives
+// alpha and beta itself from the partial sums, in the same fixed order, so no
+// launch is spent on scalars and every sum has one order: a solve repeats bit
+// for bit. The partial sums of r . z alternate between two buffers by
+// iteration parity, so the direction reads the old and the new sum while the
+// update of the next iteration has not yet written. The host launches
+// iterations in runs and reads r . r between runs only.
 
 #include "solve_kernels.hpp"
 
@@ -15,175 +20,255 @@ namespace sbk {
 
 namespace {
 
-constexpr int kRowsPerBlock = 4;  // one wavefront per row
+constexpr int kWaves = 4;          // wavefronts per product block
+constexpr int kLanesPerRow = 32;   // a row's lanes in the product (rows of the block hold up to 81 entries)
+constexpr int kGroups = 64 / kLanesPerRow;  // rows per wavefront at a time
+constexpr int kSlots = (81 + kLanesPerRow - 1) / kLanesPerRow;  // a lane's entries of a full row
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+// The sum of v[0, m) in a fixed order, returned to every thread of the
+// (256-thread) block; s is the block's scratch.
+__device__ double block_sum(const double* __restrict__ v, int m, double* s) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < m; i += 256) acc += v[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double out = s[0];
+  __syncthreads();
+  return out;
 }
 
-__global__ __launch_bounds__(256) void spmv_kernel(const int* __restrict__ ptr, const int* __restrict__ index,
-                                                   const double* __restrict__ values, long long n,
-                                                   const double* __restrict__ x, double* __restrict__ y) {
-  const int lane = threadIdx.x & 63;
-  const long long row = static_cast<long long>(blockIdx.x) * kRowsPerBlock + (threadIdx.x >> 6);
-  if (row >= n) return;  // the whole wavefront
-  const int k0 = ptr[row], k1 = ptr[row + 1];
+// Two per-thread sums reduced over the block in a fixed tree; thread 0 gets
+// the totals.
+__device__ void block_reduce2(double& a, double& b, double* s1, double* s2) {
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) {
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+      s2[threadIdx.x] += s2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  a = s1[0];
+  b = s2[0];
+}
+
+// This block's contiguous range of [0, n) when n is cut into gridDim.x parts.
+__device__ __forceinline__ void block_range(long long n, long long& i0, long long& i1) {
+  const long long per = (n + gridDim.x - 1) / gridDim.x;
+  i0 = static_cast<long long>(blockIdx.x) * per;
+  i1 = i0 + per < n ? i0 + per : n;
+}
+
+// The row's sum of K(row, :) x in a fixed order, in every lane of the row's
+// group: lane l of the group takes entries l, l + kLanesPerRow, ..., then a
+// butterfly within the group. A row beyond n sums to 0 (every lane of the
+// wavefront takes part in the butterfly).
+__device__ __forceinline__ double row_product(const int* __restrict__ ptr, const int* __restrict__ index,
+                                              const double* __restrict__ values, const double* __restrict__ x,
+                                              long long row, long long n, int l) {
   double acc = 0.0;
-  for (int k = k0 + lane; k < k1; k += 64) acc = fma(values[k], x[index[k]], acc);
-  acc = wave_sum(acc);
-  if (lane == 0) y[row] = acc;
+  if (row < n) {
+    // the lane's first kSlots entries: every index and value load issued
+    // before the gathers of x, and those before the sums (a row of the block
+    // holds at most 81 entries; longer rows finish in the loop after)
+    const int k0 = ptr[row] + l, k1 = ptr[row + 1];
+    int col[kSlots];
+    double v[kSlots], xv[kSlots];
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+      const int k = k0 + s * kLanesPerRow;
+      col[s] = k < k1 ? index[k] : -1;
+      v[s] = k < k1 ? values[k] : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) xv[s] = col[s] >= 0 ? x[col[s]] : 0.0;
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s)
+      if (col[s] >= 0) acc = fma(v[s], xv[s], acc);
+    for (int k = k0 + kSlots * kLanesPerRow; k < k1; k += kLanesPerRow) acc = fma(values[k], x[index[k]], acc);
+  }
+#pragma unroll
+  for (int off = kLanesPerRow / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  return acc;
+}
+
+// q = K p, kGroups rows per wavefront at a time, dealt round-robin over the
+// launch; pq[block] = sum over the block's rows of p q (each group's rows in
+// order, then the groups and wavefronts in order).
+__global__ __launch_bounds__(256) void product_kernel(CgArgs A) {
+  __shared__ double ws[kWaves * kGroups];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int g = lane / kLanesPerRow, l = lane % kLanesPerRow;
+  const long long stride = static_cast<long long>(gridDim.x) * kWaves * kGroups;
+  double dot = 0.0;
+  for (long long base = (static_cast<long long>(blockIdx.x) * kWaves + wave) * kGroups; base < A.n;
+       base += stride) {
+    const long long row = base + g;
+    const double qr = row_product(A.ptr, A.index, A.values, A.p, row, A.n, l);
+    if (l == 0 && row < A.n) {
+      A.q[row] = qr;
+      dot = fma(A.p[row], qr, dot);
+    }
+  }
+  if (l == 0) ws[wave * kGroups + g] = dot;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sum = 0.0;
+    for (int i = 0; i < kWaves * kGroups; ++i) sum += ws[i];
+    A.pq[blockIdx.x] = sum;
+  }
+}
+
+// alpha = rz / p.q from the partial sums (0 once the iteration broke down or
+// r = 0), then x += alpha p, r -= alpha q, z = D^-1 r over the block's range,
+// with the block's r . z (into the buffer of iteration k's parity) and r . r.
+__global__ __launch_bounds__(256) void update_kernel(CgArgs A, double* __restrict__ x, long long k) {
+  __shared__ double s1[256], s2[256];
+  const double pq = block_sum(A.pq, A.nrows, s1);
+  const double rz = block_sum(A.rz + ((k + 1) & 1) * A.nparts, A.nparts, s1);
+  bool broke = A.scalars[kCgBreak] != 0.0;
+  double alpha = 0.0;
+  if (!broke && rz != 0.0) {
+    if (pq > 0.0 && std::isfinite(pq))
+      alpha = rz / pq;
+    else
+      broke = true;  // p . K p <= 0: not positive definite (every block sees the same sums)
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    A.scalars[kCgAlpha] = alpha;
+    if (broke) A.scalars[kCgBreak] = 1.0;
+  }
+  long long i0, i1;
+  block_range(A.n, i0, i1);
+  double acc_rz = 0.0, acc_rr = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i1; i += 256) {
+    x[i] = fma(alpha, A.p[i], x[i]);
+    const double ri = fma(-alpha, A.q[i], A.r[i]);
+    const double zi = A.dinv[i] * ri;
+    A.r[i] = ri;
+    A.z[i] = zi;
+    acc_rz = fma(ri, zi, acc_rz);
+    acc_rr = fma(ri, ri, acc_rr);
+  }
+  block_reduce2(acc_rz, acc_rr, s1, s2);
+  if (threadIdx.x == 0) {
+    A.rz[(k & 1) * A.nparts + blockIdx.x] = acc_rz;
+    A.rr[blockIdx.x] = acc_rr;
+  }
+}
+
+// beta = r.z (new) / r.z (old) from the partial sums, p = z + beta p over the
+// block's range; block 0 leaves r . z, r . r and beta for the host.
+__global__ __launch_bounds__(256) void direction_kernel(CgArgs A, long long k) {
+  __shared__ double s[256];
+  const double rz_new = block_sum(A.rz + (k & 1) * A.nparts, A.nparts, s);
+  const double rz_old = block_sum(A.rz + ((k + 1) & 1) * A.nparts, A.nparts, s);
+  const bool broke = A.scalars[kCgBreak] != 0.0;
+  const double beta = (!broke && rz_old != 0.0) ? rz_new / rz_old : 0.0;
+  if (blockIdx.x == 0) {
+    const double rr = block_sum(A.rr, A.nparts, s);
+    if (threadIdx.x == 0) {
+      A.scalars[kCgRz] = rz_new;
+      A.scalars[kCgRr] = rr;
+      A.scalars[kCgBeta] = beta;
+    }
+  }
+  long long i0, i1;
+  block_range(A.n, i0, i1);
+  for (long long i = i0 + threadIdx.x; i < i1; i += 256) A.p[i] = fma(beta, A.p[i], A.z[i]);
 }
 
 // dinv[i] = 1 / K_ii; a missing, non-positive or non-finite diagonal sets
 // the break flag (the block is not positive definite).
-__global__ __launch_bounds__(256) void diagonal_kernel(const int* __restrict__ ptr, const int* __restrict__ index,
-                                                       const double* __restrict__ values, long long n,
-                                                       double* __restrict__ dinv, double* __restrict__ scalars) {
+__global__ __launch_bounds__(256) void diagonal_kernel(CgArgs A) {
   const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= n) return;
+  if (i >= A.n) return;
   double d = 0.0;
-  for (int k = ptr[i]; k < ptr[i + 1]; ++k)
-    if (index[k] == i) d = values[k];
+  for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k)
+    if (A.index[k] == i) d = A.values[k];
   if (!(d > 0.0) || !std::isfinite(d)) {
-    scalars[kCgBreak] = 1.0;
-    dinv[i] = 0.0;
+    A.scalars[kCgBreak] = 1.0;
+    A.dinv[i] = 0.0;
   } else {
-    dinv[i] = 1.0 / d;
+    A.dinv[i] = 1.0 / d;
   }
 }
 
-__global__ __launch_bounds__(256) void start_kernel(const double* __restrict__ b, const double* __restrict__ dinv,
-                                                    long long n, double* __restrict__ x, double* __restrict__ r,
-                                                    double* __restrict__ z, double* __restrict__ p) {
-  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= n) return;
-  const double bi = b[i];
-  x[i] = 0.0;
-  r[i] = bi;
-  z[i] = dinv[i] * bi;
-  p[i] = z[i];
-}
-
-// partial[blk] = sum a b and, when c is given, partial[nparts + blk] =
-// sum c d, over the block's fixed range.
-__global__ __launch_bounds__(256) void dot_partials_kernel(const double* __restrict__ a,
-                                                           const double* __restrict__ b,
-                                                           const double* __restrict__ c,
-                                                           const double* __restrict__ d, long long n,
-                                                           double* __restrict__ partial) {
+// x = 0, r = b, z = D^-1 b, p = z over the block's range; its r . z goes to
+// the buffer iteration 0 reads as the old sum, its r . r to rr.
+__global__ __launch_bounds__(256) void start_kernel(CgArgs A, const double* __restrict__ b, double* __restrict__ x) {
   __shared__ double s1[256], s2[256];
-  const long long per = (n + gridDim.x - 1) / gridDim.x;
-  const long long i0 = static_cast<long long>(blockIdx.x) * per;
-  const long long i1 = i0 + per < n ? i0 + per : n;
-  double acc1 = 0.0, acc2 = 0.0;
+  long long i0, i1;
+  block_range(A.n, i0, i1);
+  double acc_rz = 0.0, acc_rr = 0.0;
   for (long long i = i0 + threadIdx.x; i < i1; i += 256) {
-    acc1 = fma(a[i], b[i], acc1);
-    if (c) acc2 = fma(c[i], d[i], acc2);
+    const double bi = b[i];
+    const double zi = A.dinv[i] * bi;
+    x[i] = 0.0;
+    A.r[i] = bi;
+    A.z[i] = zi;
+    A.p[i] = zi;
+    acc_rz = fma(bi, zi, acc_rz);
+    acc_rr = fma(bi, bi, acc_rr);
   }
-  s1[threadIdx.x] = acc1;
-  s2[threadIdx.x] = acc2;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (static_cast<int>(threadIdx.x) < w) {
-      s1[threadIdx.x] += s1[threadIdx.x + w];
-      s2[threadIdx.x] += s2[threadIdx.x + w];
-    }
-    __syncthreads();
-  }
+  block_reduce2(acc_rz, acc_rr, s1, s2);
   if (threadIdx.x == 0) {
-    partial[blockIdx.x] = s1[0];
-    if (c) partial[gridDim.x + blockIdx.x] = s2[0];
+    A.rz[A.nparts + blockIdx.x] = acc_rz;
+    A.rr[blockIdx.x] = acc_rr;
   }
 }
 
-enum ScalarStep : int { kStepStart = 0, kStepAlpha = 1, kStepBeta = 2, kStepResidual = 3 };
+__global__ __launch_bounds__(256) void start_scalars_kernel(CgArgs A) {
+  __shared__ double s[256];
+  const double rz = block_sum(A.rz + A.nparts, A.nparts, s);
+  const double rr = block_sum(A.rr, A.nparts, s);
+  if (threadIdx.x == 0) {
+    A.scalars[kCgRz] = rz;
+    A.scalars[kCgRr] = rr;
+    A.scalars[kCgBb] = rr;
+    A.scalars[kCgAlpha] = 0.0;
+    A.scalars[kCgBeta] = 0.0;
+  }
+}
 
-// One block: the partial sums in a fixed tree, then the step's scalars.
-__global__ __launch_bounds__(256) void scalars_kernel(const double* __restrict__ partial, int nparts, int step,
-                                                      double* __restrict__ sc) {
+// y = K x, kGroups rows per wavefront.
+__global__ __launch_bounds__(256) void spmv_kernel(const int* __restrict__ ptr, const int* __restrict__ index,
+                                                   const double* __restrict__ values, long long n,
+                                                   const double* __restrict__ x, double* __restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const long long base = (static_cast<long long>(blockIdx.x) * kWaves + (threadIdx.x >> 6)) * kGroups;
+  if (base >= n) return;  // the whole wavefront
+  const long long row = base + lane / kLanesPerRow;
+  const double v = row_product(ptr, index, values, x, row, n, lane % kLanesPerRow);
+  if (lane % kLanesPerRow == 0 && row < n) y[row] = v;
+}
+
+// r = b - q over the block's range and its r . r into partial.
+__global__ __launch_bounds__(256) void residual_kernel(CgArgs A, const double* __restrict__ b) {
   __shared__ double s1[256], s2[256];
-  double acc1 = 0.0, acc2 = 0.0;
-  const bool two = step == kStepStart || step == kStepBeta;
-  for (int i = threadIdx.x; i < nparts; i += 256) {
-    acc1 += partial[i];
-    if (two) acc2 += partial[nparts + i];
+  long long i0, i1;
+  block_range(A.n, i0, i1);
+  double acc = 0.0, unused = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i1; i += 256) {
+    const double ri = b[i] - A.q[i];
+    A.r[i] = ri;
+    acc = fma(ri, ri, acc);
   }
-  s1[threadIdx.x] = acc1;
-  s2[threadIdx.x] = acc2;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (static_cast<int>(threadIdx.x) < w) {
-      s1[threadIdx.x] += s1[threadIdx.x + w];
-      s2[threadIdx.x] += s2[threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x != 0) return;
-  const double v1 = s1[0], v2 = s2[0];
-  switch (step) {
-    case kStepStart:  // r.z, r.r (= b.b)
-      sc[kCgRz] = v1;
-      sc[kCgRr] = v2;
-      sc[kCgBb] = v2;
-      sc[kCgAlpha] = 0.0;
-      sc[kCgBeta] = 0.0;
-      break;
-    case kStepAlpha: {  // p.q
-      const double rz = sc[kCgRz];
-      double alpha = 0.0;
-      if (rz != 0.0) {  // r = 0 is converged: no step
-        if (!(v1 > 0.0) || !std::isfinite(v1)) {
-          sc[kCgBreak] = 1.0;
-        } else {
-          alpha = rz / v1;
-        }
-      }
-      // after a break every further step is zero
-      sc[kCgAlpha] = sc[kCgBreak] != 0.0 ? 0.0 : alpha;
-      break;
-    }
-    case kStepBeta: {  // new r.z, r.r
-      const double rz = sc[kCgRz];
-      sc[kCgBeta] = (rz != 0.0 && sc[kCgBreak] == 0.0) ? v1 / rz : 0.0;
-      sc[kCgRz] = v1;
-      sc[kCgRr] = v2;
-      break;
-    }
-    default:  // the true residual's r.r
-      sc[kCgRr] = v1;
-      break;
-  }
+  block_reduce2(acc, unused, s1, s2);
+  if (threadIdx.x == 0) A.partial[blockIdx.x] = acc;
 }
 
-// x += alpha p, r -= alpha q, z = D^-1 r.
-__global__ __launch_bounds__(256) void update_kernel(const double* __restrict__ sc, const double* __restrict__ p,
-                                                     const double* __restrict__ q, const double* __restrict__ dinv,
-                                                     long long n, double* __restrict__ x, double* __restrict__ r,
-                                                     double* __restrict__ z) {
-  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= n) return;
-  const double alpha = sc[kCgAlpha];
-  x[i] = fma(alpha, p[i], x[i]);
-  const double ri = fma(-alpha, q[i], r[i]);
-  r[i] = ri;
-  z[i] = dinv[i] * ri;
-}
-
-// p = z + beta p.
-__global__ __launch_bounds__(256) void direction_kernel(const double* __restrict__ sc, const double* __restrict__ z,
-                                                        long long n, double* __restrict__ p) {
-  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= n) return;
-  p[i] = fma(sc[kCgBeta], p[i], z[i]);
-}
-
-// r = b - y.
-__global__ __launch_bounds__(256) void difference_kernel(const double* __restrict__ b, const double* __restrict__ y,
-                                                         long long n, double* __restrict__ r) {
-  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (i < n) r[i] = b[i] - y[i];
+__global__ __launch_bounds__(256) void residual_scalars_kernel(CgArgs A) {
+  __shared__ double s[256];
+  const double rr = block_sum(A.partial, A.nparts, s);
+  if (threadIdx.x == 0) A.scalars[kCgRr] = rr;
 }
 
 __global__ void clear_break_kernel(double* sc) {
@@ -194,60 +279,74 @@ unsigned int grid_of(long long n, long long per) { return static_cast<unsigned i
 
 }  // namespace
 
+int cg_vector_blocks(long long n) {
+  // two entries per thread, at most 1024 blocks (each block of the update and
+  // the direction sums every partial)
+  const long long b = (n + 511) / 512;
+  return static_cast<int>(b < 1 ? 1 : (b > 1024 ? 1024 : b));
+}
+
+int cg_product_blocks(long long n) {
+  // about 4 rows per group, at most 2048 blocks (8 wavefronts per SIMD)
+  const long long b = (n + 4 * kWaves * kGroups - 1) / (4 * kWaves * kGroups);
+  return static_cast<int>(b < 1 ? 1 : (b > 2048 ? 2048 : b));
+}
+
+long long cg_work_doubles(long long n) {
+  return 5 * n + cg_product_blocks(n) + 5ll * cg_vector_blocks(n) + kCgScalars;
+}
+
+void cg_layout(CgArgs& a, double* work) {
+  a.nparts = cg_vector_blocks(a.n);
+  a.nrows = cg_product_blocks(a.n);
+  a.dinv = work;
+  a.r = a.dinv + a.n;
+  a.z = a.r + a.n;
+  a.p = a.z + a.n;
+  a.q = a.p + a.n;
+  a.pq = a.q + a.n;
+  a.rz = a.pq + a.nrows;
+  a.rr = a.rz + 2 * a.nparts;
+  a.partial = a.rr + a.nparts;
+  a.scalars = a.partial + 2 * a.nparts;
+}
+
 int launch_block_spmv(const int* ptr, const int* index, const double* values, long long n, const double* x,
                       double* y, hipStream_t s) {
   (void)hipGetLastError();
   if (n > 0)
-    hipLaunchKernelGGL(spmv_kernel, dim3(grid_of(n, kRowsPerBlock)), dim3(64 * kRowsPerBlock), 0, s, ptr, index,
-                       values, n, x, y);
+    hipLaunchKernelGGL(spmv_kernel, dim3(grid_of(n, kWaves * kGroups)), dim3(64 * kWaves), 0, s, ptr, index, values,
+                       n, x, y);
   return static_cast<int>(hipGetLastError());
 }
 
 int launch_cg_start(const CgArgs& a, const double* b, double* x, hipStream_t s) {
   (void)hipGetLastError();
   hipLaunchKernelGGL(clear_break_kernel, dim3(1), dim3(64), 0, s, a.scalars);
-  if (a.n > 0) {
-    const unsigned int g = grid_of(a.n, 256);
-    hipLaunchKernelGGL(diagonal_kernel, dim3(g), dim3(256), 0, s, a.ptr, a.index, a.values, a.n, a.dinv, a.scalars);
-    hipLaunchKernelGGL(start_kernel, dim3(g), dim3(256), 0, s, b, a.dinv, a.n, x, a.r, a.z, a.p);
-  }
-  hipLaunchKernelGGL(dot_partials_kernel, dim3(a.nparts), dim3(256), 0, s, a.r, a.z, a.r, a.r, a.n, a.partial);
-  hipLaunchKernelGGL(scalars_kernel, dim3(1), dim3(256), 0, s, a.partial, a.nparts, static_cast<int>(kStepStart),
-                     a.scalars);
+  if (a.n > 0) hipLaunchKernelGGL(diagonal_kernel, dim3(grid_of(a.n, 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(start_kernel, dim3(a.nparts), dim3(256), 0, s, a, b, x);
+  hipLaunchKernelGGL(start_scalars_kernel, dim3(1), dim3(256), 0, s, a);
   return static_cast<int>(hipGetLastError());
 }
 
-int launch_cg_iterations(const CgArgs& a, double* x, int count, hipStream_t s) {
+int launch_cg_iterations(const CgArgs& a, double* x, long long first, int count, hipStream_t s) {
   (void)hipGetLastError();
   if (a.n == 0) return 0;
-  const unsigned int g = grid_of(a.n, 256);
-  for (int it = 0; it < count; ++it) {
-    hipLaunchKernelGGL(spmv_kernel, dim3(grid_of(a.n, kRowsPerBlock)), dim3(64 * kRowsPerBlock), 0, s, a.ptr,
-                       a.index, a.values, a.n, a.p, a.q);
-    hipLaunchKernelGGL(dot_partials_kernel, dim3(a.nparts), dim3(256), 0, s, a.p, a.q,
-                       static_cast<const double*>(nullptr), static_cast<const double*>(nullptr), a.n, a.partial);
-    hipLaunchKernelGGL(scalars_kernel, dim3(1), dim3(256), 0, s, a.partial, a.nparts, static_cast<int>(kStepAlpha),
-                       a.scalars);
-    hipLaunchKernelGGL(update_kernel, dim3(g), dim3(256), 0, s, a.scalars, a.p, a.q, a.dinv, a.n, x, a.r, a.z);
-    hipLaunchKernelGGL(dot_partials_kernel, dim3(a.nparts), dim3(256), 0, s, a.r, a.z, a.r, a.r, a.n, a.partial);
-    hipLaunchKernelGGL(scalars_kernel, dim3(1), dim3(256), 0, s, a.partial, a.nparts, static_cast<int>(kStepBeta),
-                       a.scalars);
-    hipLaunchKernelGGL(direction_kernel, dim3(g), dim3(256), 0, s, a.scalars, a.z, a.n, a.p);
+  for (long long k = first; k < first + count; ++k) {
+    hipLaunchKernelGGL(product_kernel, dim3(a.nrows), dim3(64 * kWaves), 0, s, a);
+    hipLaunchKernelGGL(update_kernel, dim3(a.nparts), dim3(256), 0, s, a, x, k);
+    hipLaunchKernelGGL(direction_kernel, dim3(a.nparts), dim3(256), 0, s, a, k);
   }
   return static_cast<int>(hipGetLastError());
 }
 
 int launch_cg_residual(const CgArgs& a, const double* b, const double* x, hipStream_t s) {
   (void)hipGetLastError();
-  if (a.n > 0) {
-    hipLaunchKernelGGL(spmv_kernel, dim3(grid_of(a.n, kRowsPerBlock)), dim3(64 * kRowsPerBlock), 0, s, a.ptr,
-                       a.index, a.values, a.n, x, a.q);
-    hipLaunchKernelGGL(difference_kernel, dim3(grid_of(a.n, 256)), dim3(256), 0, s, b, a.q, a.n, a.r);
-  }
-  hipLaunchKernelGGL(dot_partials_kernel, dim3(a.nparts), dim3(256), 0, s, a.r, a.r,
-                     static_cast<const double*>(nullptr), static_cast<const double*>(nullptr), a.n, a.partial);
-  hipLaunchKernelGGL(scalars_kernel, dim3(1), dim3(256), 0, s, a.partial, a.nparts, static_cast<int>(kStepResidual),
-                     a.scalars);
+  if (a.n > 0)
+    hipLaunchKernelGGL(spmv_kernel, dim3(grid_of(a.n, kWaves * kGroups)), dim3(64 * kWaves), 0, s, a.ptr, a.index,
+                       a.values, a.n, x, a.q);
+  hipLaunchKernelGGL(residual_kernel, dim3(a.nparts), dim3(256), 0, s, a, b);
+  hipLaunchKernelGGL(residual_scalars_kernel, dim3(1), dim3(256), 0, s, a);
   return static_cast<int>(hipGetLastError());
 }
 

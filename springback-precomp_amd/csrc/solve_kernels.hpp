@@ -23,25 +23,40 @@ struct CgArgs {
   const int* index;     // the block's column indices
   const double* values; // the block's values (rows; K_ff symmetric)
   long long n;
-  int nparts;           // partial sums per dot product (fixed: the sums' order)
+  int nparts;           // vector blocks: fixed contiguous ranges, one partial sum each
+  int nrows;            // product blocks (4 wavefronts each, rows dealt round-robin)
   double* dinv;         // n: 1 / K_ii
   double* r;            // n
   double* z;            // n
   double* p;            // n
   double* q;            // n: K p
-  double* partial;      // 2 nparts
+  double* pq;           // nrows: partial sums of p . q
+  double* rz;           // 2 nparts: partial sums of r . z, by iteration parity
+  double* rr;           // nparts: partial sums of r . r
+  double* partial;      // 2 nparts (start and residual)
   double* scalars;      // kCgScalars
 };
+
+// Product blocks for a block of n rows, and the doubles of work a solve of n
+// unknowns takes (vectors, partial sums, scalars).
+int cg_vector_blocks(long long n);
+int cg_product_blocks(long long n);
+long long cg_work_doubles(long long n);
+// The work's pieces laid out from `work`.
+void cg_layout(CgArgs& a, double* work);
 
 // The diagonal's inverse (a missing or non-positive diagonal sets kCgBreak),
 // x = 0, r = b, z = D^-1 r, p = z and the first r . z, r . r, b . b.
 int launch_cg_start(const CgArgs& a, const double* b, double* x, hipStream_t s);
-// `count` iterations: q = K p, alpha = rz / p.q, x += alpha p, r -= alpha q,
-// z = D^-1 r, beta = r.z / rz, p = z + beta p.
-int launch_cg_iterations(const CgArgs& a, double* x, int count, hipStream_t s);
+// Iterations [first, first + count): q = K p, alpha = rz / p.q, x += alpha p,
+// r -= alpha q, z = D^-1 r, beta = r.z / rz, p = z + beta p; three launches
+// each (the product with p . q, the update with r . z and r . r, the
+// direction), every block deriving alpha and beta from the partial sums in
+// the same fixed order.
+int launch_cg_iterations(const CgArgs& a, double* x, long long first, int count, hipStream_t s);
 // The true residual's r . r (r = b - K x) into scalars[kCgRr].
 int launch_cg_residual(const CgArgs& a, const double* b, const double* x, hipStream_t s);
-// y = K x over the block's rows (one wavefront per row, fixed-order sums).
+// y = K x over the block's rows (fixed-order sums).
 int launch_block_spmv(const int* ptr, const int* index, const double* values, long long n, const double* x,
                       double* y, hipStream_t s);
 

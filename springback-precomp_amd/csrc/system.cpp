@@ -49,13 +49,6 @@ void release_partition(sparlab_system* s) {
   s->part_nnz = 0;
 }
 
-// The block solve's partial sums per dot product (fixed for a block size).
-int cg_parts(int64_t n) { return static_cast<int>(n / 4096 + 1 < 1024 ? n / 4096 + 1 : 1024); }
-
-std::size_t cg_work_doubles(int64_t n) {
-  return 5 * static_cast<std::size_t>(n) + 2 * static_cast<std::size_t>(cg_parts(n)) + sbk::kCgScalars;
-}
-
 void release(sparlab_system* s) {
   release_partition(s);
   dfree(s->part_map);
@@ -316,7 +309,7 @@ int sparlab_system_partition_device(sparlab_system* s, const int32_t* free_dofs,
   }
   const std::size_t nnz = static_cast<std::size_t>(head[0]);
   if (!dalloc(s->part_index, nnz) || !dalloc(s->part_source, nnz) ||
-      !dalloc(s->part_work, cg_work_doubles(num_free))) {
+      !dalloc(s->part_work, static_cast<std::size_t>(sbk::cg_work_doubles(num_free)))) {
     release_partition(s);
     return sbk::capi_fail(SPARLAB_E_DEVICE, "device allocation failed");
   }
@@ -372,15 +365,7 @@ int sparlab_system_solve_free_block_device(sparlab_system* s, const double* bloc
   a.index = s->part_index;
   a.values = block_values;
   a.n = n;
-  a.nparts = cg_parts(n);
-  const std::size_t un = static_cast<std::size_t>(n);
-  a.dinv = s->part_work;
-  a.r = a.dinv + un;
-  a.z = a.r + un;
-  a.p = a.z + un;
-  a.q = a.p + un;
-  a.partial = a.q + un;
-  a.scalars = a.partial + 2 * static_cast<std::size_t>(a.nparts);
+  sbk::cg_layout(a, s->part_work);
   double sc[sbk::kCgScalars];
   auto read = [&]() {
     return hipMemcpyAsync(sc, a.scalars, sizeof(sc), hipMemcpyDeviceToHost, st) == hipSuccess &&
@@ -396,7 +381,7 @@ int sparlab_system_solve_free_block_device(sparlab_system* s, const double* bloc
   bool converged = sc[sbk::kCgRr] <= goal;
   while (!converged && sc[sbk::kCgBreak] == 0.0 && done < max_iterations) {
     const int run = static_cast<int>(max_iterations - done < kRun ? max_iterations - done : kRun);
-    if (sbk::launch_cg_iterations(a, x, run, st) != 0 || !read())
+    if (sbk::launch_cg_iterations(a, x, done, run, st) != 0 || !read())
       return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's iterations failed");
     done += run;
     converged = sc[sbk::kCgRr] <= goal;
