@@ -294,8 +294,10 @@ int sparlab_system_free_block_device(sparlab_system* system, const double* value
  * (PartitionedFactor::factorize and solve, src/fem/Forming.cpp:104-172,
  * called at :1001-1016 with b = -r). The method differs: conjugate
  * gradients preconditioned with the block's diagonal, from x = 0, until the
- * iteration's residual satisfies ||b - K x|| <= tolerance ||b|| (checked every
- * 16 iterations) or max_iterations pass. *iterations (host) receives the
+ * true residual satisfies ||b - K x|| <= tolerance ||b|| or max_iterations
+ * pass (the iteration's own residual is read every 16 iterations; when it
+ * passes, the true one is computed, and the iteration restarts from it when
+ * that does not). *iterations (host) receives the
  * iterations run and *relative_residual (host) the final true residual
  * ||b - K x|| / ||b||. SPARLAB_E_SOLVER when the block is not positive
  * definite (a non-positive diagonal or curvature; the reference then falls

@@ -238,6 +238,39 @@ __global__ __launch_bounds__(256) void start_scalars_kernel(CgArgs A) {
   }
 }
 
+// A restart from the residual in r: z = D^-1 r, p = z over the block's
+// range, its r . z into the buffer `slot` (the one the next iteration reads
+// as the old sum) and its r . r.
+__global__ __launch_bounds__(256) void restart_kernel(CgArgs A, int slot) {
+  __shared__ double s1[256], s2[256];
+  long long i0, i1;
+  block_range(A.n, i0, i1);
+  double acc_rz = 0.0, acc_rr = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i1; i += 256) {
+    const double ri = A.r[i];
+    const double zi = A.dinv[i] * ri;
+    A.z[i] = zi;
+    A.p[i] = zi;
+    acc_rz = fma(ri, zi, acc_rz);
+    acc_rr = fma(ri, ri, acc_rr);
+  }
+  block_reduce2(acc_rz, acc_rr, s1, s2);
+  if (threadIdx.x == 0) {
+    A.rz[slot * A.nparts + blockIdx.x] = acc_rz;
+    A.rr[blockIdx.x] = acc_rr;
+  }
+}
+
+__global__ __launch_bounds__(256) void restart_scalars_kernel(CgArgs A, int slot) {
+  __shared__ double s[256];
+  const double rz = block_sum(A.rz + slot * A.nparts, A.nparts, s);
+  const double rr = block_sum(A.rr, A.nparts, s);
+  if (threadIdx.x == 0) {
+    A.scalars[kCgRz] = rz;
+    A.scalars[kCgRr] = rr;
+  }
+}
+
 // y = K x, kGroups rows per wavefront.
 __global__ __launch_bounds__(256) void spmv_kernel(const int* __restrict__ ptr, const int* __restrict__ index,
                                                    const double* __restrict__ values, long long n,
@@ -337,6 +370,14 @@ int launch_cg_iterations(const CgArgs& a, double* x, long long first, int count,
     hipLaunchKernelGGL(update_kernel, dim3(a.nparts), dim3(256), 0, s, a, x, k);
     hipLaunchKernelGGL(direction_kernel, dim3(a.nparts), dim3(256), 0, s, a, k);
   }
+  return static_cast<int>(hipGetLastError());
+}
+
+int launch_cg_restart(const CgArgs& a, long long next, hipStream_t s) {
+  (void)hipGetLastError();
+  const int slot = static_cast<int>((next + 1) & 1);
+  hipLaunchKernelGGL(restart_kernel, dim3(a.nparts), dim3(256), 0, s, a, slot);
+  hipLaunchKernelGGL(restart_scalars_kernel, dim3(1), dim3(256), 0, s, a, slot);
   return static_cast<int>(hipGetLastError());
 }
 

@@ -54,7 +54,11 @@ int launch_cg_start(const CgArgs& a, const double* b, double* x, hipStream_t s);
 // direction), every block deriving alpha and beta from the partial sums in
 // the same fixed order.
 int launch_cg_iterations(const CgArgs& a, double* x, long long first, int count, hipStream_t s);
-// The true residual's r . r (r = b - K x) into scalars[kCgRr].
+// A restart from the residual in r (after launch_cg_residual): z = D^-1 r,
+// p = z, with r . z where iteration `next` reads it, and r . z, r . r in the
+// scalars.
+int launch_cg_restart(const CgArgs& a, long long next, hipStream_t s);
+// The true residual r = b - K x into r and its r . r into scalars[kCgRr].
 int launch_cg_residual(const CgArgs& a, const double* b, const double* x, hipStream_t s);
 // y = K x over the block's rows (fixed-order sums).
 int launch_block_spmv(const int* ptr, const int* index, const double* values, long long n, const double* x,

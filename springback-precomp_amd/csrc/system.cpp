@@ -375,23 +375,33 @@ int sparlab_system_solve_free_block_device(sparlab_system* s, const double* bloc
   *relative_residual = 0.0;
   if (sbk::launch_cg_start(a, b, x, st) != 0 || !read())
     return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's start failed");
-  const double goal = tolerance * tolerance * sc[sbk::kCgBb];
+  const double bb = sc[sbk::kCgBb];
+  const double goal = tolerance * tolerance * bb;
   int64_t done = 0;
   constexpr int64_t kRun = 16;  // iterations between two reads of the residual
-  bool converged = sc[sbk::kCgRr] <= goal;
+  // The iteration's residual drifts from b - K x; when it meets the
+  // tolerance the true residual is computed, and the iteration restarts from
+  // it (residual replacement) unless that meets the tolerance too.
+  bool converged = sc[sbk::kCgRr] <= goal;  // r = b at the start: exact
   while (!converged && sc[sbk::kCgBreak] == 0.0 && done < max_iterations) {
     const int run = static_cast<int>(max_iterations - done < kRun ? max_iterations - done : kRun);
     if (sbk::launch_cg_iterations(a, x, done, run, st) != 0 || !read())
       return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's iterations failed");
     done += run;
-    converged = sc[sbk::kCgRr] <= goal;
+    if (sc[sbk::kCgBreak] == 0.0 && sc[sbk::kCgRr] <= goal) {
+      if (sbk::launch_cg_residual(a, b, x, st) != 0 || !read())
+        return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's residual failed");
+      converged = sc[sbk::kCgRr] <= goal;
+      if (!converged && (sbk::launch_cg_restart(a, done, st) != 0 || !read()))
+        return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's restart failed");
+    }
   }
   *iterations = done;
   if (sc[sbk::kCgBreak] != 0.0)
     return sbk::capi_fail(SPARLAB_E_SOLVER, "the free-free block is not positive definite (conjugate gradients)");
-  const double bb = sc[sbk::kCgBb];
-  if (sbk::launch_cg_residual(a, b, x, st) != 0 || !read())
+  if (!converged && (sbk::launch_cg_residual(a, b, x, st) != 0 || !read()))
     return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's residual failed");
+  // after a converged run sc holds the true residual's r . r
   *relative_residual = bb > 0.0 ? std::sqrt(sc[sbk::kCgRr] / bb) : std::sqrt(sc[sbk::kCgRr]);
   if (!converged) return sbk::capi_fail(SPARLAB_E_SOLVER, "the block solve did not converge");
   return SPARLAB_OK;
