@@ -341,6 +341,13 @@ def assembly_leg(G, W, torch, dev, reps):
     _, solve_its, solve_rel = sysm.solve_free_block_device(k_block, rhs, x=x_sol, tolerance=1e-10, stream=stream)
     torch.cuda.synchronize()
     solve_ms = (time.perf_counter() - t0) * 1e3
+    # the same system by BiCGStab (the method for a non-symmetric tangent)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, bicg_its, bicg_rel = sysm.solve_free_block_device(k_block, rhs, x=x_sol, tolerance=1e-10, symmetric=False,
+                                                         stream=stream)
+    torch.cuda.synchronize()
+    bicg_ms = (time.perf_counter() - t0) * 1e3
     del k_values, k_block
 
     def candidates():
@@ -404,7 +411,11 @@ def assembly_leg(G, W, torch, dev, reps):
                             "iterations": int(solve_its), "relative_residual": solve_rel, "ms": solve_ms,
                             "us_per_iteration": solve_ms * 1e3 / max(solve_its, 1),
                             "GBps": (12 * block_nnz + 112 * int(free_mask.sum())) * max(solve_its, 1)
-                                    / (solve_ms * 1e-3) / 1e9}}
+                                    / (solve_ms * 1e-3) / 1e9,
+                            "bicgstab": {"note": "the same system by BiCGStab (symmetric = 0, the method for a "
+                                                 "non-symmetric tangent): two products per iteration",
+                                         "iterations": int(bicg_its), "relative_residual": bicg_rel, "ms": bicg_ms,
+                                         "us_per_iteration": bicg_ms * 1e3 / max(bicg_its, 1)}}}
 
 
 def copy_probe(torch, mib):

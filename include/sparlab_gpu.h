@@ -290,23 +290,26 @@ int sparlab_system_free_block_device(sparlab_system* system, const double* value
                                      void* stream);
 /* Solves K_ff x = b on the partition's block (block_values from
  * sparlab_system_free_block_device; b, x: num_free, device) where the
- * reference factorises and solves a symmetric tangent
- * (PartitionedFactor::factorize and solve, src/fem/Forming.cpp:104-172,
- * called at :1001-1016 with b = -r). The method differs: conjugate
- * gradients preconditioned with the block's diagonal, from x = 0, until the
- * true residual satisfies ||b - K x|| <= tolerance ||b|| or max_iterations
- * pass (the iteration's own residual is read every 16 iterations; when it
- * passes, the true one is computed, and the iteration restarts from it when
- * that does not). *iterations (host) receives the
+ * reference factorises and solves the tangent (PartitionedFactor::factorize
+ * and solve, src/fem/Forming.cpp:104-172, called at :1001-1016 with b = -r
+ * and symmetric = the system's and the contact's symmetry). The method
+ * differs: preconditioned with the block's diagonal, conjugate gradients for
+ * symmetric != 0 (the reference: CHOLMOD LLT, else LDL^T) and BiCGStab for a
+ * non-symmetric block (the reference: UMFPACK or SparseLU), from x = 0, until
+ * the true residual satisfies ||b - K x|| <= tolerance ||b|| or
+ * max_iterations pass (the iteration's own residual is read every 16
+ * iterations; when it passes, the true one is computed, and the iteration
+ * restarts from it when that does not). *iterations (host) receives the
  * iterations run and *relative_residual (host) the final true residual
- * ||b - K x|| / ||b||. SPARLAB_E_SOLVER when the block is not positive
- * definite (a non-positive diagonal or curvature; the reference then falls
- * back to LDL^T or LU) or the iteration does not converge; x then holds the
- * last iterate. The block is read by rows (K_ff symmetric: either layout).
- * Every sum has a fixed order: a solve repeats bit for bit. Synchronizes
- * stream between runs of iterations. */
-int sparlab_system_solve_free_block_device(sparlab_system* system, const double* block_values, const double* b,
-                                           double* x, double tolerance, int64_t max_iterations,
+ * ||b - K x|| / ||b||. SPARLAB_E_SOLVER when the iteration breaks down (a
+ * symmetric block that is not positive definite: a non-positive diagonal or
+ * curvature; the reference then falls back to LDL^T or LU) or does not
+ * converge; x then holds the last iterate. The block is read by rows: its
+ * row-major values give K_ff, the reference's column-major ones K_ff^T (the
+ * same for a symmetric block). Every sum has a fixed order: a solve repeats
+ * bit for bit. Synchronizes stream between runs of iterations. */
+int sparlab_system_solve_free_block_device(sparlab_system* system, const double* block_values, int symmetric,
+                                           const double* b, double* x, double tolerance, int64_t max_iterations,
                                            int64_t* iterations, double* relative_residual, void* stream);
 
 /* Per-element stress measures (the reference's ElastoplasticStress,

@@ -98,31 +98,43 @@ __device__ __forceinline__ double row_product(const int* __restrict__ ptr, const
   return acc;
 }
 
-// q = K p, kGroups rows per wavefront at a time, dealt round-robin over the
-// launch; pq[block] = sum over the block's rows of p q (each group's rows in
+// out = K in, kGroups rows per wavefront at a time, dealt round-robin over
+// the launch; partial[block] = sum over the block's rows of w out and, when
+// square is given, square[block] = sum of out out (each group's rows in
 // order, then the groups and wavefronts in order).
-__global__ __launch_bounds__(256) void product_kernel(CgArgs A) {
-  __shared__ double ws[kWaves * kGroups];
+__global__ __launch_bounds__(256) void product_kernel(const int* __restrict__ ptr, const int* __restrict__ index,
+                                                      const double* __restrict__ values, long long n,
+                                                      const double* __restrict__ in, double* __restrict__ out,
+                                                      const double* __restrict__ w, double* __restrict__ partial,
+                                                      double* __restrict__ square) {
+  __shared__ double ws[kWaves * kGroups], ws2[kWaves * kGroups];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int g = lane / kLanesPerRow, l = lane % kLanesPerRow;
   const long long stride = static_cast<long long>(gridDim.x) * kWaves * kGroups;
-  double dot = 0.0;
-  for (long long base = (static_cast<long long>(blockIdx.x) * kWaves + wave) * kGroups; base < A.n;
-       base += stride) {
+  double dot = 0.0, sq = 0.0;
+  for (long long base = (static_cast<long long>(blockIdx.x) * kWaves + wave) * kGroups; base < n; base += stride) {
     const long long row = base + g;
-    const double qr = row_product(A.ptr, A.index, A.values, A.p, row, A.n, l);
-    if (l == 0 && row < A.n) {
-      A.q[row] = qr;
-      dot = fma(A.p[row], qr, dot);
+    const double o = row_product(ptr, index, values, in, row, n, l);
+    if (l == 0 && row < n) {
+      out[row] = o;
+      dot = fma(w[row], o, dot);
+      sq = fma(o, o, sq);
     }
   }
-  if (l == 0) ws[wave * kGroups + g] = dot;
+  if (l == 0) {
+    ws[wave * kGroups + g] = dot;
+    ws2[wave * kGroups + g] = sq;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double sum = 0.0;
-    for (int i = 0; i < kWaves * kGroups; ++i) sum += ws[i];
-    A.pq[blockIdx.x] = sum;
+    double sum = 0.0, sum2 = 0.0;
+    for (int i = 0; i < kWaves * kGroups; ++i) {
+      sum += ws[i];
+      sum2 += ws2[i];
+    }
+    partial[blockIdx.x] = sum;
+    if (square) square[blockIdx.x] = sum2;
   }
 }
 
@@ -304,6 +316,154 @@ __global__ __launch_bounds__(256) void residual_scalars_kernel(CgArgs A) {
   if (threadIdx.x == 0) A.scalars[kCgRr] = rr;
 }
 
+// ---------------------------------------------------------------------------
+// BiCGStab (van der Vorst), preconditioned with the diagonal, for a block
+// that is not symmetric. Iteration k: p = r + beta (p - omega v) (p = r at a
+// cycle's start), y = D^-1 p, v = K y, alpha = rho / r^.v, s = r - alpha v,
+// x += alpha y, z = D^-1 s, t = K z, omega = t.s / t.t, x += omega z,
+// r = s - omega t, rho' = r^ . r. The scalars come from the partial sums in
+// every block that needs them, in one order; a zero rho, r^ . v or omega
+// with r != 0 is a breakdown (the flag; every later step is zero).
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void bicg_start_kernel(CgArgs A, const double* __restrict__ b,
+                                                         double* __restrict__ x) {
+  __shared__ double s1[256], s2[256];
+  long long i0, i1;
+  block_range(A.n, i0, i1);
+  double acc = 0.0, unused = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i1; i += 256) {
+    const double bi = b[i];
+    x[i] = 0.0;
+    A.r[i] = bi;
+    A.rhat[i] = bi;
+    A.p[i] = 0.0;
+    A.v[i] = 0.0;
+    acc = fma(bi, bi, acc);
+  }
+  block_reduce2(acc, unused, s1, s2);
+  if (threadIdx.x == 0) {
+    A.rho[blockIdx.x] = acc;  // iteration 0's r^ . r
+    A.rr[blockIdx.x] = acc;
+  }
+}
+
+// rho (the slot's r^ . r) and r . r into the scalars; b . b too at the start.
+__global__ __launch_bounds__(256) void bicg_scalars_kernel(CgArgs A, int slot, int start) {
+  __shared__ double s[256];
+  const double rho = block_sum(A.rho + slot * A.nparts, A.nparts, s);
+  const double rr = block_sum(A.rr, A.nparts, s);
+  if (threadIdx.x == 0) {
+    A.scalars[kCgRz] = rho;
+    A.scalars[kCgRr] = rr;
+    if (start) {
+      A.scalars[kCgBb] = rr;
+      A.scalars[kCgAlpha] = 0.0;
+      A.scalars[kCgBeta] = 0.0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void bicg_direction_kernel(CgArgs A, long long cycle, long long k) {
+  __shared__ double sh[256];
+  const double rr = block_sum(A.rr, A.nparts, sh);
+  const double rho = block_sum(A.rho + (k & 1) * A.nparts, A.nparts, sh);
+  const double rho_prev = block_sum(A.rho + ((k + 1) & 1) * A.nparts, A.nparts, sh);
+  const double rv = block_sum(A.pv, A.nrows, sh);
+  const double ts = block_sum(A.pts, A.nrows, sh);
+  const double tt = block_sum(A.pts + A.nrows, A.nrows, sh);
+  const bool first = k == cycle;
+  bool broke = A.scalars[kCgBreak] != 0.0;
+  double beta = 0.0, omega = 0.0;
+  if (!first) {
+    const double alpha = rho_prev / rv;
+    omega = ts / tt;
+    if (!broke && rr != 0.0) {
+      if (rho == 0.0 || omega == 0.0 || !std::isfinite(alpha) || !std::isfinite(omega))
+        broke = true;
+      else
+        beta = (rho / rho_prev) * (alpha / omega);
+    }
+    if (broke || !std::isfinite(omega)) omega = 0.0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && broke) A.scalars[kCgBreak] = 1.0;
+  long long i0, i1;
+  block_range(A.n, i0, i1);
+  for (long long i = i0 + threadIdx.x; i < i1; i += 256) {
+    const double pi = first ? A.r[i] : fma(beta, fma(-omega, A.v[i], A.p[i]), A.r[i]);
+    A.p[i] = pi;
+    A.y[i] = A.dinv[i] * pi;
+  }
+}
+
+__global__ __launch_bounds__(256) void bicg_half_kernel(CgArgs A, double* __restrict__ x, long long k) {
+  __shared__ double sh[256];
+  const double rr = block_sum(A.rr, A.nparts, sh);
+  const double rho = block_sum(A.rho + (k & 1) * A.nparts, A.nparts, sh);
+  const double rv = block_sum(A.pv, A.nrows, sh);
+  bool broke = A.scalars[kCgBreak] != 0.0;
+  double alpha = 0.0;
+  if (!broke && rr != 0.0) {
+    if (rv != 0.0 && std::isfinite(rho / rv))
+      alpha = rho / rv;
+    else
+      broke = true;  // every block sees the same sums
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    A.scalars[kCgAlpha] = alpha;
+    if (broke) A.scalars[kCgBreak] = 1.0;
+  }
+  long long i0, i1;
+  block_range(A.n, i0, i1);
+  for (long long i = i0 + threadIdx.x; i < i1; i += 256) {
+    const double si = fma(-alpha, A.v[i], A.r[i]);
+    A.s[i] = si;
+    x[i] = fma(alpha, A.y[i], x[i]);
+    A.z[i] = A.dinv[i] * si;
+  }
+}
+
+__global__ __launch_bounds__(256) void bicg_full_kernel(CgArgs A, double* __restrict__ x, long long k) {
+  __shared__ double s1[256], s2[256];
+  const double ts = block_sum(A.pts, A.nrows, s1);
+  const double tt = block_sum(A.pts + A.nrows, A.nrows, s1);
+  const bool broke = A.scalars[kCgBreak] != 0.0;
+  const double omega = (!broke && tt != 0.0 && std::isfinite(ts / tt)) ? ts / tt : 0.0;
+  long long i0, i1;
+  block_range(A.n, i0, i1);
+  double acc_rho = 0.0, acc_rr = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i1; i += 256) {
+    x[i] = fma(omega, A.z[i], x[i]);
+    const double ri = fma(-omega, A.t[i], A.s[i]);
+    A.r[i] = ri;
+    acc_rho = fma(A.rhat[i], ri, acc_rho);
+    acc_rr = fma(ri, ri, acc_rr);
+  }
+  block_reduce2(acc_rho, acc_rr, s1, s2);
+  if (threadIdx.x == 0) {
+    A.rho[((k + 1) & 1) * A.nparts + blockIdx.x] = acc_rho;
+    A.rr[blockIdx.x] = acc_rr;
+  }
+}
+
+// A new cycle from the residual in r: r^ = r, its r . r as rho (slot) and rr.
+__global__ __launch_bounds__(256) void bicg_restart_kernel(CgArgs A, int slot) {
+  __shared__ double s1[256], s2[256];
+  long long i0, i1;
+  block_range(A.n, i0, i1);
+  double acc = 0.0, unused = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i1; i += 256) {
+    const double ri = A.r[i];
+    A.rhat[i] = ri;
+    acc = fma(ri, ri, acc);
+  }
+  block_reduce2(acc, unused, s1, s2);
+  if (threadIdx.x == 0) {
+    A.rho[slot * A.nparts + blockIdx.x] = acc;
+    A.rr[blockIdx.x] = acc;
+  }
+}
+
 __global__ void clear_break_kernel(double* sc) {
   if (threadIdx.x == 0) sc[kCgBreak] = 0.0;
 }
@@ -326,7 +486,7 @@ int cg_product_blocks(long long n) {
 }
 
 long long cg_work_doubles(long long n) {
-  return 5 * n + cg_product_blocks(n) + 5ll * cg_vector_blocks(n) + kCgScalars;
+  return 10 * n + 4ll * cg_product_blocks(n) + 7ll * cg_vector_blocks(n) + kCgScalars;
 }
 
 void cg_layout(CgArgs& a, double* work) {
@@ -342,6 +502,14 @@ void cg_layout(CgArgs& a, double* work) {
   a.rr = a.rz + 2 * a.nparts;
   a.partial = a.rr + a.nparts;
   a.scalars = a.partial + 2 * a.nparts;
+  a.rhat = a.scalars + kCgScalars;
+  a.v = a.rhat + a.n;
+  a.y = a.v + a.n;
+  a.s = a.y + a.n;
+  a.t = a.s + a.n;
+  a.pv = a.t + a.n;
+  a.pts = a.pv + a.nrows;
+  a.rho = a.pts + 2 * a.nrows;
 }
 
 int launch_block_spmv(const int* ptr, const int* index, const double* values, long long n, const double* x,
@@ -366,10 +534,46 @@ int launch_cg_iterations(const CgArgs& a, double* x, long long first, int count,
   (void)hipGetLastError();
   if (a.n == 0) return 0;
   for (long long k = first; k < first + count; ++k) {
-    hipLaunchKernelGGL(product_kernel, dim3(a.nrows), dim3(64 * kWaves), 0, s, a);
+    hipLaunchKernelGGL(product_kernel, dim3(a.nrows), dim3(64 * kWaves), 0, s, a.ptr, a.index, a.values, a.n, a.p,
+                       a.q, a.p, a.pq, static_cast<double*>(nullptr));
     hipLaunchKernelGGL(update_kernel, dim3(a.nparts), dim3(256), 0, s, a, x, k);
     hipLaunchKernelGGL(direction_kernel, dim3(a.nparts), dim3(256), 0, s, a, k);
   }
+  return static_cast<int>(hipGetLastError());
+}
+
+int launch_bicg_start(const CgArgs& a, const double* b, double* x, hipStream_t s) {
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(clear_break_kernel, dim3(1), dim3(64), 0, s, a.scalars);
+  if (a.n > 0) hipLaunchKernelGGL(diagonal_kernel, dim3(grid_of(a.n, 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(bicg_start_kernel, dim3(a.nparts), dim3(256), 0, s, a, b, x);
+  hipLaunchKernelGGL(bicg_scalars_kernel, dim3(1), dim3(256), 0, s, a, 0, 1);
+  return static_cast<int>(hipGetLastError());
+}
+
+int launch_bicg_iterations(const CgArgs& a, double* x, long long cycle, long long first, int count,
+                           hipStream_t s) {
+  (void)hipGetLastError();
+  if (a.n == 0) return 0;
+  for (long long k = first; k < first + count; ++k) {
+    hipLaunchKernelGGL(bicg_direction_kernel, dim3(a.nparts), dim3(256), 0, s, a, cycle, k);
+    hipLaunchKernelGGL(product_kernel, dim3(a.nrows), dim3(64 * kWaves), 0, s, a.ptr, a.index, a.values, a.n, a.y,
+                       a.v, a.rhat, a.pv, static_cast<double*>(nullptr));
+    hipLaunchKernelGGL(bicg_half_kernel, dim3(a.nparts), dim3(256), 0, s, a, x, k);
+    hipLaunchKernelGGL(product_kernel, dim3(a.nrows), dim3(64 * kWaves), 0, s, a.ptr, a.index, a.values, a.n, a.z,
+                       a.t, a.s, a.pts, a.pts + a.nrows);
+    hipLaunchKernelGGL(bicg_full_kernel, dim3(a.nparts), dim3(256), 0, s, a, x, k);
+  }
+  hipLaunchKernelGGL(bicg_scalars_kernel, dim3(1), dim3(256), 0, s, a, static_cast<int>((first + count) & 1), 0);
+  return static_cast<int>(hipGetLastError());
+}
+
+int launch_bicg_restart(const CgArgs& a, long long next, hipStream_t s) {
+  (void)hipGetLastError();
+  const int slot = static_cast<int>(next & 1);
+  hipLaunchKernelGGL(clear_break_kernel, dim3(1), dim3(64), 0, s, a.scalars);  // a restart after a breakdown
+  hipLaunchKernelGGL(bicg_restart_kernel, dim3(a.nparts), dim3(256), 0, s, a, slot);
+  hipLaunchKernelGGL(bicg_scalars_kernel, dim3(1), dim3(256), 0, s, a, slot, 0);
   return static_cast<int>(hipGetLastError());
 }
 

@@ -35,6 +35,12 @@ struct CgArgs {
   double* rr;           // nparts: partial sums of r . r
   double* partial;      // 2 nparts (start and residual)
   double* scalars;      // kCgScalars
+  // BiCGStab (a non-symmetric block) besides: r^ (the shadow residual), v =
+  // K y, y = D^-1 p, s, t = K z (z = D^-1 s in z, the product's input), and
+  // partial sums of r^ . v (nrows), t . s and t . t (2 nrows) and of r^ . r
+  // by iteration parity (2 nparts)
+  double *rhat, *v, *y, *s, *t;
+  double *pv, *pts, *rho;
 };
 
 // Product blocks for a block of n rows, and the doubles of work a solve of n
@@ -58,6 +64,21 @@ int launch_cg_iterations(const CgArgs& a, double* x, long long first, int count,
 // p = z, with r . z where iteration `next` reads it, and r . z, r . r in the
 // scalars.
 int launch_cg_restart(const CgArgs& a, long long next, hipStream_t s);
+// BiCGStab, preconditioned with the diagonal, for a non-symmetric block:
+// x = 0, r = r^ = b and the first r^ . r, r . r, b . b (after the diagonal's
+// inverse, as launch_cg_start).
+int launch_bicg_start(const CgArgs& a, const double* b, double* x, hipStream_t s);
+// Iterations [first, first + count) of the cycle that began at `cycle`
+// (whose first iteration takes p = r): five launches each (the direction
+// with y = D^-1 p, v = K y with r^ . v, the half step with z = D^-1 s, t = K z
+// with t . s and t . t, the full step with r^ . r and r . r), every block
+// deriving alpha, omega and beta itself from the partial sums; then r . r
+// into the scalars.
+int launch_bicg_iterations(const CgArgs& a, double* x, long long cycle, long long first, int count,
+                           hipStream_t s);
+// A new cycle from the residual in r (after launch_cg_residual): r^ = r,
+// with r^ . r where iteration `next` reads it; clears a breakdown.
+int launch_bicg_restart(const CgArgs& a, long long next, hipStream_t s);
 // The true residual r = b - K x into r and its r . r into scalars[kCgRr].
 int launch_cg_residual(const CgArgs& a, const double* b, const double* x, hipStream_t s);
 // y = K x over the block's rows (fixed-order sums).

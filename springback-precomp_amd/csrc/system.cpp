@@ -349,8 +349,8 @@ int sparlab_system_free_block_device(sparlab_system* s, const double* values, do
   return SPARLAB_OK;
 }
 
-int sparlab_system_solve_free_block_device(sparlab_system* s, const double* block_values, const double* b,
-                                           double* x, double tolerance, int64_t max_iterations,
+int sparlab_system_solve_free_block_device(sparlab_system* s, const double* block_values, int symmetric,
+                                           const double* b, double* x, double tolerance, int64_t max_iterations,
                                            int64_t* iterations, double* relative_residual, void* stream) {
   if (!s || !iterations || !relative_residual) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
   if (s->part_free < 0) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "no partition");
@@ -373,32 +373,56 @@ int sparlab_system_solve_free_block_device(sparlab_system* s, const double* bloc
   };
   *iterations = 0;
   *relative_residual = 0.0;
-  if (sbk::launch_cg_start(a, b, x, st) != 0 || !read())
-    return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's start failed");
+  const int started = symmetric ? sbk::launch_cg_start(a, b, x, st) : sbk::launch_bicg_start(a, b, x, st);
+  if (started != 0 || !read()) return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's start failed");
   const double bb = sc[sbk::kCgBb];
   const double goal = tolerance * tolerance * bb;
-  int64_t done = 0;
+  int64_t done = 0, cycle = 0;  // cycle: the first iteration since the last (re)start
   constexpr int64_t kRun = 16;  // iterations between two reads of the residual
   // The iteration's residual drifts from b - K x; when it meets the
   // tolerance the true residual is computed, and the iteration restarts from
   // it (residual replacement) unless that meets the tolerance too.
+  // BiCGStab's shadow residual can become orthogonal to the residual in
+  // floating point (r^ . r = 0: a breakdown); the iteration then restarts from
+  // the true residual with r^ = r, at most kRestarts times.
+  constexpr int kRestarts = 16;
+  int restarts = 0;
   bool converged = sc[sbk::kCgRr] <= goal;  // r = b at the start: exact
   while (!converged && sc[sbk::kCgBreak] == 0.0 && done < max_iterations) {
     const int run = static_cast<int>(max_iterations - done < kRun ? max_iterations - done : kRun);
-    if (sbk::launch_cg_iterations(a, x, done, run, st) != 0 || !read())
-      return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's iterations failed");
+    const int ran = symmetric ? sbk::launch_cg_iterations(a, x, done, run, st)
+                              : sbk::launch_bicg_iterations(a, x, cycle, done, run, st);
+    if (ran != 0 || !read()) return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's iterations failed");
     done += run;
-    if (sc[sbk::kCgBreak] == 0.0 && sc[sbk::kCgRr] <= goal) {
+    const bool broke = sc[sbk::kCgBreak] != 0.0;
+    if (broke && !symmetric && restarts < kRestarts) {
+      // every step after the breakdown was zero: x is the last good iterate
       if (sbk::launch_cg_residual(a, b, x, st) != 0 || !read())
         return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's residual failed");
       converged = sc[sbk::kCgRr] <= goal;
-      if (!converged && (sbk::launch_cg_restart(a, done, st) != 0 || !read()))
-        return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's restart failed");
+      if (!converged) {
+        if (sbk::launch_bicg_restart(a, done, st) != 0 || !read())
+          return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's restart failed");
+        cycle = done;
+        ++restarts;
+      }
+      continue;
+    }
+    if (!broke && sc[sbk::kCgRr] <= goal) {
+      if (sbk::launch_cg_residual(a, b, x, st) != 0 || !read())
+        return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's residual failed");
+      converged = sc[sbk::kCgRr] <= goal;
+      if (!converged) {
+        const int restarted = symmetric ? sbk::launch_cg_restart(a, done, st) : sbk::launch_bicg_restart(a, done, st);
+        if (restarted != 0 || !read()) return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's restart failed");
+        cycle = done;
+      }
     }
   }
   *iterations = done;
-  if (sc[sbk::kCgBreak] != 0.0)
-    return sbk::capi_fail(SPARLAB_E_SOLVER, "the free-free block is not positive definite (conjugate gradients)");
+  if (!converged && sc[sbk::kCgBreak] != 0.0)
+    return sbk::capi_fail(SPARLAB_E_SOLVER, symmetric ? "the free-free block is not positive definite (conjugate gradients)"
+                                                      : "the block solve broke down (BiCGStab)");
   if (!converged && (sbk::launch_cg_residual(a, b, x, st) != 0 || !read()))
     return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's residual failed");
   // after a converged run sc holds the true residual's r . r

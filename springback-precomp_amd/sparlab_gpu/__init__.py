@@ -628,22 +628,25 @@ class NonlinearSystem:
                                                   ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(s.cuda_stream)))
         return out[:nnz]
 
-    def solve_free_block_device(self, block_values, b, x=None, tolerance=1e-10, max_iterations=20000, stream=None):
-        """K_ff x = b on the partition's block by Jacobi-preconditioned
-        conjugate gradients (device tensors, num_free each): (x, iterations,
-        relative residual). Raises SolverError when the block is not positive
-        definite or the iteration does not converge."""
+    def solve_free_block_device(self, block_values, b, x=None, tolerance=1e-10, max_iterations=20000, symmetric=True,
+                                stream=None):
+        """K_ff x = b on the partition's block, preconditioned with its
+        diagonal: conjugate gradients (symmetric) or BiCGStab (device tensors,
+        num_free each; a non-symmetric block's values row-major, as K_ff's
+        rows): (x, iterations, relative residual). Raises SolverError when
+        the iteration breaks down or does not converge."""
         import torch
         L = lib()
         L.sparlab_system_solve_free_block_device.argtypes = [
-            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int64,
-            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+            ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]
         if x is None:
             x = torch.empty(max(b.shape[0], 1), dtype=torch.float64, device=b.device)[:b.shape[0]]
         its, rel = ctypes.c_int64(0), ctypes.c_double(0.0)
         s = stream if stream is not None else torch.cuda.current_stream(b.device)
         code = L.sparlab_system_solve_free_block_device(self.h, ctypes.c_void_p(block_values.data_ptr()),
-                                                        ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                                        int(bool(symmetric)), ctypes.c_void_p(b.data_ptr()),
+                                                        ctypes.c_void_p(x.data_ptr()),
                                                         float(tolerance), int(max_iterations), ctypes.byref(its),
                                                         ctypes.byref(rel), ctypes.c_void_p(s.cuda_stream))
         if code == -4:  # SPARLAB_E_SOLVER

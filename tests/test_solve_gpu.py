@@ -91,3 +91,53 @@ def test_block_solve_refuses_what_it_cannot_solve(device):
     # still solvable afterwards
     _, _, rel = sysm.solve_free_block_device(block, b, tolerance=TOL)
     assert rel <= TOL
+
+
+def test_nonsymmetric_block_solve_matches_direct_solve(device):
+    """BiCGStab (symmetric = 0) on a block that is not symmetric: the plate's
+    elastic block plus, on every node's 3 x 3 diagonal block, c (I + S) with S
+    a seeded skew matrix (the shape frictional contact adds: a penalty and a
+    non-symmetric tangential part), read by rows; against scipy's sparse LU
+    solve of the same matrix."""
+    import torch
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import spsolve
+    sysm, block, ptr, index, free = plate_system(device)
+    k = as_scipy(block, ptr, index)
+    rng = np.random.default_rng(21)
+    node = free // 3
+    c = 0.5 * k.diagonal().mean()
+    rows, cols, vals = [], [], []
+    for start in np.nonzero(np.r_[True, node[1:] != node[:-1]])[0]:
+        m = int(np.sum(node[start:start + 3] == node[start]))
+        a = rng.uniform(-1, 1, (m, m))
+        blk = c * (np.eye(m) + (a - a.T))
+        for i in range(m):
+            for j in range(m):
+                rows.append(start + i)
+                cols.append(start + j)
+                vals.append(blk[i, j])
+    data = k.data.copy()  # in place: the block's own entries (explicit zeros kept)
+    for r, col, v in zip(rows, cols, vals):
+        lo, hi = k.indptr[r], k.indptr[r + 1]
+        data[lo + np.searchsorted(k.indices[lo:hi], col)] += v
+    kn = sp.csr_matrix((data, k.indices, k.indptr), shape=k.shape)
+    assert abs(kn - kn.T).max() > 1e-3 * abs(kn).max()
+    d_block = torch.as_tensor(kn.data, device=device)
+    b = rng.normal(size=free.size) * 1e3
+    d_b = torch.as_tensor(b, device=device)
+    x, its, rel = sysm.solve_free_block_device(d_block, d_b, tolerance=TOL, symmetric=False)
+    torch.cuda.synchronize()
+    xs = x.cpu().numpy()
+    true_rel = np.linalg.norm(b - kn @ xs) / np.linalg.norm(b)
+    assert rel <= TOL and true_rel <= 1.5 * TOL
+    ref = spsolve(kn.tocsc(), b)
+    assert np.linalg.norm(xs - ref) <= 1e-7 * np.linalg.norm(ref), (its, rel)
+    x2, its2, rel2 = sysm.solve_free_block_device(d_block, d_b, tolerance=TOL, symmetric=False)
+    torch.cuda.synchronize()
+    assert its2 == its and rel2 == rel and torch.equal(x2, x)
+    # a symmetric block solves the same way by either method
+    xs_sym, _, rel_sym = sysm.solve_free_block_device(block, d_b, tolerance=TOL, symmetric=False)
+    torch.cuda.synchronize()
+    ref_sym = spsolve(as_scipy(block, ptr, index).tocsc(), b)
+    assert rel_sym <= TOL and np.linalg.norm(xs_sym.cpu().numpy() - ref_sym) <= 1e-7 * np.linalg.norm(ref_sym)
