@@ -307,7 +307,10 @@ int sparlab_system_free_block_device(sparlab_system* system, const double* value
  * converge; x then holds the last iterate. The block is read by rows: its
  * row-major values give K_ff, the reference's column-major ones K_ff^T (the
  * same for a symmetric block). Every sum has a fixed order: a solve repeats
- * bit for bit. Synchronizes stream between runs of iterations. */
+ * bit for bit. b and x must not overlap (b is read again for the true
+ * residual). The solve's vectors are the system's (allocated with the
+ * partition): solves on one system must not overlap. Synchronizes stream
+ * between runs of iterations. */
 int sparlab_system_solve_free_block_device(sparlab_system* system, const double* block_values, int symmetric,
                                            const double* b, double* x, double tolerance, int64_t max_iterations,
                                            int64_t* iterations, double* relative_residual, void* stream);

@@ -357,6 +357,7 @@ int sparlab_system_solve_free_block_device(sparlab_system* s, const double* bloc
   const int64_t n = s->part_free;
   if ((s->part_nnz > 0 && !block_values) || (n > 0 && (!b || !x)))
     return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (n > 0 && b < x + n && x < b + n) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "b and x overlap");
   if (!(tolerance > 0.0) || !std::isfinite(tolerance) || max_iterations < 0)
     return sbk::capi_fail(SPARLAB_E_ARGUMENT, "bad tolerance or iteration limit");
   hipStream_t st = static_cast<hipStream_t>(stream);

@@ -10,7 +10,8 @@ within the tolerance asked (1e-12); the solution within 1e-7 of scipy's
 sparse LU solution (the block's condition number times the tolerance bounds
 the difference); two solves bit-identical (fixed-order sums); b = 0 gives
 x = 0 after no iteration; a block that is not positive definite and an
-iteration limit that is too small raise SolverError.
+iteration limit that is too small raise SolverError; b and x that overlap
+are refused.
 """
 import numpy as np
 import pytest
@@ -88,6 +89,8 @@ def test_block_solve_refuses_what_it_cannot_solve(device):
         sysm.solve_free_block_device(-block, b, tolerance=TOL)
     with pytest.raises(G.SolverError, match="did not converge"):
         sysm.solve_free_block_device(block, b, tolerance=TOL, max_iterations=3)
+    with pytest.raises(RuntimeError, match="overlap"):
+        sysm.solve_free_block_device(block, b, x=b, tolerance=TOL)
     # still solvable afterwards
     _, _, rel = sysm.solve_free_block_device(block, b, tolerance=TOL)
     assert rel <= TOL
