@@ -507,6 +507,26 @@ const char* sparlab_last_error(void);
 /* The library version string. */
 const char* sparlab_version(void);
 
+/* The released part's height map before its nearest fill: the reference's
+ * formed_surface (python/precomp/fea/results.py:228-248) interpolates the
+ * deformed tool-side surface with _interpolate_structured (:262-283) and
+ * raycast_top (python/precomp/geometry/raycast.py:27-100). surface (device):
+ * the deformed positions of the surface's structured node grid, (ny_s, nx_s,
+ * 3), row y-major; cut into two triangles per cell ((a, b, c), (a, c, d),
+ * counterclockwise from the cell's lowest corner); cast along vertical rays
+ * through the grid nodes x0 + i h, y0 + j h (i < nx, j < ny). z (device,
+ * ny nx): the highest hit per node, -inf where no triangle is hit; hit
+ * (device, ny nx bytes): 1 where hit, bit for bit the reference's (numpy's
+ * order of operations). The reference then gives each node outside the
+ * surface the height of the nearest hit node (scipy's distance transform);
+ * that fill stays with the caller. A surface with triangles of zero or
+ * negative plan area is refused (SPARLAB_E_ARGUMENT, *folded (host) = their
+ * count), as the reference's PrecompError. Synchronizes stream (it reads the
+ * fold count first). */
+int sparlab_surface_raycast_device(const double* surface, int64_t nx_s, int64_t ny_s, double x0, double y0,
+                                   int64_t nx, int64_t ny, double h, double* z, uint8_t* hit, int64_t* folded,
+                                   void* stream);
+
 #ifdef __cplusplus
 }
 #endif

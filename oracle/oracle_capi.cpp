@@ -10,6 +10,7 @@
 // Status codes: 0 ok, 1 SolverError, 2 MeshError, 3 ConfigError, 4 other.
 #include "assembly.hpp"
 #include "element.hpp"
+#include "surface.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -394,6 +395,15 @@ long long oracle_free_block(int nn, int ne, const int* conn, const double* value
   std::copy(b.position.begin(), b.position.end(), position);
   free_block_values(b, values, block_values);
   return nnz;
+}
+
+// _interpolate_structured + raycast_top (results.py:262-283, raycast.py:27-100)
+// without the nearest fill: surface (ny_s, nx_s, 3) -> z, hit (ny, nx);
+// returns the folded triangles' count (nothing cast when positive).
+long long oracle_surface_raycast(const double* surface, long long nxs, long long nys, double x0, double y0,
+                                 long long nx, long long ny, double h, double* z, unsigned char* hit) {
+  const HeightGrid g{x0, y0, h, nx, ny};
+  return surface_raycast(surface, nxs, nys, g, z, hit);
 }
 
 // NonlinearSystem.hpp:240-262; returns the gross norm.

@@ -11,6 +11,7 @@
 
 #include "contact_kernels.hpp"
 #include "element_kernels.hpp"
+#include "surface_kernels.hpp"
 
 namespace {
 
@@ -598,3 +599,32 @@ int sparlab_add_node_blocks_device(const double* blocks, const int32_t* active, 
 }
 
 }  // extern "C"
+
+int sparlab_surface_raycast_device(const double* surface, int64_t nx_s, int64_t ny_s, double x0, double y0,
+                                   int64_t nx, int64_t ny, double h, double* z, uint8_t* hit, int64_t* folded,
+                                   void* stream) {
+  if (!surface || !z || !hit || !folded) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (nx_s < 2 || ny_s < 2 || nx < 1 || ny < 1) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "bad surface or grid size");
+  if (!(h > 0.0) || !std::isfinite(h) || !std::isfinite(x0) || !std::isfinite(y0))
+    return sbk::capi_fail(SPARLAB_E_ARGUMENT, "the grid spacing must be finite and > 0, its origin finite");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  unsigned long long* d_folded = nullptr;
+  if (hipMalloc(reinterpret_cast<void**>(&d_folded), sizeof(unsigned long long)) != hipSuccess)
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "device allocation failed");
+  unsigned long long count = 0;
+  const bool ok = hipMemsetAsync(d_folded, 0, sizeof(count), st) == hipSuccess &&
+                  sbk::launch_surface_folds(surface, nx_s, ny_s, d_folded, st) == 0 &&
+                  hipMemcpyAsync(&count, d_folded, sizeof(count), hipMemcpyDeviceToHost, st) == hipSuccess &&
+                  hipStreamSynchronize(st) == hipSuccess;
+  (void)hipFree(d_folded);
+  if (!ok) return sbk::capi_fail(SPARLAB_E_DEVICE, "the surface's fold check failed");
+  *folded = static_cast<int64_t>(count);
+  if (count > 0)
+    return sbk::capi_fail(SPARLAB_E_ARGUMENT, std::to_string(count) +
+                                                  " surface triangles fold over in plan view; the deformed surface is "
+                                                  "not a height field");
+  const sbk::HeightGrid g{x0, y0, h, nx, ny};
+  if (sbk::launch_surface_raycast(surface, nx_s, ny_s, g, z, hit, st) != 0)
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "the surface ray-cast failed");
+  return SPARLAB_OK;
+}

@@ -975,3 +975,30 @@ def add_node_blocks_device(blocks, active, first_value, row_len, values, column_
                                             ctypes.c_void_p(first_value.data_ptr()),
                                             ctypes.c_void_p(row_len.data_ptr()), active.shape[0], int(column_major),
                                             ctypes.c_void_p(values.data_ptr()), ctypes.c_void_p(s.cuda_stream)))
+
+
+def surface_raycast_device(surface, grid, stream=None):
+    """The released part's height map before the nearest fill (the
+    reference's formed_surface without _fill_nearest): surface, a float64
+    device tensor (ny_s, nx_s, 3) of the deformed tool-side surface's
+    structured nodes; grid = (x0, y0, nx, ny, h). Returns (z, hit) device
+    tensors (ny, nx): the highest hit per node (-inf where none) and the hit
+    flags. Raises RuntimeError for a surface that folds over in plan view."""
+    import torch
+    L = lib()
+    L.sparlab_surface_raycast_device.argtypes = [
+        ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_int64,
+        ctypes.c_int64, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+        ctypes.c_void_p]
+    if surface.dtype != torch.float64 or surface.dim() != 3 or surface.shape[2] != 3 or not surface.is_contiguous():
+        raise ValueError("surface must be a contiguous float64 (ny_s, nx_s, 3) tensor")
+    x0, y0, nx, ny, h = grid
+    z = torch.empty((int(ny), int(nx)), dtype=torch.float64, device=surface.device)
+    hit = torch.empty((int(ny), int(nx)), dtype=torch.uint8, device=surface.device)
+    folded = ctypes.c_int64(0)
+    s = stream if stream is not None else torch.cuda.current_stream(surface.device)
+    _check(L.sparlab_surface_raycast_device(ctypes.c_void_p(surface.data_ptr()), surface.shape[1], surface.shape[0],
+                                            float(x0), float(y0), int(nx), int(ny), float(h),
+                                            ctypes.c_void_p(z.data_ptr()), ctypes.c_void_p(hit.data_ptr()),
+                                            ctypes.byref(folded), ctypes.c_void_p(s.cuda_stream)))
+    return z, hit.bool()

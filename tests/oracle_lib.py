@@ -314,6 +314,26 @@ def free_block(nn, conn, values, free):
     return outer, inner[:nnz].copy(), out[:nnz].copy(), position
 
 
+def surface_raycast(surface, grid):
+    """The height map of a structured surface (ny_s, nx_s, 3) on grid (x0,
+    y0, nx, ny, h), before the nearest fill: (z (ny, nx), -inf where nothing
+    is hit; hit; the count of folded triangles)."""
+    surface = np.ascontiguousarray(surface, dtype=np.float64)
+    nys, nxs = surface.shape[:2]
+    x0, y0, nx, ny, h = grid
+    z = np.zeros((ny, nx))
+    hit = np.zeros((ny, nx), np.uint8)
+    L = lib()
+    L.oracle_surface_raycast.restype = ctypes.c_longlong
+    L.oracle_surface_raycast.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_longlong, ctypes.c_longlong,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_longlong, ctypes.c_longlong,
+                                         ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_ubyte)]
+    folded = L.oracle_surface_raycast(dp(surface), nxs, nys, x0, y0, nx, ny, h, dp(z),
+                                      hit.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
+    return z, hit.astype(bool), int(folded)
+
+
 def tool_record(shape=1, centre=(0, 0, 0), centre0=None, radius=0.0, normal=(0, 0, 1), axis=(0, 0, 1),
                 friction=0.0, tangential_ratio=1.0):
     centre0 = centre if centre0 is None else centre0
