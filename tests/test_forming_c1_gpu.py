@@ -22,8 +22,11 @@ element loop, scatter, force gather, contact and add_node_block. Bars: the
 residual norms of every iteration agree to 1e-6 relative while they are above
 1e-7 of the increment's first residual (below it both sides are at round-off),
 the iteration counts agree, and the top surface's heights after the release
-agree within 1e-6 mm. (The reference halves an increment whose evaluation
-fails; this driver's steps are small enough that none does.)
+agree within 1e-6 mm, and so do the released part's height maps on the
+reference top-node grid (the device's ray-cast of the device side's surface
+against the oracle's of the oracle side's, same hit nodes). (The reference
+halves an increment whose evaluation fails; this driver's steps are small
+enough that none does.)
 """
 import numpy as np
 import pytest
@@ -217,6 +220,25 @@ def newton(side, u0, free, outer, inner, centre, centre0, tool_on=True, max_it=2
     return u, norms, steps
 
 
+def height_maps(coords, top, u_d, u_o):
+    """(z, hit) of the deformed top surface on the reference top-node grid,
+    from the device (u_d) and from the oracle (u_o)."""
+    import torch
+    xs, ys = np.unique(coords[top, 0]), np.unique(coords[top, 1])
+    order = np.lexsort((coords[top, 0], coords[top, 1]))  # y-major, then x
+    grid = (xs[0], ys[0], xs.size, ys.size, xs[1] - xs[0])
+
+    def surface(u):
+        cur = coords[top] + u.reshape(-1, 3)[top]
+        return np.ascontiguousarray(cur[order].reshape(ys.size, xs.size, 3))
+
+    z_d, hit_d = G.surface_raycast_device(torch.as_tensor(surface(u_d), device="cuda"), grid)
+    torch.cuda.synchronize()
+    z_o, hit_o, folded = O.surface_raycast(surface(u_o), grid)
+    assert folded == 0
+    return z_d.cpu().numpy(), hit_d.cpu().numpy(), z_o, hit_o
+
+
 def test_c1_newton_histories_and_release_heights():
     coords, conn, m, top, free = c1_model()
     nn = coords.shape[0]
@@ -255,6 +277,12 @@ def test_c1_newton_histories_and_release_heights():
     heights_d = coords[top, 2] + u_d[3 * top + 2]
     heights_o = coords[top, 2] + u_o[3 * top + 2]
     assert np.abs(heights_d - heights_o).max() <= 1e-9  # 1e-6 mm
+    # the released part's height maps (formed_surface before its nearest
+    # fill) on the reference top-node grid: the device side's by the device
+    # ray-cast, the oracle side's by the oracle's
+    z_d, hit_d, z_o, hit_o = height_maps(coords, top, u_d, u_o)
+    assert np.array_equal(hit_d, hit_o) and hit_o.mean() > 0.9
+    assert np.abs(z_d[hit_d] - z_o[hit_o]).max() <= 1e-9  # 1e-6 mm
 
 
 def test_c1_with_the_device_block_solve():
