@@ -184,6 +184,22 @@ int sparlab_elements_evaluate_stage_device(const sparlab_material* material, con
                                            int kinematics, int want_tangent,
                                            const sparlab_element_batch* batch, int stages, void* stream);
 
+/* sparlab_elements_evaluate_device without a workspace and without any
+ * device allocation. No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). batch->workspace is ignored; the
+ * workspace is a file-scope device array of SPARLAB_NOALLOC_SCRATCH_BYTES,
+ * one per GPU, held until the process ends. A batch that needs more
+ * (sparlab_element_workspace_doubles x 8 bytes: about 21 000 elements of
+ * 8 points) is refused with SPARLAB_E_ARGUMENT before any device call. Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (other streams, other host threads) race. The outputs equal those of
+ * sparlab_elements_evaluate_device bit for bit (the same kernels). */
+#define SPARLAB_NOALLOC_SCRATCH_BYTES (128ll << 20)
+int sparlab_elements_evaluate_noalloc_device(const sparlab_material* material, const sparlab_rule* rule,
+                                             int kinematics, int want_tangent,
+                                             const sparlab_element_batch* batch, void* stream);
+
 /* ------------------------------------------------------------------------
  * The nonlinear system: the drop-in for detail::NonlinearSystem
  * (src/fem/NonlinearSystem.hpp:85-334) with the Assembler's cached pattern

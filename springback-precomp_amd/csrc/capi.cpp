@@ -456,9 +456,16 @@ int64_t sparlab_element_workspace_doubles(int64_t num_elements, const sparlab_ru
   return static_cast<int64_t>(sbk::workspace_doubles(num_elements, r.n));
 }
 
-int sparlab_elements_evaluate_stage_device(const sparlab_material* material, const sparlab_rule* rule,
-                                           int kinematics, int want_tangent, const sparlab_element_batch* b,
-                                           int stages, void* stream) {
+}  // extern "C"
+
+namespace sbk {
+
+// The checks of an element batch that need no device: null pointers, sizes,
+// kinematics, stages, the rule and the material. Fills a's rule and material
+// and returns SPARLAB_OK, or an error code with the message set. The batch's
+// workspace pointer is not looked at (the caller supplies the workspace).
+int capi_check_element_batch(const sparlab_material* material, const sparlab_rule* rule, int kinematics,
+                             int want_tangent, const sparlab_element_batch* b, int stages, ElementArgs& a) {
   if (material == nullptr || rule == nullptr || b == nullptr) return fail(SPARLAB_E_ARGUMENT, "null argument");
   if (kinematics < 0 || kinematics > 2) return fail(SPARLAB_E_ARGUMENT, "kinematics must be 0, 1 or 2");
   if ((stages & ~(3 | SPARLAB_STAGE_INDEX_ORDER)) != 0 || (stages & 3) == 0)
@@ -467,13 +474,18 @@ int sparlab_elements_evaluate_stage_device(const sparlab_material* material, con
   if (b->num_elements == 0) return SPARLAB_OK;
   if (b->coords == nullptr || b->conn == nullptr || b->u == nullptr || b->state_in == nullptr ||
       b->state_out == nullptr || b->alpha_out == nullptr || b->force == nullptr || b->energy == nullptr ||
-      b->yielding == nullptr || b->iterations == nullptr || b->status == nullptr || b->fail_value == nullptr ||
-      b->workspace == nullptr)
+      b->yielding == nullptr || b->iterations == nullptr || b->status == nullptr || b->fail_value == nullptr)
     return fail(SPARLAB_E_ARGUMENT, "a required buffer of the element batch is null");
   if (want_tangent && b->tangent == nullptr) return fail(SPARLAB_E_ARGUMENT, "want_tangent needs a tangent buffer");
-  sbk::ElementArgs a{};
-  const int prep = sbk::capi_prepare_elements(material, rule, kinematics, a);
-  if (prep != SPARLAB_OK) return prep;
+  return capi_prepare_elements(material, rule, kinematics, a);
+}
+
+// Queues the stages of an element batch already checked by
+// capi_check_element_batch (a holds its rule and material) with `workspace`
+// (workspace_doubles(num_elements, points) doubles of device memory).
+int capi_launch_element_batch(int want_tangent, const sparlab_element_batch* b, double* workspace, int stages,
+                              void* stream, ElementArgs& a) {
+  if (b->num_elements == 0) return SPARLAB_OK;
   a.coords = b->coords;
   a.conn = b->conn;
   a.u = b->u;
@@ -490,23 +502,38 @@ int sparlab_elements_evaluate_stage_device(const sparlab_material* material, con
   a.iterations = b->iterations;
   a.status = b->status;
   a.fail_value = b->fail_value;
-  sbk::set_workspace(a, b->workspace);
+  set_workspace(a, workspace);
   if (stages & SPARLAB_STAGE_INDEX_ORDER) a.ordered = 0;
   // The local solve writes every element's status; the condensation reads it.
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (stages & 1) {
-    const int err = sbk::launch_local_solve(a, s);
+    const int err = launch_local_solve(a, s);
     if (err != 0)
       return fail(SPARLAB_E_DEVICE, std::string("the local-solve kernel did not launch: ") +
                                         hipGetErrorString(static_cast<hipError_t>(err)));
   }
   if (stages & 2) {
-    const int err = sbk::launch_condense(a, want_tangent != 0, s);
+    const int err = launch_condense(a, want_tangent != 0, s);
     if (err != 0)
       return fail(SPARLAB_E_DEVICE, std::string("the condensation kernel did not launch: ") +
                                         hipGetErrorString(static_cast<hipError_t>(err)));
   }
   return SPARLAB_OK;
+}
+
+}  // namespace sbk
+
+extern "C" {
+
+int sparlab_elements_evaluate_stage_device(const sparlab_material* material, const sparlab_rule* rule,
+                                           int kinematics, int want_tangent, const sparlab_element_batch* b,
+                                           int stages, void* stream) {
+  sbk::ElementArgs a{};
+  const int st = sbk::capi_check_element_batch(material, rule, kinematics, want_tangent, b, stages, a);
+  if (st != SPARLAB_OK) return st;
+  if (b->num_elements > 0 && b->workspace == nullptr)
+    return fail(SPARLAB_E_ARGUMENT, "a required buffer of the element batch is null");
+  return sbk::capi_launch_element_batch(want_tangent, b, b->workspace, stages, stream, a);
 }
 
 int sparlab_elements_evaluate_device(const sparlab_material* material, const sparlab_rule* rule, int kinematics,

@@ -120,6 +120,9 @@ def lib():
         _lib.sparlab_elements_evaluate_stage_device.argtypes = [
             ctypes.POINTER(CMaterial), ctypes.POINTER(CRule), ctypes.c_int, ctypes.c_int,
             ctypes.POINTER(CElementBatch), ctypes.c_int, ctypes.c_void_p]
+        _lib.sparlab_elements_evaluate_noalloc_device.argtypes = [
+            ctypes.POINTER(CMaterial), ctypes.POINTER(CRule), ctypes.c_int, ctypes.c_int,
+            ctypes.POINTER(CElementBatch), ctypes.c_void_p]
     return _lib
 
 
@@ -326,6 +329,25 @@ def evaluate_elements(material, rule, batch, kinematics=FINITE_LOGARITHMIC, want
     _check(lib().sparlab_elements_evaluate_stage_device(ctypes.byref(material.c), ctypes.byref(rule.c), kinematics,
                                                         int(want_tangent), ctypes.byref(batch.c), int(stages),
                                                         ctypes.c_void_p(s.cuda_stream)))
+    return batch
+
+
+NOALLOC_SCRATCH_BYTES = 128 << 20  # SPARLAB_NOALLOC_SCRATCH_BYTES
+
+
+def evaluate_elements_noalloc(material, rule, batch, kinematics=FINITE_LOGARITHMIC, want_tangent=True, stream=None):
+    """Queues the evaluation of every element of `batch` through
+    sparlab_elements_evaluate_noalloc_device: no workspace (the batch's own is
+    not used) and no device allocation; the library's per-device scratch array
+    holds batches of up to NOALLOC_SCRATCH_BYTES of workspace. Calls that
+    overlap on one GPU race."""
+    import torch
+    if want_tangent and batch.tangent is None:
+        raise ValueError("the batch was allocated without a tangent buffer")
+    s = stream if stream is not None else torch.cuda.current_stream(batch.coords.device)
+    _check(lib().sparlab_elements_evaluate_noalloc_device(ctypes.byref(material.c), ctypes.byref(rule.c), kinematics,
+                                                          int(want_tangent), ctypes.byref(batch.c),
+                                                          ctypes.c_void_p(s.cuda_stream)))
     return batch
 
 
