@@ -524,7 +524,7 @@ def main():
                                         f"{one['seconds']:.1f} s wall"}
         fl, fc = cb["flops_local"], cb["flops_condense"]
         achieved = fl * ne / (ms_local * 1e-3) / 1e12
-        roof = {"bound": "mfma", "pipe": "fp64 (vector ALU; MI355X FP64 vector peak = FP64 matrix peak)",
+        roof = {"bound": "fp64", "pipe": "FP64 vector ALU (no MFMA instructions; MI355X FP64 vector peak = FP64 matrix peak)",
                 "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic("local_solve_kernel", ne),
                 "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated; "

@@ -197,15 +197,19 @@ __global__ __launch_bounds__(256) void direction_kernel(CgArgs A, long long k) {
   for (long long i = i0 + threadIdx.x; i < i1; i += 256) A.p[i] = fma(beta, A.p[i], A.z[i]);
 }
 
-// dinv[i] = 1 / K_ii; a missing, non-positive or non-finite diagonal sets
-// the break flag (the block is not positive definite).
-__global__ __launch_bounds__(256) void diagonal_kernel(CgArgs A) {
+// Conjugate gradients (positive = 1): dinv[i] = 1 / K_ii, and a missing,
+// non-positive or non-finite diagonal sets the break flag (the block is not
+// positive definite). BiCGStab (positive = 0), whose block need not be
+// definite: dinv[i] = 1 / |K_ii|, or 1 where K_ii is zero or not finite.
+__global__ __launch_bounds__(256) void diagonal_kernel(CgArgs A, int positive) {
   const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
   if (i >= A.n) return;
   double d = 0.0;
   for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k)
     if (A.index[k] == i) d = A.values[k];
-  if (!(d > 0.0) || !std::isfinite(d)) {
+  if (!positive) {
+    A.dinv[i] = (d != 0.0 && std::isfinite(d)) ? 1.0 / fabs(d) : 1.0;
+  } else if (!(d > 0.0) || !std::isfinite(d)) {
     A.scalars[kCgBreak] = 1.0;
     A.dinv[i] = 0.0;
   } else {
@@ -524,7 +528,7 @@ int launch_block_spmv(const int* ptr, const int* index, const double* values, lo
 int launch_cg_start(const CgArgs& a, const double* b, double* x, hipStream_t s) {
   (void)hipGetLastError();
   hipLaunchKernelGGL(clear_break_kernel, dim3(1), dim3(64), 0, s, a.scalars);
-  if (a.n > 0) hipLaunchKernelGGL(diagonal_kernel, dim3(grid_of(a.n, 256)), dim3(256), 0, s, a);
+  if (a.n > 0) hipLaunchKernelGGL(diagonal_kernel, dim3(grid_of(a.n, 256)), dim3(256), 0, s, a, 1);
   hipLaunchKernelGGL(start_kernel, dim3(a.nparts), dim3(256), 0, s, a, b, x);
   hipLaunchKernelGGL(start_scalars_kernel, dim3(1), dim3(256), 0, s, a);
   return static_cast<int>(hipGetLastError());
@@ -545,7 +549,7 @@ int launch_cg_iterations(const CgArgs& a, double* x, long long first, int count,
 int launch_bicg_start(const CgArgs& a, const double* b, double* x, hipStream_t s) {
   (void)hipGetLastError();
   hipLaunchKernelGGL(clear_break_kernel, dim3(1), dim3(64), 0, s, a.scalars);
-  if (a.n > 0) hipLaunchKernelGGL(diagonal_kernel, dim3(grid_of(a.n, 256)), dim3(256), 0, s, a);
+  if (a.n > 0) hipLaunchKernelGGL(diagonal_kernel, dim3(grid_of(a.n, 256)), dim3(256), 0, s, a, 0);
   hipLaunchKernelGGL(bicg_start_kernel, dim3(a.nparts), dim3(256), 0, s, a, b, x);
   hipLaunchKernelGGL(bicg_scalars_kernel, dim3(1), dim3(256), 0, s, a, 0, 1);
   return static_cast<int>(hipGetLastError());

@@ -396,6 +396,15 @@ int sparlab_system_solve_free_block_device(sparlab_system* s, const double* bloc
     if (ran != 0 || !read()) return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's iterations failed");
     done += run;
     const bool broke = sc[sbk::kCgBreak] != 0.0;
+    if (broke && symmetric) {
+      // p . K p <= 0 (round-off near the solution, or an indefinite block):
+      // every step after the break was zero, so x is the last good iterate;
+      // it is the solution if its true residual meets the tolerance
+      if (sbk::launch_cg_residual(a, b, x, st) != 0 || !read())
+        return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's residual failed");
+      converged = sc[sbk::kCgRr] <= goal;
+      break;
+    }
     if (broke && !symmetric && restarts < kRestarts) {
       // every step after the breakdown was zero: x is the last good iterate
       if (sbk::launch_cg_residual(a, b, x, st) != 0 || !read())

@@ -264,6 +264,24 @@ def _ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
 
+def _require(t, name, length, dtype="float64", device=None, at_least=False):
+    """Refuses (ValueError) a tensor the C ABI would read or write out of
+    bounds: the wrong dtype, not contiguous, on another device, or fewer
+    (or, unless at_least, other than) `length` elements. The library's
+    device entry points take no sizes for these buffers."""
+    import torch
+    want = getattr(torch, dtype)
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor")
+    if t.dtype != want or not t.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous {dtype} tensor")
+    if t.device.type != "cuda" or (device is not None and t.device != device):
+        raise ValueError(f"{name} must be on the device {device if device is not None else 'cuda'}")
+    n = t.numel()
+    if (n < length) if at_least else (n != length):
+        raise ValueError(f"{name} holds {n} entries; {'at least ' if at_least else ''}{length} are needed")
+
+
 class ElementBatch:
     """Device buffers of one element batch (torch tensors on a HIP device).
 
@@ -592,6 +610,11 @@ class NonlinearSystem:
         L.sparlab_system_stiffness_gross_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        _require(values, "values", self.nnz)
+        _require(x, "x", 3 * self.num_nodes, device=values.device)
+        _require(free, "free", free.numel() if isinstance(free, torch.Tensor) else 0, "int32", values.device)
+        if product is not None:
+            _require(product, "product", 3 * self.num_nodes, device=values.device)
         out = torch.empty(1, dtype=torch.float64, device=x.device)
         s = stream if stream is not None else torch.cuda.current_stream(x.device)
         p = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
@@ -609,8 +632,7 @@ class NonlinearSystem:
         L = lib()
         L.sparlab_system_partition_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                                       ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p]
-        if free.dtype != torch.int32:
-            raise ValueError("free DOFs must be int32")
+        _require(free, "free", free.numel() if isinstance(free, torch.Tensor) else 0, "int32")
         nnz = ctypes.c_int64(0)
         s = stream if stream is not None else torch.cuda.current_stream(free.device)
         self._block = (-1, 0)  # a refused list leaves no partition
@@ -643,8 +665,11 @@ class NonlinearSystem:
         L = lib()
         L.sparlab_system_free_block_device.argtypes = [ctypes.c_void_p] * 4
         nf, nnz = getattr(self, "_block", (-1, 0))  # without a partition the library refuses
+        _require(values, "values", self.nnz)
         if out is None:
             out = torch.empty(max(nnz, 1), dtype=torch.float64, device=values.device)
+        else:
+            _require(out, "out", nnz, device=values.device, at_least=True)
         s = stream if stream is not None else torch.cuda.current_stream(values.device)
         _check(L.sparlab_system_free_block_device(self.h, ctypes.c_void_p(values.data_ptr()),
                                                   ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(s.cuda_stream)))
@@ -662,6 +687,12 @@ class NonlinearSystem:
         L.sparlab_system_solve_free_block_device.argtypes = [
             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
             ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]
+        nf, nnz = getattr(self, "_block", (-1, 0))  # without a partition the library refuses
+        if nf >= 0:
+            _require(block_values, "block_values", nnz, at_least=True)
+            _require(b, "b", nf, device=block_values.device)
+            if x is not None:
+                _require(x, "x", nf, device=block_values.device)
         if x is None:
             x = torch.empty(max(b.shape[0], 1), dtype=torch.float64, device=b.device)[:b.shape[0]]
         its, rel = ctypes.c_int64(0), ctypes.c_double(0.0)

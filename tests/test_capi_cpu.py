@@ -229,3 +229,28 @@ def test_oracle_free_block_is_the_submatrix(which):
     assert np.array_equal(np.sort(position[both]), np.arange(b_inner.size))
     if which == "all":
         assert np.array_equal(b_values, values)
+
+
+def test_device_wrappers_refuse_tensors_the_library_would_overrun():
+    """The wrappers of device entry points that take no sizes check dtype,
+    contiguity, device and length before calling the library (a wrong tensor
+    would otherwise be read or written out of bounds on the GPU)."""
+    import types
+    import torch
+    fake = types.SimpleNamespace(nnz=10, num_nodes=4, _block=(5, 7), h=None)
+    cpu = torch.zeros(10, dtype=torch.float64)
+    with pytest.raises(ValueError, match="device"):
+        G.NonlinearSystem.free_block_device(fake, cpu)
+    with pytest.raises(ValueError, match="float64"):
+        G.NonlinearSystem.free_block_device(fake, torch.zeros(10, dtype=torch.float32))
+    with pytest.raises(ValueError, match="contiguous"):
+        G.NonlinearSystem.free_block_device(fake, torch.zeros(20, dtype=torch.float64)[::2])
+    with pytest.raises(ValueError):
+        G.NonlinearSystem.solve_free_block_device(fake, torch.zeros(7, dtype=torch.float64), cpu[:5])
+    with pytest.raises(ValueError):
+        G.NonlinearSystem.stiffness_gross_device(fake, torch.zeros(9, dtype=torch.float64), cpu[:12],
+                                                 torch.zeros(3, dtype=torch.int32))
+    with pytest.raises(ValueError, match="int32"):
+        G.NonlinearSystem.partition_device(fake, torch.zeros(3, dtype=torch.int64))
+    with pytest.raises(ValueError, match="device|needed"):
+        G._require(torch.zeros(3, dtype=torch.float64), "t", 4, device=None)

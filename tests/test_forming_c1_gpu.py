@@ -117,6 +117,7 @@ class DeviceSide:
         n = outer.size - 1
         if not hasattr(self, "solves"):
             self.solves = []
+        checked = []  # the first symmetric solve of this solver checks the block's symmetry
 
         def solve(values, r):
             if not symmetric:  # the same entries in row order (the pattern is symmetric)
@@ -124,7 +125,8 @@ class DeviceSide:
                 rows.sort_indices()
                 values = rows.data
             block = self.sys.free_block_device(self.t(values))
-            if symmetric and not self.solves:
+            if symmetric and not checked:
+                checked.append(True)
                 ptr, index = self.sys.partition_pattern_device(self.dev)
                 k = sp.csr_matrix((block.cpu().numpy(), index.cpu().numpy(), ptr.cpu().numpy()),
                                   shape=(free_idx.size, free_idx.size))

@@ -144,3 +144,8 @@ def test_nonsymmetric_block_solve_matches_direct_solve(device):
     torch.cuda.synchronize()
     ref_sym = spsolve(as_scipy(block, ptr, index).tocsc(), b)
     assert rel_sym <= TOL and np.linalg.norm(xs_sym.cpu().numpy() - ref_sym) <= 1e-7 * np.linalg.norm(ref_sym)
+    # negative diagonal entries (the non-symmetric block negated, which the
+    # reference's LU solves): BiCGStab preconditions with 1 / |K_ii|
+    xn, _, rel_n = sysm.solve_free_block_device(-d_block, d_b, tolerance=TOL, symmetric=False)
+    torch.cuda.synchronize()
+    assert rel_n <= TOL and np.linalg.norm(xn.cpu().numpy() + ref) <= 1e-7 * np.linalg.norm(ref)
