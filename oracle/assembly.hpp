@@ -345,7 +345,7 @@ inline ContactOut contact_evaluate(const Tool& tool, int friction_symmetric, int
     const double g = gap(tool, x, nrm, dn);
     if (!(g < 0.0)) continue;
     const double ka = kappa_a[i];
-    const double pn = -ka * g;
+    const double pn = g_plant == kPlantNormalForce ? -ka * g * (1.0 + 1.0e-6) : -ka * g;
     double force[3] = {pn * nrm[0], pn * nrm[1], pn * nrm[2]};
     double k_node[3][3];
     for (int a = 0; a < 3; ++a)
@@ -399,7 +399,7 @@ inline ContactOut contact_evaluate(const Tool& tool, int friction_symmetric, int
       for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) d_trial[a][b] = d_hat[a][b] - kt * d_delta[a][b];
       const double tn = std::sqrt(trial[0] * trial[0] + trial[1] * trial[1] + trial[2] * trial[2]);
-      const double limit = mu * pn;
+      const double limit = g_plant == kPlantSlipRadius ? mu * ka : mu * pn;
       double ft[3], d_ft[3][3], k_sym[3][3];
       if (tn <= limit) {
         for (int a = 0; a < 3; ++a) ft[a] = trial[a];

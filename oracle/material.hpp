@@ -53,6 +53,7 @@ struct MaterialData {
 
 // Material.cpp:73-90
 template <class T> T yield(const MaterialData& m, T a) {
+  if (g_plant == kPlantVoceYield) return T(m.sy0) + T(m.H) * a + T(m.Q) * -expm1_(T(-1.01 * m.delta) * a);
   return T(m.sy0) + T(m.H) * a + T(m.Q) * -expm1_(T(-m.delta) * a);
 }
 template <class T> T yield_slope(const MaterialData& m, T a) {
@@ -61,6 +62,7 @@ template <class T> T yield_slope(const MaterialData& m, T a) {
 template <class T> T isotropic_energy(const MaterialData& m, T a) {
   T out = T(0.5) * T(m.H) * a * a;
   if (m.Q != 0.0) out = out + T(m.Q) * (a + expm1_(T(-m.delta) * a) / T(m.delta));
+  if (g_plant == kPlantVoceEnergySign && m.Q != 0.0) out = out - T(2.0 * m.Q) * (a + expm1_(T(-m.delta) * a) / T(m.delta));
   return out;
 }
 
@@ -342,6 +344,7 @@ template <class T> Decay<T> decay(double gamma, T dl, int integration) {
     d.chi = d.theta * dl;
     d.dtheta = T(-gamma) * d.theta * d.theta;
     d.dchi = d.theta * d.theta;
+    if (g_plant == kPlantBackwardEulerChi) { d.chi = dl; d.dchi = T(1.0); }
   }
   return d;
 }
@@ -456,7 +459,7 @@ T kinematic_energy(const GeneralReturn<T>& r, const PlasticState<T>& st) {
   T out = T(0.0);
   for (int i = 0; i < r.count; ++i) {
     const T nrm = tensor_norm(st.alpha[i]);
-    out = out + T(0.75) * nrm * nrm / T(r.mod[i]);
+    out = out + T(g_plant == kPlantKinematicEnergy ? 0.5 : 0.75) * nrm * nrm / T(r.mod[i]);
   }
   return out;
 }

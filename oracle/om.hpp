@@ -36,6 +36,22 @@ struct ModelError : std::runtime_error {
 };
 
 // ---------------------------------------------------------------------------
+// Planted mistakes (tests only): a test sets one through oracle_set_plant to
+// show that its known answer fails when the oracle is wrong in that way, and
+// sets 0 again. 0, the default, is the reference's algorithm.
+// ---------------------------------------------------------------------------
+enum Plant : int {
+  kPlantNone = 0,
+  kPlantVoceEnergySign = 1,   // isotropic_energy: the Voce term subtracted
+  kPlantKinematicEnergy = 2,  // kinematic_energy: 1/2 instead of 3/4
+  kPlantBackwardEulerChi = 3, // backward Euler: chi = dl (and dchi = 1), theta kept
+  kPlantSlipRadius = 4,       // Coulomb slip limit mu kappa A instead of mu p_N
+  kPlantNormalForce = 5,      // contact normal force (1 + 1e-6) kappa A |g|
+  kPlantVoceYield = 6,        // Voce saturation rate 1 % high in the yield stress
+};
+inline int g_plant = kPlantNone;
+
+// ---------------------------------------------------------------------------
 // The op-counting scalar
 // ---------------------------------------------------------------------------
 inline std::uint64_t& flop_counter() {

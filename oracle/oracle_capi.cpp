@@ -125,6 +125,18 @@ extern "C" {
 
 int oracle_state_record() { return kStateRecord; }
 
+// Tests only: plants one of the mistakes of the Plant list (0: none).
+void oracle_set_plant(int which) { g_plant = which; }
+
+// The hardening laws at a (Material.cpp:73-109): out = yield stress, its
+// slope and the isotropic hardening energy.
+void oracle_hardening(const double* mat, double a, double* out) {
+  const MaterialData m = read_material(mat);
+  out[0] = yield(m, a);
+  out[1] = yield_slope(m, a);
+  out[2] = isotropic_energy(m, a);
+}
+
 int oracle_rule_points(int in_plane, int thickness, int axis) {
   try {
     return make_rule(in_plane, thickness, axis).n;

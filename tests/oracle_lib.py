@@ -34,6 +34,38 @@ def lib():
     return _lib
 
 
+# Planted mistakes (oracle/om.hpp, enum Plant): a known-answer test runs its
+# check once more with one planted, to show that the check catches it.
+PLANT_VOCE_ENERGY_SIGN = 1
+PLANT_KINEMATIC_ENERGY = 2
+PLANT_BACKWARD_EULER_CHI = 3
+PLANT_SLIP_RADIUS = 4
+PLANT_NORMAL_FORCE = 5
+PLANT_VOCE_YIELD = 6
+
+
+class planted:
+    """with planted(PLANT_...): the oracle computes with that mistake."""
+
+    def __init__(self, which):
+        self.which = which
+
+    def __enter__(self):
+        lib().oracle_set_plant(int(self.which))
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_set_plant(0)
+        return False
+
+
+def hardening(mat, a):
+    """(yield stress, its slope, isotropic hardening energy) at abar = a."""
+    out = np.zeros(3)
+    lib().oracle_hardening(dp(np.ascontiguousarray(mat, dtype=np.float64)), ctypes.c_double(a), dp(out))
+    return out
+
+
 def dp(a):
     if a is None:
         return None
