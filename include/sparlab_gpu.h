@@ -155,7 +155,19 @@ typedef struct sparlab_element_batch {
   int32_t* status;
   double* fail_value;
   double* workspace;
+  int32_t* local_path;  /* optional (may be null): per element, the SPARLAB_LOCAL_* phases of
+                           its local iteration (a diagnostic; the reference keeps none) */
 } sparlab_element_batch;
+
+/* Bits of sparlab_element_batch.local_path: the fallbacks of the local
+ * iteration of the modes (src/fem/IncompatibleModes.cpp:398-546) an element
+ * went through. */
+#define SPARLAB_LOCAL_REGULA_FALSI 1   /* the line search's regula falsi on d . r_alpha */
+#define SPARLAB_LOCAL_NORM_SEARCH 2    /* search_norm: halvings on |r_alpha| */
+#define SPARLAB_LOCAL_INDEFINITE 4     /* sym(K_aa) not positive definite */
+#define SPARLAB_LOCAL_SHIFTED 8        /* the shifted direction (K_aa + mu I) d = -r_alpha */
+#define SPARLAB_LOCAL_MARQUARDT 16     /* Levenberg-Marquardt */
+#define SPARLAB_LOCAL_HALVED 32        /* a refused trial halved by the line search */
 
 /* Doubles of workspace an element batch needs (86 per integration point, 109
  * per element and 17 more), or a negative code for a rule the kernels do not
@@ -369,6 +381,10 @@ int sparlab_system_history(const sparlab_system* system, int trial, double* stat
 int sparlab_system_set_history(sparlab_system* system, const double* states, const double* alpha);
 /* The last evaluation's element forces (24 per element) and tangents (576). */
 int sparlab_system_element_outputs(const sparlab_system* system, double* fe, double* ke);
+/* The same for elements first .. first + count - 1 (host arrays of 24 and
+ * 576 per element; null skips), e.g. a sample of a large mesh. */
+int sparlab_system_element_outputs_range(const sparlab_system* system, int64_t first, int64_t count, double* fe,
+                                         double* ke);
 
 /* Replace the nodal reference coordinates (host, 3 per node) or the material
  * of a system (the pattern and the history are kept; the material must keep

@@ -197,7 +197,17 @@ template <class T> struct ElementResult {
   Vec<T, 9> internal;
   int internal_iterations = 0;
   int passes = 0;  // pass() calls, successful or not (diagnostic only)
+  int path = 0;    // the local iteration's phases entered (kPath* bits; diagnostic only)
 };
+
+// Bits of ElementResult::path: the phases of the local iteration
+// (IncompatibleModes.cpp:398-546) an evaluation went through.
+constexpr int kPathRegulaFalsi = 1;   // the line search's regula falsi on d . r_a
+constexpr int kPathNormSearch = 2;    // search_norm: halvings on |r_a|
+constexpr int kPathIndefinite = 4;    // sym(K_aa) not positive definite
+constexpr int kPathShifted = 8;       // the shifted direction (K_aa + mu I) d = -r_a
+constexpr int kPathMarquardt = 16;    // Levenberg-Marquardt
+constexpr int kPathHalved = 32;       // a trial that failed, halved in the line search
 
 // include/sparlab/fem/Elastoplastic.hpp:206-221 (ElastoplasticStress).
 template <class T> struct ElementStress {
@@ -506,6 +516,7 @@ class IncompatibleHex8 {
     pass(alpha);
     T residual = norm2(r_alpha());
     auto search_norm = [&](const Vec<T, 9>& d, T reach) {
+      out.path |= kPathNormSearch;
       T best_step = T(0.0);
       T best = residual;
       bool last_is_best = false;
@@ -529,6 +540,7 @@ class IncompatibleHex8 {
       T reach = T(1.0);
       bool evaluated = try_pass(axpy(alpha, T(1.0), d));
       for (int halving = 0; !evaluated && halving < kMaxLineSearch; ++halving) {
+        out.path |= kPathHalved;
         reach = reach * T(0.5);
         ++cuts;
         evaluated = try_pass(axpy(alpha, reach, d));
@@ -545,6 +557,7 @@ class IncompatibleHex8 {
       if (!(phi0 < T(0.0)) || !(phi1 > T(0.0)) || !finite_(phi1)) return search_norm(d, T(0.5) * reach);
       T s0 = T(0.0), f0 = phi0, s1 = reach, f1 = phi1, step = reach;
       int side = 0;
+      out.path |= kPathRegulaFalsi;
       for (int search = 0; search < kMaxLineSearch; ++search) {
         const T width = s1 - s0;
         step = clamp_((s0 * f1 - s1 * f0) / (f1 - f0), s0 + T(0.1) * width, s1 - T(0.1) * width);
@@ -606,6 +619,7 @@ class IncompatibleHex8 {
       if (llt.ok) {
         if (line_search(newton, r0)) continue;
       } else {
+        out.path |= kPathIndefinite;
         if (try_pass(axpy(alpha, T(1.0), newton))) {
           const T full = norm2(r_alpha());
           if (full <= T(kFullStepReduction) * residual) {
@@ -614,6 +628,7 @@ class IncompatibleHex8 {
             continue;
           }
         }
+        out.path |= kPathShifted;
         Vec<T, 9> ev;
         Mat<T, 9, 9> evec;
         jacobi_eigen<T, 9>(sym, ev, evec, false);
@@ -628,6 +643,7 @@ class IncompatibleHex8 {
         for (int i = 0; i < 9; ++i) shifted(i) = -shifted(i);
         if (line_search(shifted, r0) || search_norm(newton, T(0.5))) continue;
       }
+      out.path |= kPathMarquardt;
       const Mat<T, 9, 9> normal = mul_tn(k0, k0);
       const Vec<T, 9> gradient = mul_tn(k0, r0);
       T dmax = normal(0, 0);

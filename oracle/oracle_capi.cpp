@@ -85,7 +85,7 @@ template <class F> int guarded(char* msg, int len, F&& f) {
 }
 
 // One element evaluation (IncompatibleModes.cpp:577-600) on scalar type T.
-// scalars_out: energy, yielding points, local iterations, symmetric, passes.
+// scalars_out: energy, yielding points, local iterations, symmetric, passes, path.
 template <class T>
 int element_impl(const double* x0, const double* ue, const double* states_in, const double* alpha_in,
                  const MaterialData& mat, const Rule& rule, int kinematics, int want_tangent,
@@ -115,6 +115,7 @@ int element_impl(const double* x0, const double* ue, const double* states_in, co
       scalars[2] = r.internal_iterations;
       scalars[3] = r.symmetric ? 1.0 : 0.0;
       scalars[4] = r.passes;
+      scalars[5] = r.path;
     }
   });
 }
@@ -241,7 +242,7 @@ long long oracle_element_flops(const double* x0, const double* ue, const double*
                                int axis, int kinematics, int want_tangent) {
   const Rule rule = make_rule(in_plane, thickness, axis);
   std::vector<double> f(24), k(576), a(9), so(static_cast<std::size_t>(rule.n) * kStateRecord);
-  double sc[5];
+  double sc[6];
   flop_counter() = 0;
   const int st = element_impl<Counted>(x0, ue, states_in, alpha_in, read_material(mat), rule, kinematics,
                                        want_tangent, f.data(), k.data(), a.data(), so.data(), sc, nullptr, 0);
@@ -257,7 +258,7 @@ long long oracle_element_flops_split(const double* x0, const double* ue, const d
                                      int axis, int kinematics, int want_tangent, long long* parts) {
   const Rule rule = make_rule(in_plane, thickness, axis);
   std::vector<double> f(24), k(576), a(9), so(static_cast<std::size_t>(rule.n) * kStateRecord);
-  double sc[5];
+  double sc[6];
   flop_counter() = 0;
   flop_mark() = 0;
   const int st = element_impl<Counted>(x0, ue, states_in, alpha_in, read_material(mat), rule, kinematics,
@@ -272,7 +273,7 @@ long long oracle_element_flops_split(const double* x0, const double* ue, const d
 // NonlinearSystem.hpp:208-237 (the CPU baseline). Element e reads
 // x0[e*24], ue[e*24], states_in[e*P*39], alpha_in[e*9] (alpha_in may be null)
 // and writes f[e*24], k[e*576] (k may be null), alpha_out[e*9],
-// states_out[e*P*39], scalars[e*5], status[e]. Returns the failures.
+// states_out[e*P*39], scalars[e*6], status[e]. Returns the failures.
 int oracle_element_batch(long long ne, const double* x0, const double* ue, const double* states_in,
                          const double* alpha_in, const double* mat, int in_plane, int thickness, int axis,
                          int kinematics, int want_tangent, int threads, double* f, double* k,
@@ -287,7 +288,7 @@ int oracle_element_batch(long long ne, const double* x0, const double* ue, const
     const int st = element_impl<double>(
         x0 + e * 24, ue + e * 24, states_in + e * per, alpha_in ? alpha_in + e * 9 : nullptr, m, rule,
         kinematics, want_tangent, f + e * 24, k ? k + e * 576 : nullptr, alpha_out + e * 9,
-        states_out + e * per, scalars + e * 5, nullptr, 0);
+        states_out + e * per, scalars + e * 6, nullptr, 0);
     status[e] = st;
     if (st != 0) ++failures;
   }

@@ -502,6 +502,7 @@ int capi_launch_element_batch(int want_tangent, const sparlab_element_batch* b, 
   a.iterations = b->iterations;
   a.status = b->status;
   a.fail_value = b->fail_value;
+  a.local_path = b->local_path;
   set_workspace(a, workspace);
   if (stages & SPARLAB_STAGE_INDEX_ORDER) a.ordered = 0;
   // The local solve writes every element's status; the condensation reads it.

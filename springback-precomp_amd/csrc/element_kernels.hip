@@ -292,8 +292,17 @@ enum Phase : int {
 enum LsContext : int { kLsSpd = 0, kLsShifted = 1 };
 enum SnContext : int { kSnFromLs = 0, kSnAfterShift = 1 };
 
+// Bits of an element's local path (the phases of the local iteration it went
+// through; SPARLAB_LOCAL_* in the C ABI).
+constexpr int kPathRegulaFalsi = 1;
+constexpr int kPathNormSearch = 2;
+constexpr int kPathIndefinite = 4;
+constexpr int kPathShifted = 8;
+constexpr int kPathMarquardt = 16;
+constexpr int kPathHalved = 32;
+
 struct Machine {
-  int phase, ls_ctx, sn_ctx, iterations, halving, search, side;
+  int phase, ls_ctx, sn_ctx, iterations, halving, search, side, path;
   bool last_is_best, moved;
   double alpha[9], d[9], newton[9], r0[9];
   double residual, s;  // s: the trial is alpha + s d
@@ -355,11 +364,13 @@ __device__ int machine_top(Machine& M, TeamSlot& T, const LocalSums& cur, int p,
     for (int i = 0; i < 9; ++i) M.d[i] = M.newton[i];
     M.s = 1.0;
     M.phase = kIndefFull;
+    M.path |= kPathIndefinite;
   }
   return kContinue;
 }
 
 __device__ __forceinline__ void sn_start(Machine& M, double reach) {
+  M.path |= kPathNormSearch;
   M.best_step = 0.0;
   M.best = M.residual;
   M.last_is_best = false;
@@ -382,6 +393,7 @@ __device__ __forceinline__ void lm_start(Machine& M, TeamSlot& T, int p) {
   for (int i = 1; i < 9; ++i) dmax = fmax(dmax, T.w()[i * 10]);
   M.mu = kLocalMarquardt * dmax;
   M.moved = false;
+  M.path |= kPathMarquardt;
   M.search = 0;
 }
 
@@ -440,6 +452,7 @@ __device__ int machine_step(Machine& M, TeamSlot& T, bool ok, const LocalSums& c
     case kLsReach:
       if (!ok) {
         if (M.halving < kMaxLineSearch) {
+          M.path |= kPathHalved;
           M.reach = M.reach * 0.5;
           ++M.halving;
           M.s = M.reach;
@@ -470,6 +483,7 @@ __device__ int machine_step(Machine& M, TeamSlot& T, bool ok, const LocalSums& c
         M.step = raw < lo ? lo : (hi < raw ? hi : raw);
         M.s = M.step;
         M.phase = kLsFalsi;
+        M.path |= kPathRegulaFalsi;
         return kContinue;
       }
     case kLsFalsi: {
@@ -567,6 +581,7 @@ __device__ int machine_step(Machine& M, TeamSlot& T, bool ok, const LocalSums& c
         }
       }
       // shifted direction: (K_aa + mu I) d = -r0, mu from sym's spectrum
+      M.path |= kPathShifted;
       for (int i = 0; i < 81; ++i) {
         const int r = i / 9, c = i % 9;
         T.w()[i] = 0.5 * (T.k0[r * 9 + c] + T.k0[c * 9 + r]);
@@ -707,6 +722,7 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
       const int code = team_imin(st != kOk ? (p << 8) | st : 1 << 20);
       if (code != (1 << 20)) {
         if (p == (code >> 8)) { A.status[e] = code & 0xFF; A.fail_value[e] = fv; }
+        if (p == 0 && A.local_path) A.local_path[e] = 0;
         failed_element_outputs(A, e, p);
         continue;  // the team stays idle and takes another element
       }
@@ -714,6 +730,7 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
 #pragma unroll
         for (int i = 0; i < 9; ++i) M.alpha[i] = A.alpha_in ? A.alpha_in[e * 9 + i] : 0.0;
         M.iterations = 0;
+        M.path = 0;
         M.s = 0.0;
         M.phase = kFirst;
       }
@@ -739,6 +756,7 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
         if (p == 0) {
           A.status[e] = kOk;
           A.iterations[e] = M.iterations;
+          if (A.local_path) A.local_path[e] = M.path;
         }
         if (p < 9) A.alpha_out[e * 9 + p] = M.alpha[p];
         if (p == 0) A.alpha_out[e * 9 + 8] = M.alpha[8];
@@ -760,6 +778,7 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
           A.status[e] = status == kOk ? kUnsupported : status;
           A.fail_value[e] = value;
           A.iterations[e] = M.iterations;
+          if (A.local_path) A.local_path[e] = M.path;
         }
         failed_element_outputs(A, e, p);
         team_sync();

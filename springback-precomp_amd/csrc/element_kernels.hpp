@@ -36,6 +36,7 @@ struct ElementArgs {
   int* iterations;    // local Newton iterations per element
   int* status;        // per element: 0 or an ElementStatus
   double* fail_value; // per element: the determinant of a refused Jacobian
+  int* local_path;    // per element (optional): the local iteration's phases (kPath* bits)
   // Workspace (device scratch of one call), three regions:
   //   point_ws  [npts][42]: S (6), then the tangent (36, row-major) of each
   //             point's final local pass (written by the local solve)

@@ -632,4 +632,17 @@ int sparlab_system_element_outputs(const sparlab_system* s, double* fe, double* 
   return SPARLAB_OK;
 }
 
+int sparlab_system_element_outputs_range(const sparlab_system* s, int64_t first, int64_t count, double* fe,
+                                         double* ke) {
+  if (!s) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null system");
+  if (first < 0 || count < 0 || first > s->ne || count > s->ne - first)
+    return sbk::capi_fail(SPARLAB_E_ARGUMENT, "the element range lies outside the mesh");
+  if (count == 0) return SPARLAB_OK;
+  if (hipDeviceSynchronize() != hipSuccess) return sbk::capi_fail(SPARLAB_E_DEVICE, "synchronize failed");
+  if ((fe && hipMemcpy(fe, s->fe + 24 * first, 24 * count * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) ||
+      (ke && hipMemcpy(ke, s->ke + 576 * first, 576 * count * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "download failed");
+  return SPARLAB_OK;
+}
+
 }  // extern "C"

@@ -82,7 +82,7 @@ class CElementBatch(ctypes.Structure):
         ("state_out", ctypes.c_void_p), ("alpha_out", ctypes.c_void_p),
         ("force", ctypes.c_void_p), ("tangent", ctypes.c_void_p), ("energy", ctypes.c_void_p),
         ("yielding", ctypes.c_void_p), ("iterations", ctypes.c_void_p), ("status", ctypes.c_void_p),
-        ("fail_value", ctypes.c_void_p), ("workspace", ctypes.c_void_p),
+        ("fail_value", ctypes.c_void_p), ("workspace", ctypes.c_void_p), ("local_path", ctypes.c_void_p),
     ]
 
 
@@ -308,6 +308,7 @@ class ElementBatch:
         self.iterations = torch.empty((ne,), **i32)
         self.status = torch.empty((ne,), **i32)
         self.fail_value = torch.empty((ne,), **f64)
+        self.local_path = torch.empty((ne,), **i32)  # SPARLAB_LOCAL_* bits per element
         # device scratch of the element kernels (86 doubles per point, 108 per element)
         self.workspace = torch.empty((workspace_doubles(ne, points),), **f64)
         for name in ("coords", "u", "state_in"):
@@ -320,7 +321,7 @@ class ElementBatch:
             _ptr(coords), _ptr(conn), _ptr(u), coords.shape[0], ne, _ptr(state_in), _ptr(alpha_in),
             _ptr(self.state_out), _ptr(self.alpha_out), _ptr(self.force), _ptr(self.tangent),
             _ptr(self.energy), _ptr(self.yielding), _ptr(self.iterations), _ptr(self.status),
-            _ptr(self.fail_value), _ptr(self.workspace))
+            _ptr(self.fail_value), _ptr(self.workspace), _ptr(self.local_path))
 
 
 LOCAL_SOLVE = 1
@@ -738,10 +739,20 @@ class NonlinearSystem:
         out = stress.cpu().numpy()
         return {name: out[:, off:off + width] if width > 1 else out[:, off] for name, off, width in STRESS_LAYOUT}
 
-    def element_outputs(self):
-        fe = np.zeros((self.num_elements, 24))
-        ke = np.zeros((self.num_elements, 24, 24))
-        _check(lib().sparlab_system_element_outputs(self.h, _np_ptr(fe), _np_ptr(ke)))
+    def element_outputs(self, first=None, count=None):
+        """The last evaluation's element forces (E, 24) and tangents (E, 24,
+        24), of every element or of elements first .. first + count - 1."""
+        if first is None:
+            fe = np.zeros((self.num_elements, 24))
+            ke = np.zeros((self.num_elements, 24, 24))
+            _check(lib().sparlab_system_element_outputs(self.h, _np_ptr(fe), _np_ptr(ke)))
+            return fe, ke
+        L = lib()
+        L.sparlab_system_element_outputs_range.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+                                                           ctypes.c_void_p, ctypes.c_void_p]
+        fe = np.zeros((count, 24))
+        ke = np.zeros((count, 24, 24))
+        _check(L.sparlab_system_element_outputs_range(self.h, int(first), int(count), _np_ptr(fe), _np_ptr(ke)))
         return fe, ke
 
 

@@ -177,14 +177,15 @@ def element(x0, ue, mat, states=None, alpha=None, rule=(2, 0, 2), kinematics=2, 
     assert states.shape == (npts, STATE)
     a_in = None if alpha is None else np.ascontiguousarray(alpha, dtype=np.float64)
     f, k, a_out = np.zeros(24), np.zeros(576), np.zeros(9)
-    s_out, sc = np.zeros((npts, STATE)), np.zeros(5)
+    s_out, sc = np.zeros((npts, STATE)), np.zeros(6)
     buf = ctypes.create_string_buffer(1024)
     st = lib().oracle_element(dp(x0), dp(ue), dp(states), dp(a_in), dp(mat), rule[0], rule[1], rule[2],
                               kinematics, int(want_tangent), dp(f), dp(k), dp(a_out), dp(s_out), dp(sc),
                               buf, 1024)
     _check(st, buf)
     return dict(force=f, tangent=k.reshape(24, 24), alpha=a_out, states=s_out, energy=sc[0],
-                yielding=int(sc[1]), iterations=int(sc[2]), symmetric=bool(sc[3]), passes=int(sc[4]))
+                yielding=int(sc[1]), iterations=int(sc[2]), symmetric=bool(sc[3]), passes=int(sc[4]),
+                path=int(sc[5]))
 
 
 def element_flops(x0, ue, mat, states=None, alpha=None, rule=(2, 0, 2), kinematics=2, want_tangent=True):
@@ -225,7 +226,7 @@ def element_batch(x0, ue, mat, states, alpha=None, rule=(2, 0, 2), kinematics=2,
     k = np.zeros((ne, 24, 24)) if want_tangent else None
     a_out = np.zeros((ne, 9))
     s_out = np.zeros((ne, npts, STATE))
-    sc = np.zeros((ne, 5))
+    sc = np.zeros((ne, 6))
     status = np.zeros(ne, dtype=np.int32)
     lib().oracle_element_batch(ctypes.c_longlong(ne), dp(x0), dp(ue), dp(states), dp(a_in), dp(mat),
                                rule[0], rule[1], rule[2], kinematics, int(want_tangent), threads, dp(f),

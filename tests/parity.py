@@ -39,7 +39,8 @@ def run_gpu(material, rule, coords, conn, u, states_ref, alpha=None, kinematics=
                 alpha=b.alpha_out.cpu().numpy(),
                 states=G.states_to_reference(b.state_out.cpu().numpy(), material, ne, npts),
                 energy=b.energy.cpu().numpy(), yielding=b.yielding.cpu().numpy(),
-                iterations=b.iterations.cpu().numpy(), status=b.status.cpu().numpy())
+                iterations=b.iterations.cpu().numpy(), status=b.status.cpu().numpy(),
+                local_path=b.local_path.cpu().numpy())
 
 
 def rel(a, b):

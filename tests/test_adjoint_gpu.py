@@ -147,34 +147,51 @@ def test_autograd_function(device):
     assert np.array_equal(theta.grad.numpy(), ref["parameters"].cpu().numpy())
 
 
-def test_spiral_forming_state_matches_central_differences(device):
+@pytest.mark.parametrize("size", [(20, 25, 4), (200, 250, 4)], ids=["20x25x4", "200x250x4"])
+def test_spiral_forming_state_matches_central_differences(device, size):
     """The differentiation workload's state (spiral incremental forming, mid
-    path) at 20 x 25 x 4: coordinate and parameter products against central
-    differences, h = 1e-6 (of the element size / of each parameter)."""
+    path) at 20 x 25 x 4 and at its full 200 x 250 x 4 (200 000 elements):
+    coordinate and parameter products against central differences, h = 1e-6
+    of the element size and of each parameter (1e-4 for the parameters of the
+    full sheet, see below)."""
     import torch
+    nx, ny, nz = size
+    print(f"spiral forming state {nx} x {ny} x {nz}", flush=True)
     m = W.microbench_material()
-    coords, conn, u1, u2 = W.differentiation_workload(nx=20, ny=25, nz=4)
+    coords, conn, u1, u2 = W.differentiation_workload(nx=nx, ny=ny, nz=nz)
     s = G.NonlinearSystem(coords, conn, m, G.Rule(2, 0, 2))
-    s.evaluate(u1)
+    u = torch.as_tensor(u1, device=device)
+    f = torch.zeros(3 * s.num_nodes, dtype=torch.float64, device=device)
+    values = torch.empty(s.nnz, dtype=torch.float64, device=device)
+    s.evaluate_device(u, f, values)
+    s.result()
     s.commit()
-    ev = s.evaluate(u2)
-    assert ev.yielding_points > 0
     u = torch.as_tensor(u2, device=device)
+    s.evaluate_device(u, f, values)
+    ev = s.result()
+    assert ev.yielding_points > 0 and ev.failed_elements == 0
+    del values
     rng = np.random.default_rng(12)
     v = rng.normal(size=3 * s.num_nodes)
     g = s.vjp_device(u, torch.as_tensor(v, device=device), coords=True, parameters=True)
     gx, gp = g["coords"].cpu().numpy(), g["parameters"].cpu().numpy()
-    d = rng.normal(size=coords.shape) * (0.020 / 20)
+    d = rng.normal(size=coords.shape) * (0.020 / nx)
     h = 1e-6
     fd = v @ (fint_at(s, u, coords + h * d) - fint_at(s, u, coords - h * d)) / (2.0 * h)
     s.set_coords(coords)
     assert abs(fd - gx @ d.reshape(-1)) <= 1e-6 * abs(gx @ d.reshape(-1))
     theta = G.material_parameters(m)
     dt = rng.normal(size=7) * theta[:7]
+    # The parameter step: 1e-6, except on the full sheet, where a central
+    # difference at 1e-6 is limited by round-off (eps sum |v_i f_i| / h, about
+    # 1e-6 of the product there: the oracle's own differences at h = 1e-6,
+    # 1e-5, 1e-4 give -3.0035957, -3.0035991, -3.0035987, with no local
+    # iteration count changing); at 1e-4 the truncation error is about 1e-8.
+    hp = 1e-4 if nx * ny * nz > 100_000 else h
     tp, tm = theta.copy(), theta.copy()
-    tp[:7] += h * dt
-    tm[:7] -= h * dt
-    fd = v @ (fint_at(s, u, material=G.with_parameters(m, tp)) - fint_at(s, u, material=G.with_parameters(m, tm))) / (2.0 * h)
+    tp[:7] += hp * dt
+    tm[:7] -= hp * dt
+    fd = v @ (fint_at(s, u, material=G.with_parameters(m, tp)) - fint_at(s, u, material=G.with_parameters(m, tm))) / (2.0 * hp)
     s.set_material(m)
     an = gp[:7] @ dt
     assert abs(fd - an) <= 1e-6 * abs(an)
