@@ -121,7 +121,8 @@ int build_rule(const sparlab_rule& r, sbk::RuleArg& out, std::string& why) {
   const auto& rz = r.thickness_axis == 2 ? through : plane;
   const std::size_t total = rx.size() * ry.size() * rz.size();
   if (total > static_cast<std::size_t>(sbk::kMaxPoints)) {
-    why = "the element kernels take integration rules of at most 8 points (got " + std::to_string(total) + ")";
+    why = "the element kernels take integration rules of at most " + std::to_string(sbk::kMaxPoints) + " points (got " +
+          std::to_string(total) + ")";
     return -2;
   }
   int q = 0;

@@ -2,9 +2,12 @@
 //
 // Execution model: a team of 8 consecutive lanes of a 64-wide wavefront owns
 // one element; lane p of the team owns integration point p of the element's
-// rule (rules of up to 8 points). Team-wide sums are DPP butterflies inside
-// the 8 lanes (quad_perm xor 1, quad_perm xor 2, row_half_mirror), so every
-// lane of a team ends with the bitwise same total, in a fixed order.
+// rule (rules of up to 8 points; rules of 9 to 32 points, the reference's
+// 2 x 2 x m box rules, take teams of 32 lanes). Team-wide sums are DPP
+// butterflies inside the team (quad_perm xor 1, quad_perm xor 2,
+// row_half_mirror; for 32 lanes also row_mirror and a swizzle between the
+// two rows of 16), so every lane of a team ends with the bitwise same total,
+// in a fixed order.
 //
 // The constitutive algorithms follow the reference's material point:
 //   src/material/Material.cpp:73-90 (hardening), src/material/Plasticity.cpp
@@ -19,8 +22,9 @@
 
 namespace sbk {
 
-constexpr int kTeam = 8;           // lanes per element
-constexpr int kMaxPoints = 8;      // integration points per element
+constexpr int kTeam = 8;           // lanes per element (rules of up to 8 points)
+constexpr int kWideTeam = 32;      // lanes per element for rules of 9..32 points
+constexpr int kMaxPoints = 32;     // integration points per element
 constexpr int kMaxTerms = 4;       // backstresses (include/sparlab/material/Material.hpp:68)
 constexpr double kEps = 2.220446049250313e-16;
 
@@ -81,43 +85,68 @@ constexpr int kXor1 = 0xB1;        // quad_perm [1,0,3,2]
 constexpr int kXor2 = 0x4E;        // quad_perm [2,3,0,1]
 constexpr int kHalfMirror = 0x141; // row_half_mirror: lane i <-> 7 - i in each 8
 
-// Every lane of the team receives the same sum (commutativity makes the
-// three butterfly steps symmetric).
+constexpr int kRowMirror = 0x140;  // row_mirror: lane i <-> 15 - i in each 16
+// ds_swizzle in bit mode over 32 lanes: lane i reads lane i ^ 16
+constexpr int kSwap16 = 0x1F | (0x10 << 10);
+
+__device__ __forceinline__ double swap16_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_swizzle(static_cast<int>(b), kSwap16);
+  const int hi = __builtin_amdgcn_ds_swizzle(static_cast<int>(b >> 32), kSwap16);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+__device__ __forceinline__ int swap16_i32(int v) { return __builtin_amdgcn_ds_swizzle(v, kSwap16); }
+
+// A team is TW consecutive lanes (8 or 32) of the wavefront; every lane of
+// the team receives the same result, in a fixed order (commutativity makes
+// each butterfly step symmetric). The 8-lane steps are quad_perm xor 1, xor
+// 2 and row_half_mirror; 32 lanes add row_mirror and a swap of the two rows
+// of 16.
+template <int TW = kTeam, class Op>
+__device__ __forceinline__ double team_fold(double v, Op op) {
+  v = op(v, dpp_f64<kXor1>(v));
+  v = op(v, dpp_f64<kXor2>(v));
+  v = op(v, dpp_f64<kHalfMirror>(v));
+  if constexpr (TW == kWideTeam) {
+    v = op(v, dpp_f64<kRowMirror>(v));
+    v = op(v, swap16_f64(v));
+  }
+  return v;
+}
+template <int TW = kTeam, class Op>
+__device__ __forceinline__ int team_ifold(int v, Op op) {
+  v = op(v, dpp_i32<kXor1>(v));
+  v = op(v, dpp_i32<kXor2>(v));
+  v = op(v, dpp_i32<kHalfMirror>(v));
+  if constexpr (TW == kWideTeam) {
+    v = op(v, dpp_i32<kRowMirror>(v));
+    v = op(v, swap16_i32(v));
+  }
+  return v;
+}
+template <int TW = kTeam>
 __device__ __forceinline__ double team_sum(double v) {
-  v = v + dpp_f64<kXor1>(v);
-  v = v + dpp_f64<kXor2>(v);
-  v = v + dpp_f64<kHalfMirror>(v);
-  return v;
+  return team_fold<TW>(v, [](double a, double b) { return a + b; });
 }
+template <int TW = kTeam>
 __device__ __forceinline__ int team_or(int v) {
-  v = v | dpp_i32<kXor1>(v);
-  v = v | dpp_i32<kXor2>(v);
-  v = v | dpp_i32<kHalfMirror>(v);
-  return v;
+  return team_ifold<TW>(v, [](int a, int b) { return a | b; });
 }
+template <int TW = kTeam>
 __device__ __forceinline__ double team_max(double v) {
-  v = fmax(v, dpp_f64<kXor1>(v));
-  v = fmax(v, dpp_f64<kXor2>(v));
-  v = fmax(v, dpp_f64<kHalfMirror>(v));
-  return v;
+  return team_fold<TW>(v, [](double a, double b) { return fmax(a, b); });
 }
+template <int TW = kTeam>
 __device__ __forceinline__ double team_min(double v) {
-  v = fmin(v, dpp_f64<kXor1>(v));
-  v = fmin(v, dpp_f64<kXor2>(v));
-  v = fmin(v, dpp_f64<kHalfMirror>(v));
-  return v;
+  return team_fold<TW>(v, [](double a, double b) { return fmin(a, b); });
 }
+template <int TW = kTeam>
 __device__ __forceinline__ int team_isum(int v) {
-  v = v + dpp_i32<kXor1>(v);
-  v = v + dpp_i32<kXor2>(v);
-  v = v + dpp_i32<kHalfMirror>(v);
-  return v;
+  return team_ifold<TW>(v, [](int a, int b) { return a + b; });
 }
+template <int TW = kTeam>
 __device__ __forceinline__ int team_imin(int v) {
-  v = min(v, dpp_i32<kXor1>(v));
-  v = min(v, dpp_i32<kXor2>(v));
-  v = min(v, dpp_i32<kHalfMirror>(v));
-  return v;
+  return team_ifold<TW>(v, [](int a, int b) { return min(a, b); });
 }
 // Orders the team's lanes' LDS traffic. A team lives in one wavefront, whose
 // LDS instructions execute in issue order, so wavefront-scope fences suffice:

@@ -31,8 +31,15 @@
 
 namespace sbk {
 
-constexpr int kWaveTeams = 8;
+constexpr int kWaveTeams = 8;      // teams of 8 lanes per wavefront
 constexpr int kDefaultChunk = 64;  // elements per wavefront in the local solve
+// Lanes per element: 8 (a lane per point) for rules of up to 8 points, 32 for
+// rules of 9..32 points (the reference's 2 x 2 x m box rules). In a 32-lane
+// team lane p still owns point p; the element's 9 x 9 algebra (the local
+// Newton's machine, K_aa's factorisation) runs on lanes 0..7 as in an 8-lane
+// team, and the condensation's row blocks on lanes 0..7.
+template <int TW>
+constexpr int teams_per_wave() { return 64 / TW; }
 
 // IncompatibleModes.hpp:98-127
 constexpr int kMaxLocalIterations = 25;
@@ -89,7 +96,7 @@ __device__ __forceinline__ void mode_operator(const double F[3][3], const double
 
 // The team's sums of a pass from each lane's point response (w = 0 and zero
 // response on lanes without a point); the K_aa totals go to kaa (row-major).
-template <int KIN>
+template <int KIN, int TW>
 __device__ __forceinline__ void team_sums(const double F[3][3], const double h[3][3], const PointResponse& r,
                                           const double gm[3][3], double w, int p, double* kaa, LocalSums& out,
                                           bool want_kaa_only) {
@@ -116,9 +123,9 @@ __device__ __forceinline__ void team_sums(const double F[3][3], const double h[3
         ss = ss + fabs(ba[c][n]) * fabs(r.S[c]);
         sf = sf + fabs(ba[c][n]);
       }
-      out.ra[n] = team_sum(w * sr);
-      smax = fmax(smax, team_sum(w * ss));
-      fmaxv = fmax(fmaxv, team_sum(ff * sf));
+      out.ra[n] = team_sum<TW>(w * sr);
+      smax = fmax(smax, team_sum<TW>(w * ss));
+      fmaxv = fmax(fmaxv, team_sum<TW>(ff * sf));
     }
     out.scale_max = smax;
     out.floor_max = fmaxv;
@@ -131,7 +138,7 @@ __device__ __forceinline__ void team_sums(const double F[3][3], const double h[3
       double sr = ba[0][n] * r.S[0];
 #pragma unroll
       for (int c = 1; c < 6; ++c) sr = sr + ba[c][n] * r.S[c];
-      out.ra[n] = team_sum(w * sr);
+      out.ra[n] = team_sum<TW>(w * sr);
     }
   }
   // K_aa = sum_q w B_a^T mat B_a (+ the geometric alpha-alpha block), a row
@@ -169,8 +176,8 @@ __device__ __forceinline__ void team_sums(const double F[3][3], const double h[3
       if constexpr (KIN != kSmall) {
         if (mm % 3 == n % 3) v = v + w * geo[mm / 3][n / 3];
       }
-      const double tot = team_sum(v);
-      if (((mm * 9 + n) & 7) == p) kaa[mm * 9 + n] = tot;
+      const double tot = team_sum<TW>(v);
+      if (((mm * 9 + n) % TW) == p) kaa[mm * 9 + n] = tot;
     }
   }
   team_sync();
@@ -193,7 +200,7 @@ __device__ __forceinline__ void zero_response(PointResponse& r) {
 // workspace, energy and yielding of a pass that meets the stopping test.
 // trial(alpha) fills the pass's mode parameters (from the team's LDS machine);
 // it is called again after the return rather than holding them across it.
-template <int KIN, int NT, class Trial>
+template <int KIN, int NT, int TW, class Trial>
 __device__ __forceinline__ int team_local_pass(const ElementArgs& A, const DevMaterial& m, long long e, int p,
                                                LaneLds geom, LaneLds logs, Trial trial, double* kaa,
                                                LocalSums& out) {
@@ -212,7 +219,7 @@ __device__ __forceinline__ int team_local_pass(const ElementArgs& A, const DevMa
   }
   reload_barrier();
   const int stage = (fail == kReturnFailed || fail == kNonFiniteTrial) ? 1 : 0;
-  const int first = team_imin(fail != 0 ? (stage << 12) | (p << 8) | fail : 1 << 20);
+  const int first = team_imin<TW>(fail != 0 ? (stage << 16) | (p << 8) | fail : 1 << 20);
   if (first != (1 << 20)) return first & 0xFF;
   // F and h again from the LDS data and the machine's alpha, rather than held
   // in registers across the return
@@ -231,9 +238,9 @@ __device__ __forceinline__ int team_local_pass(const ElementArgs& A, const DevMa
 #pragma unroll
       for (int j = 0; j < 3; ++j) { F[i][j] = 0.0; h[i][j] = 0.0; gm[i][j] = 0.0; }
   }
-  const double energy = team_sum(w * r.energy);
-  const int yielding = team_isum(r.yielding ? 1 : 0);
-  team_sums<KIN>(F, h, r, gm, w, p, kaa, out, false);
+  const double energy = team_sum<TW>(w * r.energy);
+  const int yielding = team_isum<TW>(r.yielding ? 1 : 0);
+  team_sums<KIN, TW>(F, h, r, gm, w, p, kaa, out, false);
   if (out.stop) {
     if (has_point) {
       double* rec = A.point_ws + q * kPointRecord;
@@ -255,7 +262,7 @@ __device__ __forceinline__ int team_local_pass(const ElementArgs& A, const DevMa
 // SolverError); the condensation zeroes its force and tangent. Every output
 // of a failed element is thereby fixed, whatever its team did before.
 __device__ __forceinline__ void failed_element_outputs(const ElementArgs& A, long long e, int p) {
-  A.alpha_out[e * 9 + p] = A.alpha_in ? A.alpha_in[e * 9 + p] : 0.0;
+  if (p < kTeam) A.alpha_out[e * 9 + p] = A.alpha_in ? A.alpha_in[e * 9 + p] : 0.0;
   if (p == 0) {
     A.alpha_out[e * 9 + 8] = A.alpha_in ? A.alpha_in[e * 9 + 8] : 0.0;
     A.energy[e] = 0.0;
@@ -318,6 +325,7 @@ struct TeamSlot {
   LuSlot lu;
   double k0[81];
   Machine m;
+  int step;  // the machine's step result, for the lanes of a 32-lane team beyond 8
   __device__ double* w() { return lu.a; }
 };
 
@@ -668,16 +676,16 @@ __device__ int machine_step(Machine& M, TeamSlot& T, bool ok, const LocalSums& c
   return kFailed;
 }
 
-template <int KIN, int NT>
+template <int KIN, int NT, int TW>
 __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
-  __shared__ TeamSlot slots[kWaveTeams];
+  __shared__ TeamSlot slots[teams_per_wave<TW>()];
   __shared__ double geom_lds[mode_geom_fields<KIN>()][64];
   __shared__ double log_lds[KIN == kLog ? kLogStash : 1][64];
   const int lane = threadIdx.x;
   const LaneLds geom{&geom_lds[0][lane]};
   const LaneLds logs{&log_lds[0][lane]};
-  const int t = lane >> 3;
-  const int p = lane & 7;
+  const int t = lane / TW;
+  const int p = lane % TW;
   TeamSlot& T = slots[t];
   Machine& M = T.m;
   // the wavefront's positions: a balanced partition of [0, ne) into gridDim.x
@@ -692,7 +700,7 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
     // Hand the chunk's next elements to the idle teams, in team order.
     const bool idle = M.phase == kIdle;
     const unsigned long long leaders = __ballot(idle && p == 0);
-    const int before = __popcll(leaders & ((1ull << (t * 8)) - 1ull));
+    const int before = __popcll(leaders & ((1ull << (t * TW)) - 1ull));
     const int free_teams = __popcll(leaders);
     bool fresh = false;
     if (idle) {
@@ -719,7 +727,7 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
         put_mode_geom<KIN>(mg, geom);
       }
       // the first failing point in point order, as the reference's set-up loop
-      const int code = team_imin(st != kOk ? (p << 8) | st : 1 << 20);
+      const int code = team_imin<TW>(st != kOk ? (p << 8) | st : 1 << 20);
       if (code != (1 << 20)) {
         if (p == (code >> 8)) { A.status[e] = code & 0xFF; A.fail_value[e] = fv; }
         if (p == 0 && A.local_path) A.local_path[e] = 0;
@@ -747,11 +755,22 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
         }
       };
       LocalSums cur;
-      const int fail = team_local_pass<KIN, NT>(A, A.mat, e, p, geom, logs, trial, T.lu.a, cur);
+      const int fail = team_local_pass<KIN, NT, TW>(A, A.mat, e, p, geom, logs, trial, T.lu.a, cur);
       const bool ok = fail == 0;
       int status = ok ? kOk : fail;
       double value = 0.0;
-      const int res = machine_step(M, T, ok, cur, p, status, value);
+      int res;
+      if constexpr (TW == kTeam) {
+        res = machine_step(M, T, ok, cur, p, status, value);
+      } else {
+        // the machine on lanes 0..7; its result to the rest of the team
+        if (p < kTeam) {
+          res = machine_step(M, T, ok, cur, p, status, value);
+          if (p == 0) T.step = res;
+        }
+        team_sync();
+        res = T.step;
+      }
       if (res == kConverged) {
         if (p == 0) {
           A.status[e] = kOk;
@@ -762,10 +781,12 @@ __global__ __launch_bounds__(64) void local_solve_kernel(ElementArgs A) {
         if (p == 0) A.alpha_out[e * 9 + 8] = M.alpha[8];
         // K_aa's factors and r_a of the final iteration, for the condensation
         double* er = A.elem_ws + e * kElemRecord;
-        for (int i = p; i < 81; i += 8) er[kErFactors + i] = T.lu.a[i];
-        er[kErResidual + p] = cur.ra[p];
-        er[kErRows + p] = static_cast<double>(T.lu.rt[p]);
-        er[kErCols + p] = static_cast<double>(T.lu.ct[p]);
+        for (int i = p; i < 81; i += TW) er[kErFactors + i] = T.lu.a[i];
+        if (p < kTeam) {
+          er[kErResidual + p] = cur.ra[p];
+          er[kErRows + p] = static_cast<double>(T.lu.rt[p]);
+          er[kErCols + p] = static_cast<double>(T.lu.ct[p]);
+        }
         if (p == 0) {
           er[kErResidual + 8] = cur.ra[8];
           er[kErRows + 8] = static_cast<double>(T.lu.rt[8]);
@@ -848,13 +869,14 @@ __device__ __forceinline__ bool trial_strain(const ModeGeom& mg, const double al
 
 // One team per element, lane p = point p: cost[e] = the element's expected
 // work (0 .. 2 points). Block 0 also clears the class counts and cursors.
-template <int KIN, int NT>
+template <int KIN, int NT, int TW>
 __global__ __launch_bounds__(64) void cost_key_kernel(ElementArgs A) {
   const int lane = threadIdx.x;
-  const int t = lane >> 3;
-  const int p = lane & 7;
-  if (blockIdx.x == 0 && lane < 2 * kCostClasses) A.counts[lane] = 0;
-  const long long e = static_cast<long long>(blockIdx.x) * kWaveTeams + t;
+  const int t = lane / TW;
+  const int p = lane % TW;
+  if (blockIdx.x == 0)
+    for (int i = lane; i < 2 * kCostClasses; i += 64) A.counts[i] = 0;
+  const long long e = static_cast<long long>(blockIdx.x) * teams_per_wave<TW>() + t;
   int c = 0;
   if (e < A.ne && p < A.rule.n) {
     double x0[8][3], ue[24];
@@ -874,7 +896,7 @@ __global__ __launch_bounds__(64) void cost_key_kernel(ElementArgs A) {
       }
     }
   }
-  const int key = team_isum(c);
+  const int key = team_isum<TW>(c);
   if (e < A.ne && p == 0) A.cost[e] = key;
 }
 
@@ -953,17 +975,18 @@ struct CondenseSlot {
 // accumulations run as three point loops (f_u, K_ua and K_au together; K_uu
 // in two column halves after the solve): each loop re-reads the records, and
 // fewer loops would hold more accumulators than the registers take.
-template <int KIN, bool TANGENT, bool SYM>
+template <int KIN, bool TANGENT, bool SYM, int TW>
 __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
-  __shared__ CondenseSlot slots[kWaveTeams];
+  __shared__ CondenseSlot slots[teams_per_wave<TW>()];
   const int lane = threadIdx.x;
-  const int t = lane >> 3;
-  const int p = lane & 7;
+  const int t = lane / TW;
+  const int p = lane % TW;
   CondenseSlot& T = slots[t];
-  const long long e = static_cast<long long>(blockIdx.x) * kWaveTeams + t;
+  const long long e = static_cast<long long>(blockIdx.x) * teams_per_wave<TW>() + t;
   if (e >= A.ne) return;
   if (A.status[e] != kOk) {
     // a failed element: zero force and tangent (lane a: rows 3a..3a+2)
+    if (p >= kTeam) return;
     for (int k = 0; k < 3; ++k) A.force[e * 24 + 3 * p + k] = 0.0;
     if constexpr (TANGENT)
       for (int i = 0; i < 72; ++i) A.tangent[e * 576 + 72 * p + i] = 0.0;
@@ -995,18 +1018,23 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
   }
   // K_aa's factors of the final local iteration
   double* er = A.elem_ws + e * kElemRecord;
-  for (int i = p; i < 81; i += 8) T.lu.a[i] = er[kErFactors + i];
-  T.lu.rt[p] = static_cast<int>(er[kErRows + p]);
-  T.lu.ct[p] = static_cast<int>(er[kErCols + p]);
+  for (int i = p; i < 81; i += TW) T.lu.a[i] = er[kErFactors + i];
+  if (p < kTeam) {
+    T.lu.rt[p] = static_cast<int>(er[kErRows + p]);
+    T.lu.ct[p] = static_cast<int>(er[kErCols + p]);
+  }
   if (p == 0) {
     T.lu.rt[8] = static_cast<int>(er[kErRows + 8]);
     T.lu.ct[8] = static_cast<int>(er[kErCols + 8]);
   }
   // r_a: column 24 of the right-hand sides
-  T.region[p * kRhs + 24] = er[kErResidual + p];
+  if (p < kTeam) T.region[p * kRhs + 24] = er[kErResidual + p];
   if (p == 0) T.region[8 * kRhs + 24] = er[kErResidual + 8];
   team_sync_global();  // the geometry records (global) and the factors (LDS)
   if (p == 0) lu_perm(T.lu.rt, T.lu.ct, T.perm);
+  // the row blocks of the 8 nodes: lanes 0..7 (a 32-lane team's other lanes
+  // have written their points' geometry records and are done)
+  if (TW != kTeam && p >= kTeam) return;
   const int a = p;
   double ru[3], kua[3][9];
   if constexpr (!TANGENT || SYM) {
@@ -1198,12 +1226,12 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
 // ElastoplasticStress; a failing element (set-up, then kinematics, then the
 // returns, each in point order, as the reference's loops) gets zeros and its
 // status.
-template <int KIN, int NT>
+template <int KIN, int NT, int TW>
 __global__ __launch_bounds__(64) void stress_kernel(ElementArgs A, double* __restrict__ out) {
   const int lane = threadIdx.x;
-  const int t = lane >> 3;
-  const int p = lane & 7;
-  const long long e = static_cast<long long>(blockIdx.x) * kWaveTeams + t;
+  const int t = lane / TW;
+  const int p = lane % TW;
+  const long long e = static_cast<long long>(blockIdx.x) * teams_per_wave<TW>() + t;
   if (e >= A.ne) return;  // the whole team
   const int n = A.rule.n;
   // cauchy 6, piola_kirchhoff 6, kirchhoff 6, log strain 6 of the lane's point
@@ -1338,29 +1366,29 @@ __global__ __launch_bounds__(64) void stress_kernel(ElementArgs A, double* __res
     }
   }
   double* o = out + e * kStressFields;
-  const int code = team_imin(st != kOk ? (stage << 12) | (p << 8) | st : 1 << 20);
+  const int code = team_imin<TW>(st != kOk ? (stage << 16) | (p << 8) | st : 1 << 20);
   if (code != (1 << 20)) {
-    for (int i = p; i < kStressFields; i += kWaveTeams) o[i] = 0.0;
+    for (int i = p; i < kStressFields; i += TW) o[i] = 0.0;
     if (p == 0 && A.status) A.status[e] = code & 0xFF;
-    if (p == ((code >> 8) & 0xF) && A.fail_value) A.fail_value[e] = fv;
+    if (p == ((code >> 8) & 0xFF) && A.fail_value) A.fail_value[e] = fv;
     return;
   }
   const double count = static_cast<double>(n);
   double avg[24];
 #pragma unroll
-  for (int i = 0; i < 24; ++i) avg[i] = team_sum(v[i]) / count;
-  max_strain = team_max(max_strain);
-  min_jacobian = KIN == kSmall ? 1.0 : team_min(min_jacobian);
-  max_abar = team_max(max_abar);
+  for (int i = 0; i < 24; ++i) avg[i] = team_sum<TW>(v[i]) / count;
+  max_strain = team_max<TW>(max_strain);
+  min_jacobian = KIN == kSmall ? 1.0 : team_min<TW>(min_jacobian);
+  max_abar = team_max<TW>(max_abar);
   const double a = avg[0] - avg[1], b = avg[1] - avg[2], c = avg[2] - avg[0];
   const double vm = sqrt(0.5 * (a * a + b * b + c * c) + 3.0 * (avg[3] * avg[3] + avg[4] * avg[4] + avg[5] * avg[5]));
   const double tail[4] = {vm, max_abar, max_strain, min_jacobian};
 #pragma unroll
   for (int i = 0; i < 24; ++i)
-    if ((i & 7) == p) o[i] = avg[i];
+    if ((i % TW) == p) o[i] = avg[i];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
-    if (((24 + i) & 7) == p) o[24 + i] = tail[i];
+    if (((24 + i) % TW) == p) o[24 + i] = tail[i];
   if (p == 0 && A.status) A.status[e] = kOk;
 }
 
@@ -1381,98 +1409,137 @@ int resident_waves() {
   return cached[dev];
 }
 
-template <int KIN, int NT>
+template <int KIN, int NT, int TW>
 void launch_local_solve_nt(const ElementArgs& a, unsigned int blocks, hipStream_t stream) {
   if (a.ordered) {
     const int deal = static_cast<long long>(blocks) < 4ll * resident_waves() ? 1 : 0;
-    const unsigned int teams = static_cast<unsigned int>((a.ne + kWaveTeams - 1) / kWaveTeams);
+    constexpr int kTeams = teams_per_wave<TW>();
+    const unsigned int teams = static_cast<unsigned int>((a.ne + kTeams - 1) / kTeams);
     const long long span = 2048;  // elements per block of the counting sort
     const unsigned int sort_blocks = static_cast<unsigned int>(std::min<long long>(1024, (a.ne + span - 1) / span));
-    hipLaunchKernelGGL((cost_key_kernel<KIN, NT>), dim3(teams), dim3(64), 0, stream, a);
+    hipLaunchKernelGGL((cost_key_kernel<KIN, NT, TW>), dim3(teams), dim3(64), 0, stream, a);
     hipLaunchKernelGGL(cost_count_kernel, dim3(sort_blocks), dim3(256), 0, stream, a.cost, a.ne, a.counts);
     hipLaunchKernelGGL(cost_scan_kernel, dim3(1), dim3(64), 0, stream, a.counts);
     hipLaunchKernelGGL(cost_scatter_kernel, dim3(sort_blocks), dim3(256), 0, stream, a.cost, a.ne, a.counts,
                        static_cast<long long>(blocks), deal, a.order);
   }
-  hipLaunchKernelGGL((local_solve_kernel<KIN, NT>), dim3(blocks), dim3(64), 0, stream, a);
+  hipLaunchKernelGGL((local_solve_kernel<KIN, NT, TW>), dim3(blocks), dim3(64), 0, stream, a);
 }
 
-template <int KIN>
+template <int KIN, int TW>
 void launch_local_solve_kin(const ElementArgs& a, bool two_terms, unsigned int blocks, hipStream_t stream) {
   if (two_terms)
-    launch_local_solve_nt<KIN, 2>(a, blocks, stream);
+    launch_local_solve_nt<KIN, 2, TW>(a, blocks, stream);
   else
-    launch_local_solve_nt<KIN, kMaxTerms>(a, blocks, stream);
+    launch_local_solve_nt<KIN, kMaxTerms, TW>(a, blocks, stream);
 }
+
+template <int TW>
+void launch_local_solve_tw(const ElementArgs& a, bool two, unsigned int blocks, hipStream_t stream) {
+  switch (a.kinematics) {
+    case kSmall: launch_local_solve_kin<kSmall, TW>(a, two, blocks, stream); break;
+    case kFinite: launch_local_solve_kin<kFinite, TW>(a, two, blocks, stream); break;
+    default: launch_local_solve_kin<kLog, TW>(a, two, blocks, stream); break;
+  }
+}
+
+// Lanes per element of a rule: 8 up to 8 points, 32 up to 32.
+int team_width(int points) { return points <= kTeam ? kTeam : kWideTeam; }
 
 int launch_local_solve(const ElementArgs& args, hipStream_t stream) {
   (void)hipGetLastError();
   ElementArgs a = args;
+  if (a.rule.n < 1 || a.rule.n > kMaxPoints) return static_cast<int>(hipErrorInvalidValue);
+  const int tw = team_width(a.rule.n);
+  const int teams = 64 / tw;
   if (a.chunk <= 0) {
-    // Elements per wavefront when the caller sets none: 64, or fewer for a
-    // launch too small to give every SIMD's wavefront slot two chunks (a small
-    // mesh would otherwise run on a few wavefronts, each working through 64
-    // elements in turn); a multiple of 8, at least 8.
+    // Elements per wavefront when the caller sets none: 64 (16 in 32-lane
+    // teams), or fewer for a launch too small to give every SIMD's wavefront
+    // slot two chunks (a small mesh would otherwise run on a few wavefronts,
+    // each working through its elements in turn); a multiple of the teams per
+    // wavefront, at least one per team.
+    const int full = tw == kTeam ? kDefaultChunk : kDefaultChunk / 4;
     const long long per = a.ne / (2ll * resident_waves());
-    a.chunk = per >= kDefaultChunk ? kDefaultChunk : (per < kWaveTeams ? kWaveTeams : static_cast<int>(per / kWaveTeams) * kWaveTeams);
+    a.chunk = per >= full ? full : (per < teams ? teams : static_cast<int>(per / teams) * teams);
   }
-  if (a.chunk % kWaveTeams != 0) return static_cast<int>(hipErrorInvalidValue);
+  if (a.chunk % teams != 0) return static_cast<int>(hipErrorInvalidValue);
   const unsigned int blocks = static_cast<unsigned int>((a.ne + a.chunk - 1) / a.chunk);
   if (blocks == 0) return 0;
   // the smallest instantiation that holds the material's kinematic history
   const int fields = a.mat.terms > a.mat.kin_fields ? a.mat.terms : a.mat.kin_fields;
   if (fields > kMaxTerms) return static_cast<int>(hipErrorInvalidValue);
   const bool two = fields <= 2;
-  switch (a.kinematics) {
-    case kSmall: launch_local_solve_kin<kSmall>(a, two, blocks, stream); break;
-    case kFinite: launch_local_solve_kin<kFinite>(a, two, blocks, stream); break;
-    default: launch_local_solve_kin<kLog>(a, two, blocks, stream); break;
-  }
+  if (tw == kTeam)
+    launch_local_solve_tw<kTeam>(a, two, blocks, stream);
+  else
+    launch_local_solve_tw<kWideTeam>(a, two, blocks, stream);
   return static_cast<int>(hipGetLastError());
 }
 
-template <int KIN>
+template <int KIN, int TW>
 void launch_condense_kin(const ElementArgs& a, bool tangent, unsigned int blocks, hipStream_t stream) {
   if (!tangent)
-    hipLaunchKernelGGL((condense_kernel<KIN, false, true>), dim3(blocks), dim3(64), 0, stream, a);
+    hipLaunchKernelGGL((condense_kernel<KIN, false, true, TW>), dim3(blocks), dim3(64), 0, stream, a);
   else if (a.mat.symmetric)
-    hipLaunchKernelGGL((condense_kernel<KIN, true, true>), dim3(blocks), dim3(64), 0, stream, a);
+    hipLaunchKernelGGL((condense_kernel<KIN, true, true, TW>), dim3(blocks), dim3(64), 0, stream, a);
   else
-    hipLaunchKernelGGL((condense_kernel<KIN, true, false>), dim3(blocks), dim3(64), 0, stream, a);
+    hipLaunchKernelGGL((condense_kernel<KIN, true, false, TW>), dim3(blocks), dim3(64), 0, stream, a);
 }
 
-template <int KIN>
+template <int KIN, int TW>
 void launch_stress_kin(const ElementArgs& a, bool two_terms, double* out, unsigned int blocks, hipStream_t stream) {
   if (two_terms)
-    hipLaunchKernelGGL((stress_kernel<KIN, 2>), dim3(blocks), dim3(64), 0, stream, a, out);
+    hipLaunchKernelGGL((stress_kernel<KIN, 2, TW>), dim3(blocks), dim3(64), 0, stream, a, out);
   else
-    hipLaunchKernelGGL((stress_kernel<KIN, kMaxTerms>), dim3(blocks), dim3(64), 0, stream, a, out);
+    hipLaunchKernelGGL((stress_kernel<KIN, kMaxTerms, TW>), dim3(blocks), dim3(64), 0, stream, a, out);
+}
+
+template <int TW>
+void launch_stress_tw(const ElementArgs& a, bool two, double* out, unsigned int blocks, hipStream_t stream) {
+  switch (a.kinematics) {
+    case kSmall: launch_stress_kin<kSmall, TW>(a, two, out, blocks, stream); break;
+    case kFinite: launch_stress_kin<kFinite, TW>(a, two, out, blocks, stream); break;
+    default: launch_stress_kin<kLog, TW>(a, two, out, blocks, stream); break;
+  }
 }
 
 int launch_element_stress(const ElementArgs& a, double* out, hipStream_t stream) {
   (void)hipGetLastError();
-  const unsigned int blocks = static_cast<unsigned int>((a.ne + kWaveTeams - 1) / kWaveTeams);
+  if (a.rule.n < 1 || a.rule.n > kMaxPoints) return static_cast<int>(hipErrorInvalidValue);
+  const int tw = team_width(a.rule.n);
+  const int teams = 64 / tw;
+  const unsigned int blocks = static_cast<unsigned int>((a.ne + teams - 1) / teams);
   if (blocks == 0) return 0;
   const int fields = a.mat.terms > a.mat.kin_fields ? a.mat.terms : a.mat.kin_fields;
-  if (fields > kMaxTerms || a.rule.n > kWaveTeams) return static_cast<int>(hipErrorInvalidValue);
+  if (fields > kMaxTerms) return static_cast<int>(hipErrorInvalidValue);
   const bool two = fields <= 2;
-  switch (a.kinematics) {
-    case kSmall: launch_stress_kin<kSmall>(a, two, out, blocks, stream); break;
-    case kFinite: launch_stress_kin<kFinite>(a, two, out, blocks, stream); break;
-    default: launch_stress_kin<kLog>(a, two, out, blocks, stream); break;
-  }
+  if (tw == kTeam)
+    launch_stress_tw<kTeam>(a, two, out, blocks, stream);
+  else
+    launch_stress_tw<kWideTeam>(a, two, out, blocks, stream);
   return static_cast<int>(hipGetLastError());
+}
+
+template <int TW>
+void launch_condense_tw(const ElementArgs& a, bool tangent, unsigned int blocks, hipStream_t stream) {
+  switch (a.kinematics) {
+    case kSmall: launch_condense_kin<kSmall, TW>(a, tangent, blocks, stream); break;
+    case kFinite: launch_condense_kin<kFinite, TW>(a, tangent, blocks, stream); break;
+    default: launch_condense_kin<kLog, TW>(a, tangent, blocks, stream); break;
+  }
 }
 
 int launch_condense(const ElementArgs& a, bool tangent, hipStream_t stream) {
   (void)hipGetLastError();
-  const unsigned int blocks = static_cast<unsigned int>((a.ne + kWaveTeams - 1) / kWaveTeams);
+  if (a.rule.n < 1 || a.rule.n > kMaxPoints) return static_cast<int>(hipErrorInvalidValue);
+  const int tw = team_width(a.rule.n);
+  const int teams = 64 / tw;
+  const unsigned int blocks = static_cast<unsigned int>((a.ne + teams - 1) / teams);
   if (blocks == 0) return 0;
-  switch (a.kinematics) {
-    case kSmall: launch_condense_kin<kSmall>(a, tangent, blocks, stream); break;
-    case kFinite: launch_condense_kin<kFinite>(a, tangent, blocks, stream); break;
-    default: launch_condense_kin<kLog>(a, tangent, blocks, stream); break;
-  }
+  if (tw == kTeam)
+    launch_condense_tw<kTeam>(a, tangent, blocks, stream);
+  else
+    launch_condense_tw<kWideTeam>(a, tangent, blocks, stream);
   return static_cast<int>(hipGetLastError());
 }
 

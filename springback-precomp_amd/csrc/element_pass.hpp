@@ -27,11 +27,6 @@ enum ElementStatus {
   kUnsupported = 10,       // the launch does not fit the kernel
 };
 
-struct Rule8_unused {
-  double xi[kMaxPoints], eta[kMaxPoints], zeta[kMaxPoints], w[kMaxPoints];
-  int n;
-};
-
 // The alpha-independent data of the lane's point.
 struct PointGeom {
   double g[3][11];  // [G | G~]

@@ -90,8 +90,18 @@ _lib = None
 
 
 def workspace_doubles(num_elements, points):
-    """Doubles of device scratch an element batch needs (sparlab_element_workspace_doubles)."""
-    return num_elements * points * (42 + 44) + num_elements * 108 + (2 * num_elements + 2 * 17 + 1) // 2
+    """Doubles of device scratch an element batch of `points`-point elements
+    needs (sparlab_element_workspace_doubles: 86 per point, 108 per element
+    and the local solve's schedule)."""
+    rule = {8: (2, 0, 2), 1: (1, 0, 2), 27: (3, 0, 2), 64: (4, 0, 2)}.get(points)
+    if rule is None:
+        if points % 4 != 0:
+            raise ValueError(f"no rule of {points} points")
+        rule = (2, points // 4, 2)
+    n = lib().sparlab_element_workspace_doubles(int(num_elements), ctypes.byref(CRule(*rule)))
+    if n < 0:
+        raise RuntimeError(f"sparlab_gpu error {n}: {lib().sparlab_last_error().decode()}")
+    return n
 
 
 def lib():

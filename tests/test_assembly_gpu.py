@@ -81,25 +81,29 @@ def jittered_mesh(nx, ny, nz, lx, ly, lz, seed):
     return coords, conn
 
 
-def test_system_evaluate_commit_and_history(device):
+@pytest.mark.parametrize("rule", [(2, 0, 2), (2, 5, 2), (2, 7, 2)], ids=["2x2x2", "2x2x5", "2x2x7"])
+def test_system_evaluate_commit_and_history(device, rule):
     """NonlinearSystem::evaluate / commit on a small plastic mesh vs the oracle
     element by element, the oracle's scatter and gather; two steps so the
-    second starts from a committed plastic history."""
+    second starts from a committed plastic history. The 2 x 2 x 2 rule and
+    the box rules through the thickness of the reference's fine-mesh decks
+    (2 x 2 x 5, e.g. benchmarks/physics_audit/challenge/cases/kin_h0833/deck/
+    deck.json, and 2 x 2 x 7)."""
     coords, conn = jittered_mesh(5, 4, 2, 0.01, 0.008, 0.002, seed=3)
     m = G.Material(70e9, 0.33, yield_stress=95e6, saturation_stress=150e6, saturation_rate=10.0,
                    hill=dict(r0=0.7, r45=0.6, r90=0.8), backstresses=((6000e6, 60.0), (1500e6, 10.0)))
     om = oracle_material(m)
-    sysm = G.NonlinearSystem(coords, conn, m)
+    sysm = G.NonlinearSystem(coords, conn, m, G.Rule(*rule))
     nn, ne = coords.shape[0], conn.shape[0]
     rng = np.random.default_rng(8)
-    states = np.zeros((ne, 8, 39))
+    states = np.zeros((ne, G.Rule(*rule).points, 39))
     alpha = np.zeros((ne, 9))
     for step in range(2):
         u = rng.normal(0, 2e-5, (nn, 3)) + (step + 1) * 1e-4 * coords[:, [2, 0, 1]]
         ev = sysm.evaluate(u, want_tangent=True, column_major=True)
         x0 = coords[conn]
         ue = u[conn].reshape(ne, 24)
-        ref = O.element_batch(x0, ue, om, states, alpha=alpha, kinematics=2, threads=4)
+        ref = O.element_batch(x0, ue, om, states, alpha=alpha, rule=rule, kinematics=2, threads=4)
         assert (ref["status"] == 0).all()
         f_ref, gross_ref = O.gather_force(nn, conn, ref["force"])
         k_ref = O.scatter(nn, conn, ref["tangent"].reshape(ne, 576))
@@ -118,7 +122,7 @@ def test_system_evaluate_commit_and_history(device):
         states, alpha = ref["states"], ref["alpha"]
     assert ev.yielding_points > 0
     # restart: set the history of a fresh system to the committed one
-    other = G.NonlinearSystem(coords, conn, m)
+    other = G.NonlinearSystem(coords, conn, m, G.Rule(*rule))
     other.set_history(committed, committed_alpha)
     s2, a2 = other.history()
     assert np.array_equal(a2, committed_alpha)

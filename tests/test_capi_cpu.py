@@ -53,11 +53,11 @@ def test_material_derivation_matches_oracle():
 
 
 def test_rules_and_state_layout():
-    for rule in ((2, 0, 2), (2, 2, 2), (1, 0, 2), (2, 1, 2), (2, 1, 0)):
+    for rule in ((2, 0, 2), (2, 2, 2), (1, 0, 2), (2, 1, 2), (2, 1, 0), (2, 5, 2), (2, 7, 2), (2, 7, 0), (3, 0, 2)):
         r = G.Rule(*rule)
         assert r.points == O.rule_points(*rule)
     with pytest.raises(RuntimeError):
-        G.Rule(2, 5, 2).points            # 20 points: more than the kernels take
+        G.Rule(4, 0, 2).points            # 64 points: more than the kernels take
     with pytest.raises(RuntimeError):
         G.Rule(5, 0, 2).points
     m = W.microbench_material()

@@ -98,44 +98,50 @@ def single_hex_batch(scales, extra_z=0.2):
 
 @pytest.mark.parametrize("law", ["elastic", "J2", "Hill48", "Chaboche", "ChabocheBE", "Chaboche4"])
 @pytest.mark.parametrize("kinematics", [0, 1, 2])
-def test_single_hex_laws(law, kinematics):
+@pytest.mark.parametrize("box", [(2, 0, 2), (2, 5, 2)], ids=["2x2x2", "2x2x5"])
+def test_single_hex_laws(law, kinematics, box):
     """The laws x kinematics of test_incompatible_modes.cpp:433-508, from a
-    committed first step as there, at several load levels."""
+    committed first step as there, at several load levels; the 2 x 2 x 2 rule
+    and the 2 x 2 x 5 box rule of the reference's fine-mesh decks."""
     m = materials()[law]
-    rule = G.Rule(2, 0, 2)
+    rule = G.Rule(*box)
     base = {0: 4.0e-3, 1: 0.02, 2: 0.1}[kinematics] if law != "elastic" else {0: 1e-3, 1: 0.05, 2: 0.05}[kinematics]
     scales = base * np.array([0.25, 0.5, 1.0, 1.5])
     om = oracle_material(m)
     ne = len(scales)
     # committed states: the first step at deformation(x, s), as the reference test
-    states = np.zeros((ne, 8, 39))
+    states = np.zeros((ne, rule.points, 39))
     alpha = np.zeros((ne, 9))
     for i, s in enumerate(scales):
-        first = O.element(DISTORTED, deformation(DISTORTED, s), om, kinematics=kinematics)
+        first = O.element(DISTORTED, deformation(DISTORTED, s), om, rule=box, kinematics=kinematics)
         if m.c.yield_stress > 0:
             states[i] = first["states"]
         alpha[i] = first["alpha"]
     coords, conn, u = single_hex_batch(scales)
     x0 = coords[conn]
     ue = u[conn].reshape(ne, 24)
-    ref = O.element_batch(x0, ue, om, states, alpha=alpha, kinematics=kinematics, threads=4)
+    ref = O.element_batch(x0, ue, om, states, alpha=alpha, rule=box, kinematics=kinematics, threads=4)
     gpu = run_gpu(m, rule, coords, conn, u, states, alpha=alpha, kinematics=kinematics)
     assert (ref["status"] == 0).all()
-    compare(gpu, ref, f"{law}/{kinematics}")
+    compare(gpu, ref, f"{law}/{kinematics}/{box}")
 
 
-def test_microbench_sample_parity():
+@pytest.mark.parametrize("box", [(2, 0, 2), (2, 5, 2), (2, 7, 2)], ids=["2x2x2", "2x2x5", "2x2x7"])
+def test_microbench_sample_parity(box):
     """A 4096-element sample of the element microbenchmark (Hill48 + Chaboche,
-    logarithmic strain), tangent and residual-only."""
+    logarithmic strain), tangent and residual-only; with the 2 x 2 x 2 rule
+    and the 2 x 2 x 5 and 2 x 2 x 7 box rules (their 20 and 28 points, each
+    with a committed state drawn as the microbenchmark draws them)."""
     m = W.microbench_material()
     om = oracle_material(m)
-    w = W.microbench(4096, seed=7)
+    rule = G.Rule(*box)
+    w = W.microbench(4096, seed=7, points=rule.points)
     x0, ue = W.element_arrays(w)
-    ref = O.element_batch(x0, ue, om, w["states"], kinematics=2, threads=8)
-    gpu = run_gpu(m, G.Rule(), w["coords"], w["conn"], w["u"], w["states"], kinematics=2)
-    res = compare(gpu, ref, "microbench")
-    print("microbench parity", res)
-    gpu_r = run_gpu(m, G.Rule(), w["coords"], w["conn"], w["u"], w["states"], kinematics=2, want_tangent=False)
+    ref = O.element_batch(x0, ue, om, w["states"], rule=box, kinematics=2, threads=8)
+    gpu = run_gpu(m, rule, w["coords"], w["conn"], w["u"], w["states"], kinematics=2)
+    res = compare(gpu, ref, f"microbench {box}")
+    print("microbench parity", box, res)
+    gpu_r = run_gpu(m, rule, w["coords"], w["conn"], w["u"], w["states"], kinematics=2, want_tangent=False)
     assert np.array_equal(gpu_r["force"], gpu["force"]), "residual-only force differs from the tangent pass"
 
 

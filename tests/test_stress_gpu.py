@@ -35,9 +35,11 @@ def jittered_mesh(seed):
 
 @pytest.mark.parametrize("law", sorted(LAWS))
 @pytest.mark.parametrize("kinematics", [0, 1, 2])
-def test_stress_matches_oracle(device, law, kinematics):
+@pytest.mark.parametrize("box", [(2, 0, 2), (2, 7, 2)], ids=["2x2x2", "2x2x7"])
+def test_stress_matches_oracle(device, law, kinematics, box):
     import torch
     m = LAWS[law]()
+    npts = G.Rule(*box).points
     om = oracle_material(m)
     coords, conn = jittered_mesh(4)
     nn, ne = coords.shape[0], conn.shape[0]
@@ -45,13 +47,13 @@ def test_stress_matches_oracle(device, law, kinematics):
     u = rng.normal(0, 3e-6, (nn, 3)) + 2e-2 * coords[:, [2, 0, 1]]
     x0 = coords[conn]
     ue = u[conn].reshape(ne, 24)
-    ev = O.element_batch(x0, ue, om, np.zeros((ne, 8, O.STATE)), kinematics=kinematics, want_tangent=False,
-                         threads=8)
+    ev = O.element_batch(x0, ue, om, np.zeros((ne, npts, O.STATE)), rule=box, kinematics=kinematics,
+                         want_tangent=False, threads=8)
     assert (ev["status"] == 0).all() and (ev["scalars"][:, 1] > 0).any()
     states, alpha = ev["states"], ev["alpha"]
-    ref = O.element_stress_batch(x0, ue, om, states, alpha, kinematics=kinematics, threads=8)
+    ref = O.element_stress_batch(x0, ue, om, states, alpha, rule=box, kinematics=kinematics, threads=8)
     assert (ref["status"] == 0).all()
-    sysm = G.NonlinearSystem(coords, conn, m, G.Rule(2, 0, 2), kinematics)
+    sysm = G.NonlinearSystem(coords, conn, m, G.Rule(*box), kinematics)
     sysm.set_history(states, alpha)
     stress, status = sysm.stress_device(torch.as_tensor(u.reshape(-1), device=device))
     torch.cuda.synchronize()

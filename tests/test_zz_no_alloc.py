@@ -48,7 +48,9 @@ def test_noalloc_refuses_a_batch_beyond_its_scratch():
     import ctypes
     m = W.microbench_material()
     rule = G.Rule()
-    ne = G.NOALLOC_SCRATCH_BYTES // (8 * G.workspace_doubles(1, 8)) + 1000
+    per_element = G.workspace_doubles(2, 8) - G.workspace_doubles(1, 8)
+    ne = G.NOALLOC_SCRATCH_BYTES // (8 * per_element) + 1000
+    assert 8 * G.workspace_doubles(ne, 8) > G.NOALLOC_SCRATCH_BYTES
     fake = ctypes.c_void_p(4096)
     b = G.CElementBatch(fake, fake, fake, 8 * ne, ne, fake, None, fake, fake, fake, fake, fake, fake, fake, fake,
                         fake, None)
