@@ -379,6 +379,41 @@ int sparlab_system_commit(sparlab_system* system);
  * element); set_committed / set_internal. Host arrays; null skips. */
 int sparlab_system_history(const sparlab_system* system, int trial, double* states, double* alpha);
 int sparlab_system_set_history(sparlab_system* system, const double* states, const double* alpha);
+/* What NonlinearSystem answers about itself (src/fem/NonlinearSystem.hpp:
+ * 104-114, 273-311): symmetric() (no recovering backstress), plastic() (the
+ * material yields), elastoplastic(e) and averaged(e), the same for every
+ * element here (one material; the incompatible-mode Hex8 averages no
+ * dilatation), and the layout of the history. */
+typedef struct sparlab_system_info {
+  int64_t num_nodes;
+  int64_t num_elements;
+  int64_t nnz;
+  int32_t points;            /* integration points per element */
+  int32_t state_fields;      /* doubles of history per point (sparlab_state_fields) */
+  int32_t kinematic_fields;  /* 6-vectors of kinematic history per point */
+  int32_t general;           /* 1: they are the general return's backstresses alpha_i; 0: the radial return's beta */
+  int32_t symmetric;         /* NonlinearSystem::symmetric() */
+  int32_t plastic;           /* NonlinearSystem::plastic() */
+  int32_t elastoplastic;     /* NonlinearSystem::elastoplastic(e), every e */
+  int32_t averaged;          /* NonlinearSystem::averaged(e), every e: 0 */
+  int32_t modes;             /* internal-mode parameters per element: 9 */
+  int32_t kinematics;        /* SPARLAB_SMALL_STRAIN, _FINITE or _FINITE_LOGARITHMIC */
+} sparlab_system_info;
+int sparlab_system_info_get(const sparlab_system* system, sparlab_system_info* out);
+
+/* NonlinearSystem::max_plastic_strain (src/fem/NonlinearSystem.hpp:181-188):
+ * the largest equivalent plastic strain of the committed history, reduced on
+ * the device (0 for an elastic material). Synchronous on `stream`. */
+int sparlab_system_max_plastic_strain(sparlab_system* system, double* out, void* stream);
+
+/* committed(e) / internal(e) (trial = 0), or the last evaluation's states and
+ * modes (Evaluation::states / modes, trial = 1), for elements first ..
+ * first + count - 1 (src/fem/NonlinearSystem.hpp:39-46, 108-145): states per
+ * element, point and field ([e - first][q][f], state_fields per point, the
+ * library's fields), alpha 9 per element. Host arrays; null skips. */
+int sparlab_system_history_range(const sparlab_system* system, int trial, int64_t first, int64_t count,
+                                 double* states, double* alpha);
+
 /* The last evaluation's element forces (24 per element) and tangents (576). */
 int sparlab_system_element_outputs(const sparlab_system* system, double* fe, double* ke);
 /* The same for elements first .. first + count - 1 (host arrays of 24 and

@@ -53,6 +53,28 @@ def build(verbose=False):
     return LIB
 
 
+ADAPTER_TEST = os.path.join(ROOT, "tests", "cpp", "_build", "adapter_sequence")
+
+
+def build_adapter_test(verbose=False):
+    """The C++ driver of tests/cpp/adapter_sequence.cpp (the forming driver's
+    calls through include/sparlab_gpu_system.hpp), linked to the library."""
+    os.makedirs(os.path.dirname(ADAPTER_TEST), exist_ok=True)
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(HIPCC)))
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(rocm, "include"),
+           os.path.join(ROOT, "tests", "cpp", "adapter_sequence.cpp"), "-o", ADAPTER_TEST,
+           "-L", LIBDIR, "-lsparlab_gpu", "-L", os.path.join(rocm, "lib"), "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/../../../springback-precomp_amd/lib", "-Wl,-rpath," + os.path.join(rocm, "lib")]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"the adapter test driver did not build:\n{res.stderr}")
+    if verbose:
+        print(f"built {ADAPTER_TEST}")
+    return ADAPTER_TEST
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_adapter_test(verbose=True)
     sys.exit(0)

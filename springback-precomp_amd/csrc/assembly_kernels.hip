@@ -288,6 +288,48 @@ int launch_reduce(const double* x, long long n, int square, double* partial, int
   return static_cast<int>(hipGetLastError());
 }
 
+// The largest of x[i * stride] over i in [0, n) (max is exact in any order):
+// per-block maxima, then one block over them; 0 for n = 0 (the values are
+// non-negative: equivalent plastic strains).
+__global__ __launch_bounds__(256) void max_partials_kernel(const double* __restrict__ x, long long n,
+                                                           double* __restrict__ partial) {
+  __shared__ double s[256];
+  const long long per = (n + gridDim.x - 1) / gridDim.x;
+  const long long b0 = static_cast<long long>(blockIdx.x) * per;
+  const long long b1 = b0 + per < n ? b0 + per : n;
+  double acc = 0.0;
+  for (long long i = b0 + threadIdx.x; i < b1; i += blockDim.x) acc = fmax(acc, x[i]);
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(256) void max_kernel(const double* __restrict__ x, int n, double* __restrict__ out) {
+  __shared__ double s[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc = fmax(acc, x[i]);
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = s[0];
+}
+
+int launch_reduce_max(const double* x, long long n, double* partial, int nparts, double* out, hipStream_t s) {
+  (void)hipGetLastError();
+  if (nparts < 1) nparts = 1;
+  if (nparts > 1024) nparts = 1024;
+  hipLaunchKernelGGL(max_partials_kernel, dim3(nparts), dim3(256), 0, s, x, n, partial);
+  hipLaunchKernelGGL(max_kernel, dim3(1), dim3(256), 0, s, partial, nparts, out);
+  return static_cast<int>(hipGetLastError());
+}
+
 int launch_status_summary(const int* status, const int* yielding, long long n, long long* summary, hipStream_t s) {
   (void)hipGetLastError();
   hipLaunchKernelGGL(status_init_kernel, dim3(1), dim3(64), 0, s, summary);

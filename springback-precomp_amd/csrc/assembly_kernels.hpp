@@ -51,6 +51,8 @@ int launch_stiffness_gross(const AbsProductArgs& a, hipStream_t s);
 int launch_gather_force(const ForceArgs& a, hipStream_t s);
 // out[0] = sum x (square = 0) or sqrt(sum x^2) (square = 1), fixed-order tree
 // over nparts contiguous slices (nparts <= 1024 partial sums).
+// out[0] = max(0, x[0..n)) by per-block maxima (nparts blocks, at most 1024).
+int launch_reduce_max(const double* x, long long n, double* partial, int nparts, double* out, hipStream_t s);
 int launch_reduce(const double* x, long long n, int square, double* partial, int nparts, double* out,
                   hipStream_t s);
 // summary[0] failed elements, [1] lowest failed element, [2] yielding points

@@ -156,7 +156,7 @@ int sparlab_system_create(const double* coords, int64_t num_nodes, const int32_t
             dalloc(s->iterations, ne) && dalloc(s->status, ne) &&
             dalloc(s->ws, static_cast<std::size_t>(sbk::workspace_doubles(num_elements, s->points))) &&
             dalloc(s->partial, 1024) &&
-            dalloc(s->scalars, 2) && dalloc(s->summary, 3) && upload(s->row_ptr, s->pattern.row_ptr) &&
+            dalloc(s->scalars, 3) && dalloc(s->summary, 3) && upload(s->row_ptr, s->pattern.row_ptr) &&
             upload(s->inc_pos, s->pattern.inc_pos) && upload(s->node_ptr, s->pattern.node_ptr) &&
             upload(s->node_incidence, s->pattern.node_incidence) && upload(s->diag_block, s->pattern.diag_block) &&
             upload(s->block_col, s->pattern.block_col_node) && upload(s->block_mirror, s->pattern.block_mirror) &&
@@ -630,6 +630,70 @@ int sparlab_system_element_outputs(const sparlab_system* s, double* fe, double* 
   if (hipDeviceSynchronize() != hipSuccess) return sbk::capi_fail(SPARLAB_E_DEVICE, "synchronize failed");
   if ((fe && hipMemcpy(fe, s->fe, 24 * s->ne * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) ||
       (ke && hipMemcpy(ke, s->ke, 576 * s->ne * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "download failed");
+  return SPARLAB_OK;
+}
+
+int sparlab_system_info_get(const sparlab_system* s, sparlab_system_info* out) {
+  if (!s || !out) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  *out = sparlab_system_info{};
+  out->num_nodes = s->nn;
+  out->num_elements = s->ne;
+  out->nnz = static_cast<int64_t>(s->pattern.col_idx.size());
+  out->points = s->points;
+  out->state_fields = s->fields;
+  out->kinematic_fields = s->args.mat.kin_fields;
+  out->general = s->args.mat.general;
+  // NonlinearSystem::initialise_history (src/fem/NonlinearSystem.hpp:273-311):
+  // one material, so every element is elastoplastic or none is; a recovering
+  // backstress makes the tangent non-symmetric; the incompatible-mode element
+  // averages no dilatation.
+  out->plastic = s->args.mat.enabled;
+  out->elastoplastic = s->args.mat.enabled;
+  out->symmetric = s->args.mat.symmetric;
+  out->averaged = 0;
+  out->modes = 9;
+  out->kinematics = s->kinematics;
+  return SPARLAB_OK;
+}
+
+int sparlab_system_max_plastic_strain(sparlab_system* s, double* out, void* stream) {
+  if (!s || !out) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  *out = 0.0;
+  if (!s->args.mat.enabled || s->ne == 0) return SPARLAB_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const long long npts = static_cast<long long>(s->ne) * s->points;
+  const double* abar = s->state[s->cur] + static_cast<std::size_t>(6 + 6 * s->args.mat.kin_fields) * npts;
+  const int parts = static_cast<int>(std::min<long long>(1024, npts / 4096 + 1));
+  if (sbk::launch_reduce_max(abar, npts, s->partial, parts, s->scalars + 2, st) != 0 ||
+      hipMemcpyAsync(out, s->scalars + 2, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "the plastic strain maximum failed");
+  return SPARLAB_OK;
+}
+
+int sparlab_system_history_range(const sparlab_system* s, int trial, int64_t first, int64_t count, double* states,
+                                 double* alpha) {
+  if (!s) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null system");
+  if (first < 0 || count < 0 || first > s->ne || count > s->ne - first)
+    return sbk::capi_fail(SPARLAB_E_ARGUMENT, "the element range lies outside the mesh");
+  if (count == 0) return SPARLAB_OK;
+  const int k = trial ? 1 - s->cur : s->cur;
+  const std::size_t npts = static_cast<std::size_t>(s->ne) * s->points;
+  const std::size_t width = static_cast<std::size_t>(count) * s->points;
+  if (hipDeviceSynchronize() != hipSuccess) return sbk::capi_fail(SPARLAB_E_DEVICE, "synchronize failed");
+  if (states) {
+    // field f of the range's points is one contiguous run of the field's row
+    std::vector<double> rows(static_cast<std::size_t>(s->fields) * width);
+    if (hipMemcpy2D(rows.data(), width * sizeof(double), s->state[k] + static_cast<std::size_t>(first) * s->points,
+                    npts * sizeof(double), width * sizeof(double), static_cast<std::size_t>(s->fields),
+                    hipMemcpyDeviceToHost) != hipSuccess)
+      return sbk::capi_fail(SPARLAB_E_DEVICE, "download failed");
+    for (std::size_t q = 0; q < width; ++q)
+      for (int f = 0; f < s->fields; ++f) states[q * s->fields + f] = rows[static_cast<std::size_t>(f) * width + q];
+  }
+  if (alpha && hipMemcpy(alpha, s->alpha[k] + 9 * first, 9 * static_cast<std::size_t>(count) * sizeof(double),
+                         hipMemcpyDeviceToHost) != hipSuccess)
     return sbk::capi_fail(SPARLAB_E_DEVICE, "download failed");
   return SPARLAB_OK;
 }
