@@ -117,6 +117,66 @@ def cpu_single_core_leg(workload, sample):
     return dict(evals_per_s=sample / dt, us_per_eval=dt / sample * 1e6, seconds=dt, core=cpus[0])
 
 
+def box_rule_leg(G, W, torch, dev, steps, ne=262144):
+    """The element microbenchmark with the 2 x 2 x 5 box rule of the
+    reference's fine-mesh decks (20 points per element, 32-lane teams):
+    `ne` elements, device evals/s and the algorithmic FLOP rate (op-counting
+    oracle on 200 elements), beside the oracle on one host core (the figure
+    comparable with the reference's 265 us per 20-point log-strain evaluation,
+    docs/forming.md:470)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from parity import oracle_material
+    box = (2, 5, 2)
+    rule = G.Rule(*box)
+    m = W.microbench_material()
+    w = W.microbench(ne, seed=20241204, points=rule.points)
+    st = G.states_from_reference(w["states"], m)
+    t = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    b = G.ElementBatch(t(w["coords"]), t(w["conn"], torch.int32), t(w["u"]), t(st), None, rule.points, True)
+    del st
+    stream = torch.cuda.current_stream(dev)
+    G.evaluate_elements(m, rule, b, G.FINITE_LOGARITHMIC, True, stream)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for k in range(steps):
+        ev[k][0].record(stream)
+        G.evaluate_elements(m, rule, b, G.FINITE_LOGARITHMIC, True, stream, stages=G.LOCAL_SOLVE)
+        ev[k][1].record(stream)
+        G.evaluate_elements(m, rule, b, G.FINITE_LOGARITHMIC, True, stream, stages=G.CONDENSATION)
+        ev[k][2].record(stream)
+    torch.cuda.synchronize()
+    ms_local = sum(e[0].elapsed_time(e[1]) for e in ev) / steps
+    ms_cond = sum(e[1].elapsed_time(e[2]) for e in ev) / steps
+    ms = ms_local + ms_cond
+    failures = int((b.status != 0).sum().item())
+    plastic = float((b.yielding > 0).double().mean().item())
+    om = oracle_material(m)
+    x0 = w["coords"][w["conn"][:2000]]
+    ue = w["u"][w["conn"][:2000]].reshape(2000, 24)
+    sts = w["states"][:2000]
+    parts = np.array([O.element_flops_split(x0[i], ue[i], om, states=sts[i], rule=box, kinematics=2)
+                      for i in range(200)], dtype=np.float64)
+    cpus = sorted(os.sched_getaffinity(0))
+    os.sched_setaffinity(0, {cpus[0]})
+    try:
+        t0 = time.perf_counter()
+        O.element_batch(x0, ue, om, sts, rule=box, kinematics=2, want_tangent=True, threads=1)
+        one = (time.perf_counter() - t0) / 2000 * 1e6
+    finally:
+        os.sched_setaffinity(0, set(cpus))
+    flops = float(parts[:, 0].mean())
+    return {"workload": f"element microbench with the 2 x 2 x 5 box rule: {ne} elements, 20 points each "
+                        "(32-lane teams), log strain, Hill48 + 2 Chaboche + Voce",
+            "elements": ne, "integration_points": rule.points, "ms": ms, "local_solve_ms": ms_local,
+            "condensation_ms": ms_cond, "evals_per_s": ne / (ms * 1e-3), "failures": failures,
+            "plastic_fraction": plastic, "flops_per_elem": flops, "flops_sample": 200,
+            "achieved_TFLOPs": flops * ne / (ms * 1e-3) / 1e12,
+            "frac": flops * ne / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+            "cpu_single_core_us_per_eval": one}
+
+
 def batched_leg(G, W, torch, dev, steps):
     """The batched form beside the plain drop-in on the 64-problem forming
     workload (workloads.batched_problems: 4 part families, 2 000-40 000 Hex8
@@ -442,6 +502,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation leg")
     ap.add_argument("--no-assembly", action="store_true", help="skip the assembly + contact leg")
+    ap.add_argument("--no-box", action="store_true", help="skip the 2 x 2 x 5 box-rule leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain MIB-MiB device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -541,10 +602,19 @@ def main():
               f"FLOP/elem ({fl:.0f} local + {fc:.0f} condensation); oracle plastic fraction "
               f"{cb['plastic_fraction']:.3f}", file=sys.stderr)
 
+    del batch
+    torch.cuda.empty_cache()
+    box = None
+    if not args.no_box:
+        box = box_rule_leg(G, W, torch, dev, max(1, min(args.steps, 5)))
+        torch.cuda.empty_cache()
+        print(f"# 2 x 2 x 5 rule: {box['elements']} elements, {box['ms']:.2f} ms ({box['evals_per_s'] / 1e6:.2f} M evals/s, "
+              f"{box['achieved_TFLOPs']:.2f} TFLOP/s, {box['frac'] * 100:.1f} % of FP64 peak; local solve "
+              f"{box['local_solve_ms']:.2f} ms, condensation {box['condensation_ms']:.2f} ms); oracle on one core "
+              f"{box['cpu_single_core_us_per_eval']:.1f} us/eval", file=sys.stderr, flush=True)
+
     batched = None
     if not args.no_batched:
-        del batch
-        torch.cuda.empty_cache()
         batched = batched_leg(G, W, torch, dev, max(1, min(args.steps, 5)))
         print(f"# batched form: {batched['elements']} elements in {batched['problems']} problems; plain "
               f"{batched['plain_ms']:.2f} ms, batched {batched['batched_ms']:.2f} ms per evaluation of all "
@@ -585,7 +655,7 @@ def main():
                                "log strain, Hill48 + 2 Chaboche + Voce, f_int + K_t + trial history",
                    "elements": ne, "integration_points": npts, "plastic_fraction": plastic,
                    "mean_local_iterations": mean_iters, "failures": failures},
-        "roofline": roof, "cpu_baseline": cpu, "batched": batched, "differentiation": adjoint,
+        "roofline": roof, "cpu_baseline": cpu, "box_rule": box, "batched": batched, "differentiation": adjoint,
         "assembly": assembly,
     }
     print(json.dumps(line))
