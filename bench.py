@@ -241,6 +241,111 @@ def batched_leg(G, W, torch, dev, steps):
             "yielding_points": int(sc["yielding"].sum())}
 
 
+def rim_free_dofs(coords):
+    """Every DOF but those of the rim's nodes (a sheet clamped at its edge)."""
+    x, y = coords[:, 0], coords[:, 1]
+    rim = np.isclose(x, x.min()) | np.isclose(x, x.max()) | np.isclose(y, y.min()) | np.isclose(y, y.max())
+    return np.nonzero(np.repeat(~rim, 3))[0].astype(np.int32)
+
+
+def newton_leg(G, W, torch, dev, tolerance=1e-8, max_iterations=20000):
+    """One Newton iteration of all 64 forming problems (the batched form's
+    workload, rims clamped): evaluation (f_int + row-major K_t), the free-free
+    block, r = f_int on the free DOFs, the block solve K_ff du = -r (BiCGStab:
+    the Hill48 + Chaboche tangent is not symmetric) to `tolerance`, u += du.
+    Plain: each problem's own system, one after another. Batched: the union
+    evaluated in its captured graph and every problem's solve in one
+    segmented solve. Each problem's du must equal its plain solve's bit for
+    bit (x, iterations and residual: the same sums in the same order)."""
+    probs = W.batched_problems()
+    m = W.microbench_material()
+    rule = G.Rule(2, 0, 2)
+    stream = torch.cuda.current_stream(dev)
+    frees = [rim_free_dofs(p["coords"]) for p in probs]
+    # plain
+    plain = []
+    for p, fr in zip(probs, frees):
+        s = G.NonlinearSystem(p["coords"], p["conn"], m, rule)
+        u0 = torch.as_tensor(p["u"].reshape(-1), device=dev)
+        d_free = torch.as_tensor(fr, device=dev)
+        nnz = s.partition_device(d_free)
+        plain.append(dict(s=s, u0=u0, u=u0.clone(), f=torch.zeros_like(u0),
+                          v=torch.zeros(s.nnz, dtype=torch.float64, device=dev), free=d_free.long(),
+                          blk=torch.empty(max(nnz, 1), dtype=torch.float64, device=dev),
+                          x=torch.zeros(fr.size, dtype=torch.float64, device=dev)))
+
+    def plain_iteration():
+        its = []
+        for q in plain:
+            q["u"].copy_(q["u0"])
+            q["s"].evaluate_device(q["u"], q["f"], q["v"], column_major=False, stream=stream)
+            q["s"].result(stream)
+            q["s"].free_block_device(q["v"], out=q["blk"], stream=stream)
+            rhs = -q["f"][q["free"]]
+            try:
+                _, it, _ = q["s"].solve_free_block_device(q["blk"], rhs, x=q["x"], tolerance=tolerance,
+                                                          max_iterations=max_iterations, symmetric=False,
+                                                          stream=stream)
+            except G.SolverError:
+                it = -1
+            q["u"][q["free"]] += q["x"]
+            its.append(it)
+        return its
+
+    plain_iteration()  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_its = plain_iteration()
+    torch.cuda.synchronize()
+    ms_plain = (time.perf_counter() - t0) * 1e3
+    plain_x = [q["x"].cpu().numpy() for q in plain]
+    del plain
+    torch.cuda.empty_cache()
+    # batched
+    b = G.BatchedSystem([(p["coords"], p["conn"]) for p in probs], m, rule)
+    u0 = torch.as_tensor(np.concatenate([p["u"] for p in probs]).reshape(-1), device=dev)
+    u = u0.clone()
+    fint = torch.zeros_like(u)
+    values = torch.zeros(b.system.nnz, dtype=torch.float64, device=dev)
+    free = np.concatenate([f + 3 * b.node_offset[i] for i, f in enumerate(frees)]).astype(np.int32)
+    d_free = torch.as_tensor(free, device=dev)
+    nnz = b.system.partition_device(d_free)
+    blk = torch.empty(max(nnz, 1), dtype=torch.float64, device=dev)
+    x = torch.zeros(free.size, dtype=torch.float64, device=dev)
+    rows = b.free_segments(free)
+    lfree = d_free.long()
+    b.autotune(u, stream)
+
+    def batched_iteration():
+        u.copy_(u0)
+        b.evaluate_device(u, fint, values, column_major=False, stream=stream)
+        sc = b.result(stream)
+        b.system.free_block_device(values, out=blk, stream=stream)
+        rhs = -fint[lfree]
+        _, its, _, status = b.system.solve_free_block_segments_device(blk, rhs, rows, x=x, tolerance=tolerance,
+                                                                      max_iterations=max_iterations,
+                                                                      symmetric=False, stream=stream)
+        u[lfree] += x
+        return its, status, sc
+
+    batched_iteration()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its, status, sc = batched_iteration()
+    torch.cuda.synchronize()
+    ms_batched = (time.perf_counter() - t0) * 1e3
+    xb = x.cpu().numpy()
+    identical = all(np.array_equal(xb[rows[p]:rows[p + 1]], plain_x[p]) for p in range(len(probs))) and \
+        [int(i) if st == 0 else -1 for i, st in zip(its, status)] == [int(i) for i in plain_its]
+    return {"workload": "one Newton iteration of the 64 forming problems (rims clamped): evaluation, free-free "
+                        f"block, BiCGStab to {tolerance:g}, update",
+            "problems": len(probs), "free_dofs": int(free.size), "tolerance": tolerance,
+            "plain_ms": ms_plain, "batched_ms": ms_batched, "speedup": ms_plain / ms_batched,
+            "iterations_min": int(np.min(its)), "iterations_median": float(np.median(its)),
+            "iterations_max": int(np.max(its)), "failed_solves": int((status != 0).sum()),
+            "failed_elements": int(sc["failed"].sum()), "identical_to_plain": bool(identical)}
+
+
 def differentiation_leg(G, W, torch, dev):
     """The differentiation form on the 200 x 250 x 4 sheet mid-way through a
     spiral incremental-forming path: the forward evaluation and the adjoint
@@ -503,6 +608,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation leg")
     ap.add_argument("--no-assembly", action="store_true", help="skip the assembly + contact leg")
     ap.add_argument("--no-box", action="store_true", help="skip the 2 x 2 x 5 box-rule leg")
+    ap.add_argument("--no-newton", action="store_true", help="skip the batched Newton-iteration leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain MIB-MiB device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -647,6 +753,17 @@ def main():
               f"({assembly['block_solve']['us_per_iteration']:.1f} us each, {assembly['block_solve']['GBps']:.0f} GB/s)",
               file=sys.stderr, flush=True)
 
+    newton = None
+    if not args.no_newton:
+        torch.cuda.empty_cache()
+        newton = newton_leg(G, W, torch, dev)
+        torch.cuda.empty_cache()
+        print(f"# batched Newton iteration: {newton['problems']} problems, {newton['free_dofs']} free DOFs; plain "
+              f"{newton['plain_ms']:.1f} ms, batched {newton['batched_ms']:.1f} ms ({newton['speedup']:.2f}x); "
+              f"BiCGStab iterations {newton['iterations_min']}..{newton['iterations_max']} (median "
+              f"{newton['iterations_median']:.0f}); identical to plain: {newton['identical_to_plain']}",
+              file=sys.stderr, flush=True)
+
     line = {
         "metric": METRIC, "value": value, "unit": "element evals/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -655,7 +772,7 @@ def main():
                                "log strain, Hill48 + 2 Chaboche + Voce, f_int + K_t + trial history",
                    "elements": ne, "integration_points": npts, "plastic_fraction": plastic,
                    "mean_local_iterations": mean_iters, "failures": failures},
-        "roofline": roof, "cpu_baseline": cpu, "box_rule": box, "batched": batched, "differentiation": adjoint,
+        "roofline": roof, "cpu_baseline": cpu, "box_rule": box, "batched": batched, "newton": newton, "differentiation": adjoint,
         "assembly": assembly,
     }
     print(json.dumps(line))

@@ -342,6 +342,26 @@ int sparlab_system_free_block_device(sparlab_system* system, const double* value
 int sparlab_system_solve_free_block_device(sparlab_system* system, const double* block_values, int symmetric,
                                            const double* b, double* x, double tolerance, int64_t max_iterations,
                                            int64_t* iterations, double* relative_residual, void* stream);
+/* The batched form's block solve (the reference's per-problem
+ * PartitionedFactor::factorize and solve, src/fem/Forming.cpp:104-172, for
+ * every problem of a batch at once): the partition's block is block-diagonal
+ * over num_segments problems, rows [segment_rows[p], segment_rows[p + 1])
+ * (host, num_segments + 1 ascending offsets from 0 to num_free; a block row
+ * with a column outside its segment is refused, SPARLAB_E_ARGUMENT). Each
+ * problem is solved as sparlab_system_solve_free_block_device solves its rows
+ * alone (the same block cut, sums, scalars, reads and restarts: the same
+ * iterates bit for bit), all in one set of launches per iteration; a problem
+ * that has converged or failed is left as it is. Per problem (host arrays of
+ * num_segments): iterations, relative_residual, and status (SPARLAB_OK, or
+ * SPARLAB_E_SOLVER where that problem's plain solve would fail).
+ * SPARLAB_E_SOLVER when any problem failed (every result written). The
+ * solve's vectors are the system's: solves on one system must not overlap.
+ * Synchronizes stream between runs of iterations. */
+int sparlab_system_solve_free_block_segments_device(sparlab_system* system, const double* block_values,
+                                                    int symmetric, int32_t num_segments,
+                                                    const int64_t* segment_rows, const double* b, double* x,
+                                                    double tolerance, int64_t max_iterations, int64_t* iterations,
+                                                    double* relative_residual, int32_t* status, void* stream);
 
 /* Per-element stress measures (the reference's ElastoplasticStress,
  * include/sparlab/fem/Elastoplastic.hpp:206-221), SPARLAB_STRESS_FIELDS

@@ -1,9 +1,13 @@
-// Conjugate gradients on the free-free block of a symmetric tangent: the
-// device solve of K_ff x = b beside the reference's direct factorisations
-// (PartitionedFactor::factorize and solve, src/fem/Forming.cpp:104-172).
+// Conjugate gradients on the free-free block of a symmetric tangent (and
+// BiCGStab on a non-symmetric one): the device solve of K_ff x = b beside the
+// reference's direct factorisations (PartitionedFactor::factorize and solve,
+// src/fem/Forming.cpp:104-172). Every launch below acts on the problems whose
+// active flag is set (all of them in a plain solve).
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <vector>
 
 namespace sbk {
 
@@ -41,6 +45,25 @@ struct CgArgs {
   // by iteration parity (2 nparts)
   double *rhat, *v, *y, *s, *t;
   double *pv, *pts, *rho;
+  // Segments: a block-diagonal block of nseg independent problems (rows
+  // [seg_row[p], seg_row[p + 1])), each iterated with its own scalars
+  // (scalars + p kCgScalars) and its own partial sums, cut into the vector
+  // and product blocks a plain solve of its rows would use (vfirst, pfirst:
+  // each problem's first block; vseg, pseg: each block's problem), so every
+  // problem's sums run in the order of its own plain solve. active[p] = 0
+  // leaves problem p as it is; cycles[p] is its BiCGStab cycle start. A plain
+  // solve is one segment with these arrays null.
+  int nseg;
+  const long long* seg_row;
+  const int *vseg, *vfirst, *pseg, *pfirst;
+  int* active;
+  long long* cycles;
+};
+
+// The blocks of the problems of a segmented solve (host side).
+struct CgSegments {
+  std::vector<long long> rows;  // nseg + 1 row offsets
+  std::vector<int> vfirst, pfirst, vseg, pseg;
 };
 
 // Product blocks for a block of n rows, and the doubles of work a solve of n
@@ -48,8 +71,20 @@ struct CgArgs {
 int cg_vector_blocks(long long n);
 int cg_product_blocks(long long n);
 long long cg_work_doubles(long long n);
-// The work's pieces laid out from `work`.
+// The work's pieces laid out from `work` (a plain solve: one segment).
 void cg_layout(CgArgs& a, double* work);
+// The blocks of problems whose rows start at rows[p] (rows: nseg + 1
+// ascending offsets, the last the block's row count).
+CgSegments cg_segments(const std::vector<long long>& rows);
+// Doubles of work and bytes of device metadata a segmented solve takes.
+long long cg_segment_work_doubles(const CgSegments& g);
+long long cg_segment_meta_bytes(const CgSegments& g);
+// The segmented solve's pieces from `work` and `meta` (device), the metadata
+// copied in on stream s (every problem active, cycles 0); waits for the
+// copies. 0 on success.
+int cg_segment_layout(CgArgs& a, const CgSegments& g, double* work, void* meta, hipStream_t s);
+// flag[0] = 1 (device) when a row has a column outside its problem's rows.
+int launch_segment_check(const CgArgs& a, int* flag, hipStream_t s);
 
 // The diagonal's inverse (a missing or non-positive diagonal sets kCgBreak),
 // x = 0, r = b, z = D^-1 r, p = z and the first r . z, r . r, b . b.

@@ -61,6 +61,8 @@ void release(sparlab_system* s) {
   dfree(s->block_col); dfree(s->block_mirror); dfree(s->kproduct); dfree(s->kgathered);
   dfree(s->ws); dfree(s->partial); dfree(s->scalars); dfree(s->summary);
   dfree(s->grad_x); dfree(s->grad_u); dfree(s->grad_p);
+  dfree(s->seg_work); dfree(s->seg_meta);
+  s->seg_work_doubles = s->seg_meta_bytes = 0;
 }
 
 }  // namespace
@@ -349,6 +351,152 @@ int sparlab_system_free_block_device(sparlab_system* s, const double* values, do
   return SPARLAB_OK;
 }
 
+namespace {
+
+// How one problem's block solve ended.
+enum SolveEnd : int32_t { kSolved = 0, kNotDefinite = 1, kBrokeDown = 2, kNotConverged = 3 };
+
+// The block solve of a.nseg problems (a plain solve: one) on stream st, each
+// problem through the same steps as it would take alone: runs of iterations,
+// the true residual once the iteration's residual meets the tolerance, a
+// restart from it when it does not (residual replacement), BiCGStab's
+// restarts after a breakdown. Every launch acts on the problems that need it
+// (the active flags; a plain solve has none and needs every launch). Per
+// problem: iterations, relative residual and how it ended. False when a
+// device call failed.
+bool run_block_solve(sbk::CgArgs& a, int symmetric, const double* b, double* x, double tolerance,
+                     int64_t max_iterations, hipStream_t st, int64_t* iterations, double* relative_residual,
+                     int32_t* end) {
+  const int nseg = a.nseg;
+  std::vector<double> sc(static_cast<std::size_t>(nseg) * sbk::kCgScalars);
+  auto read = [&]() {
+    return hipMemcpyAsync(sc.data(), a.scalars, sc.size() * sizeof(double), hipMemcpyDeviceToHost, st) ==
+               hipSuccess &&
+           hipStreamSynchronize(st) == hipSuccess;
+  };
+  auto at = [&](int p, int k) { return sc[static_cast<std::size_t>(p) * sbk::kCgScalars + k]; };
+  std::vector<int> on(nseg, 1);  // the device's active flags (all set by the layout)
+  auto select = [&](const std::vector<int>& want) {
+    if (!a.active || want == on) return true;
+    on = want;
+    return hipMemcpyAsync(a.active, on.data(), on.size() * sizeof(int), hipMemcpyHostToDevice, st) == hipSuccess &&
+           hipStreamSynchronize(st) == hipSuccess;
+  };
+  std::vector<int64_t> cycle(nseg, 0);  // each problem's first iteration since its last (re)start
+  auto upload_cycles = [&]() {
+    return !a.cycles || (hipMemcpyAsync(a.cycles, cycle.data(), cycle.size() * sizeof(int64_t),
+                                        hipMemcpyHostToDevice, st) == hipSuccess &&
+                         hipStreamSynchronize(st) == hipSuccess);
+  };
+  const int started = symmetric ? sbk::launch_cg_start(a, b, x, st) : sbk::launch_bicg_start(a, b, x, st);
+  if (started != 0 || !read()) return false;
+  std::vector<double> bb(nseg), goal(nseg);
+  std::vector<char> converged(nseg), stopped(nseg);
+  std::vector<int> restarts(nseg, 0);
+  for (int p = 0; p < nseg; ++p) {
+    bb[p] = at(p, sbk::kCgBb);
+    goal[p] = tolerance * tolerance * bb[p];
+    converged[p] = at(p, sbk::kCgRr) <= goal[p];  // r = b at the start: exact
+    stopped[p] = converged[p] || at(p, sbk::kCgBreak) != 0.0;
+    iterations[p] = 0;
+    relative_residual[p] = 0.0;
+  }
+  int64_t done = 0;
+  constexpr int64_t kRun = 16;  // iterations between two reads of the residual
+  // The iteration's residual drifts from b - K x; when it meets the
+  // tolerance the true residual is computed, and the iteration restarts from
+  // it (residual replacement) unless that meets the tolerance too.
+  // BiCGStab's shadow residual can become orthogonal to the residual in
+  // floating point (r^ . r = 0: a breakdown); the iteration then restarts from
+  // the true residual with r^ = r, at most kRestarts times. A break of
+  // conjugate gradients (p . K p <= 0: round-off near the solution, or an
+  // indefinite block) left every step after it zero, so x is the last good
+  // iterate: the solution if its true residual meets the tolerance.
+  constexpr int kRestarts = 16;
+  enum Next : char { kNone, kBreakCg, kBreakBicg, kMet };
+  std::vector<char> next(nseg);
+  std::vector<int> run_set(nseg), sub(nseg);
+  while (done < max_iterations) {
+    bool any = false;
+    for (int p = 0; p < nseg; ++p) any |= (run_set[p] = !stopped[p]) != 0;
+    if (!any) break;
+    if (!select(run_set)) return false;
+    const int run = static_cast<int>(max_iterations - done < kRun ? max_iterations - done : kRun);
+    const int ran = symmetric ? sbk::launch_cg_iterations(a, x, done, run, st)
+                              : sbk::launch_bicg_iterations(a, x, cycle[0], done, run, st);
+    if (ran != 0 || !read()) return false;
+    done += run;
+    bool residual = false;
+    for (int p = 0; p < nseg; ++p) {
+      next[p] = kNone;
+      if (!run_set[p]) continue;
+      iterations[p] = done;
+      const bool broke = at(p, sbk::kCgBreak) != 0.0;
+      if (broke && symmetric)
+        next[p] = kBreakCg;
+      else if (broke && restarts[p] < kRestarts)
+        next[p] = kBreakBicg;
+      else if (broke)
+        stopped[p] = 1;  // BiCGStab out of restarts
+      else if (at(p, sbk::kCgRr) <= goal[p])
+        next[p] = kMet;
+      residual |= next[p] != kNone;
+    }
+    if (!residual) continue;
+    for (int p = 0; p < nseg; ++p) sub[p] = next[p] != kNone;
+    if (!select(sub) || sbk::launch_cg_residual(a, b, x, st) != 0 || !read()) return false;
+    bool restart = false;
+    for (int p = 0; p < nseg; ++p) {
+      sub[p] = 0;
+      if (next[p] == kNone) continue;
+      converged[p] = at(p, sbk::kCgRr) <= goal[p];
+      if (converged[p] || next[p] == kBreakCg) {
+        stopped[p] = 1;
+        continue;
+      }
+      sub[p] = 1;
+      cycle[p] = done;
+      if (next[p] == kBreakBicg) ++restarts[p];
+      restart = true;
+    }
+    if (!restart) continue;
+    if (!select(sub) || (!symmetric && !upload_cycles())) return false;
+    const int restarted = symmetric ? sbk::launch_cg_restart(a, done, st) : sbk::launch_bicg_restart(a, done, st);
+    if (restarted != 0 || !read()) return false;
+  }
+  // the ends; a solve out of iterations gets its true residual
+  bool residual = false;
+  for (int p = 0; p < nseg; ++p) {
+    sub[p] = 0;
+    if (converged[p]) {
+      end[p] = kSolved;
+    } else if (at(p, sbk::kCgBreak) != 0.0) {
+      end[p] = symmetric ? kNotDefinite : kBrokeDown;
+    } else {
+      end[p] = kNotConverged;
+      sub[p] = 1;
+      residual = true;
+    }
+  }
+  if (residual && (!select(sub) || sbk::launch_cg_residual(a, b, x, st) != 0 || !read())) return false;
+  for (int p = 0; p < nseg; ++p) {
+    if (end[p] == kNotDefinite || end[p] == kBrokeDown) continue;
+    // after a converged run the scalars hold the true residual's r . r
+    const double rr = at(p, sbk::kCgRr);
+    relative_residual[p] = bb[p] > 0.0 ? std::sqrt(rr / bb[p]) : std::sqrt(rr);
+  }
+  return true;
+}
+
+const char* solve_end_message(int32_t end, int symmetric) {
+  if (end == kNotDefinite) return "the free-free block is not positive definite (conjugate gradients)";
+  if (end == kBrokeDown) return "the block solve broke down (BiCGStab)";
+  (void)symmetric;
+  return "the block solve did not converge";
+}
+
+}  // namespace
+
 int sparlab_system_solve_free_block_device(sparlab_system* s, const double* block_values, int symmetric,
                                            const double* b, double* x, double tolerance, int64_t max_iterations,
                                            int64_t* iterations, double* relative_residual, void* stream) {
@@ -367,77 +515,70 @@ int sparlab_system_solve_free_block_device(sparlab_system* s, const double* bloc
   a.values = block_values;
   a.n = n;
   sbk::cg_layout(a, s->part_work);
-  double sc[sbk::kCgScalars];
-  auto read = [&]() {
-    return hipMemcpyAsync(sc, a.scalars, sizeof(sc), hipMemcpyDeviceToHost, st) == hipSuccess &&
-           hipStreamSynchronize(st) == hipSuccess;
-  };
-  *iterations = 0;
-  *relative_residual = 0.0;
-  const int started = symmetric ? sbk::launch_cg_start(a, b, x, st) : sbk::launch_bicg_start(a, b, x, st);
-  if (started != 0 || !read()) return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's start failed");
-  const double bb = sc[sbk::kCgBb];
-  const double goal = tolerance * tolerance * bb;
-  int64_t done = 0, cycle = 0;  // cycle: the first iteration since the last (re)start
-  constexpr int64_t kRun = 16;  // iterations between two reads of the residual
-  // The iteration's residual drifts from b - K x; when it meets the
-  // tolerance the true residual is computed, and the iteration restarts from
-  // it (residual replacement) unless that meets the tolerance too.
-  // BiCGStab's shadow residual can become orthogonal to the residual in
-  // floating point (r^ . r = 0: a breakdown); the iteration then restarts from
-  // the true residual with r^ = r, at most kRestarts times.
-  constexpr int kRestarts = 16;
-  int restarts = 0;
-  bool converged = sc[sbk::kCgRr] <= goal;  // r = b at the start: exact
-  while (!converged && sc[sbk::kCgBreak] == 0.0 && done < max_iterations) {
-    const int run = static_cast<int>(max_iterations - done < kRun ? max_iterations - done : kRun);
-    const int ran = symmetric ? sbk::launch_cg_iterations(a, x, done, run, st)
-                              : sbk::launch_bicg_iterations(a, x, cycle, done, run, st);
-    if (ran != 0 || !read()) return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's iterations failed");
-    done += run;
-    const bool broke = sc[sbk::kCgBreak] != 0.0;
-    if (broke && symmetric) {
-      // p . K p <= 0 (round-off near the solution, or an indefinite block):
-      // every step after the break was zero, so x is the last good iterate;
-      // it is the solution if its true residual meets the tolerance
-      if (sbk::launch_cg_residual(a, b, x, st) != 0 || !read())
-        return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's residual failed");
-      converged = sc[sbk::kCgRr] <= goal;
-      break;
-    }
-    if (broke && !symmetric && restarts < kRestarts) {
-      // every step after the breakdown was zero: x is the last good iterate
-      if (sbk::launch_cg_residual(a, b, x, st) != 0 || !read())
-        return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's residual failed");
-      converged = sc[sbk::kCgRr] <= goal;
-      if (!converged) {
-        if (sbk::launch_bicg_restart(a, done, st) != 0 || !read())
-          return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's restart failed");
-        cycle = done;
-        ++restarts;
-      }
-      continue;
-    }
-    if (!broke && sc[sbk::kCgRr] <= goal) {
-      if (sbk::launch_cg_residual(a, b, x, st) != 0 || !read())
-        return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's residual failed");
-      converged = sc[sbk::kCgRr] <= goal;
-      if (!converged) {
-        const int restarted = symmetric ? sbk::launch_cg_restart(a, done, st) : sbk::launch_bicg_restart(a, done, st);
-        if (restarted != 0 || !read()) return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's restart failed");
-        cycle = done;
-      }
-    }
+  int32_t end = kSolved;
+  if (!run_block_solve(a, symmetric, b, x, tolerance, max_iterations, st, iterations, relative_residual, &end))
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's launches failed");
+  if (end != kSolved) return sbk::capi_fail(SPARLAB_E_SOLVER, solve_end_message(end, symmetric));
+  return SPARLAB_OK;
+}
+
+int sparlab_system_solve_free_block_segments_device(sparlab_system* s, const double* block_values, int symmetric,
+                                                    int32_t num_segments, const int64_t* segment_rows,
+                                                    const double* b, double* x, double tolerance,
+                                                    int64_t max_iterations, int64_t* iterations,
+                                                    double* relative_residual, int32_t* status, void* stream) {
+  if (!s || !segment_rows || !iterations || !relative_residual || !status)
+    return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (s->part_free < 0) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "no partition");
+  const int64_t n = s->part_free;
+  if ((s->part_nnz > 0 && !block_values) || (n > 0 && (!b || !x)))
+    return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (n > 0 && b < x + n && x < b + n) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "b and x overlap");
+  if (!(tolerance > 0.0) || !std::isfinite(tolerance) || max_iterations < 0)
+    return sbk::capi_fail(SPARLAB_E_ARGUMENT, "bad tolerance or iteration limit");
+  if (num_segments < 1) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "at least one segment");
+  std::vector<long long> rows(segment_rows, segment_rows + num_segments + 1);
+  if (rows.front() != 0 || rows.back() != n)
+    return sbk::capi_fail(SPARLAB_E_ARGUMENT, "the segments must cover the block's rows from 0 to num_free");
+  for (int p = 0; p < num_segments; ++p)
+    if (rows[p + 1] < rows[p]) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "the segment rows must ascend");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const sbk::CgSegments g = sbk::cg_segments(rows);
+  const std::size_t work = static_cast<std::size_t>(sbk::cg_segment_work_doubles(g));
+  const std::size_t meta = static_cast<std::size_t>(sbk::cg_segment_meta_bytes(g)) + sizeof(int);
+  if (work > s->seg_work_doubles) {
+    dfree(s->seg_work);
+    s->seg_work_doubles = 0;
+    if (!dalloc(s->seg_work, work)) return sbk::capi_fail(SPARLAB_E_DEVICE, "device allocation failed");
+    s->seg_work_doubles = work;
   }
-  *iterations = done;
-  if (!converged && sc[sbk::kCgBreak] != 0.0)
-    return sbk::capi_fail(SPARLAB_E_SOLVER, symmetric ? "the free-free block is not positive definite (conjugate gradients)"
-                                                      : "the block solve broke down (BiCGStab)");
-  if (!converged && (sbk::launch_cg_residual(a, b, x, st) != 0 || !read()))
-    return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's residual failed");
-  // after a converged run sc holds the true residual's r . r
-  *relative_residual = bb > 0.0 ? std::sqrt(sc[sbk::kCgRr] / bb) : std::sqrt(sc[sbk::kCgRr]);
-  if (!converged) return sbk::capi_fail(SPARLAB_E_SOLVER, "the block solve did not converge");
+  if (meta > s->seg_meta_bytes) {
+    dfree(s->seg_meta);
+    s->seg_meta_bytes = 0;
+    if (!dalloc(s->seg_meta, meta)) return sbk::capi_fail(SPARLAB_E_DEVICE, "device allocation failed");
+    s->seg_meta_bytes = meta;
+  }
+  sbk::CgArgs a{};
+  a.ptr = s->part_ptr;
+  a.index = s->part_index;
+  a.values = block_values;
+  if (sbk::cg_segment_layout(a, g, s->seg_work, s->seg_meta, st) != 0)
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "copying the segments failed");
+  // the block must be block-diagonal over the segments (its pattern's check:
+  // a flag after the metadata)
+  int* flag = reinterpret_cast<int*>(static_cast<char*>(static_cast<void*>(s->seg_meta)) + meta - sizeof(int));
+  int bad = 0;
+  if (hipMemsetAsync(flag, 0, sizeof(int), st) != hipSuccess || sbk::launch_segment_check(a, flag, st) != 0 ||
+      hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "the segments' check failed");
+  if (bad) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "the free-free block couples two segments");
+  if (!run_block_solve(a, symmetric, b, x, tolerance, max_iterations, st, iterations, relative_residual, status))
+    return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's launches failed");
+  for (int p = 0; p < num_segments; ++p)
+    if (status[p] != kSolved)
+      return sbk::capi_fail(SPARLAB_E_SOLVER, "segment " + std::to_string(p) + ": " +
+                                                  solve_end_message(status[p], symmetric));
   return SPARLAB_OK;
 }
 

@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 #include <string>
 
@@ -46,6 +47,11 @@ struct sparlab_system {
   int32_t *part_ptr = nullptr, *part_index = nullptr, *part_source = nullptr;
   int64_t part_free = -1, part_nnz = 0;
   double* part_work = nullptr;  // the block solve's vectors, partial sums and scalars
+  // the segmented block solve's work and metadata (grown as needed)
+  double* seg_work = nullptr;
+  std::size_t seg_work_doubles = 0;
+  unsigned char* seg_meta = nullptr;
+  std::size_t seg_meta_bytes = 0;
   double* ws = nullptr;         // the element kernels' workspace
   double* partial = nullptr;    // 1024 partial sums
   double* scalars = nullptr;    // [0] energy, [1] gross

@@ -718,6 +718,46 @@ class NonlinearSystem:
         _check(code)
         return x, its.value, rel.value
 
+    def solve_free_block_segments_device(self, block_values, b, segment_rows, x=None, tolerance=1e-10,
+                                         max_iterations=20000, symmetric=True, stream=None):
+        """The batched form's block solve: the partition's block is
+        block-diagonal over the segments (segment_rows: num_segments + 1
+        ascending row offsets from 0 to num_free), each solved as
+        solve_free_block_device solves its rows alone, all in one set of
+        launches per iteration. Returns (x, iterations, relative residuals,
+        status) with per-segment numpy arrays (status 0: converged; else the
+        plain solve's SolverError code) and never raises for a segment's
+        failure."""
+        import torch
+        L = lib()
+        L.sparlab_system_solve_free_block_segments_device.argtypes = [
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+            ctypes.c_void_p, ctypes.c_double, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+            ctypes.c_void_p]
+        nf, nnz = getattr(self, "_block", (-1, 0))
+        if nf >= 0:
+            _require(block_values, "block_values", nnz, at_least=True)
+            _require(b, "b", nf, device=block_values.device)
+            if x is not None:
+                _require(x, "x", nf, device=block_values.device)
+        rows = np.ascontiguousarray(segment_rows, dtype=np.int64)
+        nseg = rows.shape[0] - 1
+        if nseg < 1:
+            raise ValueError("at least one segment")
+        if x is None:
+            x = torch.empty(max(b.shape[0], 1), dtype=torch.float64, device=b.device)[:b.shape[0]]
+        its = np.zeros(nseg, dtype=np.int64)
+        rel = np.zeros(nseg, dtype=np.float64)
+        status = np.zeros(nseg, dtype=np.int32)
+        s = stream if stream is not None else torch.cuda.current_stream(b.device)
+        code = L.sparlab_system_solve_free_block_segments_device(
+            self.h, ctypes.c_void_p(block_values.data_ptr()), int(bool(symmetric)), nseg, _np_ptr(rows),
+            ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()), float(tolerance), int(max_iterations),
+            _np_ptr(its), _np_ptr(rel), _np_ptr(status), ctypes.c_void_p(s.cuda_stream))
+        if code != -4:  # SPARLAB_E_SOLVER: a segment failed, reported in status
+            _check(code)
+        return x, its, rel, status
+
     def stress_device(self, u, stress=None, status=None, stream=None):
         """elastoplastic_stress of every element at displacement u (device
         tensor, 3 per node) from the committed history and modes: (stress
@@ -861,6 +901,13 @@ class BatchedSystem:
 
     def commit(self):
         self.system.commit()
+
+    def free_segments(self, free):
+        """The block rows of each problem when the union's free DOFs are
+        `free` (ascending union DOF numbers): num_problems + 1 offsets, for
+        system.solve_free_block_segments_device."""
+        free = np.asarray(free, dtype=np.int64)
+        return np.searchsorted(free, 3 * self.node_offset).astype(np.int64)
 
 
 VJP_PARAMETERS = 11
