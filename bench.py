@@ -513,6 +513,26 @@ def assembly_leg(G, W, torch, dev, reps):
                                                          stream=stream)
     torch.cuda.synchronize()
     bicg_ms = (time.perf_counter() - t0) * 1e3
+    # the same two solves preconditioned by the multigrid V-cycle: the first
+    # solve after the partition builds the hierarchy (host, once), every
+    # solve sets its values up from the block (device)
+    sysm.set_block_preconditioner("multigrid")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sysm.solve_free_block_device(k_block, rhs, x=x_sol, tolerance=1e-10, stream=stream)
+    torch.cuda.synchronize()
+    mg_first_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    _, mg_its, mg_rel = sysm.solve_free_block_device(k_block, rhs, x=x_sol, tolerance=1e-10, stream=stream)
+    torch.cuda.synchronize()
+    mg_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    _, mgb_its, mgb_rel = sysm.solve_free_block_device(k_block, rhs, x=x_sol, tolerance=1e-10, symmetric=False,
+                                                       stream=stream)
+    torch.cuda.synchronize()
+    mgb_ms = (time.perf_counter() - t0) * 1e3
+    mg_rows, mg_nnz = sysm.block_preconditioner_levels()
+    sysm.set_block_preconditioner("diagonal")
     del k_values, k_block
 
     def candidates():
@@ -580,7 +600,15 @@ def assembly_leg(G, W, torch, dev, reps):
                             "bicgstab": {"note": "the same system by BiCGStab (symmetric = 0, the method for a "
                                                  "non-symmetric tangent): two products per iteration",
                                          "iterations": int(bicg_its), "relative_residual": bicg_rel, "ms": bicg_ms,
-                                         "us_per_iteration": bicg_ms * 1e3 / max(bicg_its, 1)}}}
+                                         "us_per_iteration": bicg_ms * 1e3 / max(bicg_its, 1)},
+                            "multigrid": {"note": "the same solves preconditioned by a smoothed-aggregation "
+                                                  "multigrid V-cycle (rigid-body modes, Chebyshev degree 2, dense "
+                                                  "coarsest level); ms includes the set-up from the block's values",
+                                          "levels_rows": mg_rows, "levels_nonzeros": mg_nnz,
+                                          "first_solve_ms": mg_first_ms, "iterations": int(mg_its),
+                                          "relative_residual": mg_rel, "ms": mg_ms,
+                                          "bicgstab_iterations": int(mgb_its), "bicgstab_relative_residual": mgb_rel,
+                                          "bicgstab_ms": mgb_ms}}}
 
 
 def copy_probe(torch, mib):
@@ -750,7 +778,13 @@ def main():
               f"free-free block: partition {assembly['free_block']['partition_ms']:.2f} ms, values "
               f"{assembly['free_block']['values_us']:.1f} us ({assembly['free_block']['values_GBps']:.0f} GB/s); block "
               f"solve {assembly['block_solve']['iterations']} CG iterations in {assembly['block_solve']['ms']:.1f} ms "
-              f"({assembly['block_solve']['us_per_iteration']:.1f} us each, {assembly['block_solve']['GBps']:.0f} GB/s)",
+              f"({assembly['block_solve']['us_per_iteration']:.1f} us each, {assembly['block_solve']['GBps']:.0f} GB/s); "
+              f"multigrid: CG {assembly['block_solve']['multigrid']['iterations']} iterations in "
+              f"{assembly['block_solve']['multigrid']['ms']:.1f} ms, BiCGStab "
+              f"{assembly['block_solve']['multigrid']['bicgstab_iterations']} in "
+              f"{assembly['block_solve']['multigrid']['bicgstab_ms']:.1f} ms (first solve with the hierarchy's build "
+              f"{assembly['block_solve']['multigrid']['first_solve_ms']:.0f} ms; levels "
+              f"{assembly['block_solve']['multigrid']['levels_rows']})",
               file=sys.stderr, flush=True)
 
     newton = None

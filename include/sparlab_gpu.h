@@ -107,7 +107,8 @@ typedef struct sparlab_rule {
 int sparlab_material_finalize(sparlab_material* material);
 
 /* Points of the rule (elastoplastic_points, src/fem/Elastoplastic.cpp:222-224),
- * or a negative code. The element kernels take rules of up to 8 points. */
+ * or a negative code. The element kernels take rules of up to 32 points (8-lane teams up to
+ * 8 points, 32-lane teams above: the 2 x 2 x 5 and 2 x 2 x 7 box rules). */
 int sparlab_rule_points(const sparlab_rule* rule);
 
 /* Doubles of history per integration point in the library's layout: plastic
@@ -342,6 +343,32 @@ int sparlab_system_free_block_device(sparlab_system* system, const double* value
 int sparlab_system_solve_free_block_device(sparlab_system* system, const double* block_values, int symmetric,
                                            const double* b, double* x, double tolerance, int64_t max_iterations,
                                            int64_t* iterations, double* relative_residual, void* stream);
+/* The preconditioner of sparlab_system_solve_free_block_device (no
+ * counterpart in the reference, which factorises the block directly):
+ * DIAGONAL (the default) or MULTIGRID, smoothed-aggregation algebraic
+ * multigrid, one V-cycle per application. The hierarchy is built at the first
+ * solve after a partition or new coordinates: aggregates of the block's node
+ * graph (greedy, every node with its neighbours), each aggregate's six
+ * rigid-body modes at the nodes' coordinates orthonormalised as the
+ * tentative prolongator T, P = T - 4 / (3 rho) D^-1 A T, A_c = P^T A P, until
+ * a level has at most 256 rows, which is solved by its dense inverse; every
+ * solve sets the values up again from its block (the device: fixed-order
+ * sums, a solve repeats bit for bit) and smooths with Chebyshev of degree 2
+ * on D^-1 A over [1.2 rho / 30, 1.2 rho], rho a 25-step power estimate; a
+ * BiCGStab solve (symmetric = 0) builds it on the block's symmetric part
+ * (A + A^T) / 2, whose spectrum is real.
+ * SPARLAB_E_SOLVER from the solve when no hierarchy can be built or its
+ * coarsest level is singular. The segmented solve always preconditions with
+ * the diagonal. */
+#define SPARLAB_PRECONDITIONER_DIAGONAL 0
+#define SPARLAB_PRECONDITIONER_MULTIGRID 1
+#define SPARLAB_MULTIGRID_MAX_LEVELS 12
+int sparlab_system_set_block_preconditioner(sparlab_system* system, int kind);
+/* The hierarchy of the partition (after a multigrid solve): *count levels
+ * (0 without one), and each level's rows and nonzeros (host arrays of
+ * SPARLAB_MULTIGRID_MAX_LEVELS, either may be null). */
+int sparlab_system_block_preconditioner_levels(sparlab_system* system, int32_t* count, int64_t* rows,
+                                               int64_t* nonzeros);
 /* The batched form's block solve (the reference's per-problem
  * PartitionedFactor::factorize and solve, src/fem/Forming.cpp:104-172, for
  * every problem of a batch at once): the partition's block is block-diagonal

@@ -221,6 +221,13 @@ __global__ __launch_bounds__(256) void update_kernel(CgArgs A, double* __restric
   long long i0, i1;
   part_range(q, i0, i1);
   double acc_rz = 0.0, acc_rr = 0.0;
+  if (A.external) {  // z and r . z come from the preconditioner
+    for (long long i = i0 + threadIdx.x; i < i1; i += 256) {
+      x[i] = fma(alpha, A.p[i], x[i]);
+      A.r[i] = fma(-alpha, A.q[i], A.r[i]);
+    }
+    return;
+  }
   for (long long i = i0 + threadIdx.x; i < i1; i += 256) {
     x[i] = fma(alpha, A.p[i], x[i]);
     const double ri = fma(-alpha, A.q[i], A.r[i]);
@@ -233,6 +240,28 @@ __global__ __launch_bounds__(256) void update_kernel(CgArgs A, double* __restric
   block_reduce2(acc_rz, acc_rr, s1, s2);
   if (threadIdx.x == 0) {
     A.rz[(k & 1) * A.nparts + blockIdx.x] = acc_rz;
+    A.rr[blockIdx.x] = acc_rr;
+  }
+}
+
+// After an external preconditioner: the block's r . z into the buffer
+// `slot`, its r . r, and p = z when set_p (a start or restart).
+__global__ __launch_bounds__(256) void precond_dot_kernel(CgArgs A, int slot, int set_p) {
+  const Part q = vector_part(A);
+  if (resting(A, q.p)) return;
+  __shared__ double s1[256], s2[256];
+  long long i0, i1;
+  part_range(q, i0, i1);
+  double acc_rz = 0.0, acc_rr = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i1; i += 256) {
+    const double ri = A.r[i], zi = A.z[i];
+    if (set_p) A.p[i] = zi;
+    acc_rz = fma(ri, zi, acc_rz);
+    acc_rr = fma(ri, ri, acc_rr);
+  }
+  block_reduce2(acc_rz, acc_rr, s1, s2);
+  if (threadIdx.x == 0) {
+    A.rz[slot * A.nparts + blockIdx.x] = acc_rz;
     A.rr[blockIdx.x] = acc_rr;
   }
 }
@@ -298,6 +327,13 @@ __global__ __launch_bounds__(256) void start_kernel(CgArgs A, const double* __re
   long long i0, i1;
   part_range(q, i0, i1);
   double acc_rz = 0.0, acc_rr = 0.0;
+  if (A.external) {  // z, p and r . z come from the preconditioner
+    for (long long i = i0 + threadIdx.x; i < i1; i += 256) {
+      x[i] = 0.0;
+      A.r[i] = b[i];
+    }
+    return;
+  }
   for (long long i = i0 + threadIdx.x; i < i1; i += 256) {
     const double bi = b[i];
     const double zi = A.dinv[i] * bi;
@@ -496,7 +532,7 @@ __global__ __launch_bounds__(256) void bicg_direction_kernel(CgArgs A, long long
   for (long long i = i0 + threadIdx.x; i < i1; i += 256) {
     const double pi = first ? A.r[i] : fma(beta, fma(-omega, A.v[i], A.p[i]), A.r[i]);
     A.p[i] = pi;
-    A.y[i] = A.dinv[i] * pi;
+    if (!A.external) A.y[i] = A.dinv[i] * pi;
   }
 }
 
@@ -526,7 +562,7 @@ __global__ __launch_bounds__(256) void bicg_half_kernel(CgArgs A, double* __rest
     const double si = fma(-alpha, A.v[i], A.r[i]);
     A.s[i] = si;
     x[i] = fma(alpha, A.y[i], x[i]);
-    A.z[i] = A.dinv[i] * si;
+    if (!A.external) A.z[i] = A.dinv[i] * si;
   }
 }
 
@@ -650,6 +686,9 @@ void cg_layout(CgArgs& a, double* work) {
   a.seg_row = nullptr;
   a.vseg = a.vfirst = a.pseg = a.pfirst = a.active = nullptr;
   a.cycles = nullptr;
+  a.external = 0;
+  a.precond = nullptr;
+  a.precond_ctx = nullptr;
   lay_out(a, work);
 }
 
@@ -685,6 +724,9 @@ int cg_segment_layout(CgArgs& a, const CgSegments& g, double* work, void* meta, 
   const int nseg = static_cast<int>(g.rows.size()) - 1;
   a.n = g.rows.back();
   a.nseg = nseg;
+  a.external = 0;
+  a.precond = nullptr;
+  a.precond_ctx = nullptr;
   a.nparts = g.vfirst.back();
   a.nrows = g.pfirst.back();
   lay_out(a, work);
@@ -738,6 +780,10 @@ int launch_cg_start(const CgArgs& a, const double* b, double* x, hipStream_t s) 
   hipLaunchKernelGGL(clear_break_kernel, dim3(1), dim3(256), 0, s, a);
   hipLaunchKernelGGL(diagonal_kernel, dim3(a.nparts), dim3(256), 0, s, a, 1);
   hipLaunchKernelGGL(start_kernel, dim3(a.nparts), dim3(256), 0, s, a, b, x);
+  if (a.external) {
+    if (a.precond(a.precond_ctx, a.r, a.z, s) != 0) return 1;
+    hipLaunchKernelGGL(precond_dot_kernel, dim3(a.nparts), dim3(256), 0, s, a, 1, 1);
+  }
   hipLaunchKernelGGL(start_scalars_kernel, dim3(a.nseg), dim3(256), 0, s, a);
   return static_cast<int>(hipGetLastError());
 }
@@ -749,6 +795,10 @@ int launch_cg_iterations(const CgArgs& a, double* x, long long first, int count,
     hipLaunchKernelGGL(product_kernel, dim3(a.nrows), dim3(64 * kWaves), 0, s, a, a.p, a.q, a.p, a.pq,
                        static_cast<double*>(nullptr));
     hipLaunchKernelGGL(update_kernel, dim3(a.nparts), dim3(256), 0, s, a, x, k);
+    if (a.external) {
+      if (a.precond(a.precond_ctx, a.r, a.z, s) != 0) return 1;
+      hipLaunchKernelGGL(precond_dot_kernel, dim3(a.nparts), dim3(256), 0, s, a, static_cast<int>(k & 1), 0);
+    }
     hipLaunchKernelGGL(direction_kernel, dim3(a.nparts), dim3(256), 0, s, a, k);
   }
   return static_cast<int>(hipGetLastError());
@@ -769,9 +819,11 @@ int launch_bicg_iterations(const CgArgs& a, double* x, long long cycle, long lon
   if (a.n == 0) return 0;
   for (long long k = first; k < first + count; ++k) {
     hipLaunchKernelGGL(bicg_direction_kernel, dim3(a.nparts), dim3(256), 0, s, a, cycle, k);
+    if (a.external && a.precond(a.precond_ctx, a.p, a.y, s) != 0) return 1;
     hipLaunchKernelGGL(product_kernel, dim3(a.nrows), dim3(64 * kWaves), 0, s, a, a.y, a.v, a.rhat, a.pv,
                        static_cast<double*>(nullptr));
     hipLaunchKernelGGL(bicg_half_kernel, dim3(a.nparts), dim3(256), 0, s, a, x, k);
+    if (a.external && a.precond(a.precond_ctx, a.s, a.z, s) != 0) return 1;
     hipLaunchKernelGGL(product_kernel, dim3(a.nrows), dim3(64 * kWaves), 0, s, a, a.z, a.t, a.s, a.pts,
                        a.pts + a.nrows);
     hipLaunchKernelGGL(bicg_full_kernel, dim3(a.nparts), dim3(256), 0, s, a, x, k);
@@ -793,7 +845,12 @@ int launch_bicg_restart(const CgArgs& a, long long next, hipStream_t s) {
 int launch_cg_restart(const CgArgs& a, long long next, hipStream_t s) {
   (void)hipGetLastError();
   const int slot = static_cast<int>((next + 1) & 1);
-  hipLaunchKernelGGL(restart_kernel, dim3(a.nparts), dim3(256), 0, s, a, slot);
+  if (a.external) {
+    if (a.precond(a.precond_ctx, a.r, a.z, s) != 0) return 1;
+    hipLaunchKernelGGL(precond_dot_kernel, dim3(a.nparts), dim3(256), 0, s, a, slot, 1);
+  } else {
+    hipLaunchKernelGGL(restart_kernel, dim3(a.nparts), dim3(256), 0, s, a, slot);
+  }
   hipLaunchKernelGGL(restart_scalars_kernel, dim3(a.nseg), dim3(256), 0, s, a, slot);
   return static_cast<int>(hipGetLastError());
 }

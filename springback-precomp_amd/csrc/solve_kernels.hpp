@@ -58,6 +58,13 @@ struct CgArgs {
   const int *vseg, *vfirst, *pseg, *pfirst;
   int* active;
   long long* cycles;
+  // An external preconditioner in place of the diagonal: z = M^-1 r (and
+  // BiCGStab's y = M^-1 p, z = M^-1 s) by precond(precond_ctx, in, out,
+  // stream) between the kernels, which then leave those vectors to it
+  // (external = 1). A plain solve only.
+  int external;
+  int (*precond)(const void* ctx, const double* in, double* out, hipStream_t s);
+  const void* precond_ctx;
 };
 
 // The blocks of the problems of a segmented solve (host side).

@@ -43,6 +43,8 @@ void dfree(T*& p) {
 }
 
 void release_partition(sparlab_system* s) {
+  delete s->amg;
+  s->amg = nullptr;
   dfree(s->part_counts); dfree(s->part_sums); dfree(s->part_bad);
   dfree(s->part_ptr); dfree(s->part_index); dfree(s->part_source); dfree(s->part_work);
   s->part_free = -1;
@@ -488,6 +490,35 @@ bool run_block_solve(sbk::CgArgs& a, int symmetric, const double* b, double* x, 
   return true;
 }
 
+// The partition's multigrid hierarchy (built at the first solve after the
+// partition or new coordinates) set up for the block's values.
+int prepare_multigrid(sparlab_system* s, const double* block_values, int symmetric, hipStream_t st) {
+  if (!s->amg) {
+    const std::size_t ndof = 3 * static_cast<std::size_t>(s->nn);
+    std::vector<int32_t> map(ndof);
+    std::vector<double> coords(ndof);
+    if (hipMemcpyAsync(map.data(), s->part_map, ndof * sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(coords.data(), s->coords, ndof * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return sbk::capi_fail(SPARLAB_E_DEVICE, "reading the partition failed");
+    std::vector<long long> free_dofs;
+    free_dofs.reserve(static_cast<std::size_t>(s->part_free));
+    for (std::size_t d = 0; d < ndof; ++d)
+      if (map[d] >= 0) free_dofs.push_back(static_cast<long long>(d));
+    auto* h = new sbk::AmgHierarchy();
+    const std::string why = h->build(s->part_ptr, s->part_index, s->part_free, s->part_nnz, free_dofs, coords, st);
+    if (!why.empty()) {
+      delete h;
+      return sbk::capi_fail(SPARLAB_E_SOLVER, "the multigrid hierarchy could not be built: " + why);
+    }
+    s->amg = h;
+  }
+  if (s->amg->setup(block_values, !symmetric, st) != 0) return sbk::capi_fail(SPARLAB_E_DEVICE, "the multigrid set-up failed");
+  if (s->amg->singular(st))
+    return sbk::capi_fail(SPARLAB_E_SOLVER, "the multigrid hierarchy's coarsest level is singular");
+  return SPARLAB_OK;
+}
+
 const char* solve_end_message(int32_t end, int symmetric) {
   if (end == kNotDefinite) return "the free-free block is not positive definite (conjugate gradients)";
   if (end == kBrokeDown) return "the block solve broke down (BiCGStab)";
@@ -515,10 +546,39 @@ int sparlab_system_solve_free_block_device(sparlab_system* s, const double* bloc
   a.values = block_values;
   a.n = n;
   sbk::cg_layout(a, s->part_work);
+  if (s->preconditioner == SPARLAB_PRECONDITIONER_MULTIGRID) {
+    const int prepared = prepare_multigrid(s, block_values, symmetric, st);
+    if (prepared != SPARLAB_OK) return prepared;
+    a.external = 1;
+    a.precond = sbk::amg_apply;
+    a.precond_ctx = s->amg;
+  }
   int32_t end = kSolved;
   if (!run_block_solve(a, symmetric, b, x, tolerance, max_iterations, st, iterations, relative_residual, &end))
     return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's launches failed");
   if (end != kSolved) return sbk::capi_fail(SPARLAB_E_SOLVER, solve_end_message(end, symmetric));
+  return SPARLAB_OK;
+}
+
+int sparlab_system_set_block_preconditioner(sparlab_system* s, int kind) {
+  if (!s) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  if (kind != SPARLAB_PRECONDITIONER_DIAGONAL && kind != SPARLAB_PRECONDITIONER_MULTIGRID)
+    return sbk::capi_fail(SPARLAB_E_ARGUMENT, "unknown preconditioner");
+  s->preconditioner = kind;
+  return SPARLAB_OK;
+}
+
+int sparlab_system_block_preconditioner_levels(sparlab_system* s, int32_t* count, int64_t* rows,
+                                               int64_t* nonzeros) {
+  if (!s || !count) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
+  *count = 0;
+  if (!s->amg) return SPARLAB_OK;
+  const std::vector<long long> r = s->amg->level_rows(), z = s->amg->level_nnz();
+  *count = static_cast<int32_t>(r.size());
+  for (std::size_t l = 0; l < r.size() && l < SPARLAB_MULTIGRID_MAX_LEVELS; ++l) {
+    if (rows) rows[l] = r[l];
+    if (nonzeros) nonzeros[l] = z[l];
+  }
   return SPARLAB_OK;
 }
 
@@ -694,6 +754,8 @@ int sparlab_system_set_coords(sparlab_system* s, const double* coords) {
       hipSuccess)
     return sbk::capi_fail(SPARLAB_E_DEVICE, "upload failed");
   s->adjoint_ready = false;
+  delete s->amg;  // its rigid-body modes were the old coordinates'
+  s->amg = nullptr;
   return SPARLAB_OK;
 }
 

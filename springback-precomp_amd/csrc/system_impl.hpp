@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <string>
 
+#include "amg.hpp"
 #include "assembly.hpp"
 #include "assembly_kernels.hpp"
 #include "element_kernels.hpp"
@@ -47,6 +48,10 @@ struct sparlab_system {
   int32_t *part_ptr = nullptr, *part_index = nullptr, *part_source = nullptr;
   int64_t part_free = -1, part_nnz = 0;
   double* part_work = nullptr;  // the block solve's vectors, partial sums and scalars
+  // the block solve's preconditioner (SPARLAB_PRECONDITIONER_*) and the
+  // partition's multigrid hierarchy (built at its first solve)
+  int preconditioner = 0;
+  sbk::AmgHierarchy* amg = nullptr;
   // the segmented block solve's work and metadata (grown as needed)
   double* seg_work = nullptr;
   std::size_t seg_work_doubles = 0;

@@ -718,6 +718,26 @@ class NonlinearSystem:
         _check(code)
         return x, its.value, rel.value
 
+    def set_block_preconditioner(self, kind):
+        """The preconditioner of solve_free_block_device: "diagonal" (the
+        default) or "multigrid" (smoothed-aggregation algebraic multigrid,
+        one V-cycle per application; its hierarchy is built at the first
+        solve after a partition or new coordinates)."""
+        code = {"diagonal": PRECONDITIONER_DIAGONAL, "multigrid": PRECONDITIONER_MULTIGRID}.get(kind, kind)
+        lib().sparlab_system_set_block_preconditioner.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        _check(lib().sparlab_system_set_block_preconditioner(self.h, int(code)))
+
+    def block_preconditioner_levels(self):
+        """The multigrid hierarchy of the partition: (rows, nonzeros) per
+        level, empty before a multigrid solve."""
+        L = lib()
+        L.sparlab_system_block_preconditioner_levels.argtypes = [ctypes.c_void_p] * 4
+        count = ctypes.c_int32(0)
+        rows = np.zeros(MULTIGRID_MAX_LEVELS, dtype=np.int64)
+        nnz = np.zeros(MULTIGRID_MAX_LEVELS, dtype=np.int64)
+        _check(L.sparlab_system_block_preconditioner_levels(self.h, ctypes.byref(count), _np_ptr(rows), _np_ptr(nnz)))
+        return rows[:count.value].tolist(), nnz[:count.value].tolist()
+
     def solve_free_block_segments_device(self, block_values, b, segment_rows, x=None, tolerance=1e-10,
                                          max_iterations=20000, symmetric=True, stream=None):
         """The batched form's block solve: the partition's block is
@@ -909,6 +929,10 @@ class BatchedSystem:
         free = np.asarray(free, dtype=np.int64)
         return np.searchsorted(free, 3 * self.node_offset).astype(np.int64)
 
+
+PRECONDITIONER_DIAGONAL = 0
+PRECONDITIONER_MULTIGRID = 1
+MULTIGRID_MAX_LEVELS = 12
 
 VJP_PARAMETERS = 11
 PARAMETER_NAMES = ("r0", "r45", "r90", "C1", "gamma1", "C2", "gamma2", "C3", "gamma3", "C4", "gamma4")

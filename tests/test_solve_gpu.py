@@ -1,17 +1,22 @@
 """The device solve of the free-free block (sparlab_system_solve_free_block_device)
 against a direct solve of the same system: where the reference factorises
 and solves a symmetric tangent (PartitionedFactor::factorize and solve,
-src/fem/Forming.cpp:104-172), the device runs Jacobi-preconditioned
-conjugate gradients, so the check is the mathematics of K_ff x = b, not a
-copy of the factorisation's round-off.
+src/fem/Forming.cpp:104-172), the device runs preconditioned conjugate
+gradients (BiCGStab for a non-symmetric block), the preconditioner the
+block's diagonal or a smoothed-aggregation multigrid V-cycle, so the check is
+the mathematics of K_ff x = b, not a copy of the factorisation's round-off.
+Every check runs with both preconditioners.
 
 Bars: the reported and the recomputed true residual ||b - K x|| / ||b|| both
 within the tolerance asked (1e-12); the solution within 1e-7 of scipy's
 sparse LU solution (the block's condition number times the tolerance bounds
-the difference); two solves bit-identical (fixed-order sums); b = 0 gives
-x = 0 after no iteration; a block that is not positive definite and an
-iteration limit that is too small raise SolverError; b and x that overlap
-are refused.
+the difference); two solves bit-identical (fixed-order sums, the multigrid
+set-up included); b = 0 gives x = 0 after no iteration; a block that is not
+positive definite and an iteration limit that is too small raise
+SolverError; b and x that overlap are refused. The multigrid hierarchy of a
+40 x 30 x 4 plate has at least three levels, takes at most a tenth of the
+diagonal's iterations, and is rebuilt alike after new coordinates and a new
+partition.
 """
 import numpy as np
 import pytest
@@ -24,11 +29,15 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def plate_system(device, material=None, nx=12, ny=10, nz=2):
+PRECONDITIONERS = ["diagonal", "multigrid"]
+
+
+def plate_system(device, material=None, nx=12, ny=10, nz=2, precond="diagonal"):
     import torch
     coords, conn = W.structured_hex_mesh(nx, ny, nz, 0.012, 0.010, 0.002)
     m = material or G.Material(70e9, 0.33)
     sysm = G.NonlinearSystem(coords, conn, m)
+    sysm.set_block_preconditioner(precond)
     nn = coords.shape[0]
     u = torch.zeros(3 * nn, dtype=torch.float64, device=device)
     fint = torch.empty(3 * nn, dtype=torch.float64, device=device)
@@ -49,10 +58,11 @@ def as_scipy(block, ptr, index):
     return sp.csr_matrix((block.cpu().numpy(), index.cpu().numpy(), ptr.cpu().numpy()), shape=(n, n))
 
 
-def test_block_solve_matches_direct_solve(device):
+@pytest.mark.parametrize("precond", PRECONDITIONERS)
+def test_block_solve_matches_direct_solve(device, precond):
     import torch
     from scipy.sparse.linalg import spsolve
-    sysm, block, ptr, index, free = plate_system(device)
+    sysm, block, ptr, index, free = plate_system(device, precond=precond)
     k = as_scipy(block, ptr, index)
     assert abs(k - k.T).max() <= 1e-12 * abs(k).max()  # the elastic tangent is symmetric
     rng = np.random.default_rng(3)
@@ -72,23 +82,25 @@ def test_block_solve_matches_direct_solve(device):
     assert its2 == its and rel2 == rel and torch.equal(x2, x)
 
 
-def test_block_solve_of_a_zero_right_hand_side(device):
+@pytest.mark.parametrize("precond", PRECONDITIONERS)
+def test_block_solve_of_a_zero_right_hand_side(device, precond):
     import torch
-    sysm, block, ptr, index, free = plate_system(device, nx=4, ny=3)
+    sysm, block, ptr, index, free = plate_system(device, nx=4, ny=3, precond=precond)
     b = torch.zeros(free.size, dtype=torch.float64, device=device)
     x, its, rel = sysm.solve_free_block_device(block, b, tolerance=TOL)
     torch.cuda.synchronize()
     assert its == 0 and rel == 0.0 and (x == 0).all()
 
 
-def test_block_solve_refuses_what_it_cannot_solve(device):
+@pytest.mark.parametrize("precond", PRECONDITIONERS)
+def test_block_solve_refuses_what_it_cannot_solve(device, precond):
     import torch
-    sysm, block, ptr, index, free = plate_system(device, nx=6, ny=5)
+    sysm, block, ptr, index, free = plate_system(device, nx=6, ny=5, precond=precond)
     b = torch.as_tensor(np.random.default_rng(8).normal(size=free.size), device=device)
     with pytest.raises(G.SolverError, match="not positive definite"):
         sysm.solve_free_block_device(-block, b, tolerance=TOL)
     with pytest.raises(G.SolverError, match="did not converge"):
-        sysm.solve_free_block_device(block, b, tolerance=TOL, max_iterations=3)
+        sysm.solve_free_block_device(block, b, tolerance=TOL, max_iterations=1)
     with pytest.raises(RuntimeError, match="overlap"):
         sysm.solve_free_block_device(block, b, x=b, tolerance=TOL)
     # still solvable afterwards
@@ -96,7 +108,8 @@ def test_block_solve_refuses_what_it_cannot_solve(device):
     assert rel <= TOL
 
 
-def test_nonsymmetric_block_solve_matches_direct_solve(device):
+@pytest.mark.parametrize("precond", PRECONDITIONERS)
+def test_nonsymmetric_block_solve_matches_direct_solve(device, precond):
     """BiCGStab (symmetric = 0) on a block that is not symmetric: the plate's
     elastic block plus, on every node's 3 x 3 diagonal block, c (I + S) with S
     a seeded skew matrix (the shape frictional contact adds: a penalty and a
@@ -105,7 +118,7 @@ def test_nonsymmetric_block_solve_matches_direct_solve(device):
     import torch
     import scipy.sparse as sp
     from scipy.sparse.linalg import spsolve
-    sysm, block, ptr, index, free = plate_system(device)
+    sysm, block, ptr, index, free = plate_system(device, precond=precond)
     k = as_scipy(block, ptr, index)
     rng = np.random.default_rng(21)
     node = free // 3
@@ -149,3 +162,33 @@ def test_nonsymmetric_block_solve_matches_direct_solve(device):
     xn, _, rel_n = sysm.solve_free_block_device(-d_block, d_b, tolerance=TOL, symmetric=False)
     torch.cuda.synchronize()
     assert rel_n <= TOL and np.linalg.norm(xn.cpu().numpy() + ref) <= 1e-7 * np.linalg.norm(ref)
+
+
+def test_multigrid_hierarchy_and_iterations(device):
+    import torch
+    from scipy.sparse.linalg import spsolve
+    coords, conn = W.structured_hex_mesh(40, 30, 4, 0.012, 0.010, 0.002)  # plate_system's
+    sysm, block, ptr, index, free = plate_system(device, nx=40, ny=30, nz=4)
+    k = as_scipy(block, ptr, index)
+    b = torch.as_tensor(np.random.default_rng(5).normal(size=free.size) * 1e3, device=device)
+    x_d, its_d, rel_d = sysm.solve_free_block_device(block, b, tolerance=1e-10)
+    sysm.set_block_preconditioner("multigrid")
+    x_m, its_m, rel_m = sysm.solve_free_block_device(block, b, tolerance=1e-10)
+    torch.cuda.synchronize()
+    rows, nnz = sysm.block_preconditioner_levels()
+    print(f"multigrid levels {rows} (nonzeros {nnz}); iterations {its_m} against the diagonal's {its_d}", flush=True)
+    assert len(rows) >= 3 and rows[0] == free.size and rows[-1] <= 256
+    assert all(rows[i + 1] < rows[i] for i in range(len(rows) - 1))
+    assert its_m * 10 <= its_d
+    ref = spsolve(k.tocsc(), b.cpu().numpy())
+    for x in (x_d, x_m):
+        assert np.linalg.norm(x.cpu().numpy() - ref) <= 1e-6 * np.linalg.norm(ref)
+    # new coordinates (the same ones) and a new partition rebuild the hierarchy alike
+    sysm.set_coords(coords)
+    assert sysm.block_preconditioner_levels() == ([], [])
+    x2, its2, rel2 = sysm.solve_free_block_device(block, b, tolerance=1e-10)
+    sysm.partition_device(torch.as_tensor(free, device=device))
+    x3, its3, rel3 = sysm.solve_free_block_device(block, b, tolerance=1e-10)
+    torch.cuda.synchronize()
+    assert its2 == its_m == its3 and rel2 == rel_m == rel3
+    assert torch.equal(x2, x_m) and torch.equal(x3, x_m)
