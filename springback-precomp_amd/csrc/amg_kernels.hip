@@ -152,9 +152,27 @@ struct Csr {
   const double* values;
 };
 
-// C = X Y on C's pattern, one 64-lane workgroup per row of C (windows of
-// kCap columns). smoothed: C is the smoothed prolongator, written as
-// T - omega D^-1 (A T) (X = A, Y = T).
+// The slot of column c among cols[0, mw) (ascending), or -1.
+__device__ __forceinline__ int find_slot(const int* cols, int mw, int c) {
+  int lo = 0, hi = mw;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (cols[mid] < c)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return (lo < mw && cols[lo] == c) ? lo : -1;
+}
+
+// C = X Y on C's pattern, one 64-lane workgroup (a single wavefront) per row
+// of C, in windows of kCap columns. X's entries of the row are read 64 at a
+// time (one per lane, with the bounds of their rows of Y) and taken in order;
+// the first 64 entries of the next entry's row of Y are loaded while the
+// current one is added. A wavefront's LDS accesses run in program order, so
+// an entry's additions see the previous entry's without a barrier.
+// smoothed: C is the smoothed prolongator, written as T - omega D^-1 (A T)
+// (X = A, Y = T).
 __global__ __launch_bounds__(64) void product_kernel(long long rows, Csr X, Csr Y, const int* __restrict__ cptr,
                                                      const int* __restrict__ cindex, double* __restrict__ cvalues,
                                                      int smoothed, AmgLevel L) {
@@ -181,31 +199,47 @@ __global__ __launch_bounds__(64) void product_kernel(long long rows, Csr X, Csr 
     }
     __syncthreads();
     const int lo_col = cols[0], hi_col = cols[mw - 1];
-    for (int t = x0; t < x1; ++t) {
-      const int k = X.index[t];
-      const double a = X.values[t];
-      const int y1 = Y.ptr[k + 1];
-      for (int u = Y.ptr[k] + lane; u < y1; u += 64) {
-        const int c = Y.index[u];
-        if (c < lo_col || c > hi_col) continue;
-        int lo = 0, hi = mw;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (cols[mid] < c)
-            lo = mid + 1;
-          else
-            hi = mid;
+    for (int b0 = x0; b0 < x1; b0 += 64) {
+      const int t = b0 + lane;
+      const bool in = t < x1;
+      const int k = in ? X.index[t] : 0;
+      const double a_l = in ? X.values[t] : 0.0;
+      const int yb_l = in ? Y.ptr[k] : 0;
+      const int ye_l = in ? Y.ptr[k + 1] : 0;
+      const int cnt = x1 - b0 < 64 ? x1 - b0 : 64;
+      int yb = __shfl(yb_l, 0, 64), ye = __shfl(ye_l, 0, 64);
+      int c_next = yb + lane < ye ? Y.index[yb + lane] : -1;
+      double v_next = yb + lane < ye ? Y.values[yb + lane] : 0.0;
+      for (int j = 0; j < cnt; ++j) {
+        const double a = __shfl(a_l, j, 64);
+        const int cb = yb, ce = ye;
+        const int c = c_next;
+        const double v = v_next;
+        if (j + 1 < cnt) {
+          yb = __shfl(yb_l, j + 1, 64);
+          ye = __shfl(ye_l, j + 1, 64);
+          c_next = yb + lane < ye ? Y.index[yb + lane] : -1;
+          v_next = yb + lane < ye ? Y.values[yb + lane] : 0.0;
         }
-        if (lo < mw && cols[lo] == c) acc[lo] = fma(a, Y.values[u], acc[lo]);
+        if (c >= lo_col && c <= hi_col) {
+          const int slot = find_slot(cols, mw, c);
+          if (slot >= 0) acc[slot] = fma(a, v, acc[slot]);
+        }
+        for (int u = cb + 64 + lane; u < ce; u += 64) {  // rows of Y longer than 64
+          const int cu = Y.index[u];
+          if (cu < lo_col || cu > hi_col) continue;
+          const int slot = find_slot(cols, mw, cu);
+          if (slot >= 0) acc[slot] = fma(a, Y.values[u], acc[slot]);
+        }
       }
-      __syncthreads();  // the next entry of X may add to the slots this one did
     }
+    __syncthreads();
     for (int s = lane; s < mw; s += 64) {
       double v = acc[s];
       if (smoothed) {
         const int j = w0 + s - own;  // the tentative column's place in T's row
-        const double t = (j >= 0 && j < tn) ? L.tp_values[t0 + j] : 0.0;
-        v = t - scale * v;
+        const double tv = (j >= 0 && j < tn) ? L.tp_values[t0 + j] : 0.0;
+        v = tv - scale * v;
       }
       cvalues[c0 + w0 + s] = v;
     }
@@ -222,8 +256,8 @@ __global__ __launch_bounds__(256) void gather_kernel(long long n, const int* __r
 // [A | I] -> [I | A^-1] by Gauss-Jordan with partial pivoting (the largest
 // |entry| of the column, the lowest row among equals); one workgroup.
 __global__ __launch_bounds__(1024) void dense_inverse_kernel(AmgLevel L) {
-  __shared__ double best[1024];
-  __shared__ int besti[1024];
+  __shared__ double best[16];
+  __shared__ int besti[16];
   __shared__ double factor[kAmgDenseMax];
   const int n = static_cast<int>(L.n);
   const int w = 2 * n;
@@ -247,22 +281,28 @@ __global__ __launch_bounds__(1024) void dense_inverse_kernel(AmgLevel L) {
         vi = i;
       }
     }
-    best[tid] = v;
-    besti[tid] = vi;
-    __syncthreads();
-    for (int half = 512; half > 0; half >>= 1) {
-      if (tid < half) {
-        const double o = best[tid + half];
-        const int oi = besti[tid + half];
-        if (o > best[tid] || (o == best[tid] && oi < besti[tid])) {
-          best[tid] = o;
-          besti[tid] = oi;
-        }
+    // the largest within each wavefront (lowest row among equals), then
+    // across the 16 wavefronts
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(v, off, 64);
+      const int oi = __shfl_xor(vi, off, 64);
+      if (o > v || (o == v && oi < vi)) {
+        v = o;
+        vi = oi;
       }
-      __syncthreads();
     }
-    const int p = besti[0];
-    const double pv = best[0];
+    if ((tid & 63) == 0) {
+      best[tid >> 6] = v;
+      besti[tid >> 6] = vi;
+    }
+    __syncthreads();
+    double pv = best[0];
+    int p = besti[0];
+    for (int wv = 1; wv < 16; ++wv)
+      if (best[wv] > pv || (best[wv] == pv && besti[wv] < p)) {
+        pv = best[wv];
+        p = besti[wv];
+      }
     if (!(pv > 0.0) || !std::isfinite(pv)) {  // every thread sees the same pivot
       if (tid == 0) L.coef[kAmgSingular] = 1.0;
       return;
@@ -279,11 +319,14 @@ __global__ __launch_bounds__(1024) void dense_inverse_kernel(AmgLevel L) {
     for (int c = tid; c < w; c += 1024) M[static_cast<long long>(j) * w + c] *= inv;
     for (int i = tid; i < n; i += 1024) factor[i] = M[static_cast<long long>(i) * w + j];
     __syncthreads();
-    for (long long e = tid; e < static_cast<long long>(n) * w; e += 1024) {
-      const int i = static_cast<int>(e / w);
-      if (i == j) continue;
-      const int c = static_cast<int>(e % w);
-      M[e] = fma(-factor[i], M[static_cast<long long>(j) * w + c], M[e]);
+    // a wavefront per row (16 at a time); the pivot row's columns left of j
+    // are zero already
+    const double* pj = M + static_cast<long long>(j) * w;
+    for (int i = tid >> 6; i < n; i += 16) {
+      const double f = factor[i];
+      if (i == j || f == 0.0) continue;
+      double* mi = M + static_cast<long long>(i) * w;
+      for (int c = j + (tid & 63); c < w; c += 64) mi[c] = fma(-f, pj[c], mi[c]);
     }
     __syncthreads();
   }
