@@ -390,6 +390,28 @@ __global__ __launch_bounds__(64) void adjoint_kernel(AdjointArgs Q) {
   team_sync();
   double x0[8][3], ue[24];
   load_element(A.coords, A.conn, A.u, e, x0, ue);
+  // the geometry records of the element's points at the converged modes (F,
+  // [G | G~], w), which the coupling rows read
+  if (p < n) {
+    PointGeom pg;
+    double fv = 0.0;
+    setup_point(x0, ue, A.rule.xi[p], A.rule.eta[p], A.rule.zeta[p], A.rule.w[p], pg, fv);
+    double alpha[9], gm[3][3], h[3][3], F[3][3];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) alpha[i] = A.alpha_out[e * 9 + i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) gm[i][j] = pg.g[i][8 + j];
+    mode_kinematics<KIN>(pg.hu, gm, alpha, h, F);
+    double* gr = A.geom_ws + (q0 + p) * kGeomRecord;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) gr[kGrF + i] = F[i / 3][i % 3];
+#pragma unroll
+    for (int i = 0; i < 33; ++i) gr[kGrG + i] = pg.g[i / 11][i % 11];
+    gr[kGrW] = pg.w;
+  }
+  team_sync_global();
   double w33[33];
 #pragma unroll
   for (int a = 0; a < 8; ++a) {

@@ -17,11 +17,12 @@
 //                       tangent (the point workspace), energy and yielding.
 //                       The last such pass of a converged element is its
 //                       final pass. Writes alpha and a status per element.
-//   condense_kernel     reads S and the tangent of the final pass back, forms
-//                       r_a, K_aa (the same arithmetic as the pass) and its
-//                       full-pivot LU, the coupling blocks K_ua, K_au, K_uu
-//                       and the static condensation
-//                       f* = f_u - K_ua K_aa^-1 r_a, K* = K_uu - K_ua K_aa^-1 K_au.
+//   condense_*_kernel   reads S and the tangent of the final pass back and
+//                       K_aa's factors of the final local iteration, forms the
+//                       coupling blocks K_ua, K_au, K_uu and the static
+//                       condensation f* = f_u - K_ua K_aa^-1 r_a,
+//                       K* = K_uu - K_ua K_aa^-1 K_au (team kernel: rules of
+//                       up to 8 points; wide kernel: 9..32 points).
 #include <algorithm>
 
 #include "element_kernels.hpp"
@@ -957,10 +958,10 @@ __global__ __launch_bounds__(256) void cost_scatter_kernel(const int* __restrict
 // ---------------------------------------------------------------------------
 // The condensation (IncompatibleModes.cpp:279-306, 549-561)
 // ---------------------------------------------------------------------------
-// A team's LDS: K_aa's factors and their composed permutations; K_au's 24
-// columns and r_a as a 25th (row-major, 9 x 25), solved in place into the
-// coupling K_aa^-1 K_au and K_aa^-1 r_a.
+// Two kernels, by the rule. Both write K* row-major or column-major
+// (ElementArgs::tangent_column_major).
 constexpr int kRhs = 25;
+
 struct CondenseSlot {
   LuSlot lu;
   LuPerm perm;
@@ -976,7 +977,7 @@ struct CondenseSlot {
 // in two column halves after the solve): each loop re-reads the records, and
 // fewer loops would hold more accumulators than the registers take.
 template <int KIN, bool TANGENT, bool SYM, int TW>
-__global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
+__global__ __launch_bounds__(64) void condense_team_kernel(ElementArgs A) {
   __shared__ CondenseSlot slots[teams_per_wave<TW>()];
   const int lane = threadIdx.x;
   const int t = lane / TW;
@@ -1139,7 +1140,9 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
     A.force[e * 24 + 3 * a + k] = ru[k] - s;
   }
   if constexpr (!TANGENT) return;
-  double* out = A.tangent + e * 576 + (3 * a) * 24;
+  // K*(r, c) at out[r * 24 + c] (row-major) or out[c * 24 + r]
+  double* out = A.tangent + e * 576;
+  const int rs = A.tangent_column_major ? 1 : 24, cs = A.tangent_column_major ? 24 : 1;
   // K_uu rows of node a in two column halves, K* = K_uu - K_ua coupling
 #pragma unroll
   for (int half = 0; half < 2; ++half) {
@@ -1193,23 +1196,272 @@ __global__ __launch_bounds__(64) void condense_kernel(ElementArgs A) {
         double s = kua[k][0] * T.region[j];
 #pragma unroll
         for (int nn = 1; nn < 9; ++nn) s = s + kua[k][nn] * T.region[nn * kRhs + j];
-        out[k * 24 + j] = kuu[k][jj] - s;
+        out[(3 * a + k) * rs + j * cs] = kuu[k][jj] - s;
       }
   }
   if constexpr (SYM) {
     // K = (K + K^T) / 2 (IncompatibleModes.cpp:557) through the output rows
     team_sync_global();
     double ks[3][24];
-    const double* base = A.tangent + e * 576;
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
-      for (int j = 0; j < 24; ++j) ks[k][j] = 0.5 * (out[k * 24 + j] + base[j * 24 + 3 * a + k]);
+      for (int j = 0; j < 24; ++j) ks[k][j] = 0.5 * (out[(3 * a + k) * rs + j * cs] + out[j * rs + (3 * a + k) * cs]);
     team_sync_global();
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
-      for (int j = 0; j < 24; ++j) out[k * 24 + j] = ks[k][j];
+      for (int j = 0; j < 24; ++j) out[(3 * a + k) * rs + j * cs] = ks[k][j];
+  }
+}
+
+
+// The wide kernel (rules of 9..32 points, where the team kernel's lane per
+// node would re-read many more point records): one element per workgroup (a
+// single wavefront). Over the 11 augmented nodes
+// (the 8 nodes, then the 3 mode groups; 3 components each) the element's
+// matrix K[(a,k),(b,l)] = sum_q w_q (B_q^T mat_q B_q + delta_kl g_a^T S_q g_b)
+// is accumulated in 3 x 3 blocks, point by point in point order, from
+// operands staged in LDS kChunk points at a time: B_q (B_NL at the converged
+// modes, 6 x 33), W_q = w_q mat_q B_q and w_q S_q g_b. Lane l = 8 a + b owns
+// the block K_uu(a, b); lanes 0..23 also own K_ua(a, m) (a = l / 3, m = l % 3),
+// lanes 24..47 K_au(m, b) (m = (l - 24) / 8, b = (l - 24) % 8; K_ua^T for a
+// symmetric material), lanes 48..55 rows 3a..3a+2 of f_u. K_aa's factors of
+// the final local iteration (the element record) then solve [K_au | r_a] in
+// place, a column per lane, and each lane forms its block of
+// K* = K_uu - K_ua K_aa^-1 K_au in LDS, from where the output is written in
+// either layout (with K* = (K* + K*^T) / 2 for a symmetric material,
+// IncompatibleModes.cpp:557). Each point record is read once.
+constexpr int kChunk = 2;
+constexpr int kAug = 33;  // 11 augmented nodes x 3 components
+constexpr int kOps = kChunk * 6 * kAug;
+
+template <int NP>
+struct CondenseLds {
+  double geo[NP][kGeomRecord];  // per point: F (9), [G | G~] (33), w
+  double ops[2 * kOps];         // B_q, then W_q, of the chunk; afterwards K* (576) and K_ua (216)
+  double rec[kChunk][kPointRecord];  // the chunk's point records: S, mat
+  double sg[kChunk][kAug];      // w_q S_q g_b, b = 0..10
+  LuSlot lu;
+  LuPerm perm;
+  double region[9 * kRhs];      // [K_au | r_a], then K_aa^-1 [K_au | r_a]
+};
+
+template <int KIN, bool TANGENT, bool SYM, int TW>
+__global__ __launch_bounds__(64) void condense_wide_kernel(ElementArgs A) {
+  constexpr int NP = kMaxPoints;
+  __shared__ CondenseLds<NP> L;
+  const int l = threadIdx.x;
+  const long long e = blockIdx.x;
+  if (e >= A.ne) return;
+  if (A.status[e] != kOk) {
+    // a failed element: zero force and tangent
+    if (l < 24) A.force[e * 24 + l] = 0.0;
+    if constexpr (TANGENT)
+      for (int i = l; i < 576; i += 64) A.tangent[e * 576 + i] = 0.0;
+    return;
+  }
+  const int n = A.rule.n;
+  const long long q0 = e * n;
+  // the geometry of point l at the converged modes (F with the pass's own
+  // arithmetic)
+  if (l < n) {
+    double x0[8][3], ue[24];
+    load_element(A.coords, A.conn, A.u, e, x0, ue);
+    PointGeom pg;
+    double fv = 0.0;
+    setup_point(x0, ue, A.rule.xi[l], A.rule.eta[l], A.rule.zeta[l], A.rule.w[l], pg, fv);
+    double alpha[9], gm[3][3], h[3][3], F[3][3];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) alpha[i] = A.alpha_out[e * 9 + i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) gm[i][j] = pg.g[i][8 + j];
+    mode_kinematics<KIN>(pg.hu, gm, alpha, h, F);
+    double* gr = L.geo[l];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) gr[kGrF + i] = F[i / 3][i % 3];
+#pragma unroll
+    for (int i = 0; i < 33; ++i) gr[kGrG + i] = pg.g[i / 11][i % 11];
+    gr[kGrW] = pg.w;
+  }
+  // K_aa's factors of the final local iteration and r_a (the 25th column)
+  const double* er = A.elem_ws + e * kElemRecord;
+  for (int i = l; i < 81; i += 64) L.lu.a[i] = er[kErFactors + i];
+  if (l < 9) {
+    L.lu.rt[l] = static_cast<int>(er[kErRows + l]);
+    L.lu.ct[l] = static_cast<int>(er[kErCols + l]);
+    L.region[l * kRhs + 24] = er[kErResidual + l];
+  }
+  // the lane's blocks: K_uu(a, b), and (r2, c2) in augmented nodes
+  const int a = l >> 3, b = l & 7;
+  int r2 = -1, c2 = 0;
+  if (l < 24) {
+    r2 = l / 3;
+    c2 = 8 + l % 3;
+  } else if (TANGENT && !SYM && l < 48) {
+    r2 = 8 + (l - 24) / 8;
+    c2 = (l - 24) % 8;
+  }
+  const int fa = l - 48;  // lanes 48..55: f_u rows of node fa
+  const bool fu = fa >= 0 && fa < 8;
+  double k1[3][3], k2[3][3], f3[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    f3[i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) k1[i][j] = k2[i][j] = 0.0;
+  }
+  double* B = L.ops;
+  double* W = L.ops + kOps;
+  for (int c0 = 0; c0 < n; c0 += kChunk) {
+    const int cn = n - c0 < kChunk ? n - c0 : kChunk;
+    __syncthreads();  // the geometry is written; the previous chunk's operands are used
+    for (int t = l; t < cn * kPointRecord; t += 64)
+      L.rec[t / kPointRecord][t % kPointRecord] = A.point_ws[(q0 + c0) * kPointRecord + t];
+    for (int t = l; t < cn * 6 * kAug; t += 64) {
+      const int q = t / (6 * kAug), r = t % (6 * kAug);
+      B[t] = bnl_rec(L.geo[c0 + q], r / kAug, (r % kAug) / 3, (r % kAug) % 3);
+    }
+    __syncthreads();
+    // W_q = w_q mat_q B_q (mat row-major in the record), w_q S_q g_b
+    for (int t = l; t < cn * 6 * kAug; t += 64) {
+      const int q = t / (6 * kAug), r = t % (6 * kAug), c = r / kAug, j = r % kAug;
+      const double* m = L.rec[q] + kWsTangent + c * 6;
+      const double* bq = B + q * 6 * kAug + j;
+      double s = m[0] * bq[0];
+#pragma unroll
+      for (int d = 1; d < 6; ++d) s = s + m[d] * bq[d * kAug];
+      W[t] = L.geo[c0 + q][kGrW] * s;
+    }
+    if constexpr (KIN != kSmall) {
+      for (int t = l; t < cn * 11; t += 64) {
+        const int q = t / 11, node = t % 11;
+        double gsa[3];
+        stress_times_gradient(L.geo[c0 + q], L.rec[q], node, gsa);
+        const double w = L.geo[c0 + q][kGrW];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) L.sg[q][3 * node + j] = w * gsa[j];
+      }
+    }
+    __syncthreads();
+    for (int q = 0; q < cn; ++q) {
+      const double* bq = B + q * 6 * kAug;
+      const double* wq = W + q * 6 * kAug;
+      const double* g = L.geo[c0 + q] + kGrG;
+      if constexpr (TANGENT) {
+        double s[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) s[i][j] = bq[3 * a + i] * wq[3 * b + j];
+#pragma unroll
+        for (int c = 1; c < 6; ++c)
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) s[i][j] = s[i][j] + bq[c * kAug + 3 * a + i] * wq[c * kAug + 3 * b + j];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) k1[i][j] = k1[i][j] + s[i][j];
+        if constexpr (KIN != kSmall) {
+          const double geo = g[0 * 11 + a] * L.sg[q][3 * b] + g[1 * 11 + a] * L.sg[q][3 * b + 1] +
+                             g[2 * 11 + a] * L.sg[q][3 * b + 2];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) k1[i][i] = k1[i][i] + geo;
+        }
+      }
+      if (r2 >= 0) {
+        double s[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) s[i][j] = bq[3 * r2 + i] * wq[3 * c2 + j];
+#pragma unroll
+        for (int c = 1; c < 6; ++c)
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) s[i][j] = s[i][j] + bq[c * kAug + 3 * r2 + i] * wq[c * kAug + 3 * c2 + j];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) k2[i][j] = k2[i][j] + s[i][j];
+        if constexpr (KIN != kSmall) {
+          const double geo = g[0 * 11 + r2] * L.sg[q][3 * c2] + g[1 * 11 + r2] * L.sg[q][3 * c2 + 1] +
+                             g[2 * 11 + r2] * L.sg[q][3 * c2 + 2];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) k2[i][i] = k2[i][i] + geo;
+        }
+      }
+      if (fu) {
+        const double w = L.geo[c0 + q][kGrW];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          double s = bq[3 * fa + i] * L.rec[q][kWsStress];
+#pragma unroll
+          for (int c = 1; c < 6; ++c) s = s + bq[c * kAug + 3 * fa + i] * L.rec[q][kWsStress + c];
+          f3[i] = f3[i] + w * s;
+        }
+      }
+    }
+  }
+  __syncthreads();  // the operands are used: K_ua takes their place
+  double* kua = L.ops + 576;  // K_ua, 24 x 9 row-major
+  if (l < 24) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) kua[(3 * r2 + i) * 9 + 3 * (c2 - 8) + j] = k2[i][j];
+  } else if (TANGENT && !SYM && l < 48) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) L.region[(3 * (r2 - 8) + i) * kRhs + 3 * c2 + j] = k2[i][j];
+  }
+  if (l == 0) lu_perm(L.lu.rt, L.lu.ct, L.perm);
+  __syncthreads();
+  if constexpr (TANGENT && SYM) {
+    // K_au = K_ua^T
+    for (int t = l; t < 216; t += 64) L.region[(t % 9) * kRhs + t / 9] = kua[t];
+    __syncthreads();
+  }
+  // K_aa^-1 [K_au | r_a] in place, a column per lane
+  if (l == 24 || (TANGENT && l < 24)) lu_solve_column(L.lu.a, L.perm, L.region, kRhs, l);
+  __syncthreads();
+  // f* = f_u - K_ua K_aa^-1 r_a
+  if (fu) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double* row = kua + (3 * fa + i) * 9;
+      double s = row[0] * L.region[24];
+#pragma unroll
+      for (int nn = 1; nn < 9; ++nn) s = s + row[nn] * L.region[nn * kRhs + 24];
+      A.force[e * 24 + 3 * fa + i] = f3[i] - s;
+    }
+  }
+  if constexpr (!TANGENT) return;
+  // K*(a, b) = K_uu(a, b) - K_ua K_aa^-1 K_au
+  double* ks = L.ops;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double* row = kua + (3 * a + i) * 9;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double s = row[0] * L.region[3 * b + j];
+#pragma unroll
+      for (int nn = 1; nn < 9; ++nn) s = s + row[nn] * L.region[nn * kRhs + 3 * b + j];
+      ks[(3 * a + i) * 24 + 3 * b + j] = k1[i][j] - s;
+    }
+  }
+  __syncthreads();
+  double* out = A.tangent + e * 576;
+  const bool cm = A.tangent_column_major != 0;
+  for (int t = l; t < 576; t += 64) {
+    const int r = cm ? t % 24 : t / 24, c = cm ? t / 24 : t % 24;
+    out[t] = SYM ? 0.5 * (ks[r * 24 + c] + ks[c * 24 + r]) : ks[r * 24 + c];
   }
 }
 
@@ -1476,14 +1728,25 @@ int launch_local_solve(const ElementArgs& args, hipStream_t stream) {
   return static_cast<int>(hipGetLastError());
 }
 
+template <int KIN, bool TANGENT, bool SYM, int TW>
+void launch_condense_one(const ElementArgs& a, hipStream_t stream) {
+  if constexpr (TW == kTeam) {
+    const unsigned int blocks = static_cast<unsigned int>((a.ne + kTeam - 1) / kTeam);  // 8 teams per wavefront
+    hipLaunchKernelGGL((condense_team_kernel<KIN, TANGENT, SYM, TW>), dim3(blocks), dim3(64), 0, stream, a);
+  } else {
+    const unsigned int blocks = static_cast<unsigned int>(a.ne);  // an element per workgroup
+    hipLaunchKernelGGL((condense_wide_kernel<KIN, TANGENT, SYM, TW>), dim3(blocks), dim3(64), 0, stream, a);
+  }
+}
+
 template <int KIN, int TW>
-void launch_condense_kin(const ElementArgs& a, bool tangent, unsigned int blocks, hipStream_t stream) {
+void launch_condense_kin(const ElementArgs& a, bool tangent, hipStream_t stream) {
   if (!tangent)
-    hipLaunchKernelGGL((condense_kernel<KIN, false, true, TW>), dim3(blocks), dim3(64), 0, stream, a);
+    launch_condense_one<KIN, false, true, TW>(a, stream);
   else if (a.mat.symmetric)
-    hipLaunchKernelGGL((condense_kernel<KIN, true, true, TW>), dim3(blocks), dim3(64), 0, stream, a);
+    launch_condense_one<KIN, true, true, TW>(a, stream);
   else
-    hipLaunchKernelGGL((condense_kernel<KIN, true, false, TW>), dim3(blocks), dim3(64), 0, stream, a);
+    launch_condense_one<KIN, true, false, TW>(a, stream);
 }
 
 template <int KIN, int TW>
@@ -1521,25 +1784,22 @@ int launch_element_stress(const ElementArgs& a, double* out, hipStream_t stream)
 }
 
 template <int TW>
-void launch_condense_tw(const ElementArgs& a, bool tangent, unsigned int blocks, hipStream_t stream) {
+void launch_condense_tw(const ElementArgs& a, bool tangent, hipStream_t stream) {
   switch (a.kinematics) {
-    case kSmall: launch_condense_kin<kSmall, TW>(a, tangent, blocks, stream); break;
-    case kFinite: launch_condense_kin<kFinite, TW>(a, tangent, blocks, stream); break;
-    default: launch_condense_kin<kLog, TW>(a, tangent, blocks, stream); break;
+    case kSmall: launch_condense_kin<kSmall, TW>(a, tangent, stream); break;
+    case kFinite: launch_condense_kin<kFinite, TW>(a, tangent, stream); break;
+    default: launch_condense_kin<kLog, TW>(a, tangent, stream); break;
   }
 }
 
 int launch_condense(const ElementArgs& a, bool tangent, hipStream_t stream) {
   (void)hipGetLastError();
   if (a.rule.n < 1 || a.rule.n > kMaxPoints) return static_cast<int>(hipErrorInvalidValue);
-  const int tw = team_width(a.rule.n);
-  const int teams = 64 / tw;
-  const unsigned int blocks = static_cast<unsigned int>((a.ne + teams - 1) / teams);
-  if (blocks == 0) return 0;
-  if (tw == kTeam)
-    launch_condense_tw<kTeam>(a, tangent, blocks, stream);
+  if (a.ne == 0) return 0;
+  if (team_width(a.rule.n) == kTeam)
+    launch_condense_tw<kTeam>(a, tangent, stream);
   else
-    launch_condense_tw<kWideTeam>(a, tangent, blocks, stream);
+    launch_condense_tw<kWideTeam>(a, tangent, stream);
   return static_cast<int>(hipGetLastError());
 }
 

@@ -30,7 +30,8 @@ struct ElementArgs {
   double* alpha_out;  // 9 per element
   double* state_out;  // trial history, same layout as state_in
   double* force;      // 24 per element
-  double* tangent;    // 576 per element (row-major 24 x 24), or null
+  double* tangent;    // 576 per element (24 x 24), or null
+  int tangent_column_major;  // the tangent's layout: 0 row-major, 1 column-major
   double* energy;     // per element
   int* yielding;      // per element
   int* iterations;    // local Newton iterations per element
@@ -40,8 +41,8 @@ struct ElementArgs {
   // Workspace (device scratch of one call), three regions:
   //   point_ws  [npts][42]: S (6), then the tangent (36, row-major) of each
   //             point's final local pass (written by the local solve)
-  //   geom_ws   [npts][44]: F (9), [G | G~] (33), w of each point at the
-  //             converged alpha (condensation scratch)
+  //   geom_ws   [npts][44]: reserved (the condensation keeps each point's
+  //             F (9), [G | G~] (33) and w at the converged alpha in LDS)
   //   elem_ws   [ne][108]:  K_aa's full-pivot LU factors (81, row-major),
   //             r_a (9), row and column transpositions (9 + 9, as doubles)
   //             of the final local iteration

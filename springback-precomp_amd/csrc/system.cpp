@@ -42,6 +42,15 @@ void dfree(T*& p) {
   p = nullptr;
 }
 
+// 24 x 24 element tangents, column-major to row-major, in place (host).
+void transpose_tangents(double* ke, int64_t count) {
+  for (int64_t e = 0; e < count; ++e) {
+    double* m = ke + 576 * e;
+    for (int i = 0; i < 24; ++i)
+      for (int j = i + 1; j < 24; ++j) std::swap(m[i * 24 + j], m[j * 24 + i]);
+  }
+}
+
 void release_partition(sparlab_system* s) {
   delete s->amg;
   s->amg = nullptr;
@@ -101,6 +110,9 @@ int system_enqueue(sparlab_system* s, const double* u, int want_tangent, int col
   a.state_out = s->state[1 - s->cur];
   a.force = s->fe;
   a.tangent = want_tangent ? s->ke : nullptr;
+  // the element tangents in the layout of the assembled values (the
+  // assembly then copies node blocks straight through)
+  a.tangent_column_major = column_major ? 1 : 0;
   a.energy = s->energy;
   a.yielding = s->yielding;
   a.iterations = s->iterations;
@@ -112,7 +124,7 @@ int system_enqueue(sparlab_system* s, const double* u, int want_tangent, int col
   ForceArgs f{s->fe, s->node_ptr, s->node_incidence, static_cast<long long>(s->nn), fint, s->gross};
   if (launch_gather_force(f, st) != 0) return capi_fail(SPARLAB_E_DEVICE, "force gather launch failed");
   if (want_tangent) {
-    if (launch_assemble_values(assembly_args(s, s->ke, column_major, values, 0), st) != 0)
+    if (launch_assemble_values(assembly_args(s, s->ke, column_major, values, column_major), st) != 0)
       return capi_fail(SPARLAB_E_DEVICE, "assembly launch failed");
   }
   const int parts = static_cast<int>(std::min<int64_t>(1024, (3 * s->nn + 4095) / 4096 + 1));
@@ -123,6 +135,7 @@ int system_enqueue(sparlab_system* s, const double* u, int want_tangent, int col
     return capi_fail(SPARLAB_E_DEVICE, "reduction launch failed");
   s->evaluated = true;
   s->last_tangent = want_tangent != 0;
+  if (want_tangent) s->ke_column_major = column_major != 0;
   s->adjoint_ready = true;
   return SPARLAB_OK;
 }
@@ -834,6 +847,7 @@ int sparlab_system_element_outputs(const sparlab_system* s, double* fe, double* 
   if ((fe && hipMemcpy(fe, s->fe, 24 * s->ne * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) ||
       (ke && hipMemcpy(ke, s->ke, 576 * s->ne * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
     return sbk::capi_fail(SPARLAB_E_DEVICE, "download failed");
+  if (ke && s->ke_column_major) transpose_tangents(ke, s->ne);
   return SPARLAB_OK;
 }
 
@@ -911,6 +925,7 @@ int sparlab_system_element_outputs_range(const sparlab_system* s, int64_t first,
   if ((fe && hipMemcpy(fe, s->fe + 24 * first, 24 * count * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) ||
       (ke && hipMemcpy(ke, s->ke + 576 * first, 576 * count * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
     return sbk::capi_fail(SPARLAB_E_DEVICE, "download failed");
+  if (ke && s->ke_column_major) transpose_tangents(ke, count);
   return SPARLAB_OK;
 }
 

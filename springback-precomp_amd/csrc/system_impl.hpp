@@ -63,6 +63,7 @@ struct sparlab_system {
   long long* summary = nullptr; // failed count, lowest failed element, yielding points
   bool evaluated = false;
   bool last_tangent = false;
+  bool ke_column_major = false;  // the layout of the last evaluation's element tangents
   // the adjoint: per-element products and whether the workspace still holds
   // the last evaluation (cleared by commit and by new coordinates, material
   // or history)
