@@ -327,7 +327,7 @@ int sparlab_system_free_block_device(sparlab_system* system, const double* value
  * non-symmetric block (the reference: UMFPACK or SparseLU), from x = 0, until
  * the true residual satisfies ||b - K x|| <= tolerance ||b|| or
  * max_iterations pass (the iteration's own residual is read every 16
- * iterations; when it passes, the true one is computed, and the iteration
+ * iterations, every 4 with the multigrid preconditioner; when it passes, the true one is computed, and the iteration
  * restarts from it when that does not). *iterations (host) receives the
  * iterations run and *relative_residual (host) the final true residual
  * ||b - K x|| / ||b||. SPARLAB_E_SOLVER when the iteration breaks down (a

@@ -382,6 +382,10 @@ enum SolveEnd : int32_t { kSolved = 0, kNotDefinite = 1, kBrokeDown = 2, kNotCon
 bool run_block_solve(sbk::CgArgs& a, int symmetric, const double* b, double* x, double tolerance,
                      int64_t max_iterations, hipStream_t st, int64_t* iterations, double* relative_residual,
                      int32_t* end) {
+  // iterations between two reads of the residual: 16 with the diagonal, 4
+  // with the multigrid cycle (an iteration costs a V-cycle or two, a read a
+  // synchronisation; a solve then overshoots its tolerance by fewer cycles)
+  const int64_t kRun = a.external ? 4 : 16;
   const int nseg = a.nseg;
   std::vector<double> sc(static_cast<std::size_t>(nseg) * sbk::kCgScalars);
   auto read = [&]() {
@@ -417,7 +421,6 @@ bool run_block_solve(sbk::CgArgs& a, int symmetric, const double* b, double* x, 
     relative_residual[p] = 0.0;
   }
   int64_t done = 0;
-  constexpr int64_t kRun = 16;  // iterations between two reads of the residual
   // The iteration's residual drifts from b - K x; when it meets the
   // tolerance the true residual is computed, and the iteration restarts from
   // it (residual replacement) unless that meets the tolerance too.

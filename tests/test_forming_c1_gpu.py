@@ -106,13 +106,14 @@ class DeviceSide:
         if tool_on and self.trial_hist is not None:
             self.hist = self.trial_hist
 
-    def device_solver(self, free, tolerance, outer, inner, symmetric):
+    def device_solver(self, free, tolerance, outer, inner, symmetric, precond="diagonal"):
         """K_ff du = -r_f on the device: the partition's free-free block of
         the tangent's values and the block solve (conjugate gradients for a
         symmetric tangent, checked; BiCGStab on the row-major values, K_ff's
         rows, otherwise)."""
         torch = self.torch
         free_idx = np.nonzero(free)[0].astype(np.int32)
+        self.sys.set_block_preconditioner(precond)
         self.sys.partition_device(self.t(free_idx, torch.int32))
         n = outer.size - 1
         if not hasattr(self, "solves"):
@@ -287,7 +288,8 @@ def test_c1_newton_histories_and_release_heights():
     assert np.abs(z_d[hit_d] - z_o[hit_o]).max() <= 1e-9  # 1e-6 mm
 
 
-def test_c1_with_the_device_block_solve():
+@pytest.mark.parametrize("precond", ["diagonal", "multigrid"])
+def test_c1_with_the_device_block_solve(precond):
     """The C1 case again with every Newton correction of the device side
     solved on the device: the partition's free-free block of the tangent and
     the block solve to 1e-10, BiCGStab for the loading (frictional contact
@@ -300,7 +302,8 @@ def test_c1_with_the_device_block_solve():
     right-hand side; late in an increment the right-hand side is so small that
     the round-off of b - K x itself is about 1e-11 of it, so a tighter
     tolerance is not always attainable), and the released heights within
-    1e-6 mm."""
+    1e-6 mm. Both preconditioners: the block's diagonal and the multigrid
+    V-cycle (built on the block's symmetric part for BiCGStab)."""
     coords, conn, m, top, free = c1_model()
     nn = coords.shape[0]
     outer, inner, _ = O.pattern(nn, conn)
@@ -309,7 +312,7 @@ def test_c1_with_the_device_block_solve():
     u_d = np.zeros(3 * nn)
     u_o = np.zeros(3 * nn)
     centre0 = (0.0, 0.0, R)
-    loading_solve = dev.device_solver(free, 1e-10, outer, inner, symmetric=False)
+    loading_solve = dev.device_solver(free, 1e-10, outer, inner, symmetric=False, precond=precond)
     for inc in range(1, 11):
         centre = (0.0, 0.0, R - STEP * inc)
         u_d, h_d, s_d = newton(dev, u_d, free, outer, inner, centre, centre0, solve=loading_solve)
@@ -320,7 +323,7 @@ def test_c1_with_the_device_block_solve():
                 assert abs(a - b) <= 1e-6 * b, (inc, h_d, h_o)
         centre0 = centre
     load_d, load_o = dev.contact_residual.copy(), ora.contact_residual.copy()
-    release_solve = dev.device_solver(free, 1e-10, outer, inner, symmetric=True)
+    release_solve = dev.device_solver(free, 1e-10, outer, inner, symmetric=True, precond=precond)
     for step in range(9, -1, -1):
         lam = step / 10.0
         u_d, h_d, s_d = newton(dev, u_d, free, outer, inner, centre0, centre0, tool_on=False, load=lam * load_d,
@@ -335,4 +338,5 @@ def test_c1_with_the_device_block_solve():
     heights_d = coords[top, 2] + u_d[3 * top + 2]
     heights_o = coords[top, 2] + u_o[3 * top + 2]
     assert np.abs(heights_d - heights_o).max() <= 1e-9  # 1e-6 mm
-    print("device block solves (symmetric, iterations, relative residual):", dev.solves)
+    print(f"device block solves with the {precond} preconditioner (symmetric, iterations, relative residual):",
+          dev.solves)
