@@ -33,7 +33,6 @@ namespace sbk {
 
 namespace {
 
-constexpr int kTeams = 8;
 
 // The point's share of w^T dr/dy for all 24 directions y of the element's
 // coordinates (WRT_U false) or displacements (true), out[3 b + l] for node b,
@@ -356,13 +355,17 @@ __device__ void point_parameters(const AdjointArgs& Q, const double x0[8][3], co
 
 }  // namespace
 
-template <int KIN>
+// A team of TW lanes per element (8 for rules of up to 8 points, 32 for the
+// 9..32-point box rules): lane p owns point p; lanes 0..7 also own node p's
+// rows (K_ua's rows for lambda, the node's gradient entries).
+template <int KIN, int TW>
 __global__ __launch_bounds__(64) void adjoint_kernel(AdjointArgs Q) {
+  constexpr int kTeams = 64 / TW;
   __shared__ LuSlot slots[kTeams];
   const ElementArgs& A = Q.e;
   const int lane = threadIdx.x;
-  const int t = lane >> 3;
-  const int p = lane & 7;
+  const int t = lane / TW;
+  const int p = lane % TW;
   LuSlot& T = slots[t];
   const long long e = static_cast<long long>(blockIdx.x) * kTeams + t;
   if (e >= A.ne) return;
@@ -370,22 +373,21 @@ __global__ __launch_bounds__(64) void adjoint_kernel(AdjointArgs Q) {
   const long long q0 = e * n;
   if (A.status[e] != kOk) {
     // a failed element contributes nothing
-    for (int k = 0; k < 3; ++k) {
-      if (Q.grad_x) Q.grad_x[e * 24 + 3 * p + k] = 0.0;
-      if (Q.grad_u) Q.grad_u[e * 24 + 3 * p + k] = 0.0;
-    }
+    if (p < 8)
+      for (int k = 0; k < 3; ++k) {
+        if (Q.grad_x) Q.grad_x[e * 24 + 3 * p + k] = 0.0;
+        if (Q.grad_u) Q.grad_u[e * 24 + 3 * p + k] = 0.0;
+      }
     if (Q.grad_p)
-      for (int k = p; k < kAdjointParams; k += kTeams) Q.grad_p[k * A.ne + e] = 0.0;
+      for (int k = p; k < kAdjointParams; k += TW) Q.grad_p[k * A.ne + e] = 0.0;
     return;
   }
   // K_aa's factors of the final local iteration
   const double* er = A.elem_ws + e * kElemRecord;
-  for (int i = p; i < 81; i += 8) T.a[i] = er[kErFactors + i];
-  T.rt[p] = static_cast<int>(er[kErRows + p]);
-  T.ct[p] = static_cast<int>(er[kErCols + p]);
-  if (p == 0) {
-    T.rt[8] = static_cast<int>(er[kErRows + 8]);
-    T.ct[8] = static_cast<int>(er[kErCols + 8]);
+  for (int i = p; i < 81; i += TW) T.a[i] = er[kErFactors + i];
+  for (int i = p; i < 9; i += TW) {
+    T.rt[i] = static_cast<int>(er[kErRows + i]);
+    T.ct[i] = static_cast<int>(er[kErCols + i]);
   }
   team_sync();
   double x0[8][3], ue[24];
@@ -420,16 +422,17 @@ __global__ __launch_bounds__(64) void adjoint_kernel(AdjointArgs Q) {
     for (int k = 0; k < 3; ++k) w33[3 * a + k] = Q.v[node * 3 + k];
   }
   {
-    // lambda = K_aa^-T K_ua^T v_e (lane a holds K_ua's rows 3a..3a+2)
+    // lambda = K_aa^-T K_ua^T v_e (lane a < 8 holds K_ua's rows 3a..3a+2)
     double ru[3], kua[3][9];
-    coupling_rows<KIN>(A, q0, n, p, ru, kua);
+    const int a = p < 8 ? p : 0;
+    coupling_rows<KIN>(A, q0, n, a, ru, kua);
     double g9[9], lam[9];
 #pragma unroll
     for (int nn = 0; nn < 9; ++nn) {
-      double s = kua[0][nn] * w33[3 * p];
-      s = s + kua[1][nn] * w33[3 * p + 1];
-      s = s + kua[2][nn] * w33[3 * p + 2];
-      g9[nn] = team_sum(s);
+      double s = kua[0][nn] * w33[3 * a];
+      s = s + kua[1][nn] * w33[3 * a + 1];
+      s = s + kua[2][nn] * w33[3 * a + 2];
+      g9[nn] = team_sum<TW>(p < 8 ? s : 0.0);
     }
     lu_solve_transposed(T, g9, lam);
 #pragma unroll
@@ -461,11 +464,12 @@ __global__ __launch_bounds__(64) void adjoint_kernel(AdjointArgs Q) {
     double mine[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < 24; ++j) {
-      const double tot = team_sum(c[j]);
+      const double tot = team_sum<TW>(c[j]);
       if (j / 3 == p) mine[j % 3] = tot;
     }
+    if (p < 8)
 #pragma unroll
-    for (int k = 0; k < 3; ++k) Q.grad_x[e * 24 + 3 * p + k] = mine[k];
+      for (int k = 0; k < 3; ++k) Q.grad_x[e * 24 + 3 * p + k] = mine[k];
   }
   if (Q.wrt & kWrtDisplacement) {
     double c[24];
@@ -478,11 +482,12 @@ __global__ __launch_bounds__(64) void adjoint_kernel(AdjointArgs Q) {
     double mine[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int j = 0; j < 24; ++j) {
-      const double tot = team_sum(c[j]);
+      const double tot = team_sum<TW>(c[j]);
       if (j / 3 == p) mine[j % 3] = tot;
     }
+    if (p < 8)
 #pragma unroll
-    for (int k = 0; k < 3; ++k) Q.grad_u[e * 24 + 3 * p + k] = mine[k];
+      for (int k = 0; k < 3; ++k) Q.grad_u[e * 24 + 3 * p + k] = mine[k];
   }
   if (Q.wrt & kWrtParams) {
     double c[kAdjointParams];
@@ -496,21 +501,31 @@ __global__ __launch_bounds__(64) void adjoint_kernel(AdjointArgs Q) {
     }
 #pragma unroll
     for (int k = 0; k < kAdjointParams; ++k) {
-      const double tot = team_sum(c[k]);
-      if (p == (k & 7)) Q.grad_p[k * A.ne + e] = tot;
+      const double tot = team_sum<TW>(c[k]);
+      if (p == k % TW) Q.grad_p[k * A.ne + e] = tot;
     }
+  }
+}
+
+template <int TW>
+void launch_adjoint_tw(const AdjointArgs& a, hipStream_t stream) {
+  constexpr int teams = 64 / TW;
+  const unsigned int blocks = static_cast<unsigned int>((a.e.ne + teams - 1) / teams);
+  switch (a.e.kinematics) {
+    case kSmall: hipLaunchKernelGGL((adjoint_kernel<kSmall, TW>), dim3(blocks), dim3(64), 0, stream, a); break;
+    case kFinite: hipLaunchKernelGGL((adjoint_kernel<kFinite, TW>), dim3(blocks), dim3(64), 0, stream, a); break;
+    default: hipLaunchKernelGGL((adjoint_kernel<kLog, TW>), dim3(blocks), dim3(64), 0, stream, a); break;
   }
 }
 
 int launch_adjoint(const AdjointArgs& a, hipStream_t stream) {
   (void)hipGetLastError();
-  const unsigned int blocks = static_cast<unsigned int>((a.e.ne + kTeams - 1) / kTeams);
-  if (blocks == 0) return 0;
-  switch (a.e.kinematics) {
-    case kSmall: hipLaunchKernelGGL(adjoint_kernel<kSmall>, dim3(blocks), dim3(64), 0, stream, a); break;
-    case kFinite: hipLaunchKernelGGL(adjoint_kernel<kFinite>, dim3(blocks), dim3(64), 0, stream, a); break;
-    default: hipLaunchKernelGGL(adjoint_kernel<kLog>, dim3(blocks), dim3(64), 0, stream, a); break;
-  }
+  if (a.e.ne == 0) return 0;
+  if (a.e.rule.n < 1 || a.e.rule.n > kMaxPoints) return static_cast<int>(hipErrorInvalidValue);
+  if (a.e.rule.n <= kTeam)
+    launch_adjoint_tw<kTeam>(a, stream);
+  else
+    launch_adjoint_tw<kWideTeam>(a, stream);
   return static_cast<int>(hipGetLastError());
 }
 

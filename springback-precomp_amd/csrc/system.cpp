@@ -794,8 +794,6 @@ int sparlab_system_vjp_device(sparlab_system* s, const double* u, const double* 
   if (!s || !u || !v) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "null argument");
   if (!s->adjoint_ready)
     return sbk::capi_fail(SPARLAB_E_ARGUMENT, "the adjoint needs the last evaluation (before commit or new inputs)");
-  if (s->points > sbk::kTeam)
-    return sbk::capi_fail(SPARLAB_E_UNSUPPORTED, "the adjoint kernel takes integration rules of at most 8 points");
   if (((wrt & SPARLAB_VJP_COORDS) && !grad_coords) || ((wrt & SPARLAB_VJP_DISPLACEMENT) && !grad_disp) ||
       ((wrt & SPARLAB_VJP_PARAMETERS) && !grad_params))
     return sbk::capi_fail(SPARLAB_E_ARGUMENT, "a requested gradient has no buffer");

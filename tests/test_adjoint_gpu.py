@@ -13,7 +13,9 @@ Checks (the reference has no adjoint; these pin it by mathematics):
     at fixed u leaves f_int unchanged, so the node gradients sum to zero);
   * parameters (r0, r45, r90, C_i, gamma_i): central finite differences with
     a relative step of 1e-6 (1e-6 relative);
-  * the torch.autograd.Function wrapper returns the same products.
+  * the torch.autograd.Function wrapper returns the same products;
+  * the 2 x 2 x 5 box rule (20 points, 32-lane teams) passes the first three
+    checks as the 2 x 2 x 2 rule does.
 """
 import numpy as np
 import pytest
@@ -37,12 +39,15 @@ def sheet(nx=4, ny=4, nz=2):
 KINEMATICS = [G.SMALL_STRAIN, G.FINITE, G.FINITE_LOGARITHMIC]
 
 
-def prepared_system(device, material=None, kinematics=G.FINITE_LOGARITHMIC):
+RULES = {"2x2x2": (2, 0, 2), "2x2x5": (2, 5, 2)}
+
+
+def prepared_system(device, material=None, kinematics=G.FINITE_LOGARITHMIC, rule="2x2x2"):
     """A system with a plastic committed history, evaluated at the next increment."""
     import torch
     coords, conn, u1, u2 = sheet()
     m = material or W.microbench_material()
-    s = G.NonlinearSystem(coords, conn, m, G.Rule(2, 0, 2), kinematics)
+    s = G.NonlinearSystem(coords, conn, m, G.Rule(*RULES[rule]), kinematics)
     s.evaluate(u1)
     s.commit()
     ev = s.evaluate(u2)
@@ -62,10 +67,11 @@ def fint_at(s, u_dev, coords=None, material=None):
     return f.cpu().numpy()
 
 
+@pytest.mark.parametrize("rule", list(RULES))
 @pytest.mark.parametrize("kinematics", KINEMATICS)
-def test_displacement_vjp_is_tangent_transpose(device, kinematics):
+def test_displacement_vjp_is_tangent_transpose(device, kinematics, rule):
     import torch
-    s, coords, conn, u, ev = prepared_system(device, kinematics=kinematics)
+    s, coords, conn, u, ev = prepared_system(device, kinematics=kinematics, rule=rule)
     row_ptr, col_idx = s.pattern()
     n = 3 * s.num_nodes
     vals = ev.tangent_values
@@ -80,10 +86,11 @@ def test_displacement_vjp_is_tangent_transpose(device, kinematics):
     assert np.abs(got - ktv).max() <= 1e-10 * np.abs(ktv).max()
 
 
+@pytest.mark.parametrize("rule", list(RULES))
 @pytest.mark.parametrize("kinematics", KINEMATICS)
-def test_coordinate_vjp_matches_central_differences(device, kinematics):
+def test_coordinate_vjp_matches_central_differences(device, kinematics, rule):
     import torch
-    s, coords, conn, u, _ = prepared_system(device, kinematics=kinematics)
+    s, coords, conn, u, _ = prepared_system(device, kinematics=kinematics, rule=rule)
     rng = np.random.default_rng(6)
     v = rng.normal(size=3 * s.num_nodes)
     g = s.vjp_device(u, torch.as_tensor(v, device=device), coords=True)["coords"].cpu().numpy()
@@ -110,10 +117,11 @@ def test_coordinate_vjp_translation_invariance(device, kinematics):
     assert np.abs(g.sum(axis=0)).max() <= 1e-9 * np.abs(g).sum(axis=0).max()
 
 
-def test_parameter_vjp_matches_central_differences(device):
+@pytest.mark.parametrize("rule", list(RULES))
+def test_parameter_vjp_matches_central_differences(device, rule):
     import torch
     m = W.microbench_material()
-    s, coords, conn, u, _ = prepared_system(device, m)
+    s, coords, conn, u, _ = prepared_system(device, m, rule=rule)
     v = np.random.default_rng(8).normal(size=3 * s.num_nodes)
     g = s.vjp_device(u, torch.as_tensor(v, device=device), coords=False, parameters=True)["parameters"].cpu().numpy()
     theta = G.material_parameters(m)

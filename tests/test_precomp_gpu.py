@@ -14,7 +14,8 @@ the load that holds a bending field in equilibrium) against central
 differences of J over equilibria re-solved by Newton with the device block
 solve at X +- h d, h = 1e-4 of the element size: elastic (a symmetric
 tangent: conjugate gradients for K^T) and plastic (Hill48 + Chaboche:
-BiCGStab on the transposed non-symmetric block).
+BiCGStab on the transposed non-symmetric block), the plastic case also with
+the 2 x 2 x 5 box rule of the reference's fine-mesh decks.
 """
 import copy
 import json
@@ -125,7 +126,7 @@ def solve(s, coords, free, u0, load, device, symmetric):
     return u, values
 
 
-@pytest.mark.parametrize("case", ["elastic", "plastic"])
+@pytest.mark.parametrize("case", ["elastic", "plastic", "plastic_2x2x5"])
 def test_shape_sensitivity_matches_central_differences(device, case):
     """The base state u* is a bending field (every node on a parabola of the
     tip deflection, the clamped end and the tip's z prescribed); the load is
@@ -139,7 +140,8 @@ def test_shape_sensitivity_matches_central_differences(device, case):
     else:
         m = W.microbench_material()
         deflection = 1.5e-4
-    s = G.NonlinearSystem(coords, conn, m)
+    rule = G.Rule(2, 5, 2) if case.endswith("2x2x5") else G.Rule(2, 0, 2)
+    s = G.NonlinearSystem(coords, conn, m, rule)
     sym = m.symmetric_tangent
     x = coords[:, 0] / 0.012
     ustar = np.zeros_like(coords)
@@ -150,7 +152,7 @@ def test_shape_sensitivity_matches_central_differences(device, case):
     f = torch.zeros_like(u0)
     s.evaluate_device(u0, f, None, want_tangent=False)
     ev = s.result()
-    assert (ev.yielding_points > 0) == (case == "plastic")
+    assert (ev.yielding_points > 0) == case.startswith("plastic")
     load = f[d_free].clone()
     u, values = solve(s, coords, free, u0, load, device, sym)
     assert torch.equal(u, u0)   # already in equilibrium
