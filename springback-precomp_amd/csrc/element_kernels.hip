@@ -962,10 +962,34 @@ __global__ __launch_bounds__(256) void cost_scatter_kernel(const int* __restrict
 // (ElementArgs::tangent_column_major).
 constexpr int kRhs = 25;
 
+// One point's records as a team stages them in LDS: the geometry record
+// then the point record, lane a of the team loading entries a, a + 8, ...
+constexpr int kStagePoint = kGeomRecord + kPointRecord;
+constexpr int kStageSlice = (kStagePoint + kTeam - 1) / kTeam;
+constexpr int kStageStride = kStageSlice * kTeam;
+
+__device__ __forceinline__ void stage_fetch(const ElementArgs& A, long long q, int a, double v[kStageSlice]) {
+  const double* gr = A.geom_ws + q * kGeomRecord;
+  const double* pr = A.point_ws + q * kPointRecord;
+#pragma unroll
+  for (int j = 0; j < kStageSlice; ++j) {
+    const int i = a + kTeam * j;
+    if (kTeam * (j + 1) <= kGeomRecord) v[j] = gr[i];
+    else if (kTeam * j >= kGeomRecord && kTeam * (j + 1) <= kStagePoint) v[j] = pr[i - kGeomRecord];
+    else v[j] = i < kGeomRecord ? gr[i] : (i < kStagePoint ? pr[i - kGeomRecord] : 0.0);
+  }
+}
+
+__device__ __forceinline__ void stage_store(double* st, int a, const double v[kStageSlice]) {
+#pragma unroll
+  for (int j = 0; j < kStageSlice; ++j) st[a + kTeam * j] = v[j];
+}
+
 struct CondenseSlot {
   LuSlot lu;
   LuPerm perm;
   double region[9 * kRhs];
+  double stage[2][kStageStride];  // the records of points q and q + 1
 };
 
 // One team (8 lanes) per element. Lane p first forms the geometry of point p
@@ -978,6 +1002,7 @@ struct CondenseSlot {
 // fewer loops would hold more accumulators than the registers take.
 template <int KIN, bool TANGENT, bool SYM, int TW>
 __global__ __launch_bounds__(64) void condense_team_kernel(ElementArgs A) {
+  static_assert(TW == kTeam, "the team condensation takes rules of up to 8 points");
   __shared__ CondenseSlot slots[teams_per_wave<TW>()];
   const int lane = threadIdx.x;
   const int t = lane / TW;
@@ -1033,14 +1058,34 @@ __global__ __launch_bounds__(64) void condense_team_kernel(ElementArgs A) {
   if (p == 0) T.region[8 * kRhs + 24] = er[kErResidual + 8];
   team_sync_global();  // the geometry records (global) and the factors (LDS)
   if (p == 0) lu_perm(T.lu.rt, T.lu.ct, T.perm);
-  // the row blocks of the 8 nodes: lanes 0..7 (a 32-lane team's other lanes
-  // have written their points' geometry records and are done)
-  if (TW != kTeam && p >= kTeam) return;
+  // the row blocks of the 8 nodes: lane a of the team
   const int a = p;
+  // Every point loop reads the records through the team's LDS stage: while
+  // the team works on point q from LDS, its lanes' loads of point q + 1's
+  // records (lane a: entries a, a + 8, ...) are in flight.
+  auto point_loop = [&](auto&& body) {
+    double pre[kStageSlice];
+    stage_fetch(A, q0, a, pre);
+    stage_store(T.stage[0], a, pre);
+    team_sync();
+    for (int q = 0; q < n; ++q) {
+      if (q + 1 < n) stage_fetch(A, q0 + q + 1, a, pre);
+      const double* st = T.stage[q & 1];
+      body(st, st + kGeomRecord);
+      if (q + 1 < n) stage_store(T.stage[(q + 1) & 1], a, pre);
+      team_sync();
+    }
+  };
   double ru[3], kua[3][9];
   if constexpr (!TANGENT || SYM) {
     // f_u rows and K_ua of node a
-    coupling_rows<KIN>(A, q0, n, a, ru, kua);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      ru[k] = 0.0;
+#pragma unroll
+      for (int j = 0; j < 9; ++j) kua[k][j] = 0.0;
+    }
+    point_loop([&](const double* gr, const double* pr) { coupling_point<KIN>(gr, pr, a, ru, kua); });
   }
   if constexpr (TANGENT) {
     // the coupling columns 3a..3a+2: K_ua^T, or K_au with
@@ -1059,9 +1104,7 @@ __global__ __launch_bounds__(64) void condense_team_kernel(ElementArgs A) {
 #pragma unroll
         for (int nn = 0; nn < 9; ++nn) { kua[k][nn] = 0.0; col[k][nn] = 0.0; }
       }
-      for (int q = 0; q < n; ++q) {
-        const double* gr = A.geom_ws + (q0 + q) * kGeomRecord;
-        const double* pr = A.point_ws + (q0 + q) * kPointRecord;
+      point_loop([&](const double* gr, const double* pr) {
         const double w = gr[kGrW];
         double bu[6][3];
         node_operator(gr, a, bu);
@@ -1116,7 +1159,7 @@ __global__ __launch_bounds__(64) void condense_team_kernel(ElementArgs A) {
             }
           }
         }
-      }
+      });
     }
 #pragma unroll
     for (int nn = 0; nn < 9; ++nn)
@@ -1151,9 +1194,7 @@ __global__ __launch_bounds__(64) void condense_team_kernel(ElementArgs A) {
     for (int k = 0; k < 3; ++k)
 #pragma unroll
       for (int j = 0; j < 12; ++j) kuu[k][j] = 0.0;
-    for (int q = 0; q < n; ++q) {
-      const double* gr = A.geom_ws + (q0 + q) * kGeomRecord;
-      const double* pr = A.point_ws + (q0 + q) * kPointRecord;
+    point_loop([&](const double* gr, const double* pr) {
       const double w = gr[kGrW];
       double bu[6][3];
       node_operator(gr, a, bu);
@@ -1187,7 +1228,7 @@ __global__ __launch_bounds__(64) void condense_team_kernel(ElementArgs A) {
           for (int k = 0; k < 3; ++k) kuu[k][3 * bb + k] = kuu[k][3 * bb + k] + v;
         }
       }
-    }
+    });
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll

@@ -73,6 +73,51 @@ __device__ __forceinline__ void stress_times_gradient(const double* gr, const do
   }
 }
 
+// One point's share of lane a's rows of f_u and K_ua (3 x 9), from its
+// geometry record gr and point record pr.
+template <int KIN>
+__device__ __forceinline__ void coupling_point(const double* gr, const double* pr, int a, double ru[3],
+                                               double kua[3][9]) {
+  const double w = gr[kGrW];
+  double bu[6][3];
+  node_operator(gr, a, bu);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double s = bu[0][k] * pr[kWsStress];
+#pragma unroll
+    for (int c = 1; c < 6; ++c) s = s + bu[c][k] * pr[kWsStress + c];
+    ru[k] = ru[k] + w * s;
+  }
+  double z[3][6];
+  node_times_tangent(bu, pr, z);
+#pragma unroll
+  for (int nn = 0; nn < 9; ++nn) {
+    double bcol[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) bcol[c] = bnl_rec(gr, c, 8 + nn / 3, nn % 3);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double s = z[k][0] * bcol[0];
+#pragma unroll
+      for (int d = 1; d < 6; ++d) s = s + z[k][d] * bcol[d];
+      kua[k][nn] = kua[k][nn] + w * s;
+    }
+  }
+  if constexpr (KIN != kSmall) {
+    // geometric (G_aug^T S G_aug) (x) I: block (a, 8 + bm)
+    const double* g = gr + kGrG;
+    double gsa[3];
+    stress_times_gradient(gr, pr, a, gsa);
+#pragma unroll
+    for (int bm = 0; bm < 3; ++bm) {
+      const double geo = gsa[0] * g[0 * 11 + 8 + bm] + gsa[1] * g[1 * 11 + 8 + bm] + gsa[2] * g[2 * 11 + 8 + bm];
+      const double v = w * geo;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) kua[k][3 * bm + k] = kua[k][3 * bm + k] + v;
+    }
+  }
+}
+
 // Lane a's rows of f_u and K_ua (3 x 9), accumulated point by point in point
 // order from the geometry and point records of element points q0..q0+n-1.
 template <int KIN>
@@ -84,48 +129,8 @@ __device__ __forceinline__ void coupling_rows(const ElementArgs& A, long long q0
 #pragma unroll
     for (int j = 0; j < 9; ++j) kua[k][j] = 0.0;
   }
-  for (int q = 0; q < n; ++q) {
-    const double* gr = A.geom_ws + (q0 + q) * kGeomRecord;
-    const double* pr = A.point_ws + (q0 + q) * kPointRecord;
-    const double w = gr[kGrW];
-    double bu[6][3];
-    node_operator(gr, a, bu);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      double s = bu[0][k] * pr[kWsStress];
-#pragma unroll
-      for (int c = 1; c < 6; ++c) s = s + bu[c][k] * pr[kWsStress + c];
-      ru[k] = ru[k] + w * s;
-    }
-    double z[3][6];
-    node_times_tangent(bu, pr, z);
-#pragma unroll
-    for (int nn = 0; nn < 9; ++nn) {
-      double bcol[6];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) bcol[c] = bnl_rec(gr, c, 8 + nn / 3, nn % 3);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        double s = z[k][0] * bcol[0];
-#pragma unroll
-        for (int d = 1; d < 6; ++d) s = s + z[k][d] * bcol[d];
-        kua[k][nn] = kua[k][nn] + w * s;
-      }
-    }
-    if constexpr (KIN != kSmall) {
-      // geometric (G_aug^T S G_aug) (x) I: block (a, 8 + bm)
-      const double* g = gr + kGrG;
-      double gsa[3];
-      stress_times_gradient(gr, pr, a, gsa);
-#pragma unroll
-      for (int bm = 0; bm < 3; ++bm) {
-        const double geo = gsa[0] * g[0 * 11 + 8 + bm] + gsa[1] * g[1 * 11 + 8 + bm] + gsa[2] * g[2 * 11 + 8 + bm];
-        const double v = w * geo;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) kua[k][3 * bm + k] = kua[k][3 * bm + k] + v;
-      }
-    }
-  }
+  for (int q = 0; q < n; ++q)
+    coupling_point<KIN>(A.geom_ws + (q0 + q) * kGeomRecord, A.point_ws + (q0 + q) * kPointRecord, a, ru, kua);
 }
 
 }  // namespace sbk
