@@ -14,6 +14,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdint>
+#include <memory>
 
 namespace sbk {
 
@@ -181,17 +182,43 @@ std::string AmgHierarchy::build(const int* block_ptr, const int* block_index, lo
                                 const std::vector<long long>& free_dofs, const std::vector<double>& coords,
                                 hipStream_t s) {
   release();
-  if (static_cast<long long>(free_dofs.size()) != n) return "the free DOF list does not match the block";
-  HostLevel cur;
-  cur.n = n;
-  cur.ptr.resize(static_cast<std::size_t>(n) + 1);
-  cur.index.resize(static_cast<std::size_t>(nnz));
-  if (hipMemcpyAsync(cur.ptr.data(), block_ptr, cur.ptr.size() * sizeof(int), hipMemcpyDeviceToHost, s) !=
-          hipSuccess ||
-      (nnz > 0 && hipMemcpyAsync(cur.index.data(), block_index, cur.index.size() * sizeof(int),
-                                 hipMemcpyDeviceToHost, s) != hipSuccess) ||
+  std::vector<int> ptr(static_cast<std::size_t>(n) + 1), index(static_cast<std::size_t>(nnz));
+  if (hipMemcpyAsync(ptr.data(), block_ptr, ptr.size() * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      (nnz > 0 &&
+       hipMemcpyAsync(index.data(), block_index, index.size() * sizeof(int), hipMemcpyDeviceToHost, s) !=
+           hipSuccess) ||
       hipStreamSynchronize(s) != hipSuccess)
     return "reading the block's pattern failed";
+  return build_levels(std::move(ptr), std::move(index), block_ptr, block_index, free_dofs, coords, s);
+}
+
+std::string AmgHierarchy::build_host(std::vector<int> ptr, std::vector<int> index,
+                                     const std::vector<long long>& free_dofs, const std::vector<double>& coords,
+                                     hipStream_t s) {
+  release();
+  bool ok = true;
+  const int* dptr = upload(ptr, s, ok);
+  const int* dindex = upload(index, s, ok);
+  if (!ok) {
+    release();
+    return "device allocation failed";
+  }
+  return build_levels(std::move(ptr), std::move(index), dptr, dindex, free_dofs, coords, s);
+}
+
+std::string AmgHierarchy::build_levels(std::vector<int> ptr, std::vector<int> index, const int* block_ptr,
+                                       const int* block_index, const std::vector<long long>& free_dofs,
+                                       const std::vector<double>& coords, hipStream_t s) {
+  const long long n = static_cast<long long>(ptr.size()) - 1;
+  const long long nnz = static_cast<long long>(index.size());
+  if (static_cast<long long>(free_dofs.size()) != n) {
+    release();
+    return "the free DOF list does not match the block";
+  }
+  HostLevel cur;
+  cur.n = n;
+  cur.ptr = std::move(ptr);
+  cur.index = std::move(index);
   // level 0's nodes: runs of rows of one mesh node
   std::vector<long long> mesh_node;  // per level-0 node
   for (long long i = 0; i < n; ++i) {
@@ -204,7 +231,10 @@ std::string AmgHierarchy::build(const int* block_ptr, const int* block_index, lo
   cur.node_ptr.push_back(static_cast<int>(n));
   if (n == 0) cur.node_ptr.assign(1, 0);
   for (long long nd : mesh_node)
-    if (3 * nd + 2 >= static_cast<long long>(coords.size())) return "a free DOF beyond the coordinates";
+    if (3 * nd + 2 >= static_cast<long long>(coords.size())) {
+      release();
+      return "a free DOF beyond the coordinates";
+    }
   double c[3] = {0.0, 0.0, 0.0}, h = 0.0;
   if (!mesh_node.empty()) {
     double lo[3], hi[3];
@@ -498,42 +528,167 @@ std::string AmgHierarchy::build(const int* block_ptr, const int* block_index, lo
   return std::string();
 }
 
-int AmgHierarchy::setup(const double* block_values, bool symmetrise, hipStream_t s) {
-  if (lv_.empty()) return 1;
-  lv_[0].values = block_values;
-  if (symmetrise) {
-    if (launch_amg_symmetric_part(nnz0_, block_values, transpose_, symmetric_, s) != 0) return 1;
-    lv_[0].values = symmetric_;
-  }
-  const int count = static_cast<int>(lv_.size());
-  for (int l = 0; l + 1 < count; ++l)
-    if (launch_amg_level_setup(lv_[l], lv_[l + 1], s) != 0) return 1;
-  return launch_amg_dense_setup(lv_[count - 1], s);
-}
-
-bool AmgHierarchy::singular(hipStream_t s) const {
-  if (lv_.empty()) return true;
-  double flag = 0.0;
-  if (hipMemcpyAsync(&flag, lv_.back().coef + kAmgSingular, sizeof(double), hipMemcpyDeviceToHost, s) !=
-          hipSuccess ||
-      hipStreamSynchronize(s) != hipSuccess)
-    return true;
-  return flag != 0.0;
-}
-
-int AmgHierarchy::apply(const double* b, double* x, hipStream_t s) const {
-  if (lv_.empty()) return 1;
-  return launch_amg_vcycle(lv_.data(), static_cast<int>(lv_.size()), b, x, s);
-}
-
 std::vector<long long> AmgHierarchy::level_rows() const {
   std::vector<long long> out;
   for (const auto& L : lv_) out.push_back(L.n);
   return out;
 }
 
-int amg_apply(const void* hierarchy, const double* b, double* x, hipStream_t s) {
-  return static_cast<const AmgHierarchy*>(hierarchy)->apply(b, x, s);
+// --- AmgBatch ----------------------------------------------------------------
+
+AmgBatch::~AmgBatch() { release(); }
+
+void AmgBatch::release() {
+  if (d_levels_) (void)hipFree(d_levels_);
+  if (d_entries_) (void)hipFree(d_entries_);
+  d_levels_ = nullptr;
+  d_entries_ = nullptr;
+  problems_.clear();
+  host_levels_.clear();
+  entries_.clear();
+  tables_ = AmgTables();
+  sym_first_ = 0;
+}
+
+void AmgBatch::add(std::unique_ptr<AmgHierarchy> h, long long row_offset, long long value_offset) {
+  problems_.push_back(Problem{std::move(h), row_offset, value_offset, 0});
+}
+
+int AmgBatch::finalize() {
+  if (problems_.empty()) return 1;
+  // every problem's levels, one array
+  host_levels_.clear();
+  int depth = 0;
+  for (auto& p : problems_) {
+    p.base = static_cast<int>(host_levels_.size());
+    for (const AmgLevel& L : p.h->lv_) host_levels_.push_back(L);
+    depth = std::max(depth, static_cast<int>(p.h->lv_.size()));
+  }
+  if (hipMalloc(&d_levels_, host_levels_.size() * sizeof(AmgLevel)) != hipSuccess) return 1;
+  // the tables, each a run of entries ended by the block count
+  entries_.clear();
+  struct Pending {
+    AmgTable* table;
+    std::size_t start;
+  };
+  std::vector<Pending> pending;
+  auto table = [&](AmgTable& t, int level, AmgTableKind kind, bool coarsest) {
+    const std::size_t start = entries_.size();
+    long long blocks = 0;
+    for (int p = 0; p < static_cast<int>(problems_.size()); ++p) {
+      const Problem& q = problems_[p];
+      const int count = static_cast<int>(q.h->lv_.size());
+      if (level >= count || (level == count - 1) != coarsest) continue;
+      AmgGroupEntry e;
+      e.L = d_levels_ + q.base + level;
+      e.N = level + 1 < count ? d_levels_ + q.base + level + 1 : nullptr;
+      e.offset = level == 0 ? q.row_offset : -1;
+      e.first = blocks;
+      e.problem = p;
+      blocks += amg_table_blocks(kind, q.h->lv_[level], q.h->nnz0_);
+      entries_.push_back(e);
+    }
+    AmgGroupEntry end;
+    end.first = blocks;
+    entries_.push_back(end);
+    t.count = static_cast<int>(entries_.size() - start) - 1;
+    t.blocks = t.count > 0 ? blocks : 0;
+    pending.push_back(Pending{&t, start});
+  };
+  tables_.levels.assign(static_cast<std::size_t>(depth), AmgLevelTables());
+  for (int l = 0; l < depth; ++l) {
+    AmgLevelTables& t = tables_.levels[l];
+    table(t.elem, l, kAmgTableElem, false);
+    table(t.rows, l, kAmgTableRows, false);
+    table(t.restrict_, l, kAmgTableRestrict, false);
+    table(t.power, l, kAmgTablePower, false);
+    table(t.one, l, kAmgTableOne, false);
+    table(t.product, l, kAmgTableProduct, false);
+    table(t.coarse_product, l, kAmgTableCoarseProduct, false);
+    table(t.gather, l, kAmgTableGather, false);
+    table(t.dense_one, l, kAmgTableOne, true);
+    table(t.dense, l, kAmgTableElem, true);
+  }
+  // level 0 of every problem, whatever its depth
+  {
+    const std::size_t start = entries_.size();
+    long long blocks = 0;
+    for (int p = 0; p < static_cast<int>(problems_.size()); ++p) {
+      const Problem& q = problems_[p];
+      AmgGroupEntry e;
+      e.L = d_levels_ + q.base;
+      e.offset = q.row_offset;
+      e.first = blocks;
+      e.problem = p;
+      e.mirror = q.h->transpose_;
+      e.dst = q.h->symmetric_;
+      e.count = q.h->nnz0_;
+      blocks += amg_table_blocks(kAmgTableSymmetric, q.h->lv_[0], q.h->nnz0_);
+      entries_.push_back(e);
+    }
+    AmgGroupEntry end;
+    end.first = blocks;
+    entries_.push_back(end);
+    tables_.symmetric.count = static_cast<int>(problems_.size());
+    tables_.symmetric.blocks = blocks;
+    sym_first_ = start;
+    pending.push_back(Pending{&tables_.symmetric, start});
+  }
+  if (hipMalloc(&d_entries_, entries_.size() * sizeof(AmgGroupEntry)) != hipSuccess) return 1;
+  for (const Pending& q : pending) q.table->entries = d_entries_ + q.start;
+  return 0;
+}
+
+int AmgBatch::setup(const double* block_values, bool symmetrise, hipStream_t s) {
+  if (problems_.empty() || !d_levels_ || !d_entries_) return 1;
+  // level 0 reads the block's values at each problem's offset, or its
+  // symmetric part
+  for (std::size_t p = 0; p < problems_.size(); ++p) {
+    const Problem& q = problems_[p];
+    const double* mine = block_values + q.value_offset;
+    host_levels_[static_cast<std::size_t>(q.base)].values = symmetrise ? q.h->symmetric_ : mine;
+    entries_[sym_first_ + p].src = mine;
+  }
+  if (hipMemcpyAsync(d_levels_, host_levels_.data(), host_levels_.size() * sizeof(AmgLevel), hipMemcpyHostToDevice,
+                     s) != hipSuccess ||
+      hipMemcpyAsync(d_entries_, entries_.data(), entries_.size() * sizeof(AmgGroupEntry), hipMemcpyHostToDevice,
+                     s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return 1;
+  return launch_amg_setup(tables_, symmetrise, s);
+}
+
+std::vector<int> AmgBatch::singular(hipStream_t s) const {
+  std::vector<double> flags(problems_.size(), 1.0);
+  bool ok = true;
+  for (std::size_t p = 0; p < problems_.size() && ok; ++p) {
+    const AmgHierarchy& h = *problems_[p].h;
+    if (h.lv_.empty()) continue;
+    ok = hipMemcpyAsync(&flags[p], h.lv_.back().coef + kAmgSingular, sizeof(double), hipMemcpyDeviceToHost, s) ==
+         hipSuccess;
+  }
+  ok = ok && hipStreamSynchronize(s) == hipSuccess;
+  std::vector<int> out(problems_.size(), 1);
+  if (!ok) return out;
+  for (std::size_t p = 0; p < problems_.size(); ++p) out[p] = flags[p] != 0.0;
+  return out;
+}
+
+int AmgBatch::apply(const double* b, double* x, hipStream_t s) const {
+  if (problems_.empty()) return 1;
+  return launch_amg_vcycle(tables_, active_, b, x, s);
+}
+
+std::vector<long long> AmgBatch::level_rows() const {
+  return problems_.empty() ? std::vector<long long>() : problems_[0].h->level_rows();
+}
+
+std::vector<long long> AmgBatch::level_nnz() const {
+  return problems_.empty() ? std::vector<long long>() : problems_[0].h->level_nnz();
+}
+
+int amg_apply(const void* batch, const double* b, double* x, hipStream_t s) {
+  return static_cast<const AmgBatch*>(batch)->apply(b, x, s);
 }
 
 }  // namespace sbk

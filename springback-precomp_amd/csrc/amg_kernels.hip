@@ -18,6 +18,39 @@
 namespace sbk {
 
 namespace {
+// Every launch is grouped: it covers one level of each of a set of
+// hierarchies (one per problem; a plain solve's set has one), each entry of
+// the group table taking its own range of the grid's blocks with exactly the
+// block decomposition a launch of its own would use. A workgroup finds its
+// entry by binary search and, when its problem rests (its active flag is 0),
+// does nothing.
+struct GroupBlock {
+  const AmgGroupEntry* e;
+  long long block;    // the block within the entry's range
+  long long nblocks;  // the range's length
+};
+
+__device__ __forceinline__ bool group_block(const AmgGroup& G, GroupBlock& out) {
+  const long long b = blockIdx.x;
+  int lo = 0, hi = G.count - 1;  // the last entry whose first block <= b
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (G.e[mid].first <= b) lo = mid; else hi = mid - 1;
+  }
+  out.e = G.e + lo;
+  out.block = b - G.e[lo].first;
+  out.nblocks = G.e[lo + 1].first - G.e[lo].first;
+  return !(G.active && G.active[G.e[lo].problem] == 0);
+}
+
+// The V-cycle's vectors of an entry's level: the caller's (at the problem's
+// offset) on level 0, the level's own above it.
+__device__ __forceinline__ const double* level_b(const AmgGroupEntry& e, const double* b) {
+  return e.offset >= 0 ? b + e.offset : e.L->b;
+}
+__device__ __forceinline__ double* level_x(const AmgGroupEntry& e, double* x) {
+  return e.offset >= 0 ? x + e.offset : e.L->x;
+}
 
 constexpr int kWaves = 4;
 constexpr int kLanesPerRow = 32;
@@ -42,23 +75,26 @@ __device__ __forceinline__ double row_dot(const int* __restrict__ ptr, const int
   return acc;
 }
 
-// The row of the calling lane in a row-wise launch (grid: ceil(n / 8) blocks
-// of 256) and whether the lane writes it.
+// The row of the calling lane in a row-wise launch (ceil(n / 8) blocks of
+// 256) and whether the lane writes it.
 struct RowLane {
   long long row;
   int l;
 };
 
-__device__ __forceinline__ RowLane row_lane() {
+__device__ __forceinline__ RowLane row_lane(long long block) {
   const int lane = threadIdx.x & 63;
   RowLane r;
-  r.row = (static_cast<long long>(blockIdx.x) * kWaves + (threadIdx.x >> 6)) * kGroups + lane / kLanesPerRow;
+  r.row = (block * kWaves + (threadIdx.x >> 6)) * kGroups + lane / kLanesPerRow;
   r.l = lane % kLanesPerRow;
   return r;
 }
 
-__global__ __launch_bounds__(256) void diag_kernel(AmgLevel L) {
-  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+__global__ __launch_bounds__(256) void diag_kernel(AmgGroup G) {
+  GroupBlock gb;
+  if (!group_block(G, gb)) return;
+  const AmgLevel& L = *gb.e->L;
+  const long long i = gb.block * 256 + threadIdx.x;
   if (i >= L.n) return;
   double d = 0.0;
   for (int k = L.ptr[i]; k < L.ptr[i + 1]; ++k)
@@ -66,29 +102,36 @@ __global__ __launch_bounds__(256) void diag_kernel(AmgLevel L) {
   L.dinv[i] = (d != 0.0 && std::isfinite(d)) ? 1.0 / d : 0.0;
 }
 
-// A fixed start vector in [-1, 1) from a hash of the row.
-__global__ __launch_bounds__(256) void power_init_kernel(AmgLevel L, double* __restrict__ x) {
-  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+// A fixed start vector in [-1, 1) from a hash of the row (into d0).
+__global__ __launch_bounds__(256) void power_init_kernel(AmgGroup G) {
+  GroupBlock gb;
+  if (!group_block(G, gb)) return;
+  const AmgLevel& L = *gb.e->L;
+  const long long i = gb.block * 256 + threadIdx.x;
   if (i == 0) L.coef[kAmgScale] = 1.0;
   if (i >= L.n) return;
   unsigned long long h = static_cast<unsigned long long>(i) * 0x9E3779B97F4A7C15ull;
   h ^= h >> 29;
-  x[i] = static_cast<double>(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  L.d0[i] = static_cast<double>(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
 }
 
-// y = D^-1 A (x / ||x||), rows dealt round-robin over the nparts blocks;
+// y = D^-1 A (x / ||x||) (x = d0, y = d1 on even steps, the other way round
+// on odd ones), rows dealt round-robin over the level's nparts blocks;
 // partial[block] = the block's sum of y^2 (groups, then wavefronts in order).
-__global__ __launch_bounds__(256) void power_step_kernel(AmgLevel L, const double* __restrict__ x,
-                                                         double* __restrict__ y) {
+__global__ __launch_bounds__(256) void power_step_kernel(AmgGroup G, int odd) {
+  GroupBlock gb;
+  if (!group_block(G, gb)) return;
+  const AmgLevel& L = *gb.e->L;
+  const double* __restrict__ x = odd ? L.d1 : L.d0;
+  double* __restrict__ y = odd ? L.d0 : L.d1;
   __shared__ double ws[kRowsPerBlock];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int g = lane / kLanesPerRow, l = lane % kLanesPerRow;
   const double scale = L.coef[kAmgScale];
-  const long long stride = static_cast<long long>(gridDim.x) * kRowsPerBlock;
+  const long long stride = gb.nblocks * kRowsPerBlock;
   double sq = 0.0;
-  for (long long base = (static_cast<long long>(blockIdx.x) * kWaves + wave) * kGroups; base < L.n;
-       base += stride) {
+  for (long long base = (gb.block * kWaves + wave) * kGroups; base < L.n; base += stride) {
     const long long row = base + g;
     const double v = row_dot(L.ptr, L.index, L.values, x, row, L.n, l);
     if (l == 0 && row < L.n) {
@@ -102,12 +145,15 @@ __global__ __launch_bounds__(256) void power_step_kernel(AmgLevel L, const doubl
   if (threadIdx.x == 0) {
     double sum = 0.0;
     for (int i = 0; i < kRowsPerBlock; ++i) sum += ws[i];
-    L.partial[blockIdx.x] = sum;
+    L.partial[gb.block] = sum;
   }
 }
 
 // ||y|| (the estimate, x having been normalised) and 1 / ||y||.
-__global__ __launch_bounds__(256) void power_finish_kernel(AmgLevel L) {
+__global__ __launch_bounds__(256) void power_finish_kernel(AmgGroup G) {
+  GroupBlock gb;
+  if (!group_block(G, gb)) return;
+  const AmgLevel& L = *gb.e->L;
   __shared__ double s[256];
   double acc = 0.0;
   for (int i = threadIdx.x; i < L.nparts; i += 256) acc += L.partial[i];
@@ -127,7 +173,10 @@ __global__ __launch_bounds__(256) void power_finish_kernel(AmgLevel L) {
 // Chebyshev on [lmax / ratio, lmax], lmax = boost rho (degree 2: d0 = r0 /
 // theta, d1 = c1 d0 + c2 r1), and the prolongator's omega = 4 / (3 rho); no
 // smoothing at all when rho is not a positive finite number.
-__global__ void coef_kernel(AmgLevel L) {
+__global__ void coef_kernel(AmgGroup G) {
+  GroupBlock gb;
+  if (!group_block(G, gb)) return;
+  const AmgLevel& L = *gb.e->L;
   if (threadIdx.x != 0) return;
   const double rho = L.coef[kAmgRho];
   double* c = L.coef;
@@ -171,14 +220,40 @@ __device__ __forceinline__ int find_slot(const int* cols, int mw, int c) {
 // the first 64 entries of the next entry's row of Y are loaded while the
 // current one is added. A wavefront's LDS accesses run in program order, so
 // an entry's additions see the previous entry's without a barrier.
-// smoothed: C is the smoothed prolongator, written as T - omega D^-1 (A T)
-// (X = A, Y = T).
-__global__ __launch_bounds__(64) void product_kernel(long long rows, Csr X, Csr Y, const int* __restrict__ cptr,
-                                                     const int* __restrict__ cindex, double* __restrict__ cvalues,
-                                                     int smoothed, AmgLevel L) {
+// kind kProductP: C = P, the smoothed prolongator, written as
+// T - omega D^-1 (A T) (X = A, Y = T); kProductAP: C = A P (X = A, Y = P);
+// kProductPtAP: C = the next level's A = P^T (A P) (X = P^T, Y = A P), over
+// the coarse rows. One block per row of C.
+enum ProductKind : int { kProductP = 0, kProductAP = 1, kProductPtAP = 2 };
+
+__global__ __launch_bounds__(64) void product_kernel(AmgGroup G, int kind) {
+  GroupBlock gb;
+  if (!group_block(G, gb)) return;
+  const AmgLevel& L = *gb.e->L;
+  const Csr A{L.ptr, L.index, L.values};
+  Csr X = A, Y{L.tp_ptr, L.tp_index, L.tp_values};
+  const int* __restrict__ cptr = L.p_ptr;
+  const int* __restrict__ cindex = L.p_index;
+  double* __restrict__ cvalues = L.p_values;
+  long long rows = L.n;
+  if (kind == kProductAP) {
+    Y = Csr{L.p_ptr, L.p_index, L.p_values};
+    cptr = L.ap_ptr;
+    cindex = L.ap_index;
+    cvalues = L.ap_values;
+  } else if (kind == kProductPtAP) {
+    const AmgLevel& N = *gb.e->N;
+    X = Csr{L.pt_ptr, L.pt_index, L.pt_values};
+    Y = Csr{L.ap_ptr, L.ap_index, L.ap_values};
+    cptr = N.ptr;
+    cindex = N.index;
+    cvalues = const_cast<double*>(N.values);
+    rows = L.nc;
+  }
+  const int smoothed = kind == kProductP;
   __shared__ int cols[kCap];
   __shared__ double acc[kCap];
-  const long long row = blockIdx.x;
+  const long long row = gb.block;
   if (row >= rows) return;
   const int lane = threadIdx.x;
   const int c0 = cptr[row], m = cptr[row + 1] - c0;
@@ -247,15 +322,21 @@ __global__ __launch_bounds__(64) void product_kernel(long long rows, Csr X, Csr 
   }
 }
 
-__global__ __launch_bounds__(256) void gather_kernel(long long n, const int* __restrict__ perm,
-                                                     const double* __restrict__ src, double* __restrict__ dst) {
-  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (i < n) dst[i] = src[perm[i]];
+// P^T's values from P's (pt_values[i] = p_values[pt_perm[i]]).
+__global__ __launch_bounds__(256) void gather_kernel(AmgGroup G) {
+  GroupBlock gb;
+  if (!group_block(G, gb)) return;
+  const AmgLevel& L = *gb.e->L;
+  const long long i = gb.block * 256 + threadIdx.x;
+  if (i < L.pt_nnz) L.pt_values[i] = L.p_values[L.pt_perm[i]];
 }
 
 // [A | I] -> [I | A^-1] by Gauss-Jordan with partial pivoting (the largest
 // |entry| of the column, the lowest row among equals); one workgroup.
-__global__ __launch_bounds__(1024) void dense_inverse_kernel(AmgLevel L) {
+__global__ __launch_bounds__(1024) void dense_inverse_kernel(AmgGroup G) {
+  GroupBlock gb;
+  if (!group_block(G, gb)) return;
+  const AmgLevel& L = *gb.e->L;
   __shared__ double best[16];
   __shared__ int besti[16];
   __shared__ double factor[kAmgDenseMax];
@@ -332,20 +413,27 @@ __global__ __launch_bounds__(1024) void dense_inverse_kernel(AmgLevel L) {
   }
 }
 
-__global__ __launch_bounds__(256) void symmetric_part_kernel(long long nnz, const double* __restrict__ v,
-                                                             const int* __restrict__ mirror,
-                                                             double* __restrict__ out) {
-  const long long k = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (k >= nnz) return;
-  const int m = mirror[k];
-  out[k] = m >= 0 ? 0.5 * (v[k] + v[m]) : 0.5 * v[k];
+// Level 0's symmetric part (A + A^T) / 2 from the entry's block values.
+__global__ __launch_bounds__(256) void symmetric_part_kernel(AmgGroup G) {
+  GroupBlock gb;
+  if (!group_block(G, gb)) return;
+  const AmgGroupEntry& e = *gb.e;
+  const long long k = gb.block * 256 + threadIdx.x;
+  if (k >= e.count) return;
+  const double* __restrict__ v = e.src;
+  const int m = e.mirror[k];
+  e.dst[k] = m >= 0 ? 0.5 * (v[k] + v[m]) : 0.5 * v[k];
 }
 
 // --- the V-cycle ------------------------------------------------------------
 
 // Pre-smoothing from x = 0, first half: r = D^-1 b, d0 = r / theta.
-__global__ __launch_bounds__(256) void pre_first_kernel(AmgLevel L, const double* __restrict__ b) {
-  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+__global__ __launch_bounds__(256) void pre_first_kernel(AmgGroup G, const double* __restrict__ b0) {
+  GroupBlock gb;
+  if (!group_block(G, gb)) return;
+  const AmgLevel& L = *gb.e->L;
+  const double* __restrict__ b = level_b(*gb.e, b0);
+  const long long i = gb.block * 256 + threadIdx.x;
   if (i >= L.n) return;
   const double ri = L.dinv[i] * b[i];
   L.r[i] = ri;
@@ -353,8 +441,12 @@ __global__ __launch_bounds__(256) void pre_first_kernel(AmgLevel L, const double
 }
 
 // Second half: r -= D^-1 A d0, d1 = c1 d0 + c2 r, x = d0 + d1.
-__global__ __launch_bounds__(256) void pre_step_kernel(AmgLevel L, double* __restrict__ x) {
-  const RowLane q = row_lane();
+__global__ __launch_bounds__(256) void pre_step_kernel(AmgGroup G, double* __restrict__ x0) {
+  GroupBlock gb;
+  if (!group_block(G, gb)) return;
+  const AmgLevel& L = *gb.e->L;
+  double* __restrict__ x = level_x(*gb.e, x0);
+  const RowLane q = row_lane(gb.block);
   const double v = row_dot(L.ptr, L.index, L.values, L.d0, q.row, L.n, q.l);
   if (q.l != 0 || q.row >= L.n) return;
   const long long i = q.row;
@@ -365,32 +457,50 @@ __global__ __launch_bounds__(256) void pre_step_kernel(AmgLevel L, double* __res
 }
 
 // r = b - A x.
-__global__ __launch_bounds__(256) void residual_kernel(AmgLevel L, const double* __restrict__ b,
-                                                       const double* __restrict__ x) {
-  const RowLane q = row_lane();
+__global__ __launch_bounds__(256) void residual_kernel(AmgGroup G, const double* __restrict__ b0,
+                                                       const double* __restrict__ x0) {
+  GroupBlock gb;
+  if (!group_block(G, gb)) return;
+  const AmgLevel& L = *gb.e->L;
+  const double* __restrict__ b = level_b(*gb.e, b0);
+  const double* __restrict__ x = level_x(*gb.e, const_cast<double*>(x0));
+  const RowLane q = row_lane(gb.block);
   const double v = row_dot(L.ptr, L.index, L.values, x, q.row, L.n, q.l);
   if (q.l == 0 && q.row < L.n) L.r[q.row] = b[q.row] - v;
 }
 
 // The next level's b = P^T r.
-__global__ __launch_bounds__(256) void restrict_kernel(AmgLevel L, double* __restrict__ nb) {
-  const RowLane q = row_lane();
+__global__ __launch_bounds__(256) void restrict_kernel(AmgGroup G) {
+  GroupBlock gb;
+  if (!group_block(G, gb)) return;
+  const AmgLevel& L = *gb.e->L;
+  double* __restrict__ nb = gb.e->N->b;
+  const RowLane q = row_lane(gb.block);
   const double v = row_dot(L.pt_ptr, L.pt_index, L.pt_values, L.r, q.row, L.nc, q.l);
   if (q.l == 0 && q.row < L.nc) nb[q.row] = v;
 }
 
 // x += P x_next.
-__global__ __launch_bounds__(256) void prolong_kernel(AmgLevel L, double* __restrict__ x,
-                                                      const double* __restrict__ nx) {
-  const RowLane q = row_lane();
+__global__ __launch_bounds__(256) void prolong_kernel(AmgGroup G, double* __restrict__ x0) {
+  GroupBlock gb;
+  if (!group_block(G, gb)) return;
+  const AmgLevel& L = *gb.e->L;
+  double* __restrict__ x = level_x(*gb.e, x0);
+  const double* __restrict__ nx = gb.e->N->x;
+  const RowLane q = row_lane(gb.block);
   const double v = row_dot(L.p_ptr, L.p_index, L.p_values, nx, q.row, L.n, q.l);
   if (q.l == 0 && q.row < L.n) x[q.row] += v;
 }
 
 // Post-smoothing from x, first half: r = D^-1 (b - A x), d0 = r / theta.
-__global__ __launch_bounds__(256) void post_first_kernel(AmgLevel L, const double* __restrict__ b,
-                                                         const double* __restrict__ x) {
-  const RowLane q = row_lane();
+__global__ __launch_bounds__(256) void post_first_kernel(AmgGroup G, const double* __restrict__ b0,
+                                                         const double* __restrict__ x0) {
+  GroupBlock gb;
+  if (!group_block(G, gb)) return;
+  const AmgLevel& L = *gb.e->L;
+  const double* __restrict__ b = level_b(*gb.e, b0);
+  const double* __restrict__ x = level_x(*gb.e, const_cast<double*>(x0));
+  const RowLane q = row_lane(gb.block);
   const double v = row_dot(L.ptr, L.index, L.values, x, q.row, L.n, q.l);
   if (q.l != 0 || q.row >= L.n) return;
   const double ri = L.dinv[q.row] * (b[q.row] - v);
@@ -399,8 +509,12 @@ __global__ __launch_bounds__(256) void post_first_kernel(AmgLevel L, const doubl
 }
 
 // Second half: r -= D^-1 A d0, d1 = c1 d0 + c2 r, x += d0 + d1.
-__global__ __launch_bounds__(256) void post_step_kernel(AmgLevel L, double* __restrict__ x) {
-  const RowLane q = row_lane();
+__global__ __launch_bounds__(256) void post_step_kernel(AmgGroup G, double* __restrict__ x0) {
+  GroupBlock gb;
+  if (!group_block(G, gb)) return;
+  const AmgLevel& L = *gb.e->L;
+  double* __restrict__ x = level_x(*gb.e, x0);
+  const RowLane q = row_lane(gb.block);
   const double v = row_dot(L.ptr, L.index, L.values, L.d0, q.row, L.n, q.l);
   if (q.l != 0 || q.row >= L.n) return;
   const long long i = q.row;
@@ -412,11 +526,16 @@ __global__ __launch_bounds__(256) void post_step_kernel(AmgLevel L, double* __re
 
 // The coarsest level: x = A^-1 b from the dense inverse (each row's sum in
 // column order).
-__global__ __launch_bounds__(256) void dense_apply_kernel(AmgLevel L, const double* __restrict__ b,
-                                                          double* __restrict__ x) {
+__global__ __launch_bounds__(256) void dense_apply_kernel(AmgGroup G, const double* __restrict__ b0,
+                                                          double* __restrict__ x0) {
+  GroupBlock gb;
+  if (!group_block(G, gb)) return;
+  const AmgLevel& L = *gb.e->L;
+  const double* __restrict__ b = level_b(*gb.e, b0);
+  double* __restrict__ x = level_x(*gb.e, x0);
   const int n = static_cast<int>(L.n);
   const long long w = 2ll * n;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+  for (long long i = gb.block * 256 + threadIdx.x; i < n; i += gb.nblocks * 256) {
     const double* row = L.dense + i * w + n;
     double acc = 0.0;
     for (int j = 0; j < n; ++j) acc = fma(row[j], b[j], acc);
@@ -424,25 +543,10 @@ __global__ __launch_bounds__(256) void dense_apply_kernel(AmgLevel L, const doub
   }
 }
 
-int vcycle(const AmgLevel* lv, int l, int count, const double* b, double* x, hipStream_t s) {
-  const AmgLevel& L = lv[l];
-  if (L.n == 0) return 0;
-  if (l == count - 1) {
-    hipLaunchKernelGGL(dense_apply_kernel, dim3(grid_of(L.n, 256)), dim3(256), 0, s, L, b, x);
-    return 0;
-  }
-  const AmgLevel& N = lv[l + 1];
-  const dim3 rows(grid_of(L.n, kRowsPerBlock)), blk(256);
-  hipLaunchKernelGGL(pre_first_kernel, dim3(grid_of(L.n, 256)), blk, 0, s, L, b);
-  hipLaunchKernelGGL(pre_step_kernel, rows, blk, 0, s, L, x);
-  hipLaunchKernelGGL(residual_kernel, rows, blk, 0, s, L, b, static_cast<const double*>(x));
-  if (L.nc > 0) {
-    hipLaunchKernelGGL(restrict_kernel, dim3(grid_of(L.nc, kRowsPerBlock)), blk, 0, s, L, N.b);
-    vcycle(lv, l + 1, count, N.b, N.x, s);
-    hipLaunchKernelGGL(prolong_kernel, rows, blk, 0, s, L, x, static_cast<const double*>(N.x));
-  }
-  hipLaunchKernelGGL(post_first_kernel, rows, blk, 0, s, L, b, static_cast<const double*>(x));
-  hipLaunchKernelGGL(post_step_kernel, rows, blk, 0, s, L, x);
+int launch(void (*kernel)(AmgGroup), const AmgTable& t, const int* active, int threads, hipStream_t s) {
+  if (t.blocks > 0)
+    hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned int>(t.blocks)), dim3(threads), 0, s,
+                       AmgGroup{t.entries, t.count, active});
   return 0;
 }
 
@@ -453,57 +557,82 @@ int amg_product_blocks(long long n) {
   return static_cast<int>(b < 1 ? 1 : (b > 2048 ? 2048 : b));
 }
 
-int launch_amg_level_setup(const AmgLevel& L, const AmgLevel& next, hipStream_t s) {
-  (void)hipGetLastError();
-  if (L.n == 0) return 0;
-  const dim3 blk(256);
-  hipLaunchKernelGGL(diag_kernel, dim3(grid_of(L.n, 256)), blk, 0, s, L);
-  hipLaunchKernelGGL(power_init_kernel, dim3(grid_of(L.n, 256)), blk, 0, s, L, L.d0);
-  double* x = L.d0;
-  double* y = L.d1;
-  for (int k = 0; k < kAmgPowerSteps; ++k) {
-    hipLaunchKernelGGL(power_step_kernel, dim3(L.nparts), blk, 0, s, L, static_cast<const double*>(x), y);
-    hipLaunchKernelGGL(power_finish_kernel, dim3(1), blk, 0, s, L);
-    double* t = x;
-    x = y;
-    y = t;
+long long amg_table_blocks(AmgTableKind kind, const AmgLevel& L, long long nnz0) {
+  switch (kind) {
+    case kAmgTableElem: return grid_of(L.n, 256);
+    case kAmgTableRows: return grid_of(L.n, kRowsPerBlock);
+    case kAmgTableRestrict: return grid_of(L.nc, kRowsPerBlock);
+    case kAmgTablePower: return L.nparts;
+    case kAmgTableOne: return 1;
+    case kAmgTableProduct: return L.n;
+    case kAmgTableCoarseProduct: return L.nc;
+    case kAmgTableGather: return grid_of(L.pt_nnz, 256);
+    case kAmgTableSymmetric: return grid_of(nnz0, 256);
+    default: return 0;
   }
-  hipLaunchKernelGGL(coef_kernel, dim3(1), dim3(64), 0, s, L);
-  const Csr A{L.ptr, L.index, L.values};
-  hipLaunchKernelGGL(product_kernel, dim3(static_cast<unsigned int>(L.n)), dim3(64), 0, s, L.n, A,
-                     Csr{L.tp_ptr, L.tp_index, L.tp_values}, L.p_ptr, L.p_index, L.p_values, 1, L);
-  hipLaunchKernelGGL(product_kernel, dim3(static_cast<unsigned int>(L.n)), dim3(64), 0, s, L.n, A,
-                     Csr{L.p_ptr, L.p_index, L.p_values}, L.ap_ptr, L.ap_index, L.ap_values, 0, L);
-  if (L.nc > 0) {
-    const long long pt_nnz = L.pt_nnz;
-    if (pt_nnz > 0)
-      hipLaunchKernelGGL(gather_kernel, dim3(grid_of(pt_nnz, 256)), blk, 0, s, pt_nnz, L.pt_perm,
-                         static_cast<const double*>(L.p_values), L.pt_values);
-    hipLaunchKernelGGL(product_kernel, dim3(static_cast<unsigned int>(L.nc)), dim3(64), 0, s, L.nc,
-                       Csr{L.pt_ptr, L.pt_index, L.pt_values}, Csr{L.ap_ptr, L.ap_index, L.ap_values}, next.ptr,
-                       next.index, const_cast<double*>(next.values), 0, L);
+}
+
+int launch_amg_setup(const AmgTables& T, bool symmetrise, hipStream_t s) {
+  (void)hipGetLastError();
+  if (symmetrise) launch(symmetric_part_kernel, T.symmetric, nullptr, 256, s);
+  for (std::size_t l = 0; l < T.levels.size(); ++l) {
+    const AmgLevelTables& t = T.levels[l];
+    launch(diag_kernel, t.elem, nullptr, 256, s);
+    launch(power_init_kernel, t.elem, nullptr, 256, s);
+    if (t.power.blocks > 0)
+      for (int k = 0; k < kAmgPowerSteps; ++k) {
+        hipLaunchKernelGGL(power_step_kernel, dim3(static_cast<unsigned int>(t.power.blocks)), dim3(256), 0, s,
+                           AmgGroup{t.power.entries, t.power.count, nullptr}, k & 1);
+        launch(power_finish_kernel, t.one, nullptr, 256, s);
+      }
+    launch(coef_kernel, t.one, nullptr, 64, s);
+    if (t.product.blocks > 0) {
+      const AmgGroup g{t.product.entries, t.product.count, nullptr};
+      hipLaunchKernelGGL(product_kernel, dim3(static_cast<unsigned int>(t.product.blocks)), dim3(64), 0, s, g,
+                         static_cast<int>(kProductP));
+      hipLaunchKernelGGL(product_kernel, dim3(static_cast<unsigned int>(t.product.blocks)), dim3(64), 0, s, g,
+                         static_cast<int>(kProductAP));
+    }
+    launch(gather_kernel, t.gather, nullptr, 256, s);
+    if (t.coarse_product.blocks > 0)
+      hipLaunchKernelGGL(product_kernel, dim3(static_cast<unsigned int>(t.coarse_product.blocks)), dim3(64), 0, s,
+                         AmgGroup{t.coarse_product.entries, t.coarse_product.count, nullptr},
+                         static_cast<int>(kProductPtAP));
+    launch(dense_inverse_kernel, t.dense_one, nullptr, 1024, s);
   }
   return static_cast<int>(hipGetLastError());
 }
 
-int launch_amg_dense_setup(const AmgLevel& L, hipStream_t s) {
+int launch_amg_vcycle(const AmgTables& T, const int* active, const double* b, double* x, hipStream_t s) {
   (void)hipGetLastError();
-  if (L.n == 0) return 0;
-  if (L.n > kAmgDenseMax) return 1;
-  hipLaunchKernelGGL(dense_inverse_kernel, dim3(1), dim3(1024), 0, s, L);
-  return static_cast<int>(hipGetLastError());
-}
-
-int launch_amg_symmetric_part(long long nnz, const double* values, const int* mirror, double* out, hipStream_t s) {
-  (void)hipGetLastError();
-  if (nnz > 0)
-    hipLaunchKernelGGL(symmetric_part_kernel, dim3(grid_of(nnz, 256)), dim3(256), 0, s, nnz, values, mirror, out);
-  return static_cast<int>(hipGetLastError());
-}
-
-int launch_amg_vcycle(const AmgLevel* levels, int count, const double* b, double* x, hipStream_t s) {
-  (void)hipGetLastError();
-  vcycle(levels, 0, count, b, x, s);
+  const int count = static_cast<int>(T.levels.size());
+  auto grid = [](const AmgTable& t) { return dim3(static_cast<unsigned int>(t.blocks)); };
+  auto group = [active](const AmgTable& t) { return AmgGroup{t.entries, t.count, active}; };
+  const double* cb = b;
+  // down: pre-smoothing, residual and restriction where a level is not the
+  // problem's coarsest, the dense solve where it is
+  for (int l = 0; l < count; ++l) {
+    const AmgLevelTables& t = T.levels[l];
+    if (t.elem.blocks > 0)
+      hipLaunchKernelGGL(pre_first_kernel, grid(t.elem), dim3(256), 0, s, group(t.elem), cb);
+    if (t.rows.blocks > 0) {
+      hipLaunchKernelGGL(pre_step_kernel, grid(t.rows), dim3(256), 0, s, group(t.rows), x);
+      hipLaunchKernelGGL(residual_kernel, grid(t.rows), dim3(256), 0, s, group(t.rows), cb,
+                         static_cast<const double*>(x));
+    }
+    launch(restrict_kernel, t.restrict_, active, 256, s);
+    if (t.dense.blocks > 0)
+      hipLaunchKernelGGL(dense_apply_kernel, grid(t.dense), dim3(256), 0, s, group(t.dense), cb, x);
+  }
+  // up: prolongation and post-smoothing
+  for (int l = count - 1; l >= 0; --l) {
+    const AmgLevelTables& t = T.levels[l];
+    if (t.rows.blocks == 0) continue;
+    hipLaunchKernelGGL(prolong_kernel, grid(t.rows), dim3(256), 0, s, group(t.rows), x);
+    hipLaunchKernelGGL(post_first_kernel, grid(t.rows), dim3(256), 0, s, group(t.rows), cb,
+                       static_cast<const double*>(x));
+    hipLaunchKernelGGL(post_step_kernel, grid(t.rows), dim3(256), 0, s, group(t.rows), x);
+  }
   return static_cast<int>(hipGetLastError());
 }
 

@@ -17,6 +17,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 namespace sbk {
 
 enum AmgCoef : int {
@@ -71,19 +73,74 @@ struct AmgLevel {
   double* dense = nullptr;
 };
 
+// One problem's part of a grouped launch (device memory). A launch covers
+// one level of each of a set of hierarchies: entry k takes the grid's blocks
+// [first, next entry's first) with the decomposition a launch of that level
+// alone would use; a table ends with an entry whose first is the launch's
+// block count.
+struct AmgGroupEntry {
+  const AmgLevel* L = nullptr;  // the problem's level (device)
+  const AmgLevel* N = nullptr;  // its next level (device; null at its coarsest)
+  long long offset = -1;        // level 0: its rows' offset in the caller's vectors; -1 above
+  long long first = 0;          // its first block
+  int problem = 0;              // its segment (the active flags)
+  int pad = 0;
+  // the symmetric-part table only: (A + A^T) / 2 of count block values
+  const double* src = nullptr;
+  const int* mirror = nullptr;
+  double* dst = nullptr;
+  long long count = 0;
+};
+
+struct AmgGroup {
+  const AmgGroupEntry* e;  // count + 1 entries
+  int count;
+  const int* active;       // per problem: 0 rests (its blocks do nothing); null: all work
+};
+
+// A group table on the device.
+struct AmgTable {
+  const AmgGroupEntry* entries = nullptr;
+  int count = 0;
+  long long blocks = 0;
+};
+
+// The blocks an entry of each kind of table takes for its level.
+enum AmgTableKind : int {
+  kAmgTableElem = 0,        // a thread per row (256 per block)
+  kAmgTableRows,            // 32 lanes per row (8 rows per block)
+  kAmgTableRestrict,        // 32 lanes per coarse row
+  kAmgTablePower,           // the level's nparts blocks
+  kAmgTableOne,             // one block
+  kAmgTableProduct,         // a block per row
+  kAmgTableCoarseProduct,   // a block per coarse row
+  kAmgTableGather,          // a thread per entry of P^T
+  kAmgTableSymmetric        // a thread per entry of level 0
+};
+long long amg_table_blocks(AmgTableKind kind, const AmgLevel& L, long long nnz0);
+
+// The tables of level l: the problems for which l is not the coarsest level
+// (smoothing, restriction, set-up) and those for which it is (dense).
+struct AmgLevelTables {
+  AmgTable elem, rows, restrict_, power, one, product, coarse_product, gather;
+  AmgTable dense_one, dense;
+};
+
+struct AmgTables {
+  std::vector<AmgLevelTables> levels;
+  AmgTable symmetric;  // level 0 of every problem
+};
+
 // Product blocks of a level of n rows (the power iteration's partial sums).
 int amg_product_blocks(long long n);
-// The numeric set-up of level `L` from its values (A_l): the diagonal, the
-// power estimate and the coefficients, P, A P, P^T and the next level's
-// values (next->values, which the host points at the next level's buffer).
-int launch_amg_level_setup(const AmgLevel& L, const AmgLevel& next, hipStream_t s);
-// The coarsest level's dense inverse (kAmgSingular on a zero pivot).
-int launch_amg_dense_setup(const AmgLevel& L, hipStream_t s);
-// out = (A + A^T) / 2 over a pattern whose entry k has its mirror at
-// mirror[k] (-1: none, the mirror taken as 0).
-int launch_amg_symmetric_part(long long nnz, const double* values, const int* mirror, double* out, hipStream_t s);
-// x = M^-1 b: one V-cycle over levels[0, count) with levels[0]'s b and x
-// replaced by the arguments.
-int launch_amg_vcycle(const AmgLevel* levels, int count, const double* b, double* x, hipStream_t s);
+// The numeric set-up of every hierarchy of the tables from their level 0
+// values (the symmetric part first when symmetrise): per level, the
+// diagonal, the power estimate and the coefficients, P, A P, P^T and the
+// next level's values; the dense inverse of each coarsest level
+// (kAmgSingular on a zero pivot).
+int launch_amg_setup(const AmgTables& T, bool symmetrise, hipStream_t s);
+// x = M^-1 b: one V-cycle of every hierarchy whose problem is active (null:
+// all), level 0 reading b and writing x at each problem's offset.
+int launch_amg_vcycle(const AmgTables& T, const int* active, const double* b, double* x, hipStream_t s);
 
 }  // namespace sbk

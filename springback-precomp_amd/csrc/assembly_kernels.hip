@@ -18,11 +18,14 @@
 //   K_e are contiguous; otherwise they are three 24-byte runs per row block.
 // gather_force_kernel: one thread per DOF: f_int and the gross |f_e| sum.
 // sum_squares_kernel: a fixed-shape tree for the norms (deterministic).
+#include <algorithm>
+
 #include "assembly_kernels.hpp"
 
 namespace sbk {
 
 constexpr int kNodeWaves = 4;  // nodes (wavefronts) per block
+constexpr int kGrossMaxGroup = 64;  // nodes per workgroup of the stiffness_gross product
 constexpr int kBatch = 4;      // incidences loaded ahead (more registers cost more than they hide)
 constexpr int kXcds = 8;       // MI355X: workgroups go round-robin to 8 XCDs, each with its own L2
 
@@ -125,79 +128,127 @@ __global__ __launch_bounds__(256) void gather_force_kernel(ForceArgs A) {
 // over row r's columns c, in ascending order, of |K(r, c)| |x(c)|, each
 // product rounded before it is added (the reference's column loop reaches a
 // row's entries in that order, with no fused multiply-add), columns with
-// x(c) = 0 skipped as there; for row-major values or the reference's
-// column-major ones (block_mirror: where the row sits in the column). One
-// wavefront per node: its lanes read the node's 9 deg values together and
-// keep the rounded products in LDS (exactly 0 for a skipped column, which
-// then adds nothing), and three lanes add their rows in column order.
+// x(c) = 0 adding an exact 0 (the reference skips them; products are never
+// negative, so the sum is the same).
+//
+// A workgroup takes A.group_nodes consecutive nodes. Its threads first form
+// every product of their rows, a (block, column) run of three rows at a time,
+// into LDS in row-major order: row-major values of rows 3a..3a+2 are
+// row_len apart; the reference's column-major ones sit together at
+// block_trans[k].x + kk block_trans[k].y (block (a, b), column 3b + kk;
+// block_trans[k].y is the length of b's columns). All of these loads and
+// the x gathers are independent. Then one thread per row adds its row's
+// products from LDS in column order.
+template <bool CM>
 __global__ __launch_bounds__(256) void abs_product_kernel(AbsProductArgs A) {
-  extern __shared__ double prod_all[];
-  const int w = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const long long n = static_cast<long long>(blockIdx.x) * kNodeWaves + w;
-  if (n >= A.num_nodes) return;  // the whole wavefront
-  double* prod = prod_all + w * 9 * A.max_degree;
-  const int first = A.row_ptr[3 * n];
-  const int row_len = A.row_ptr[3 * n + 1] - first;  // 3 deg(n)
-  const int len = 3 * row_len;
-  const int blk0 = first / 9;                        // 9 deg values per earlier node
-  for (int j = lane; j < len; j += 64) {
-    // entry (row l, block k, column kk of the block): row-major values run
-    // along the rows (j = l row_len + 3 k + kk), column-major ones along a
-    // column's three rows (j = 9 k + 3 kk + l), so that a wavefront's loads
-    // are contiguous runs either way
-    int l, k, kk;
-    if (A.column_major) {
-      k = j / 9;
-      kk = (j / 3) % 3;
-      l = j % 3;
-    } else {
-      l = j / row_len;
-      const int q = j - l * row_len;
-      k = q / 3;
-      kk = q - 3 * k;
+  extern __shared__ double prod[];
+  __shared__ int blk_start[kGrossMaxGroup + 1];  // the group's nodes' first blocks, then its end
+  // XCD-aware for the column-major gather: XCD x takes the x-th contiguous
+  // range of groups, so neighbouring nodes' column runs meet on one L2
+  const long long per = gridDim.x / kXcds;  // the grid is a multiple of kXcds
+  const long long grp = CM ? static_cast<long long>(blockIdx.x % kXcds) * per + blockIdx.x / kXcds
+                           : static_cast<long long>(blockIdx.x);
+  const long long n0 = grp * A.group_nodes;
+  if (n0 >= A.num_nodes) return;  // the whole workgroup
+  const int g = static_cast<int>(n0 + A.group_nodes <= A.num_nodes ? A.group_nodes : A.num_nodes - n0);
+  const int base = A.row_ptr[3 * n0];
+  for (int i = threadIdx.x; i <= g; i += blockDim.x) blk_start[i] = A.row_ptr[3 * (n0 + i)] / 9;
+  __syncthreads();
+  const int b0 = blk_start[0];
+  const int runs = 3 * (blk_start[g] - b0);
+  // kR runs per thread at a time: their index loads, then their value and x
+  // loads, are issued together
+  constexpr int kR = 8;
+  for (int i0 = threadIdx.x; i0 < runs; i0 += kR * blockDim.x) {
+    int blk[kR], col[kR], start[kR], row_len[kR], bcol[kR];
+    int2 t[kR];
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+      const int i = i0 + u * static_cast<int>(blockDim.x);
+      blk[u] = b0 + (i < runs ? i : runs - 1) / 3;  // past the end: the last run again, unused
+      int lo = 0, hi = g - 1;  // the node whose blocks hold blk
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (blk_start[mid] <= blk[u]) lo = mid; else hi = mid - 1;
+      }
+      start[u] = 9 * blk_start[lo];  // row_ptr of the node's first row
+      row_len[u] = 3 * (blk_start[lo + 1] - blk_start[lo]);
+      col[u] = 3 * (blk[u] - blk_start[lo]) + i % 3;  // the column within the row: 3k + kk
     }
-    const long long c = 3ll * A.block_col[blk0 + k] + kk;
-    const double xc = fabs(A.x[c]);
-    double p = 0.0;
-    if (xc != 0.0) {
-      const double v = A.column_major ? A.values[A.row_ptr[c] + 3 * A.block_mirror[blk0 + k] + l] : A.values[first + j];
-      p = __dmul_rn(fabs(v), xc);
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+      bcol[u] = A.block_col[blk[u]];
+      if constexpr (CM) t[u] = A.block_trans[blk[u]];
     }
-    prod[l * row_len + 3 * k + kk] = p;
+    double xc[kR], v[kR][3];
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+      const int c = col[u] % 3;
+      xc[u] = A.x[3ll * bcol[u] + c];
+      if constexpr (CM) {
+        const double* src = A.values + t[u].x + static_cast<long long>(c) * t[u].y;
+#pragma unroll
+        for (int l = 0; l < 3; ++l) v[u][l] = src[l];
+      } else {
+        const double* src = A.values + start[u] + col[u];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) v[u][l] = src[l * row_len[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kR; ++u) {
+      if (i0 + u * static_cast<int>(blockDim.x) >= runs) break;
+      const double x_abs = fabs(xc[u]);
+      double* dst = prod + (start[u] - base) + col[u];
+#pragma unroll
+      for (int l = 0; l < 3; ++l) dst[l * row_len[u]] = x_abs != 0.0 ? __dmul_rn(fabs(v[u][l]), x_abs) : 0.0;
+    }
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (lane < 3) {
-    double acc = 0.0;
-    for (int q = 0; q < row_len; ++q) acc = __dadd_rn(acc, prod[lane * row_len + q]);
-    A.product[3 * n + lane] = acc;
+  __syncthreads();
+  if (static_cast<int>(threadIdx.x) >= 3 * g) return;
+  const int a = static_cast<int>(threadIdx.x) / 3, l = static_cast<int>(threadIdx.x) % 3;
+  const int row_len = 3 * (blk_start[a + 1] - blk_start[a]);
+  const double* p = prod + (9 * blk_start[a] - base) + l * row_len;
+  // row_len is a multiple of 3: nine products read ahead of their adds
+  double acc = 0.0;
+  int q = 0;
+  for (; q + 9 <= row_len; q += 9) {
+    double t[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) t[j] = p[q + j];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) acc = __dadd_rn(acc, t[j]);
   }
-}
-
-// dst[i] = src[idx[i]]; an index outside [0, limit) gives NaN (which the
-// norm then shows) instead of a read outside src.
-__global__ __launch_bounds__(256) void gather_kernel(const double* __restrict__ src, long long limit,
-                                                     const int* __restrict__ idx, long long n,
-                                                     double* __restrict__ dst) {
-  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int k = idx[i];
-  dst[i] = (k >= 0 && k < limit) ? src[k] : __builtin_nan("");
+  for (; q < row_len; q += 3) {
+    const double t0 = p[q], t1 = p[q + 1], t2 = p[q + 2];
+    acc = __dadd_rn(__dadd_rn(__dadd_rn(acc, t0), t1), t2);
+  }
+  A.product[3 * n0 + threadIdx.x] = acc;
 }
 
 // Sums (of squares, or of values) of x[0, n) in a fixed order: each block of
 // the grid sums a contiguous slice with a fixed in-block tree and writes a
 // partial; one block then sums the partials the same way.
+// With idx, element i is x[idx[i]] (NaN for an index outside [0, limit)):
+// the same sums as over the gathered vector, without writing it.
 __global__ __launch_bounds__(256) void sum_partials_kernel(const double* __restrict__ x, long long n,
-                                                           double* __restrict__ partial, int square) {
+                                                           double* __restrict__ partial, int square,
+                                                           const int* __restrict__ idx, long long limit) {
   __shared__ double s[256];
   const long long per = (n + gridDim.x - 1) / gridDim.x;
   const long long b0 = static_cast<long long>(blockIdx.x) * per;
   const long long b1 = b0 + per < n ? b0 + per : n;
   double acc = 0.0;
-  for (long long i = b0 + threadIdx.x; i < b1; i += blockDim.x) acc = acc + (square ? x[i] * x[i] : x[i]);
+  for (long long i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+    double v;
+    if (idx) {
+      const int k = idx[i];
+      v = (k >= 0 && k < limit) ? x[k] : __builtin_nan("");
+    } else {
+      v = x[i];
+    }
+    acc = acc + (square ? v * v : v);
+  }
   s[threadIdx.x] = acc;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
@@ -256,17 +307,25 @@ int launch_assemble_values(const AssemblyArgs& a, hipStream_t s) {
 int launch_stiffness_gross(const AbsProductArgs& a, hipStream_t s) {
   (void)hipGetLastError();
   if (a.num_nodes > 0) {
-    const unsigned int grid = static_cast<unsigned int>((a.num_nodes + kNodeWaves - 1) / kNodeWaves);
-    const std::size_t lds = sizeof(double) * kNodeWaves * 9 * static_cast<std::size_t>(a.max_degree);
-    hipLaunchKernelGGL(abs_product_kernel, dim3(grid), dim3(64 * kNodeWaves), lds, s, a);
+    // nodes per workgroup: their products (at most 9 max_degree each) within
+    // 16 KB of LDS (8 nodes of degree 27: measured best against 4 and 16 on
+    // the 100 x 100 x 4 plate), at most kGrossMaxGroup (3 rows per node <= 256
+    // threads)
+    AbsProductArgs b = a;
+    const long long per_node = 9ll * (a.max_degree > 0 ? a.max_degree : 1) * static_cast<long long>(sizeof(double));
+    b.group_nodes = static_cast<int>(std::max(1ll, std::min<long long>(kGrossMaxGroup, 16384 / per_node)));
+    const long long groups = (a.num_nodes + b.group_nodes - 1) / b.group_nodes;
+    const unsigned int grid = static_cast<unsigned int>((groups + kXcds - 1) / kXcds * kXcds);
+    const std::size_t lds = static_cast<std::size_t>(per_node) * b.group_nodes;
+    if (a.column_major)
+      hipLaunchKernelGGL(abs_product_kernel<true>, dim3(grid), dim3(256), lds, s, b);
+    else
+      hipLaunchKernelGGL(abs_product_kernel<false>, dim3(grid), dim3(256), lds, s, b);
   }
-  if (a.num_free > 0)
-    hipLaunchKernelGGL(gather_kernel, dim3(static_cast<unsigned int>((a.num_free + 255) / 256)), dim3(256), 0, s,
-                       a.product, 3 * a.num_nodes, a.free, a.num_free, a.gathered);
-  const int parts = static_cast<int>(a.num_free / 4096 + 1 < 1024 ? a.num_free / 4096 + 1 : 1024);
+  const int parts = static_cast<int>(a.num_free / 256 + 1 < 1024 ? a.num_free / 256 + 1 : 1024);
   const int rc = static_cast<int>(hipGetLastError());
   if (rc != 0) return rc;
-  return launch_reduce(a.gathered, a.num_free, 1, a.partial, parts, a.out, s);
+  return launch_reduce_gather(a.product, a.free, 3 * a.num_nodes, a.num_free, 1, a.partial, parts, a.out, s);
 }
 
 int launch_gather_force(const ForceArgs& a, hipStream_t s) {
@@ -280,10 +339,15 @@ int launch_gather_force(const ForceArgs& a, hipStream_t s) {
 
 int launch_reduce(const double* x, long long n, int square, double* partial, int nparts, double* out,
                   hipStream_t s) {
+  return launch_reduce_gather(x, nullptr, 0, n, square, partial, nparts, out, s);
+}
+
+int launch_reduce_gather(const double* x, const int* idx, long long limit, long long n, int square, double* partial,
+                         int nparts, double* out, hipStream_t s) {
   (void)hipGetLastError();
   if (nparts < 1) nparts = 1;
   if (nparts > 1024) nparts = 1024;
-  hipLaunchKernelGGL(sum_partials_kernel, dim3(nparts), dim3(256), 0, s, x, n, partial, square);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(nparts), dim3(256), 0, s, x, n, partial, square, idx, limit);
   hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, partial, nparts, out, square);
   return static_cast<int>(hipGetLastError());
 }

@@ -28,20 +28,21 @@ struct ForceArgs {
 };
 
 // stiffness_gross: product = |K| |x| (3 num_nodes), out[0] = the 2-norm of
-// product over the free DOFs (gathered: num_free doubles of scratch).
+// product over the free DOFs.
 struct AbsProductArgs {
   const int* row_ptr;
   const int* block_col;         // per block (row-major block order): its column node
-  const int* block_mirror;      // per block (a, b): the index of a in b's neighbourhood
+  const int2* block_trans;      // per block (a, b): {where row 3a sits in column 3b of the
+                                // column-major values, the length of b's columns}
   long long num_nodes;
-  int max_degree;               // the largest neighbourhood (sizes the LDS products)
+  int max_degree;               // the largest neighbourhood (sizes the LDS staging)
+  int group_nodes;              // set by the launch: nodes per workgroup
   const double* values;         // nnz, row-major or the reference's column-major
   int column_major;
   const double* x;              // 3 num_nodes
   const int* free;              // num_free DOF indices
   long long num_free;
   double* product;              // 3 num_nodes
-  double* gathered;             // num_free
   double* partial;              // 1024
   double* out;                  // 1
 };
@@ -55,6 +56,9 @@ int launch_gather_force(const ForceArgs& a, hipStream_t s);
 int launch_reduce_max(const double* x, long long n, double* partial, int nparts, double* out, hipStream_t s);
 int launch_reduce(const double* x, long long n, int square, double* partial, int nparts, double* out,
                   hipStream_t s);
+// The same over x[idx[i]], i in [0, n) (NaN for an index outside [0, limit)).
+int launch_reduce_gather(const double* x, const int* idx, long long limit, long long n, int square, double* partial,
+                         int nparts, double* out, hipStream_t s);
 // summary[0] failed elements, [1] lowest failed element, [2] yielding points
 int launch_status_summary(const int* status, const int* yielding, long long n, long long* summary, hipStream_t s);
 

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -42,6 +43,19 @@ void dfree(T*& p) {
   p = nullptr;
 }
 
+// Per block (a, b) in row-major block order: where row 3a's entry of column
+// 3b sits in the column-major values (column 3b's start plus 3 times a's
+// place in b's neighbourhood), and the length of b's columns (3 deg(b)).
+std::vector<int2> block_trans(const sbk::HostPattern& p) {
+  std::vector<int2> t(p.block_mirror.size());
+  for (std::size_t k = 0; k < t.size(); ++k) {
+    const int32_t b = p.block_col_node[k];
+    const int32_t start = p.row_ptr[3 * static_cast<std::size_t>(b)];
+    t[k] = make_int2(start + 3 * p.block_mirror[k], p.row_ptr[3 * static_cast<std::size_t>(b) + 1] - start);
+  }
+  return t;
+}
+
 // 24 x 24 element tangents, column-major to row-major, in place (host).
 void transpose_tangents(double* ke, int64_t count) {
   for (int64_t e = 0; e < count; ++e) {
@@ -51,9 +65,16 @@ void transpose_tangents(double* ke, int64_t count) {
   }
 }
 
-void release_partition(sparlab_system* s) {
+void release_multigrid(sparlab_system* s) {
   delete s->amg;
   s->amg = nullptr;
+  delete s->amg_seg;
+  s->amg_seg = nullptr;
+  s->amg_seg_rows.clear();
+}
+
+void release_partition(sparlab_system* s) {
+  release_multigrid(s);
   dfree(s->part_counts); dfree(s->part_sums); dfree(s->part_bad);
   dfree(s->part_ptr); dfree(s->part_index); dfree(s->part_source); dfree(s->part_work);
   s->part_free = -1;
@@ -69,7 +90,7 @@ void release(sparlab_system* s) {
   dfree(s->yielding); dfree(s->iterations); dfree(s->status);
   dfree(s->row_ptr); dfree(s->inc_pos);
   dfree(s->node_ptr); dfree(s->node_incidence); dfree(s->diag_block);
-  dfree(s->block_col); dfree(s->block_mirror); dfree(s->kproduct); dfree(s->kgathered);
+  dfree(s->block_col); dfree(s->block_trans); dfree(s->kproduct);
   dfree(s->ws); dfree(s->partial); dfree(s->scalars); dfree(s->summary);
   dfree(s->grad_x); dfree(s->grad_u); dfree(s->grad_p);
   dfree(s->seg_work); dfree(s->seg_meta);
@@ -176,8 +197,8 @@ int sparlab_system_create(const double* coords, int64_t num_nodes, const int32_t
             dalloc(s->scalars, 3) && dalloc(s->summary, 3) && upload(s->row_ptr, s->pattern.row_ptr) &&
             upload(s->inc_pos, s->pattern.inc_pos) && upload(s->node_ptr, s->pattern.node_ptr) &&
             upload(s->node_incidence, s->pattern.node_incidence) && upload(s->diag_block, s->pattern.diag_block) &&
-            upload(s->block_col, s->pattern.block_col_node) && upload(s->block_mirror, s->pattern.block_mirror) &&
-            dalloc(s->kproduct, 3 * nn) && dalloc(s->kgathered, 3 * nn);
+            upload(s->block_col, s->pattern.block_col_node) && upload(s->block_trans, block_trans(s->pattern)) &&
+            dalloc(s->kproduct, 3 * nn);
   ok = ok && hipMemcpy(s->coords, coords, 3 * nn * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(s->conn, conn, 8 * ne * sizeof(int32_t), hipMemcpyHostToDevice) == hipSuccess &&
        hipMemset(s->state[0], 0, hist * sizeof(double)) == hipSuccess &&
@@ -264,7 +285,7 @@ int sparlab_system_stiffness_gross_device(sparlab_system* s, const double* value
   sbk::AbsProductArgs a{};
   a.row_ptr = s->row_ptr;
   a.block_col = s->block_col;
-  a.block_mirror = s->block_mirror;
+  a.block_trans = s->block_trans;
   a.num_nodes = s->nn;
   a.max_degree = s->pattern.max_degree;
   a.values = values;
@@ -273,7 +294,6 @@ int sparlab_system_stiffness_gross_device(sparlab_system* s, const double* value
   a.free = free_dofs;
   a.num_free = num_free;
   a.product = product ? product : s->kproduct;
-  a.gathered = s->kgathered;
   a.partial = s->partial;
   a.out = out;
   if (sbk::launch_stiffness_gross(a, static_cast<hipStream_t>(stream)) != 0)
@@ -506,32 +526,113 @@ bool run_block_solve(sbk::CgArgs& a, int symmetric, const double* b, double* x, 
   return true;
 }
 
+// The partition's free DOFs (ascending) and the node coordinates, from the
+// device.
+bool read_partition(sparlab_system* s, std::vector<long long>& free_dofs, std::vector<double>& coords,
+                    hipStream_t st) {
+  const std::size_t ndof = 3 * static_cast<std::size_t>(s->nn);
+  std::vector<int32_t> map(ndof);
+  coords.resize(ndof);
+  if (hipMemcpyAsync(map.data(), s->part_map, ndof * sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(coords.data(), s->coords, ndof * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return false;
+  free_dofs.clear();
+  free_dofs.reserve(static_cast<std::size_t>(s->part_free));
+  for (std::size_t d = 0; d < ndof; ++d)
+    if (map[d] >= 0) free_dofs.push_back(static_cast<long long>(d));
+  return true;
+}
+
+// Set up a batch for the block's values; the first singular problem, or -1.
+int set_up_multigrid(sbk::AmgBatch& batch, const double* block_values, int symmetric, hipStream_t st, int& bad) {
+  bad = -1;
+  if (batch.setup(block_values, !symmetric, st) != 0) return sbk::capi_fail(SPARLAB_E_DEVICE, "the multigrid set-up failed");
+  const std::vector<int> sing = batch.singular(st);
+  for (std::size_t p = 0; p < sing.size(); ++p)
+    if (sing[p]) {
+      bad = static_cast<int>(p);
+      break;
+    }
+  return SPARLAB_OK;
+}
+
 // The partition's multigrid hierarchy (built at the first solve after the
 // partition or new coordinates) set up for the block's values.
 int prepare_multigrid(sparlab_system* s, const double* block_values, int symmetric, hipStream_t st) {
   if (!s->amg) {
-    const std::size_t ndof = 3 * static_cast<std::size_t>(s->nn);
-    std::vector<int32_t> map(ndof);
-    std::vector<double> coords(ndof);
-    if (hipMemcpyAsync(map.data(), s->part_map, ndof * sizeof(int32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipMemcpyAsync(coords.data(), s->coords, ndof * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+    std::vector<long long> free_dofs;
+    std::vector<double> coords;
+    if (!read_partition(s, free_dofs, coords, st)) return sbk::capi_fail(SPARLAB_E_DEVICE, "reading the partition failed");
+    auto h = std::make_unique<sbk::AmgHierarchy>();
+    const std::string why = h->build(s->part_ptr, s->part_index, s->part_free, s->part_nnz, free_dofs, coords, st);
+    if (!why.empty()) return sbk::capi_fail(SPARLAB_E_SOLVER, "the multigrid hierarchy could not be built: " + why);
+    auto* b = new sbk::AmgBatch();
+    b->add(std::move(h), 0, 0);
+    if (b->finalize() != 0) {
+      delete b;
+      return sbk::capi_fail(SPARLAB_E_DEVICE, "device allocation failed");
+    }
+    s->amg = b;
+  }
+  int bad = -1;
+  const int rc = set_up_multigrid(*s->amg, block_values, symmetric, st, bad);
+  if (rc != SPARLAB_OK) return rc;
+  if (bad >= 0) return sbk::capi_fail(SPARLAB_E_SOLVER, "the multigrid hierarchy's coarsest level is singular");
+  return SPARLAB_OK;
+}
+
+// One hierarchy per segment of the partition's block (each built from its
+// rows alone, as a problem's own block), set up for the block's values.
+int prepare_segment_multigrid(sparlab_system* s, const std::vector<long long>& rows, const double* block_values,
+                              int symmetric, hipStream_t st) {
+  if (!s->amg_seg || s->amg_seg_rows != rows) {
+    delete s->amg_seg;
+    s->amg_seg = nullptr;
+    s->amg_seg_rows.clear();
+    std::vector<long long> free_dofs;
+    std::vector<double> coords;
+    std::vector<int32_t> ptr(static_cast<std::size_t>(s->part_free) + 1), index(static_cast<std::size_t>(s->part_nnz));
+    if (!read_partition(s, free_dofs, coords, st) ||
+        hipMemcpyAsync(ptr.data(), s->part_ptr, ptr.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        (!index.empty() && hipMemcpyAsync(index.data(), s->part_index, index.size() * sizeof(int32_t),
+                                          hipMemcpyDeviceToHost, st) != hipSuccess) ||
         hipStreamSynchronize(st) != hipSuccess)
       return sbk::capi_fail(SPARLAB_E_DEVICE, "reading the partition failed");
-    std::vector<long long> free_dofs;
-    free_dofs.reserve(static_cast<std::size_t>(s->part_free));
-    for (std::size_t d = 0; d < ndof; ++d)
-      if (map[d] >= 0) free_dofs.push_back(static_cast<long long>(d));
-    auto* h = new sbk::AmgHierarchy();
-    const std::string why = h->build(s->part_ptr, s->part_index, s->part_free, s->part_nnz, free_dofs, coords, st);
-    if (!why.empty()) {
-      delete h;
-      return sbk::capi_fail(SPARLAB_E_SOLVER, "the multigrid hierarchy could not be built: " + why);
+    auto* b = new sbk::AmgBatch();
+    const int nseg = static_cast<int>(rows.size()) - 1;
+    for (int p = 0; p < nseg; ++p) {
+      const long long r0 = rows[p], r1 = rows[p + 1];
+      const int v0 = ptr[static_cast<std::size_t>(r0)], v1 = ptr[static_cast<std::size_t>(r1)];
+      // the segment's rows alone: pointers from 0, columns from r0 (the
+      // segment check has shown that none leaves the segment)
+      std::vector<int> sp(static_cast<std::size_t>(r1 - r0) + 1), si(static_cast<std::size_t>(v1 - v0));
+      for (long long i = r0; i <= r1; ++i) sp[static_cast<std::size_t>(i - r0)] = ptr[static_cast<std::size_t>(i)] - v0;
+      for (int k = v0; k < v1; ++k) si[static_cast<std::size_t>(k - v0)] = index[static_cast<std::size_t>(k)] - static_cast<int>(r0);
+      std::vector<long long> sf(free_dofs.begin() + r0, free_dofs.begin() + r1);
+      auto h = std::make_unique<sbk::AmgHierarchy>();
+      const std::string why = h->build_host(std::move(sp), std::move(si), sf, coords, st);
+      if (!why.empty()) {
+        delete b;
+        return sbk::capi_fail(SPARLAB_E_SOLVER, "segment " + std::to_string(p) +
+                                                    ": the multigrid hierarchy could not be built: " + why);
+      }
+      b->add(std::move(h), r0, v0);
     }
-    s->amg = h;
+    if (b->finalize() != 0) {
+      delete b;
+      return sbk::capi_fail(SPARLAB_E_DEVICE, "device allocation failed");
+    }
+    s->amg_seg = b;
+    s->amg_seg_rows = rows;
   }
-  if (s->amg->setup(block_values, !symmetric, st) != 0) return sbk::capi_fail(SPARLAB_E_DEVICE, "the multigrid set-up failed");
-  if (s->amg->singular(st))
-    return sbk::capi_fail(SPARLAB_E_SOLVER, "the multigrid hierarchy's coarsest level is singular");
+  int bad = -1;
+  const int rc = set_up_multigrid(*s->amg_seg, block_values, symmetric, st, bad);
+  if (rc != SPARLAB_OK) return rc;
+  if (bad >= 0)
+    return sbk::capi_fail(SPARLAB_E_SOLVER,
+                          "segment " + std::to_string(bad) + ": the multigrid hierarchy's coarsest level is singular");
   return SPARLAB_OK;
 }
 
@@ -567,6 +668,7 @@ int sparlab_system_solve_free_block_device(sparlab_system* s, const double* bloc
     if (prepared != SPARLAB_OK) return prepared;
     a.external = 1;
     a.precond = sbk::amg_apply;
+    s->amg->set_active(nullptr);
     a.precond_ctx = s->amg;
   }
   int32_t end = kSolved;
@@ -649,6 +751,14 @@ int sparlab_system_solve_free_block_segments_device(sparlab_system* s, const dou
       hipStreamSynchronize(st) != hipSuccess)
     return sbk::capi_fail(SPARLAB_E_DEVICE, "the segments' check failed");
   if (bad) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "the free-free block couples two segments");
+  if (s->preconditioner == SPARLAB_PRECONDITIONER_MULTIGRID) {
+    const int prepared = prepare_segment_multigrid(s, rows, block_values, symmetric, st);
+    if (prepared != SPARLAB_OK) return prepared;
+    a.external = 1;
+    a.precond = sbk::amg_apply;
+    s->amg_seg->set_active(a.active);  // a resting problem's V-cycles do nothing
+    a.precond_ctx = s->amg_seg;
+  }
   if (!run_block_solve(a, symmetric, b, x, tolerance, max_iterations, st, iterations, relative_residual, status))
     return sbk::capi_fail(SPARLAB_E_DEVICE, "the block solve's launches failed");
   for (int p = 0; p < num_segments; ++p)
@@ -770,8 +880,7 @@ int sparlab_system_set_coords(sparlab_system* s, const double* coords) {
       hipSuccess)
     return sbk::capi_fail(SPARLAB_E_DEVICE, "upload failed");
   s->adjoint_ready = false;
-  delete s->amg;  // its rigid-body modes were the old coordinates'
-  s->amg = nullptr;
+  release_multigrid(s);  // their rigid-body modes were the old coordinates'
   return SPARLAB_OK;
 }
 

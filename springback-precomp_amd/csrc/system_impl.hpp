@@ -41,17 +41,22 @@ struct sparlab_system {
   int32_t* row_ptr = nullptr;
   uint64_t* inc_pos = nullptr;
   int32_t *node_ptr = nullptr, *node_incidence = nullptr, *diag_block = nullptr;
-  int32_t *block_col = nullptr, *block_mirror = nullptr;  // stiffness_gross maps
-  double *kproduct = nullptr, *kgathered = nullptr;      // stiffness_gross scratch (3 nn each)
+  int32_t* block_col = nullptr;    // per block: its column node
+  int2* block_trans = nullptr;     // per block: its place in the column-major values (stiffness_gross)
+  double* kproduct = nullptr;      // stiffness_gross scratch (3 nn)
   // the free-free block of the last partition (part_free < 0: none)
   int32_t *part_map = nullptr, *part_counts = nullptr, *part_sums = nullptr, *part_bad = nullptr;
   int32_t *part_ptr = nullptr, *part_index = nullptr, *part_source = nullptr;
   int64_t part_free = -1, part_nnz = 0;
   double* part_work = nullptr;  // the block solve's vectors, partial sums and scalars
   // the block solve's preconditioner (SPARLAB_PRECONDITIONER_*) and the
-  // partition's multigrid hierarchy (built at its first solve)
+  // partition's multigrid hierarchy (built at its first solve); for the
+  // segmented solve, one hierarchy per segment (built at the first solve
+  // with these segment rows)
   int preconditioner = 0;
-  sbk::AmgHierarchy* amg = nullptr;
+  sbk::AmgBatch* amg = nullptr;
+  sbk::AmgBatch* amg_seg = nullptr;
+  std::vector<long long> amg_seg_rows;
   // the segmented block solve's work and metadata (grown as needed)
   double* seg_work = nullptr;
   std::size_t seg_work_doubles = 0;

@@ -219,16 +219,24 @@ def test_contact_blocks_into_the_assembled_matrix(device):
     assert np.array_equal(values.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("renumber", [False, True])
 @pytest.mark.parametrize("column_major", [True, False])
-def test_stiffness_gross_bit_exact(device, column_major):
+def test_stiffness_gross_bit_exact(device, column_major, renumber):
     """stiffness_gross (NonlinearSystem.hpp:55-67): |K| |x| equal to the
     oracle's column loop bit for bit, from the reference's column-major values
     or the row-major values of the same (non-symmetric) K; the norm over the
-    free DOFs within 1e-14 (its sum is a tree); zero entries of x skipped."""
+    free DOFs within 1e-14 (its sum is a tree); zero entries of x skipped.
+    With the nodes numbered at random, neighbourhoods and their places in
+    each other's columns are irregular."""
     import torch
     w = W.assembly_workload(nx=24, ny=20, nz=3, contact_nodes=10)
     coords, conn = w["coords"], w["conn"]
     nn, ne = coords.shape[0], conn.shape[0]
+    if renumber:
+        perm = np.random.default_rng(3).permutation(nn)  # new number of each old node
+        new_coords = np.empty_like(coords)
+        new_coords[perm] = coords
+        coords, conn = new_coords, perm[conn].astype(conn.dtype)
     sysm = G.NonlinearSystem(coords, conn, G.Material(70e9, 0.33))
     outer, inner, _ = O.pattern(nn, conn)
     csc = O.scatter(nn, conn, w["ke"].reshape(ne, 576))

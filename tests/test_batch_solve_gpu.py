@@ -6,7 +6,10 @@ problem alone, which the block-solve tests pin by residuals.
 Bars: per problem, x, the iteration count, the relative residual and how the
 solve ended equal the plain solve's bit for bit (the same block cut, partial
 sums, scalars, reads and restarts) for conjugate gradients (J2 + Voce: a
-symmetric tangent) and BiCGStab (Hill48 + two Chaboche backstresses); a
+symmetric tangent) and BiCGStab (Hill48 + two Chaboche backstresses), with
+either preconditioner: the diagonal, or the multigrid V-cycle (each
+problem's own hierarchy, the problems' set-ups and V-cycles in shared
+launches; the problems here coarsen to one, two and three levels); a
 problem whose block is not positive definite fails alone (the others still
 equal their plain solves); an iteration limit leaves every problem with the
 plain solve's last iterate and residual; segments that cut a problem's
@@ -20,7 +23,7 @@ from sparlab_gpu import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-SIZES = ((6, 6, 2), (10, 8, 2), (4, 4, 1), (12, 12, 2), (7, 9, 2))
+SIZES = ((6, 6, 2), (10, 8, 2), (4, 4, 1), (12, 12, 2), (7, 9, 2), (28, 24, 2))
 
 
 def problems():
@@ -69,10 +72,12 @@ def setup(device, name):
     return probs, m, rule, b, frees, rows, block, rhs, m.symmetric_tangent
 
 
-def plain_solve(device, p, m, rule, free, negate=False, **kw):
-    """The problem alone: (x, iterations, relative residual, status)."""
+def plain_solve(device, p, m, rule, free, negate=False, precond="diagonal", levels=None, **kw):
+    """The problem alone: (x, iterations, relative residual, status); the
+    multigrid hierarchy's level count appended to levels."""
     import torch
     s = G.NonlinearSystem(p["coords"], p["conn"], m, rule)
+    s.set_block_preconditioner(precond)
     u = torch.as_tensor(p["u"].reshape(-1), device=device)
     f = torch.zeros_like(u)
     values = torch.zeros(s.nnz, dtype=torch.float64, device=device)
@@ -87,26 +92,38 @@ def plain_solve(device, p, m, rule, free, negate=False, **kw):
     x = torch.zeros_like(rhs)
     try:
         _, its, rel = s.solve_free_block_device(block, rhs, x=x, **kw)
+        if levels is not None and precond == "multigrid":
+            levels.append(len(s.block_preconditioner_levels()[0]))
         return x.cpu().numpy(), its, rel, 0
     except G.SolverError:
         return x.cpu().numpy(), None, None, 1
 
 
+@pytest.mark.parametrize("precond", ["diagonal", "multigrid"])
 @pytest.mark.parametrize("name", ["J2", "Hill48_Chaboche"])
-def test_segments_equal_plain_solves(device, name):
+def test_segments_equal_plain_solves(device, name, precond):
     probs, m, rule, b, frees, rows, block, rhs, sym = setup(device, name)
     assert sym == (name == "J2")
-    print(f"segmented block solve: {name}, {len(probs)} problems, {rhs.numel()} rows", flush=True)
+    b.system.set_block_preconditioner(precond)
+    print(f"segmented block solve: {name}, {precond}, {len(probs)} problems, {rhs.numel()} rows", flush=True)
     x, its, rel, status = b.system.solve_free_block_segments_device(block, rhs, rows, tolerance=1e-10,
                                                                     symmetric=sym)
     x = x.cpu().numpy()
     assert (status == 0).all(), status
     assert len(set(its.tolist())) > 1   # the problems stop at different iterations
+    levels = []
     for p, prob in enumerate(probs):
-        xp, itp, relp, stp = plain_solve(device, prob, m, rule, frees[p], tolerance=1e-10, symmetric=sym)
+        xp, itp, relp, stp = plain_solve(device, prob, m, rule, frees[p], precond=precond, levels=levels,
+                                         tolerance=1e-10, symmetric=sym)
         assert stp == 0
         assert np.array_equal(x[rows[p]:rows[p + 1]], xp), f"problem {p}: x"
         assert its[p] == itp and rel[p] == relp, (p, its[p], itp, rel[p], relp)
+    if precond == "multigrid":
+        assert {1, 2, 3} <= set(levels), levels
+        # again: the segments' hierarchies are kept and set up anew
+        x2, its2, rel2, _ = b.system.solve_free_block_segments_device(block, rhs, rows, tolerance=1e-10,
+                                                                      symmetric=sym)
+        assert np.array_equal(x2.cpu().numpy(), x) and np.array_equal(its2, its) and np.array_equal(rel2, rel)
 
 
 def test_a_failing_problem_fails_alone(device):
@@ -129,22 +146,24 @@ def test_a_failing_problem_fails_alone(device):
             assert its[p] == itp and rel[p] == relp
 
 
+@pytest.mark.parametrize("precond,limit", [("diagonal", 40), ("multigrid", 4)])
 @pytest.mark.parametrize("name", ["J2", "Hill48_Chaboche"])
-def test_iteration_limit(device, name):
+def test_iteration_limit(device, name, precond, limit):
     probs, m, rule, b, frees, rows, block, rhs, sym = setup(device, name)
+    b.system.set_block_preconditioner(precond)
     x, its, rel, status = b.system.solve_free_block_segments_device(block, rhs, rows, tolerance=1e-12,
-                                                                    max_iterations=40, symmetric=sym)
+                                                                    max_iterations=limit, symmetric=sym)
     x = x.cpu().numpy()
     assert (status != 0).sum() >= 3
     for p, prob in enumerate(probs):
-        xp, itp, relp, stp = plain_solve(device, prob, m, rule, frees[p], tolerance=1e-12, max_iterations=40,
-                                         symmetric=sym)
+        xp, itp, relp, stp = plain_solve(device, prob, m, rule, frees[p], precond=precond, tolerance=1e-12,
+                                         max_iterations=limit, symmetric=sym)
         assert stp == (1 if status[p] != 0 else 0), p
         assert np.array_equal(x[rows[p]:rows[p + 1]], xp), f"problem {p}: x"
         if stp == 0:
             assert its[p] == itp and rel[p] == relp
         else:
-            assert its[p] == 40
+            assert its[p] == limit
 
 
 def test_segments_must_not_cut_coupled_rows(device):
