@@ -256,94 +256,120 @@ def newton_leg(G, W, torch, dev, tolerance=1e-8, max_iterations=20000):
     Plain: each problem's own system, one after another. Batched: the union
     evaluated in its captured graph and every problem's solve in one
     segmented solve. Each problem's du must equal its plain solve's bit for
-    bit (x, iterations and residual: the same sums in the same order)."""
+    bit (x, iterations and residual: the same sums in the same order). Run
+    with the diagonal preconditioner and with the multigrid V-cycle (each
+    problem's own hierarchy; the batched form sets up and applies all of
+    them in shared launches); the first iteration of each form, which builds
+    the hierarchies on the host once per partition, is timed apart."""
     probs = W.batched_problems()
     m = W.microbench_material()
     rule = G.Rule(2, 0, 2)
     stream = torch.cuda.current_stream(dev)
     frees = [rim_free_dofs(p["coords"]) for p in probs]
-    # plain
-    plain = []
-    for p, fr in zip(probs, frees):
-        s = G.NonlinearSystem(p["coords"], p["conn"], m, rule)
-        u0 = torch.as_tensor(p["u"].reshape(-1), device=dev)
-        d_free = torch.as_tensor(fr, device=dev)
-        nnz = s.partition_device(d_free)
-        plain.append(dict(s=s, u0=u0, u=u0.clone(), f=torch.zeros_like(u0),
-                          v=torch.zeros(s.nnz, dtype=torch.float64, device=dev), free=d_free.long(),
-                          blk=torch.empty(max(nnz, 1), dtype=torch.float64, device=dev),
-                          x=torch.zeros(fr.size, dtype=torch.float64, device=dev)))
 
-    def plain_iteration():
-        its = []
-        for q in plain:
-            q["u"].copy_(q["u0"])
-            q["s"].evaluate_device(q["u"], q["f"], q["v"], column_major=False, stream=stream)
-            q["s"].result(stream)
-            q["s"].free_block_device(q["v"], out=q["blk"], stream=stream)
-            rhs = -q["f"][q["free"]]
-            try:
-                _, it, _ = q["s"].solve_free_block_device(q["blk"], rhs, x=q["x"], tolerance=tolerance,
-                                                          max_iterations=max_iterations, symmetric=False,
-                                                          stream=stream)
-            except G.SolverError:
-                it = -1
-            q["u"][q["free"]] += q["x"]
-            its.append(it)
-        return its
+    def run(precond):
+        # plain
+        plain = []
+        for p, fr in zip(probs, frees):
+            s = G.NonlinearSystem(p["coords"], p["conn"], m, rule)
+            s.set_block_preconditioner(precond)
+            u0 = torch.as_tensor(p["u"].reshape(-1), device=dev)
+            d_free = torch.as_tensor(fr, device=dev)
+            nnz = s.partition_device(d_free)
+            plain.append(dict(s=s, u0=u0, u=u0.clone(), f=torch.zeros_like(u0),
+                              v=torch.zeros(s.nnz, dtype=torch.float64, device=dev), free=d_free.long(),
+                              blk=torch.empty(max(nnz, 1), dtype=torch.float64, device=dev),
+                              x=torch.zeros(fr.size, dtype=torch.float64, device=dev)))
 
-    plain_iteration()  # warm-up
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    plain_its = plain_iteration()
-    torch.cuda.synchronize()
-    ms_plain = (time.perf_counter() - t0) * 1e3
-    plain_x = [q["x"].cpu().numpy() for q in plain]
-    del plain
-    torch.cuda.empty_cache()
-    # batched
-    b = G.BatchedSystem([(p["coords"], p["conn"]) for p in probs], m, rule)
-    u0 = torch.as_tensor(np.concatenate([p["u"] for p in probs]).reshape(-1), device=dev)
-    u = u0.clone()
-    fint = torch.zeros_like(u)
-    values = torch.zeros(b.system.nnz, dtype=torch.float64, device=dev)
-    free = np.concatenate([f + 3 * b.node_offset[i] for i, f in enumerate(frees)]).astype(np.int32)
-    d_free = torch.as_tensor(free, device=dev)
-    nnz = b.system.partition_device(d_free)
-    blk = torch.empty(max(nnz, 1), dtype=torch.float64, device=dev)
-    x = torch.zeros(free.size, dtype=torch.float64, device=dev)
-    rows = b.free_segments(free)
-    lfree = d_free.long()
-    b.autotune(u, stream)
+        def plain_iteration():
+            its = []
+            for q in plain:
+                q["u"].copy_(q["u0"])
+                q["s"].evaluate_device(q["u"], q["f"], q["v"], column_major=False, stream=stream)
+                q["s"].result(stream)
+                q["s"].free_block_device(q["v"], out=q["blk"], stream=stream)
+                rhs = -q["f"][q["free"]]
+                try:
+                    _, it, _ = q["s"].solve_free_block_device(q["blk"], rhs, x=q["x"], tolerance=tolerance,
+                                                              max_iterations=max_iterations, symmetric=False,
+                                                              stream=stream)
+                except G.SolverError:
+                    it = -1
+                q["u"][q["free"]] += q["x"]
+                its.append(it)
+            return its
 
-    def batched_iteration():
-        u.copy_(u0)
-        b.evaluate_device(u, fint, values, column_major=False, stream=stream)
-        sc = b.result(stream)
-        b.system.free_block_device(values, out=blk, stream=stream)
-        rhs = -fint[lfree]
-        _, its, _, status = b.system.solve_free_block_segments_device(blk, rhs, rows, x=x, tolerance=tolerance,
-                                                                      max_iterations=max_iterations,
-                                                                      symmetric=False, stream=stream)
-        u[lfree] += x
-        return its, status, sc
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plain_iteration()  # warm-up (with the multigrid hierarchies' build)
+        torch.cuda.synchronize()
+        first_plain = (time.perf_counter() - t0) * 1e3
+        t0 = time.perf_counter()
+        plain_its = plain_iteration()
+        torch.cuda.synchronize()
+        ms_plain = (time.perf_counter() - t0) * 1e3
+        plain_x = [q["x"].cpu().numpy() for q in plain]
+        del plain
+        torch.cuda.empty_cache()
+        # batched
+        b = G.BatchedSystem([(p["coords"], p["conn"]) for p in probs], m, rule)
+        b.system.set_block_preconditioner(precond)
+        u0 = torch.as_tensor(np.concatenate([p["u"] for p in probs]).reshape(-1), device=dev)
+        u = u0.clone()
+        fint = torch.zeros_like(u)
+        values = torch.zeros(b.system.nnz, dtype=torch.float64, device=dev)
+        free = np.concatenate([f + 3 * b.node_offset[i] for i, f in enumerate(frees)]).astype(np.int32)
+        d_free = torch.as_tensor(free, device=dev)
+        nnz = b.system.partition_device(d_free)
+        blk = torch.empty(max(nnz, 1), dtype=torch.float64, device=dev)
+        x = torch.zeros(free.size, dtype=torch.float64, device=dev)
+        rows = b.free_segments(free)
+        lfree = d_free.long()
+        b.autotune(u, stream)
 
-    batched_iteration()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    its, status, sc = batched_iteration()
-    torch.cuda.synchronize()
-    ms_batched = (time.perf_counter() - t0) * 1e3
-    xb = x.cpu().numpy()
-    identical = all(np.array_equal(xb[rows[p]:rows[p + 1]], plain_x[p]) for p in range(len(probs))) and \
-        [int(i) if st == 0 else -1 for i, st in zip(its, status)] == [int(i) for i in plain_its]
-    return {"workload": "one Newton iteration of the 64 forming problems (rims clamped): evaluation, free-free "
-                        f"block, BiCGStab to {tolerance:g}, update",
-            "problems": len(probs), "free_dofs": int(free.size), "tolerance": tolerance,
-            "plain_ms": ms_plain, "batched_ms": ms_batched, "speedup": ms_plain / ms_batched,
-            "iterations_min": int(np.min(its)), "iterations_median": float(np.median(its)),
-            "iterations_max": int(np.max(its)), "failed_solves": int((status != 0).sum()),
-            "failed_elements": int(sc["failed"].sum()), "identical_to_plain": bool(identical)}
+        def batched_iteration():
+            u.copy_(u0)
+            b.evaluate_device(u, fint, values, column_major=False, stream=stream)
+            sc = b.result(stream)
+            b.system.free_block_device(values, out=blk, stream=stream)
+            rhs = -fint[lfree]
+            _, its, _, status = b.system.solve_free_block_segments_device(blk, rhs, rows, x=x, tolerance=tolerance,
+                                                                          max_iterations=max_iterations,
+                                                                          symmetric=False, stream=stream)
+            u[lfree] += x
+            return its, status, sc
+
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        batched_iteration()
+        torch.cuda.synchronize()
+        first_batched = (time.perf_counter() - t0) * 1e3
+        t0 = time.perf_counter()
+        its, status, sc = batched_iteration()
+        torch.cuda.synchronize()
+        ms_batched = (time.perf_counter() - t0) * 1e3
+        xb = x.cpu().numpy()
+        identical = all(np.array_equal(xb[rows[p]:rows[p + 1]], plain_x[p]) for p in range(len(probs))) and \
+            [int(i) if st == 0 else -1 for i, st in zip(its, status)] == [int(i) for i in plain_its]
+        out = {"plain_ms": ms_plain, "batched_ms": ms_batched, "speedup": ms_plain / ms_batched,
+               "iterations_min": int(np.min(its)), "iterations_median": float(np.median(its)),
+               "iterations_max": int(np.max(its)), "failed_solves": int((status != 0).sum()),
+               "failed_elements": int(sc["failed"].sum()), "identical_to_plain": bool(identical),
+               "first_iteration_ms": {"plain": first_plain, "batched": first_batched}}
+        del b, values, blk
+        torch.cuda.empty_cache()
+        return out, int(free.size)
+
+    diag, nfree = run("diagonal")
+    mg, _ = run("multigrid")
+    out = {"workload": "one Newton iteration of the 64 forming problems (rims clamped): evaluation, free-free "
+                       f"block, BiCGStab to {tolerance:g}, update",
+           "problems": len(probs), "free_dofs": nfree, "tolerance": tolerance}
+    out.update(diag)
+    out["multigrid"] = dict(mg, note="the same iteration with each problem's block solve preconditioned by its "
+                                     "own multigrid V-cycle; first_iteration_ms includes the hierarchies' host "
+                                     "build (once per partition)")
+    return out
 
 
 def differentiation_leg(G, W, torch, dev):
@@ -797,6 +823,12 @@ def main():
               f"BiCGStab iterations {newton['iterations_min']}..{newton['iterations_max']} (median "
               f"{newton['iterations_median']:.0f}); identical to plain: {newton['identical_to_plain']}",
               file=sys.stderr, flush=True)
+        mg = newton["multigrid"]
+        print(f"# batched Newton iteration, multigrid: plain {mg['plain_ms']:.1f} ms, batched "
+              f"{mg['batched_ms']:.1f} ms ({mg['speedup']:.2f}x); BiCGStab iterations {mg['iterations_min']}.."
+              f"{mg['iterations_max']} (median {mg['iterations_median']:.0f}); first iteration (hierarchy build) "
+              f"plain {mg['first_iteration_ms']['plain']:.0f} ms, batched {mg['first_iteration_ms']['batched']:.0f} "
+              f"ms; identical to plain: {mg['identical_to_plain']}", file=sys.stderr, flush=True)
 
     line = {
         "metric": METRIC, "value": value, "unit": "element evals/s", "n_gpus": 1, "steps": args.steps,

@@ -541,13 +541,14 @@ AmgBatch::~AmgBatch() { release(); }
 void AmgBatch::release() {
   if (d_levels_) (void)hipFree(d_levels_);
   if (d_entries_) (void)hipFree(d_entries_);
+  if (d_maps_) (void)hipFree(d_maps_);
   d_levels_ = nullptr;
   d_entries_ = nullptr;
+  d_maps_ = nullptr;
   problems_.clear();
   host_levels_.clear();
   entries_.clear();
   tables_ = AmgTables();
-  sym_first_ = 0;
 }
 
 void AmgBatch::add(std::unique_ptr<AmgHierarchy> h, long long row_offset, long long value_offset) {
@@ -556,7 +557,7 @@ void AmgBatch::add(std::unique_ptr<AmgHierarchy> h, long long row_offset, long l
 
 int AmgBatch::finalize() {
   if (problems_.empty()) return 1;
-  // every problem's levels, one array
+  // every problem's levels, one array (the entries' next levels)
   host_levels_.clear();
   int depth = 0;
   for (auto& p : problems_) {
@@ -565,89 +566,85 @@ int AmgBatch::finalize() {
     depth = std::max(depth, static_cast<int>(p.h->lv_.size()));
   }
   if (hipMalloc(&d_levels_, host_levels_.size() * sizeof(AmgLevel)) != hipSuccess) return 1;
-  // the tables, each a run of entries ended by the block count
+  // the tables, each a run of entries ended by one that holds the block
+  // count (problem -1), and a map from each of its blocks to its entry
   entries_.clear();
+  std::vector<int> maps;
   struct Pending {
     AmgTable* table;
-    std::size_t start;
+    std::size_t entries, map;
   };
   std::vector<Pending> pending;
-  auto table = [&](AmgTable& t, int level, AmgTableKind kind, bool coarsest) {
-    const std::size_t start = entries_.size();
+  auto table = [&](AmgTable& t, int level, AmgTableKind kind, bool coarsest, bool all) {
+    const std::size_t start = entries_.size(), map0 = maps.size();
     long long blocks = 0;
     for (int p = 0; p < static_cast<int>(problems_.size()); ++p) {
       const Problem& q = problems_[p];
       const int count = static_cast<int>(q.h->lv_.size());
-      if (level >= count || (level == count - 1) != coarsest) continue;
+      if (!all && (level >= count || (level == count - 1) != coarsest)) continue;
       AmgGroupEntry e;
-      e.L = d_levels_ + q.base + level;
+      e.L = q.h->lv_[level];
       e.N = level + 1 < count ? d_levels_ + q.base + level + 1 : nullptr;
       e.offset = level == 0 ? q.row_offset : -1;
       e.first = blocks;
       e.problem = p;
-      blocks += amg_table_blocks(kind, q.h->lv_[level], q.h->nnz0_);
+      e.level = level;
+      if (kind == kAmgTableSymmetric) {
+        e.mirror = q.h->transpose_;
+        e.dst = q.h->symmetric_;
+        e.count = q.h->nnz0_;
+      }
+      const long long nb = amg_table_blocks(kind, q.h->lv_[level], q.h->nnz0_);
+      maps.insert(maps.end(), static_cast<std::size_t>(nb), static_cast<int>(entries_.size() - start));
+      blocks += nb;
       entries_.push_back(e);
     }
     AmgGroupEntry end;
     end.first = blocks;
+    end.problem = -1;
     entries_.push_back(end);
     t.count = static_cast<int>(entries_.size() - start) - 1;
     t.blocks = t.count > 0 ? blocks : 0;
-    pending.push_back(Pending{&t, start});
+    pending.push_back(Pending{&t, start, map0});
   };
   tables_.levels.assign(static_cast<std::size_t>(depth), AmgLevelTables());
   for (int l = 0; l < depth; ++l) {
     AmgLevelTables& t = tables_.levels[l];
-    table(t.elem, l, kAmgTableElem, false);
-    table(t.rows, l, kAmgTableRows, false);
-    table(t.restrict_, l, kAmgTableRestrict, false);
-    table(t.power, l, kAmgTablePower, false);
-    table(t.one, l, kAmgTableOne, false);
-    table(t.product, l, kAmgTableProduct, false);
-    table(t.coarse_product, l, kAmgTableCoarseProduct, false);
-    table(t.gather, l, kAmgTableGather, false);
-    table(t.dense_one, l, kAmgTableOne, true);
-    table(t.dense, l, kAmgTableElem, true);
+    table(t.elem, l, kAmgTableElem, false, false);
+    table(t.rows, l, kAmgTableRows, false, false);
+    table(t.restrict_, l, kAmgTableRestrict, false, false);
+    table(t.power, l, kAmgTablePower, false, false);
+    table(t.one, l, kAmgTableOne, false, false);
+    table(t.product, l, kAmgTableProduct, false, false);
+    table(t.coarse_product, l, kAmgTableCoarseProduct, false, false);
+    table(t.gather, l, kAmgTableGather, false, false);
+    table(t.dense_one, l, kAmgTableOne, true, false);
+    table(t.dense, l, kAmgTableElem, true, false);
   }
-  // level 0 of every problem, whatever its depth
-  {
-    const std::size_t start = entries_.size();
-    long long blocks = 0;
-    for (int p = 0; p < static_cast<int>(problems_.size()); ++p) {
-      const Problem& q = problems_[p];
-      AmgGroupEntry e;
-      e.L = d_levels_ + q.base;
-      e.offset = q.row_offset;
-      e.first = blocks;
-      e.problem = p;
-      e.mirror = q.h->transpose_;
-      e.dst = q.h->symmetric_;
-      e.count = q.h->nnz0_;
-      blocks += amg_table_blocks(kAmgTableSymmetric, q.h->lv_[0], q.h->nnz0_);
-      entries_.push_back(e);
-    }
-    AmgGroupEntry end;
-    end.first = blocks;
-    entries_.push_back(end);
-    tables_.symmetric.count = static_cast<int>(problems_.size());
-    tables_.symmetric.blocks = blocks;
-    sym_first_ = start;
-    pending.push_back(Pending{&tables_.symmetric, start});
+  table(tables_.symmetric, 0, kAmgTableSymmetric, false, true);  // level 0 of every problem
+  if (hipMalloc(&d_entries_, entries_.size() * sizeof(AmgGroupEntry)) != hipSuccess ||
+      hipMalloc(&d_maps_, std::max<std::size_t>(maps.size(), 1) * sizeof(int)) != hipSuccess ||
+      (!maps.empty() && hipMemcpy(d_maps_, maps.data(), maps.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess))
+    return 1;
+  for (const Pending& q : pending) {
+    q.table->entries = d_entries_ + q.entries;
+    q.table->map = d_maps_ + q.map;
   }
-  if (hipMalloc(&d_entries_, entries_.size() * sizeof(AmgGroupEntry)) != hipSuccess) return 1;
-  for (const Pending& q : pending) q.table->entries = d_entries_ + q.start;
   return 0;
 }
 
 int AmgBatch::setup(const double* block_values, bool symmetrise, hipStream_t s) {
   if (problems_.empty() || !d_levels_ || !d_entries_) return 1;
   // level 0 reads the block's values at each problem's offset, or its
-  // symmetric part
-  for (std::size_t p = 0; p < problems_.size(); ++p) {
-    const Problem& q = problems_[p];
-    const double* mine = block_values + q.value_offset;
-    host_levels_[static_cast<std::size_t>(q.base)].values = symmetrise ? q.h->symmetric_ : mine;
-    entries_[sym_first_ + p].src = mine;
+  // symmetric part (which the symmetric-part table forms from them)
+  for (const Problem& q : problems_)
+    host_levels_[static_cast<std::size_t>(q.base)].values =
+        symmetrise ? q.h->symmetric_ : block_values + q.value_offset;
+  for (AmgGroupEntry& e : entries_) {
+    if (e.problem < 0 || e.level != 0) continue;
+    const Problem& q = problems_[static_cast<std::size_t>(e.problem)];
+    e.L.values = host_levels_[static_cast<std::size_t>(q.base)].values;
+    if (e.mirror) e.src = block_values + q.value_offset;
   }
   if (hipMemcpyAsync(d_levels_, host_levels_.data(), host_levels_.size() * sizeof(AmgLevel), hipMemcpyHostToDevice,
                      s) != hipSuccess ||

@@ -101,9 +101,9 @@ class AmgBatch {
   std::vector<Problem> problems_;
   std::vector<AmgLevel> host_levels_;
   std::vector<AmgGroupEntry> entries_;
-  std::size_t sym_first_ = 0;
   AmgLevel* d_levels_ = nullptr;
   AmgGroupEntry* d_entries_ = nullptr;
+  int* d_maps_ = nullptr;
   AmgTables tables_;
   const int* active_ = nullptr;
   void release();

@@ -22,8 +22,8 @@ namespace {
 // hierarchies (one per problem; a plain solve's set has one), each entry of
 // the group table taking its own range of the grid's blocks with exactly the
 // block decomposition a launch of its own would use. A workgroup finds its
-// entry by binary search and, when its problem rests (its active flag is 0),
-// does nothing.
+// entry in the table's block map and, when its problem rests (its active
+// flag is 0), does nothing.
 struct GroupBlock {
   const AmgGroupEntry* e;
   long long block;    // the block within the entry's range
@@ -32,24 +32,24 @@ struct GroupBlock {
 
 __device__ __forceinline__ bool group_block(const AmgGroup& G, GroupBlock& out) {
   const long long b = blockIdx.x;
-  int lo = 0, hi = G.count - 1;  // the last entry whose first block <= b
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (G.e[mid].first <= b) lo = mid; else hi = mid - 1;
-  }
-  out.e = G.e + lo;
-  out.block = b - G.e[lo].first;
-  out.nblocks = G.e[lo + 1].first - G.e[lo].first;
-  return !(G.active && G.active[G.e[lo].problem] == 0);
+  const int j = G.map[b];
+  out.e = G.e + j;
+  out.block = b - G.e[j].first;
+  out.nblocks = G.e[j + 1].first - G.e[j].first;
+  return !(G.active && G.active[G.e[j].problem] == 0);
 }
 
+// Kernels copy their entry's level descriptor (const AmgLevel L = e.L) so
+// that stores through its pointers cannot alias it: the compiler keeps its
+// fields in registers instead of reloading them after every store.
+//
 // The V-cycle's vectors of an entry's level: the caller's (at the problem's
 // offset) on level 0, the level's own above it.
 __device__ __forceinline__ const double* level_b(const AmgGroupEntry& e, const double* b) {
-  return e.offset >= 0 ? b + e.offset : e.L->b;
+  return e.offset >= 0 ? b + e.offset : e.L.b;
 }
 __device__ __forceinline__ double* level_x(const AmgGroupEntry& e, double* x) {
-  return e.offset >= 0 ? x + e.offset : e.L->x;
+  return e.offset >= 0 ? x + e.offset : e.L.x;
 }
 
 constexpr int kWaves = 4;
@@ -61,31 +61,54 @@ constexpr int kCap = 512;  // columns of C a product workgroup holds at a time
 unsigned int grid_of(long long n, long long per) { return static_cast<unsigned int>((n + per - 1) / per); }
 
 // sum over the row of values[k] x[index[k]] (a row >= n sums to 0), in every
-// lane of the row's 32-lane group.
+// lane of the row's 32-lane group: lane l adds entries l, l + 32, ... in
+// order, then a butterfly over the group. The lane's first kSlots entries
+// have every index and value load issued before their gathers of x, and
+// those before the sums (a row of level 0 holds at most 81 entries); longer
+// rows finish in the loop after, in the same order.
+constexpr int kSlots = 3;
 __device__ __forceinline__ double row_dot(const int* __restrict__ ptr, const int* __restrict__ index,
                                           const double* __restrict__ values, const double* __restrict__ x,
                                           long long row, long long n, int l) {
   double acc = 0.0;
   if (row < n) {
-    const int k1 = ptr[row + 1];
-    for (int k = ptr[row] + l; k < k1; k += kLanesPerRow) acc = fma(values[k], x[index[k]], acc);
+    const int k0 = ptr[row] + l, k1 = ptr[row + 1];
+    int col[kSlots];
+    double v[kSlots], xv[kSlots];
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+      const int k = k0 + s * kLanesPerRow;
+      col[s] = k < k1 ? index[k] : -1;
+      v[s] = k < k1 ? values[k] : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) xv[s] = col[s] >= 0 ? x[col[s]] : 0.0;
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s)
+      if (col[s] >= 0) acc = fma(v[s], xv[s], acc);
+    for (int k = k0 + kSlots * kLanesPerRow; k < k1; k += kLanesPerRow) acc = fma(values[k], x[index[k]], acc);
   }
 #pragma unroll
   for (int off = kLanesPerRow / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
   return acc;
 }
 
-// The row of the calling lane in a row-wise launch (ceil(n / 8) blocks of
-// 256) and whether the lane writes it.
-struct RowLane {
-  long long row;
-  int l;
+// Row-wise launches deal the rows to an entry's blocks round-robin, kGroups
+// rows per wavefront at a time (a block's prologue, its entry's lookup, is
+// paid once for many rows); a row's result does not depend on the dealing.
+// Row r0 of the calling lane's first turn, the stride between its turns, and
+// the lane within its row's group.
+struct RowLoop {
+  long long first, stride;
+  int g, l;
 };
 
-__device__ __forceinline__ RowLane row_lane(long long block) {
+__device__ __forceinline__ RowLoop row_loop(const GroupBlock& gb) {
   const int lane = threadIdx.x & 63;
-  RowLane r;
-  r.row = (block * kWaves + (threadIdx.x >> 6)) * kGroups + lane / kLanesPerRow;
+  RowLoop r;
+  r.first = (gb.block * kWaves + (threadIdx.x >> 6)) * kGroups;
+  r.stride = gb.nblocks * kRowsPerBlock;
+  r.g = lane / kLanesPerRow;
   r.l = lane % kLanesPerRow;
   return r;
 }
@@ -93,26 +116,26 @@ __device__ __forceinline__ RowLane row_lane(long long block) {
 __global__ __launch_bounds__(256) void diag_kernel(AmgGroup G) {
   GroupBlock gb;
   if (!group_block(G, gb)) return;
-  const AmgLevel& L = *gb.e->L;
-  const long long i = gb.block * 256 + threadIdx.x;
-  if (i >= L.n) return;
-  double d = 0.0;
-  for (int k = L.ptr[i]; k < L.ptr[i + 1]; ++k)
-    if (L.index[k] == i) d = L.values[k];
-  L.dinv[i] = (d != 0.0 && std::isfinite(d)) ? 1.0 / d : 0.0;
+  const AmgLevel L = gb.e->L;
+  for (long long i = gb.block * 256 + threadIdx.x; i < L.n; i += gb.nblocks * 256) {
+    double d = 0.0;
+    for (int k = L.ptr[i]; k < L.ptr[i + 1]; ++k)
+      if (L.index[k] == i) d = L.values[k];
+    L.dinv[i] = (d != 0.0 && std::isfinite(d)) ? 1.0 / d : 0.0;
+  }
 }
 
 // A fixed start vector in [-1, 1) from a hash of the row (into d0).
 __global__ __launch_bounds__(256) void power_init_kernel(AmgGroup G) {
   GroupBlock gb;
   if (!group_block(G, gb)) return;
-  const AmgLevel& L = *gb.e->L;
-  const long long i = gb.block * 256 + threadIdx.x;
-  if (i == 0) L.coef[kAmgScale] = 1.0;
-  if (i >= L.n) return;
-  unsigned long long h = static_cast<unsigned long long>(i) * 0x9E3779B97F4A7C15ull;
-  h ^= h >> 29;
-  L.d0[i] = static_cast<double>(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  const AmgLevel L = gb.e->L;
+  if (gb.block == 0 && threadIdx.x == 0) L.coef[kAmgScale] = 1.0;
+  for (long long i = gb.block * 256 + threadIdx.x; i < L.n; i += gb.nblocks * 256) {
+    unsigned long long h = static_cast<unsigned long long>(i) * 0x9E3779B97F4A7C15ull;
+    h ^= h >> 29;
+    L.d0[i] = static_cast<double>(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  }
 }
 
 // y = D^-1 A (x / ||x||) (x = d0, y = d1 on even steps, the other way round
@@ -121,7 +144,7 @@ __global__ __launch_bounds__(256) void power_init_kernel(AmgGroup G) {
 __global__ __launch_bounds__(256) void power_step_kernel(AmgGroup G, int odd) {
   GroupBlock gb;
   if (!group_block(G, gb)) return;
-  const AmgLevel& L = *gb.e->L;
+  const AmgLevel L = gb.e->L;
   const double* __restrict__ x = odd ? L.d1 : L.d0;
   double* __restrict__ y = odd ? L.d0 : L.d1;
   __shared__ double ws[kRowsPerBlock];
@@ -153,7 +176,7 @@ __global__ __launch_bounds__(256) void power_step_kernel(AmgGroup G, int odd) {
 __global__ __launch_bounds__(256) void power_finish_kernel(AmgGroup G) {
   GroupBlock gb;
   if (!group_block(G, gb)) return;
-  const AmgLevel& L = *gb.e->L;
+  const AmgLevel L = gb.e->L;
   __shared__ double s[256];
   double acc = 0.0;
   for (int i = threadIdx.x; i < L.nparts; i += 256) acc += L.partial[i];
@@ -176,7 +199,7 @@ __global__ __launch_bounds__(256) void power_finish_kernel(AmgGroup G) {
 __global__ void coef_kernel(AmgGroup G) {
   GroupBlock gb;
   if (!group_block(G, gb)) return;
-  const AmgLevel& L = *gb.e->L;
+  const AmgLevel L = gb.e->L;
   if (threadIdx.x != 0) return;
   const double rho = L.coef[kAmgRho];
   double* c = L.coef;
@@ -229,7 +252,7 @@ enum ProductKind : int { kProductP = 0, kProductAP = 1, kProductPtAP = 2 };
 __global__ __launch_bounds__(64) void product_kernel(AmgGroup G, int kind) {
   GroupBlock gb;
   if (!group_block(G, gb)) return;
-  const AmgLevel& L = *gb.e->L;
+  const AmgLevel L = gb.e->L;
   const Csr A{L.ptr, L.index, L.values};
   Csr X = A, Y{L.tp_ptr, L.tp_index, L.tp_values};
   const int* __restrict__ cptr = L.p_ptr;
@@ -242,7 +265,7 @@ __global__ __launch_bounds__(64) void product_kernel(AmgGroup G, int kind) {
     cindex = L.ap_index;
     cvalues = L.ap_values;
   } else if (kind == kProductPtAP) {
-    const AmgLevel& N = *gb.e->N;
+    const AmgLevel N = *gb.e->N;
     X = Csr{L.pt_ptr, L.pt_index, L.pt_values};
     Y = Csr{L.ap_ptr, L.ap_index, L.ap_values};
     cptr = N.ptr;
@@ -326,9 +349,9 @@ __global__ __launch_bounds__(64) void product_kernel(AmgGroup G, int kind) {
 __global__ __launch_bounds__(256) void gather_kernel(AmgGroup G) {
   GroupBlock gb;
   if (!group_block(G, gb)) return;
-  const AmgLevel& L = *gb.e->L;
-  const long long i = gb.block * 256 + threadIdx.x;
-  if (i < L.pt_nnz) L.pt_values[i] = L.p_values[L.pt_perm[i]];
+  const AmgLevel L = gb.e->L;
+  for (long long i = gb.block * 256 + threadIdx.x; i < L.pt_nnz; i += gb.nblocks * 256)
+    L.pt_values[i] = L.p_values[L.pt_perm[i]];
 }
 
 // [A | I] -> [I | A^-1] by Gauss-Jordan with partial pivoting (the largest
@@ -336,7 +359,7 @@ __global__ __launch_bounds__(256) void gather_kernel(AmgGroup G) {
 __global__ __launch_bounds__(1024) void dense_inverse_kernel(AmgGroup G) {
   GroupBlock gb;
   if (!group_block(G, gb)) return;
-  const AmgLevel& L = *gb.e->L;
+  const AmgLevel L = gb.e->L;
   __shared__ double best[16];
   __shared__ int besti[16];
   __shared__ double factor[kAmgDenseMax];
@@ -418,11 +441,11 @@ __global__ __launch_bounds__(256) void symmetric_part_kernel(AmgGroup G) {
   GroupBlock gb;
   if (!group_block(G, gb)) return;
   const AmgGroupEntry& e = *gb.e;
-  const long long k = gb.block * 256 + threadIdx.x;
-  if (k >= e.count) return;
   const double* __restrict__ v = e.src;
-  const int m = e.mirror[k];
-  e.dst[k] = m >= 0 ? 0.5 * (v[k] + v[m]) : 0.5 * v[k];
+  for (long long k = gb.block * 256 + threadIdx.x; k < e.count; k += gb.nblocks * 256) {
+    const int m = e.mirror[k];
+    e.dst[k] = m >= 0 ? 0.5 * (v[k] + v[m]) : 0.5 * v[k];
+  }
 }
 
 // --- the V-cycle ------------------------------------------------------------
@@ -431,29 +454,33 @@ __global__ __launch_bounds__(256) void symmetric_part_kernel(AmgGroup G) {
 __global__ __launch_bounds__(256) void pre_first_kernel(AmgGroup G, const double* __restrict__ b0) {
   GroupBlock gb;
   if (!group_block(G, gb)) return;
-  const AmgLevel& L = *gb.e->L;
+  const AmgLevel L = gb.e->L;
   const double* __restrict__ b = level_b(*gb.e, b0);
-  const long long i = gb.block * 256 + threadIdx.x;
-  if (i >= L.n) return;
-  const double ri = L.dinv[i] * b[i];
-  L.r[i] = ri;
-  L.d0[i] = L.coef[kAmgThetaInv] * ri;
+  const double theta_inv = L.coef[kAmgThetaInv];
+  for (long long i = gb.block * 256 + threadIdx.x; i < L.n; i += gb.nblocks * 256) {
+    const double ri = L.dinv[i] * b[i];
+    L.r[i] = ri;
+    L.d0[i] = theta_inv * ri;
+  }
 }
 
 // Second half: r -= D^-1 A d0, d1 = c1 d0 + c2 r, x = d0 + d1.
 __global__ __launch_bounds__(256) void pre_step_kernel(AmgGroup G, double* __restrict__ x0) {
   GroupBlock gb;
   if (!group_block(G, gb)) return;
-  const AmgLevel& L = *gb.e->L;
+  const AmgLevel L = gb.e->L;
   double* __restrict__ x = level_x(*gb.e, x0);
-  const RowLane q = row_lane(gb.block);
-  const double v = row_dot(L.ptr, L.index, L.values, L.d0, q.row, L.n, q.l);
-  if (q.l != 0 || q.row >= L.n) return;
-  const long long i = q.row;
-  const double ri = fma(-L.dinv[i], v, L.r[i]);
-  const double d1 = fma(L.coef[kAmgC1], L.d0[i], L.coef[kAmgC2] * ri);
-  L.r[i] = ri;
-  x[i] = L.d0[i] + d1;
+  const double c1 = L.coef[kAmgC1], c2 = L.coef[kAmgC2];
+  const RowLoop q = row_loop(gb);
+  for (long long base = q.first; base < L.n; base += q.stride) {
+    const long long i = base + q.g;
+    const double v = row_dot(L.ptr, L.index, L.values, L.d0, i, L.n, q.l);
+    if (q.l != 0 || i >= L.n) continue;
+    const double ri = fma(-L.dinv[i], v, L.r[i]);
+    const double d1 = fma(c1, L.d0[i], c2 * ri);
+    L.r[i] = ri;
+    x[i] = L.d0[i] + d1;
+  }
 }
 
 // r = b - A x.
@@ -461,35 +488,44 @@ __global__ __launch_bounds__(256) void residual_kernel(AmgGroup G, const double*
                                                        const double* __restrict__ x0) {
   GroupBlock gb;
   if (!group_block(G, gb)) return;
-  const AmgLevel& L = *gb.e->L;
+  const AmgLevel L = gb.e->L;
   const double* __restrict__ b = level_b(*gb.e, b0);
   const double* __restrict__ x = level_x(*gb.e, const_cast<double*>(x0));
-  const RowLane q = row_lane(gb.block);
-  const double v = row_dot(L.ptr, L.index, L.values, x, q.row, L.n, q.l);
-  if (q.l == 0 && q.row < L.n) L.r[q.row] = b[q.row] - v;
+  const RowLoop q = row_loop(gb);
+  for (long long base = q.first; base < L.n; base += q.stride) {
+    const long long i = base + q.g;
+    const double v = row_dot(L.ptr, L.index, L.values, x, i, L.n, q.l);
+    if (q.l == 0 && i < L.n) L.r[i] = b[i] - v;
+  }
 }
 
 // The next level's b = P^T r.
 __global__ __launch_bounds__(256) void restrict_kernel(AmgGroup G) {
   GroupBlock gb;
   if (!group_block(G, gb)) return;
-  const AmgLevel& L = *gb.e->L;
+  const AmgLevel L = gb.e->L;
   double* __restrict__ nb = gb.e->N->b;
-  const RowLane q = row_lane(gb.block);
-  const double v = row_dot(L.pt_ptr, L.pt_index, L.pt_values, L.r, q.row, L.nc, q.l);
-  if (q.l == 0 && q.row < L.nc) nb[q.row] = v;
+  const RowLoop q = row_loop(gb);
+  for (long long base = q.first; base < L.nc; base += q.stride) {
+    const long long i = base + q.g;
+    const double v = row_dot(L.pt_ptr, L.pt_index, L.pt_values, L.r, i, L.nc, q.l);
+    if (q.l == 0 && i < L.nc) nb[i] = v;
+  }
 }
 
 // x += P x_next.
 __global__ __launch_bounds__(256) void prolong_kernel(AmgGroup G, double* __restrict__ x0) {
   GroupBlock gb;
   if (!group_block(G, gb)) return;
-  const AmgLevel& L = *gb.e->L;
+  const AmgLevel L = gb.e->L;
   double* __restrict__ x = level_x(*gb.e, x0);
   const double* __restrict__ nx = gb.e->N->x;
-  const RowLane q = row_lane(gb.block);
-  const double v = row_dot(L.p_ptr, L.p_index, L.p_values, nx, q.row, L.n, q.l);
-  if (q.l == 0 && q.row < L.n) x[q.row] += v;
+  const RowLoop q = row_loop(gb);
+  for (long long base = q.first; base < L.n; base += q.stride) {
+    const long long i = base + q.g;
+    const double v = row_dot(L.p_ptr, L.p_index, L.p_values, nx, i, L.n, q.l);
+    if (q.l == 0 && i < L.n) x[i] += v;
+  }
 }
 
 // Post-smoothing from x, first half: r = D^-1 (b - A x), d0 = r / theta.
@@ -497,31 +533,38 @@ __global__ __launch_bounds__(256) void post_first_kernel(AmgGroup G, const doubl
                                                          const double* __restrict__ x0) {
   GroupBlock gb;
   if (!group_block(G, gb)) return;
-  const AmgLevel& L = *gb.e->L;
+  const AmgLevel L = gb.e->L;
   const double* __restrict__ b = level_b(*gb.e, b0);
   const double* __restrict__ x = level_x(*gb.e, const_cast<double*>(x0));
-  const RowLane q = row_lane(gb.block);
-  const double v = row_dot(L.ptr, L.index, L.values, x, q.row, L.n, q.l);
-  if (q.l != 0 || q.row >= L.n) return;
-  const double ri = L.dinv[q.row] * (b[q.row] - v);
-  L.r[q.row] = ri;
-  L.d0[q.row] = L.coef[kAmgThetaInv] * ri;
+  const double theta_inv = L.coef[kAmgThetaInv];
+  const RowLoop q = row_loop(gb);
+  for (long long base = q.first; base < L.n; base += q.stride) {
+    const long long i = base + q.g;
+    const double v = row_dot(L.ptr, L.index, L.values, x, i, L.n, q.l);
+    if (q.l != 0 || i >= L.n) continue;
+    const double ri = L.dinv[i] * (b[i] - v);
+    L.r[i] = ri;
+    L.d0[i] = theta_inv * ri;
+  }
 }
 
 // Second half: r -= D^-1 A d0, d1 = c1 d0 + c2 r, x += d0 + d1.
 __global__ __launch_bounds__(256) void post_step_kernel(AmgGroup G, double* __restrict__ x0) {
   GroupBlock gb;
   if (!group_block(G, gb)) return;
-  const AmgLevel& L = *gb.e->L;
+  const AmgLevel L = gb.e->L;
   double* __restrict__ x = level_x(*gb.e, x0);
-  const RowLane q = row_lane(gb.block);
-  const double v = row_dot(L.ptr, L.index, L.values, L.d0, q.row, L.n, q.l);
-  if (q.l != 0 || q.row >= L.n) return;
-  const long long i = q.row;
-  const double ri = fma(-L.dinv[i], v, L.r[i]);
-  const double d1 = fma(L.coef[kAmgC1], L.d0[i], L.coef[kAmgC2] * ri);
-  L.r[i] = ri;
-  x[i] += L.d0[i] + d1;
+  const double c1 = L.coef[kAmgC1], c2 = L.coef[kAmgC2];
+  const RowLoop q = row_loop(gb);
+  for (long long base = q.first; base < L.n; base += q.stride) {
+    const long long i = base + q.g;
+    const double v = row_dot(L.ptr, L.index, L.values, L.d0, i, L.n, q.l);
+    if (q.l != 0 || i >= L.n) continue;
+    const double ri = fma(-L.dinv[i], v, L.r[i]);
+    const double d1 = fma(c1, L.d0[i], c2 * ri);
+    L.r[i] = ri;
+    x[i] += L.d0[i] + d1;
+  }
 }
 
 // The coarsest level: x = A^-1 b from the dense inverse (each row's sum in
@@ -530,7 +573,7 @@ __global__ __launch_bounds__(256) void dense_apply_kernel(AmgGroup G, const doub
                                                           double* __restrict__ x0) {
   GroupBlock gb;
   if (!group_block(G, gb)) return;
-  const AmgLevel& L = *gb.e->L;
+  const AmgLevel L = gb.e->L;
   const double* __restrict__ b = level_b(*gb.e, b0);
   double* __restrict__ x = level_x(*gb.e, x0);
   const int n = static_cast<int>(L.n);
@@ -546,7 +589,7 @@ __global__ __launch_bounds__(256) void dense_apply_kernel(AmgGroup G, const doub
 int launch(void (*kernel)(AmgGroup), const AmgTable& t, const int* active, int threads, hipStream_t s) {
   if (t.blocks > 0)
     hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned int>(t.blocks)), dim3(threads), 0, s,
-                       AmgGroup{t.entries, t.count, active});
+                       AmgGroup{t.entries, t.map, active});
   return 0;
 }
 
@@ -557,17 +600,30 @@ int amg_product_blocks(long long n) {
   return static_cast<int>(b < 1 ? 1 : (b > 2048 ? 2048 : b));
 }
 
+// An entry's blocks: one turn per thread (elementwise) or per row group
+// (row-wise) up to 2048 blocks, so that a small level's launch is one short
+// turn (its latency), a large one's blocks take many turns each (their
+// prologue paid once).
+long long elem_blocks(long long n) {
+  const long long b = grid_of(n, 256);
+  return b < 1 ? 1 : (b > 2048 ? 2048 : b);
+}
+long long row_blocks(long long n) {
+  const long long b = grid_of(n, kRowsPerBlock);
+  return b < 1 ? 1 : (b > 2048 ? 2048 : b);
+}
+
 long long amg_table_blocks(AmgTableKind kind, const AmgLevel& L, long long nnz0) {
   switch (kind) {
-    case kAmgTableElem: return grid_of(L.n, 256);
-    case kAmgTableRows: return grid_of(L.n, kRowsPerBlock);
-    case kAmgTableRestrict: return grid_of(L.nc, kRowsPerBlock);
+    case kAmgTableElem: return elem_blocks(L.n);
+    case kAmgTableRows: return row_blocks(L.n);
+    case kAmgTableRestrict: return row_blocks(L.nc);
     case kAmgTablePower: return L.nparts;
     case kAmgTableOne: return 1;
     case kAmgTableProduct: return L.n;
     case kAmgTableCoarseProduct: return L.nc;
-    case kAmgTableGather: return grid_of(L.pt_nnz, 256);
-    case kAmgTableSymmetric: return grid_of(nnz0, 256);
+    case kAmgTableGather: return elem_blocks(L.pt_nnz);
+    case kAmgTableSymmetric: return elem_blocks(nnz0);
     default: return 0;
   }
 }
@@ -582,12 +638,12 @@ int launch_amg_setup(const AmgTables& T, bool symmetrise, hipStream_t s) {
     if (t.power.blocks > 0)
       for (int k = 0; k < kAmgPowerSteps; ++k) {
         hipLaunchKernelGGL(power_step_kernel, dim3(static_cast<unsigned int>(t.power.blocks)), dim3(256), 0, s,
-                           AmgGroup{t.power.entries, t.power.count, nullptr}, k & 1);
+                           AmgGroup{t.power.entries, t.power.map, nullptr}, k & 1);
         launch(power_finish_kernel, t.one, nullptr, 256, s);
       }
     launch(coef_kernel, t.one, nullptr, 64, s);
     if (t.product.blocks > 0) {
-      const AmgGroup g{t.product.entries, t.product.count, nullptr};
+      const AmgGroup g{t.product.entries, t.product.map, nullptr};
       hipLaunchKernelGGL(product_kernel, dim3(static_cast<unsigned int>(t.product.blocks)), dim3(64), 0, s, g,
                          static_cast<int>(kProductP));
       hipLaunchKernelGGL(product_kernel, dim3(static_cast<unsigned int>(t.product.blocks)), dim3(64), 0, s, g,
@@ -596,7 +652,7 @@ int launch_amg_setup(const AmgTables& T, bool symmetrise, hipStream_t s) {
     launch(gather_kernel, t.gather, nullptr, 256, s);
     if (t.coarse_product.blocks > 0)
       hipLaunchKernelGGL(product_kernel, dim3(static_cast<unsigned int>(t.coarse_product.blocks)), dim3(64), 0, s,
-                         AmgGroup{t.coarse_product.entries, t.coarse_product.count, nullptr},
+                         AmgGroup{t.coarse_product.entries, t.coarse_product.map, nullptr},
                          static_cast<int>(kProductPtAP));
     launch(dense_inverse_kernel, t.dense_one, nullptr, 1024, s);
   }
@@ -607,7 +663,7 @@ int launch_amg_vcycle(const AmgTables& T, const int* active, const double* b, do
   (void)hipGetLastError();
   const int count = static_cast<int>(T.levels.size());
   auto grid = [](const AmgTable& t) { return dim3(static_cast<unsigned int>(t.blocks)); };
-  auto group = [active](const AmgTable& t) { return AmgGroup{t.entries, t.count, active}; };
+  auto group = [active](const AmgTable& t) { return AmgGroup{t.entries, t.map, active}; };
   const double* cb = b;
   // down: pre-smoothing, residual and restriction where a level is not the
   // problem's coarsest, the dense solve where it is

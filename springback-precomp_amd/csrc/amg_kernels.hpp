@@ -79,12 +79,12 @@ struct AmgLevel {
 // alone would use; a table ends with an entry whose first is the launch's
 // block count.
 struct AmgGroupEntry {
-  const AmgLevel* L = nullptr;  // the problem's level (device)
+  AmgLevel L;                   // the problem's level (a copy)
   const AmgLevel* N = nullptr;  // its next level (device; null at its coarsest)
   long long offset = -1;        // level 0: its rows' offset in the caller's vectors; -1 above
   long long first = 0;          // its first block
   int problem = 0;              // its segment (the active flags)
-  int pad = 0;
+  int level = 0;
   // the symmetric-part table only: (A + A^T) / 2 of count block values
   const double* src = nullptr;
   const int* mirror = nullptr;
@@ -94,28 +94,29 @@ struct AmgGroupEntry {
 
 struct AmgGroup {
   const AmgGroupEntry* e;  // count + 1 entries
-  int count;
+  const int* map;          // per block of the launch: its entry
   const int* active;       // per problem: 0 rests (its blocks do nothing); null: all work
 };
 
 // A group table on the device.
 struct AmgTable {
   const AmgGroupEntry* entries = nullptr;
+  const int* map = nullptr;
   int count = 0;
   long long blocks = 0;
 };
 
 // The blocks an entry of each kind of table takes for its level.
 enum AmgTableKind : int {
-  kAmgTableElem = 0,        // a thread per row (256 per block)
-  kAmgTableRows,            // 32 lanes per row (8 rows per block)
-  kAmgTableRestrict,        // 32 lanes per coarse row
+  kAmgTableElem = 0,        // a thread per row (at most 2048 blocks of 256, the rows dealt round-robin)
+  kAmgTableRows,            // 32 lanes per row (at most 2048 blocks of 8 rows, dealt round-robin)
+  kAmgTableRestrict,        // the same over the coarse rows
   kAmgTablePower,           // the level's nparts blocks
   kAmgTableOne,             // one block
   kAmgTableProduct,         // a block per row
   kAmgTableCoarseProduct,   // a block per coarse row
-  kAmgTableGather,          // a thread per entry of P^T
-  kAmgTableSymmetric        // a thread per entry of level 0
+  kAmgTableGather,          // a thread per entry of P^T, as kAmgTableElem
+  kAmgTableSymmetric        // a thread per entry of level 0, as kAmgTableElem
 };
 long long amg_table_blocks(AmgTableKind kind, const AmgLevel& L, long long nnz0);
 
