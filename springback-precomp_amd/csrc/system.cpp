@@ -12,11 +12,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "adjoint_kernels.hpp"
@@ -600,25 +602,53 @@ int prepare_segment_multigrid(sparlab_system* s, const std::vector<long long>& r
                                           hipMemcpyDeviceToHost, st) != hipSuccess) ||
         hipStreamSynchronize(st) != hipSuccess)
       return sbk::capi_fail(SPARLAB_E_DEVICE, "reading the partition failed");
-    auto* b = new sbk::AmgBatch();
+    // the segments' hierarchies are independent: built by a pool of host
+    // threads, each with a stream of its own on the caller's device
     const int nseg = static_cast<int>(rows.size()) - 1;
+    std::vector<std::unique_ptr<sbk::AmgHierarchy>> built(static_cast<std::size_t>(nseg));
+    std::vector<std::string> why(static_cast<std::size_t>(nseg));
+    std::atomic<int> next{0};
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return sbk::capi_fail(SPARLAB_E_DEVICE, "no current device");
+    auto work = [&]() {
+      hipStream_t ws = nullptr;
+      const bool ok = hipSetDevice(device) == hipSuccess &&
+                      hipStreamCreateWithFlags(&ws, hipStreamNonBlocking) == hipSuccess;
+      for (int p; (p = next++) < nseg;) {
+        if (!ok) {
+          why[static_cast<std::size_t>(p)] = "no stream for the build";
+          continue;
+        }
+        const long long r0 = rows[p], r1 = rows[p + 1];
+        const int v0 = ptr[static_cast<std::size_t>(r0)], v1 = ptr[static_cast<std::size_t>(r1)];
+        // the segment's rows alone: pointers from 0, columns from r0 (the
+        // segment check has shown that none leaves the segment)
+        std::vector<int> sp(static_cast<std::size_t>(r1 - r0) + 1), si(static_cast<std::size_t>(v1 - v0));
+        for (long long i = r0; i <= r1; ++i) sp[static_cast<std::size_t>(i - r0)] = ptr[static_cast<std::size_t>(i)] - v0;
+        for (int k = v0; k < v1; ++k)
+          si[static_cast<std::size_t>(k - v0)] = index[static_cast<std::size_t>(k)] - static_cast<int>(r0);
+        std::vector<long long> sf(free_dofs.begin() + r0, free_dofs.begin() + r1);
+        auto h = std::make_unique<sbk::AmgHierarchy>();
+        why[static_cast<std::size_t>(p)] = h->build_host(std::move(sp), std::move(si), sf, coords, ws);
+        built[static_cast<std::size_t>(p)] = std::move(h);
+      }
+      if (ws) (void)hipStreamDestroy(ws);
+    };
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nthreads = std::max(1, std::min({nseg, 16, hw > 0 ? static_cast<int>(hw) : 1}));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    auto* b = new sbk::AmgBatch();
     for (int p = 0; p < nseg; ++p) {
-      const long long r0 = rows[p], r1 = rows[p + 1];
-      const int v0 = ptr[static_cast<std::size_t>(r0)], v1 = ptr[static_cast<std::size_t>(r1)];
-      // the segment's rows alone: pointers from 0, columns from r0 (the
-      // segment check has shown that none leaves the segment)
-      std::vector<int> sp(static_cast<std::size_t>(r1 - r0) + 1), si(static_cast<std::size_t>(v1 - v0));
-      for (long long i = r0; i <= r1; ++i) sp[static_cast<std::size_t>(i - r0)] = ptr[static_cast<std::size_t>(i)] - v0;
-      for (int k = v0; k < v1; ++k) si[static_cast<std::size_t>(k - v0)] = index[static_cast<std::size_t>(k)] - static_cast<int>(r0);
-      std::vector<long long> sf(free_dofs.begin() + r0, free_dofs.begin() + r1);
-      auto h = std::make_unique<sbk::AmgHierarchy>();
-      const std::string why = h->build_host(std::move(sp), std::move(si), sf, coords, st);
-      if (!why.empty()) {
+      if (!why[static_cast<std::size_t>(p)].empty()) {
         delete b;
         return sbk::capi_fail(SPARLAB_E_SOLVER, "segment " + std::to_string(p) +
-                                                    ": the multigrid hierarchy could not be built: " + why);
+                                                    ": the multigrid hierarchy could not be built: " +
+                                                    why[static_cast<std::size_t>(p)]);
       }
-      b->add(std::move(h), r0, v0);
+      b->add(std::move(built[static_cast<std::size_t>(p)]), rows[p], ptr[static_cast<std::size_t>(rows[p])]);
     }
     if (b->finalize() != 0) {
       delete b;
