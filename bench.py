@@ -248,7 +248,7 @@ def rim_free_dofs(coords):
     return np.nonzero(np.repeat(~rim, 3))[0].astype(np.int32)
 
 
-def newton_leg(G, W, torch, dev, tolerance=1e-8, max_iterations=20000):
+def newton_leg(G, W, torch, dev, tolerance=1e-8, max_iterations=20000, precond="both", form="both"):
     """One Newton iteration of all 64 forming problems (the batched form's
     workload, rims clamped): evaluation (f_int + row-major K_t), the free-free
     block, r = f_int on the free DOFs, the block solve K_ff du = -r (BiCGStab:
@@ -260,7 +260,9 @@ def newton_leg(G, W, torch, dev, tolerance=1e-8, max_iterations=20000):
     with the diagonal preconditioner and with the multigrid V-cycle (each
     problem's own hierarchy; the batched form sets up and applies all of
     them in shared launches); the first iteration of each form, which builds
-    the hierarchies on the host once per partition, is timed apart."""
+    the hierarchies on the host once per partition, is timed apart. precond
+    and form restrict the leg to one preconditioner or one form (for a
+    profile of it alone)."""
     probs = W.batched_problems()
     m = W.microbench_material()
     rule = G.Rule(2, 0, 2)
@@ -268,7 +270,19 @@ def newton_leg(G, W, torch, dev, tolerance=1e-8, max_iterations=20000):
     frees = [rim_free_dofs(p["coords"]) for p in probs]
 
     def run(precond):
-        # plain
+        out = {}
+        plain_x = plain_its = None
+        if form in ("both", "plain"):
+            plain_x, plain_its = run_plain(precond, out)
+        if form in ("both", "batched"):
+            nfree = run_batched(precond, out, plain_x, plain_its)
+        else:
+            nfree = int(sum(f.size for f in frees))
+        if "plain_ms" in out and "batched_ms" in out:
+            out["speedup"] = out["plain_ms"] / out["batched_ms"]
+        return out, nfree
+
+    def run_plain(precond, out):
         plain = []
         for p, fr in zip(probs, frees):
             s = G.NonlinearSystem(p["coords"], p["conn"], m, rule)
@@ -303,15 +317,17 @@ def newton_leg(G, W, torch, dev, tolerance=1e-8, max_iterations=20000):
         t0 = time.perf_counter()
         plain_iteration()  # warm-up (with the multigrid hierarchies' build)
         torch.cuda.synchronize()
-        first_plain = (time.perf_counter() - t0) * 1e3
+        out.setdefault("first_iteration_ms", {})["plain"] = (time.perf_counter() - t0) * 1e3
         t0 = time.perf_counter()
         plain_its = plain_iteration()
         torch.cuda.synchronize()
-        ms_plain = (time.perf_counter() - t0) * 1e3
+        out["plain_ms"] = (time.perf_counter() - t0) * 1e3
         plain_x = [q["x"].cpu().numpy() for q in plain]
         del plain
         torch.cuda.empty_cache()
-        # batched
+        return plain_x, plain_its
+
+    def run_batched(precond, out, plain_x, plain_its):
         b = G.BatchedSystem([(p["coords"], p["conn"]) for p in probs], m, rule)
         b.system.set_block_preconditioner(precond)
         u0 = torch.as_tensor(np.concatenate([p["u"] for p in probs]).reshape(-1), device=dev)
@@ -343,30 +359,32 @@ def newton_leg(G, W, torch, dev, tolerance=1e-8, max_iterations=20000):
         t0 = time.perf_counter()
         batched_iteration()
         torch.cuda.synchronize()
-        first_batched = (time.perf_counter() - t0) * 1e3
+        out.setdefault("first_iteration_ms", {})["batched"] = (time.perf_counter() - t0) * 1e3
         t0 = time.perf_counter()
         its, status, sc = batched_iteration()
         torch.cuda.synchronize()
-        ms_batched = (time.perf_counter() - t0) * 1e3
-        xb = x.cpu().numpy()
-        identical = all(np.array_equal(xb[rows[p]:rows[p + 1]], plain_x[p]) for p in range(len(probs))) and \
-            [int(i) if st == 0 else -1 for i, st in zip(its, status)] == [int(i) for i in plain_its]
-        out = {"plain_ms": ms_plain, "batched_ms": ms_batched, "speedup": ms_plain / ms_batched,
-               "iterations_min": int(np.min(its)), "iterations_median": float(np.median(its)),
-               "iterations_max": int(np.max(its)), "failed_solves": int((status != 0).sum()),
-               "failed_elements": int(sc["failed"].sum()), "identical_to_plain": bool(identical),
-               "first_iteration_ms": {"plain": first_plain, "batched": first_batched}}
+        out["batched_ms"] = (time.perf_counter() - t0) * 1e3
+        out.update({"iterations_min": int(np.min(its)), "iterations_median": float(np.median(its)),
+                    "iterations_max": int(np.max(its)), "failed_solves": int((status != 0).sum()),
+                    "failed_elements": int(sc["failed"].sum())})
+        if plain_x is not None:
+            xb = x.cpu().numpy()
+            out["identical_to_plain"] = bool(
+                all(np.array_equal(xb[rows[p]:rows[p + 1]], plain_x[p]) for p in range(len(probs))) and
+                [int(i) if st == 0 else -1 for i, st in zip(its, status)] == [int(i) for i in plain_its])
         del b, values, blk
         torch.cuda.empty_cache()
-        return out, int(free.size)
+        return int(free.size)
 
-    diag, nfree = run("diagonal")
-    mg, _ = run("multigrid")
     out = {"workload": "one Newton iteration of the 64 forming problems (rims clamped): evaluation, free-free "
                        f"block, BiCGStab to {tolerance:g}, update",
-           "problems": len(probs), "free_dofs": nfree, "tolerance": tolerance}
-    out.update(diag)
-    out["multigrid"] = dict(mg, note="the same iteration with each problem's block solve preconditioned by its "
+           "problems": len(probs), "tolerance": tolerance}
+    if precond in ("both", "diagonal"):
+        diag, out["free_dofs"] = run("diagonal")
+        out.update(diag)
+    if precond in ("both", "multigrid"):
+        mg, out["free_dofs"] = run("multigrid")
+        out["multigrid"] = dict(mg, note="the same iteration with each problem's block solve preconditioned by its "
                                      "own multigrid V-cycle; first_iteration_ms includes the hierarchies' host "
                                      "build (once per partition)")
     return out
@@ -663,6 +681,10 @@ def main():
     ap.add_argument("--no-assembly", action="store_true", help="skip the assembly + contact leg")
     ap.add_argument("--no-box", action="store_true", help="skip the 2 x 2 x 5 box-rule leg")
     ap.add_argument("--no-newton", action="store_true", help="skip the batched Newton-iteration leg")
+    ap.add_argument("--newton-precond", choices=("both", "diagonal", "multigrid"), default="both",
+                    help="the Newton-iteration leg's preconditioners (profiling one alone)")
+    ap.add_argument("--newton-form", choices=("both", "plain", "batched"), default="both",
+                    help="the Newton-iteration leg's forms (profiling one alone; the comparison needs both)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain MIB-MiB device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -816,13 +838,15 @@ def main():
     newton = None
     if not args.no_newton:
         torch.cuda.empty_cache()
-        newton = newton_leg(G, W, torch, dev)
+        newton = newton_leg(G, W, torch, dev, precond=args.newton_precond, form=args.newton_form)
         torch.cuda.empty_cache()
+    if newton and "plain_ms" in newton and "batched_ms" in newton:
         print(f"# batched Newton iteration: {newton['problems']} problems, {newton['free_dofs']} free DOFs; plain "
               f"{newton['plain_ms']:.1f} ms, batched {newton['batched_ms']:.1f} ms ({newton['speedup']:.2f}x); "
               f"BiCGStab iterations {newton['iterations_min']}..{newton['iterations_max']} (median "
               f"{newton['iterations_median']:.0f}); identical to plain: {newton['identical_to_plain']}",
               file=sys.stderr, flush=True)
+    if newton and "multigrid" in newton and "speedup" in newton["multigrid"]:
         mg = newton["multigrid"]
         print(f"# batched Newton iteration, multigrid: plain {mg['plain_ms']:.1f} ms, batched "
               f"{mg['batched_ms']:.1f} ms ({mg['speedup']:.2f}x); BiCGStab iterations {mg['iterations_min']}.."
