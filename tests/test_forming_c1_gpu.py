@@ -6,10 +6,17 @@ Blank 20 x 20 x 1 mm, 20 x 20 x 2 incompatible-mode Hex8 (800 elements,
 1 323 nodes), edges clamped, E = 70 GPa, nu = 0.33, J2 with Voce hardening
 sigma_y = 95 + 150 (1 - exp(-10 eps_p)) MPa, logarithmic strain; a rigid
 sphere R = 5 mm pushed 1 mm at the centre in 10 increments against every
-top-surface node (penalty kappa A = 1e7 N/m per node, i.e. k = 1e4 N/mm over
-1 mm^2), Coulomb friction 0.1 with the exact tangent; then the release: the
-tool removed and its final force on the blank taken off as a load in 10 steps,
-with the clamp held.
+top-surface node, Coulomb friction 0.1 with the exact tangent. Each slave
+node's spring is the reference's kappa_j A_j = s E sqrt(A_j)
+(src/fem/RigidTool.cpp:298-321, include/sparlab/fem/RigidTool.hpp:28-35;
+A_j the node's share of its top faces' area, exactly a quarter of each flat
+bilinear face's, which the reference's 3 x 3 face quadrature integrates
+exactly), s = 1/7 so that an interior node's spring is k = 1e4 N/mm. Then the
+release as the reference runs one (src/fem/Forming.cpp:801-900): the tool
+removed, the clamp replaced by 3-2-1 supports (a bottom corner node fixed in
+x, y, z, a second in y, z, a third in z), and the imbalance of the formed
+state on the new free DOFs ramped out over 10 increments; at the end the
+statically determinate supports carry no more than round-off.
 
 The Newton driver is test code (the reference's driver is src/fem/
 Forming.cpp:891-1140 and stays out of the device path): residual = system
@@ -42,7 +49,7 @@ pytestmark = pytest.mark.gpu
 
 R = 0.005
 STEP = 1.0e-4          # 1 mm in 10 increments
-KAPPA_A = 1.0e7        # N/m per slave node
+PENALTY = 1.0 / 7.0    # s: kappa A = s E sqrt(A) = 1e7 N/m for a 1 mm^2 node (k = 1e4 N/mm)
 FRICTION = 0.1
 
 
@@ -57,12 +64,45 @@ def c1_model():
     return coords, conn, m, top, free
 
 
+def slave_springs(coords, conn, top, youngs):
+    """kappa_j A_j = s E_j sqrt(A_j) per top node (RigidTool.cpp:298-321):
+    A_j the sum over the top faces at the node of its shape function's
+    integral, a quarter of the face's area for a flat bilinear face."""
+    on_top = np.isclose(coords[:, 2], 0.0)
+    area = np.zeros(coords.shape[0])
+    for e in conn:
+        face = [a for a in e if on_top[a]]
+        if len(face) != 4:
+            continue
+        p = coords[face]
+        # the face's corners in order round it: a rectangle in the plane z = 0
+        w = np.ptp(p[:, 0]) * np.ptp(p[:, 1])
+        area[face] += 0.25 * w
+    a = area[top]
+    assert (a > 0).all()
+    return PENALTY * youngs * np.sqrt(a)
+
+
+def supports_321(coords):
+    """The release's free DOFs: all but those of 3-2-1 supports at bottom
+    corners: (-x, -y) fixed in x, y, z; (+x, -y) in y, z; (-x, +y) in z."""
+    x, y, z = coords[:, 0], coords[:, 1], coords[:, 2]
+    bottom = np.isclose(z, z.min())
+    def node(sx, sy):
+        return int(np.nonzero(bottom & np.isclose(x, sx * np.abs(x).max()) & np.isclose(y, sy * np.abs(y).max()))[0][0])
+    free = np.ones(3 * coords.shape[0], dtype=bool)
+    fixed = [3 * node(-1, -1) + k for k in range(3)] + [3 * node(1, -1) + 1, 3 * node(1, -1) + 2,
+                                                         3 * node(-1, 1) + 2]
+    free[fixed] = False
+    return free, np.array(fixed)
+
+
 class ElementFailure(RuntimeError):
     """An element evaluation failed (the reference's SolverError)."""
 
 
 class DeviceSide:
-    def __init__(self, coords, conn, m, top):
+    def __init__(self, coords, conn, m, top, kappa_a):
         import torch
         self.torch = torch
         self.dev = torch.device("cuda:0")
@@ -72,7 +112,7 @@ class DeviceSide:
         self.t = t
         self.d_coords = t(coords.ravel())
         self.d_nodes = t(top, torch.int32)
-        self.d_kappa = t(np.full(top.size, KAPPA_A))
+        self.d_kappa = t(kappa_a)
         self.hist = (t(np.zeros(top.size), torch.int32), t(np.zeros(3 * top.size)), t(np.zeros(3 * top.size)))
         row_ptr, _ = self.sys.pattern()
         self.first = t(self.sys.diagonal_blocks()[top], torch.int32)
@@ -140,8 +180,9 @@ class DeviceSide:
 
 
 class OracleSide:
-    def __init__(self, coords, conn, m, top):
+    def __init__(self, coords, conn, m, top, kappa_a):
         self.coords, self.conn, self.top = coords, conn, top
+        self.kappa_a = kappa_a
         self.om = oracle_material(m)
         self.nn, self.ne = coords.shape[0], conn.shape[0]
         self.states = np.zeros((self.ne, 8, 39))
@@ -161,8 +202,7 @@ class OracleSide:
         hist = self.hist
         if tool_on:
             tool = O.tool_record(1, centre, centre0, radius=R, friction=FRICTION)
-            c = O.contact(tool, self.coords, u.ravel(), start.ravel(), self.top, np.full(self.top.size, KAPPA_A),
-                          history=hist)
+            c = O.contact(tool, self.coords, u.ravel(), start.ravel(), self.top, self.kappa_a, history=hist)
             act = c["active"] == 1
             values = O.add_node_blocks(self.nn, self.conn, values, self.top[act], c["blocks"][act].reshape(-1, 9))
             self.contact_residual = c["residual"]
@@ -242,12 +282,45 @@ def height_maps(coords, top, u_d, u_o):
     return z_d.cpu().numpy(), hit_d.cpu().numpy(), z_o, hit_o
 
 
+def release_321(dev, ora, u_d, u_o, coords, outer, inner, centre0, bar, solve=None):
+    """The release (Forming.cpp:801-900, a release step of 10 increments):
+    the tool removed, the 3-2-1 supports' partition, the imbalance of the
+    formed state on its free DOFs, R_0f, ramped out: increment i solves
+    R_f(u) = (1 - i / 10) R_0f. Both sides through the same loop (the device
+    side by `solve` when given); residual histories compared while above
+    `bar` of an increment's first residual. Returns the released u of both
+    and the device side's support reactions."""
+    free, fixed = supports_321(coords)
+    r0_d, _ = dev.evaluate(u_d, centre0, centre0, u_d, tool_on=False)
+    r0_o, _ = ora.evaluate(u_o, centre0, centre0, u_o, tool_on=False)
+    assert np.abs(r0_d[free] - r0_o[free]).max() <= 1e-9 * np.abs(r0_o[free]).max()
+    r0_d[~free] = 0.0
+    r0_o[~free] = 0.0
+    for i in range(1, 11):
+        lam = 1.0 - i / 10.0
+        u_d, h_d, s_d = newton(dev, u_d, free, outer, inner, centre0, centre0, tool_on=False, load=-lam * r0_d,
+                               solve=solve)
+        u_o, h_o, s_o = newton(ora, u_o, free, outer, inner, centre0, centre0, tool_on=False, load=-lam * r0_o)
+        assert len(h_d) == len(h_o) and s_d == s_o, ("release", i, h_d, h_o, s_d, s_o)
+        for a, b in zip(h_d, h_o):
+            if b > bar * h_o[0]:
+                assert abs(a - b) <= 1e-6 * b, ("release", i, h_d, h_o)
+    # the supports are statically determinate: with no load left they carry
+    # round-off, and the part springs back free of them
+    r_d, _ = dev.evaluate(u_d, centre0, centre0, u_d, tool_on=False)
+    reactions = r_d[fixed]
+    assert np.abs(reactions).max() <= 1e-6 * np.abs(r0_d).max(), reactions
+    assert np.abs(r_d[free]).max() <= 1e-6 * np.abs(r0_d).max()
+    return u_d, u_o
+
+
 def test_c1_newton_histories_and_release_heights():
     coords, conn, m, top, free = c1_model()
     nn = coords.shape[0]
     outer, inner, _ = O.pattern(nn, conn)
-    dev = DeviceSide(coords, conn, m, top)
-    ora = OracleSide(coords, conn, m, top)
+    kappa_a = slave_springs(coords, conn, top, 70.0e9)
+    dev = DeviceSide(coords, conn, m, top, kappa_a)
+    ora = OracleSide(coords, conn, m, top, kappa_a)
     row_ptr, col_idx = dev.sys.pattern()
     assert np.array_equal(row_ptr, outer) and np.array_equal(col_idx, inner)
     u_d = np.zeros(3 * nn)
@@ -265,18 +338,10 @@ def test_c1_newton_histories_and_release_heights():
     # yielded, and the tool pressed the blank
     assert ora.states[..., 36].max() > 1e-3  # equivalent plastic strain
     assert u_o[3 * top + 2].min() < -5e-4
-    # release: the tool removed, its final force on the blank kept as a load
-    # and taken off in 10 steps, the clamp held
-    load_d, load_o = dev.contact_residual.copy(), ora.contact_residual.copy()
-    assert np.abs(load_d - load_o).max() <= 1e-9 * np.abs(load_o).max()
-    for step in range(9, -1, -1):
-        lam = step / 10.0
-        u_d, h_d, s_d = newton(dev, u_d, free, outer, inner, centre0, centre0, tool_on=False, load=lam * load_d)
-        u_o, h_o, s_o = newton(ora, u_o, free, outer, inner, centre0, centre0, tool_on=False, load=lam * load_o)
-        assert len(h_d) == len(h_o) and s_d == s_o, ("release", step, h_d, h_o, s_d, s_o)
-        for a, b in zip(h_d, h_o):
-            if b > 1e-7 * h_o[0]:
-                assert abs(a - b) <= 1e-6 * b, ("release", step, h_d, h_o)
+    # release onto 3-2-1 supports; the formerly clamped rim springs back
+    rim = np.isclose(np.abs(coords[top, 0]), 0.01) | np.isclose(np.abs(coords[top, 1]), 0.01)
+    u_d, u_o = release_321(dev, ora, u_d, u_o, coords, outer, inner, centre0, 1e-7)
+    assert np.abs(u_o[3 * top[rim] + 2]).max() > 1e-6  # the rim moved
     heights_d = coords[top, 2] + u_d[3 * top + 2]
     heights_o = coords[top, 2] + u_o[3 * top + 2]
     assert np.abs(heights_d - heights_o).max() <= 1e-9  # 1e-6 mm
@@ -284,7 +349,7 @@ def test_c1_newton_histories_and_release_heights():
     # fill) on the reference top-node grid: the device side's by the device
     # ray-cast, the oracle side's by the oracle's
     z_d, hit_d, z_o, hit_o = height_maps(coords, top, u_d, u_o)
-    assert np.array_equal(hit_d, hit_o) and hit_o.mean() > 0.9
+    assert np.array_equal(hit_d, hit_o) and hit_o.mean() > 0.75
     assert np.abs(z_d[hit_d] - z_o[hit_o]).max() <= 1e-9  # 1e-6 mm
 
 
@@ -295,7 +360,8 @@ def test_c1_with_the_device_block_solve(precond):
     the block solve to 1e-10, BiCGStab for the loading (frictional contact
     with its exact tangent makes the tangent non-symmetric; the reference then
     factorises with LU) and conjugate gradients for the release (the released
-    blank's tangent is symmetric; the reference: LLT). The oracle side keeps
+    blank's tangent is symmetric; the reference: LLT) on the 3-2-1 supports'
+    partition. The oracle side keeps
     scipy's direct solve. Bars: the same iteration counts and step lengths,
     residual norms within 1e-6 relative while above 1e-6 of the step's first
     residual (each iterative correction leaves a residual of 1e-10 of its
@@ -303,12 +369,15 @@ def test_c1_with_the_device_block_solve(precond):
     the round-off of b - K x itself is about 1e-11 of it, so a tighter
     tolerance is not always attainable), and the released heights within
     1e-6 mm. Both preconditioners: the block's diagonal and the multigrid
-    V-cycle (built on the block's symmetric part for BiCGStab)."""
+    V-cycle (built on the block's symmetric part for BiCGStab). The release
+    onto 3-2-1 supports solves the free-floating blank's block (positive
+    definite only through the six supports)."""
     coords, conn, m, top, free = c1_model()
     nn = coords.shape[0]
     outer, inner, _ = O.pattern(nn, conn)
-    dev = DeviceSide(coords, conn, m, top)
-    ora = OracleSide(coords, conn, m, top)
+    kappa_a = slave_springs(coords, conn, top, 70.0e9)
+    dev = DeviceSide(coords, conn, m, top, kappa_a)
+    ora = OracleSide(coords, conn, m, top, kappa_a)
     u_d = np.zeros(3 * nn)
     u_o = np.zeros(3 * nn)
     centre0 = (0.0, 0.0, R)
@@ -322,17 +391,9 @@ def test_c1_with_the_device_block_solve(precond):
             if b > 1e-6 * h_o[0]:
                 assert abs(a - b) <= 1e-6 * b, (inc, h_d, h_o)
         centre0 = centre
-    load_d, load_o = dev.contact_residual.copy(), ora.contact_residual.copy()
-    release_solve = dev.device_solver(free, 1e-10, outer, inner, symmetric=True, precond=precond)
-    for step in range(9, -1, -1):
-        lam = step / 10.0
-        u_d, h_d, s_d = newton(dev, u_d, free, outer, inner, centre0, centre0, tool_on=False, load=lam * load_d,
-                               solve=release_solve)
-        u_o, h_o, s_o = newton(ora, u_o, free, outer, inner, centre0, centre0, tool_on=False, load=lam * load_o)
-        assert len(h_d) == len(h_o) and s_d == s_o, ("release", step, h_d, h_o, s_d, s_o)
-        for a, b in zip(h_d, h_o):
-            if b > 1e-6 * h_o[0]:
-                assert abs(a - b) <= 1e-6 * b, ("release", step, h_d, h_o)
+    release_solve = dev.device_solver(supports_321(coords)[0], 1e-10, outer, inner, symmetric=True,
+                                      precond=precond)
+    u_d, u_o = release_321(dev, ora, u_d, u_o, coords, outer, inner, centre0, 1e-6, solve=release_solve)
     assert any(not sym for sym, _, _ in dev.solves) and any(sym for sym, _, _ in dev.solves)
     assert all(rel <= 1e-10 for _, _, rel in dev.solves)
     heights_d = coords[top, 2] + u_d[3 * top + 2]
