@@ -6,10 +6,10 @@
 // takes the row's entries l, l + 32, ... and a butterfly adds the lanes, so
 // every sum has one order. The sparse products C = X Y (pattern of C from
 // the host) give each row of C one 64-lane workgroup: the row's columns and
-// sums sit in LDS, X's entries of the row are taken in order, and the lanes
-// share out the entries of Y's matching row, finding each column's slot by
-// binary search; two entries of one row of Y never share a slot, so each
-// slot's sum runs in the order of X's entries.
+// sums sit in LDS; the (X entry, Y entry) pairs of up to 64 of the row's X
+// entries are loaded 64 at a time, each lane finding its column's slot by
+// binary search, and added in the order of X's entries; two entries of one
+// row of Y never share a slot, so each slot's sum runs in that order.
 
 #include "amg_kernels.hpp"
 
@@ -238,11 +238,13 @@ __device__ __forceinline__ int find_slot(const int* cols, int mw, int c) {
 }
 
 // C = X Y on C's pattern, one 64-lane workgroup (a single wavefront) per row
-// of C, in windows of kCap columns. X's entries of the row are read 64 at a
-// time (one per lane, with the bounds of their rows of Y) and taken in order;
-// the first 64 entries of the next entry's row of Y are loaded while the
-// current one is added. A wavefront's LDS accesses run in program order, so
-// an entry's additions see the previous entry's without a barrier.
+// of C, in windows of kCap columns. X's entries of the row are taken 64 at a
+// time: their (X entry, Y entry) pairs are numbered by a scan of their rows
+// of Y, loaded 64 at a time with each lane's slot (its column's place among
+// C's columns, by binary search), and added in the order of X's entries. A
+// wavefront's LDS accesses run in program order, so an entry's additions see
+// the previous entry's without a barrier; two entries of one row of Y never
+// share a slot, so each slot's sum runs in the order of X's entries.
 // kind kProductP: C = P, the smoothed prolongator, written as
 // T - omega D^-1 (A T) (X = A, Y = T); kProductAP: C = A P (X = A, Y = P);
 // kProductPtAP: C = the next level's A = P^T (A P) (X = P^T, Y = A P), over
@@ -276,6 +278,8 @@ __global__ __launch_bounds__(64) void product_kernel(AmgGroup G, int kind) {
   const int smoothed = kind == kProductP;
   __shared__ int cols[kCap];
   __shared__ double acc[kCap];
+  __shared__ int xs_start[64], xs_yb[64];
+  __shared__ double xs_a[64];
   const long long row = gb.block;
   if (row >= rows) return;
   const int lane = threadIdx.x;
@@ -298,38 +302,54 @@ __global__ __launch_bounds__(64) void product_kernel(AmgGroup G, int kind) {
     __syncthreads();
     const int lo_col = cols[0], hi_col = cols[mw - 1];
     for (int b0 = x0; b0 < x1; b0 += 64) {
+      // up to 64 entries of X's row: each lane loads one, with its row of Y's
+      // bounds; a scan of the rows' lengths numbers their (X entry, Y entry)
+      // pairs
       const int t = b0 + lane;
       const bool in = t < x1;
       const int k = in ? X.index[t] : 0;
       const double a_l = in ? X.values[t] : 0.0;
       const int yb_l = in ? Y.ptr[k] : 0;
-      const int ye_l = in ? Y.ptr[k + 1] : 0;
-      const int cnt = x1 - b0 < 64 ? x1 - b0 : 64;
-      int yb = __shfl(yb_l, 0, 64), ye = __shfl(ye_l, 0, 64);
-      int c_next = yb + lane < ye ? Y.index[yb + lane] : -1;
-      double v_next = yb + lane < ye ? Y.values[yb + lane] : 0.0;
-      for (int j = 0; j < cnt; ++j) {
-        const double a = __shfl(a_l, j, 64);
-        const int cb = yb, ce = ye;
-        const int c = c_next;
-        const double v = v_next;
-        if (j + 1 < cnt) {
-          yb = __shfl(yb_l, j + 1, 64);
-          ye = __shfl(ye_l, j + 1, 64);
-          c_next = yb + lane < ye ? Y.index[yb + lane] : -1;
-          v_next = yb + lane < ye ? Y.values[yb + lane] : 0.0;
-        }
-        if (c >= lo_col && c <= hi_col) {
-          const int slot = find_slot(cols, mw, c);
-          if (slot >= 0) acc[slot] = fma(a, v, acc[slot]);
-        }
-        for (int u = cb + 64 + lane; u < ce; u += 64) {  // rows of Y longer than 64
-          const int cu = Y.index[u];
-          if (cu < lo_col || cu > hi_col) continue;
-          const int slot = find_slot(cols, mw, cu);
-          if (slot >= 0) acc[slot] = fma(a, Y.values[u], acc[slot]);
-        }
+      const int len_l = in ? Y.ptr[k + 1] - yb_l : 0;
+      int incl = len_l;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
       }
+      const int total = __shfl(incl, 63, 64);
+      const int cnt = x1 - b0 < 64 ? x1 - b0 : 64;
+      xs_start[lane] = incl - len_l;
+      xs_a[lane] = a_l;
+      xs_yb[lane] = yb_l;
+      __syncthreads();
+      for (int f0 = 0; f0 < total; f0 += 64) {
+        // 64 pairs at a time: every load issued together, each lane's slot
+        // found, then the pairs added in X's order (the lanes of one X entry
+        // hold distinct columns), so each slot's sum runs in X's order
+        const int f = f0 + lane;
+        int jj = 1 << 30, slot = -1;
+        double a = 0.0, v = 0.0;
+        if (f < total) {
+          int lo = 0, hi = cnt - 1;  // the last X entry whose pairs start at or before f
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (xs_start[mid] <= f) lo = mid; else hi = mid - 1;
+          }
+          jj = lo;
+          const int u = xs_yb[lo] + (f - xs_start[lo]);
+          const int c = Y.index[u];
+          v = Y.values[u];
+          a = xs_a[lo];
+          if (c >= lo_col && c <= hi_col) slot = find_slot(cols, mw, c);
+        }
+        const int j_first = __shfl(jj, 0, 64);
+        const int last_lane = total - f0 < 64 ? total - f0 - 1 : 63;
+        const int j_last = __shfl(jj, last_lane, 64);
+        for (int j = j_first; j <= j_last; ++j)
+          if (jj == j && slot >= 0) acc[slot] = fma(a, v, acc[slot]);
+      }
+      __syncthreads();  // before the next chunk rewrites xs_*
     }
     __syncthreads();
     for (int s = lane; s < mw; s += 64) {

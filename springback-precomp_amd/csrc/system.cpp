@@ -183,7 +183,14 @@ int sparlab_system_create(const double* coords, int64_t num_nodes, const int32_t
   s->points = s->args.rule.n;
   s->fields = sbk::state_fields(s->args.mat.kin_fields);
   std::string why;
-  if (!sbk::build_pattern(num_nodes, num_elements, conn, s->pattern, &why)) {
+  bool built = false;
+  try {
+    built = sbk::build_pattern(num_nodes, num_elements, conn, s->pattern, &why);
+  } catch (const std::exception& ex) {
+    delete s;
+    return sbk::capi_fail(SPARLAB_E_DEVICE, std::string("the pattern: host failure: ") + ex.what());
+  }
+  if (!built) {
     delete s;
     return sbk::capi_fail(SPARLAB_E_MESH, why);
   }
@@ -621,23 +628,34 @@ int prepare_segment_multigrid(sparlab_system* s, const std::vector<long long>& r
         }
         const long long r0 = rows[p], r1 = rows[p + 1];
         const int v0 = ptr[static_cast<std::size_t>(r0)], v1 = ptr[static_cast<std::size_t>(r1)];
-        // the segment's rows alone: pointers from 0, columns from r0 (the
-        // segment check has shown that none leaves the segment)
-        std::vector<int> sp(static_cast<std::size_t>(r1 - r0) + 1), si(static_cast<std::size_t>(v1 - v0));
-        for (long long i = r0; i <= r1; ++i) sp[static_cast<std::size_t>(i - r0)] = ptr[static_cast<std::size_t>(i)] - v0;
-        for (int k = v0; k < v1; ++k)
-          si[static_cast<std::size_t>(k - v0)] = index[static_cast<std::size_t>(k)] - static_cast<int>(r0);
-        std::vector<long long> sf(free_dofs.begin() + r0, free_dofs.begin() + r1);
-        auto h = std::make_unique<sbk::AmgHierarchy>();
-        why[static_cast<std::size_t>(p)] = h->build_host(std::move(sp), std::move(si), sf, coords, ws);
-        built[static_cast<std::size_t>(p)] = std::move(h);
+        try {  // nothing may escape a worker thread
+          // the segment's rows alone: pointers from 0, columns from r0 (the
+          // segment check has shown that none leaves the segment)
+          std::vector<int> sp(static_cast<std::size_t>(r1 - r0) + 1), si(static_cast<std::size_t>(v1 - v0));
+          for (long long i = r0; i <= r1; ++i)
+            sp[static_cast<std::size_t>(i - r0)] = ptr[static_cast<std::size_t>(i)] - v0;
+          for (int k = v0; k < v1; ++k)
+            si[static_cast<std::size_t>(k - v0)] = index[static_cast<std::size_t>(k)] - static_cast<int>(r0);
+          std::vector<long long> sf(free_dofs.begin() + r0, free_dofs.begin() + r1);
+          auto h = std::make_unique<sbk::AmgHierarchy>();
+          why[static_cast<std::size_t>(p)] = h->build_host(std::move(sp), std::move(si), sf, coords, ws);
+          built[static_cast<std::size_t>(p)] = std::move(h);
+        } catch (const std::exception& ex) {
+          why[static_cast<std::size_t>(p)] = std::string("host failure: ") + ex.what();
+        }
       }
       if (ws) (void)hipStreamDestroy(ws);
     };
     const unsigned hw = std::thread::hardware_concurrency();
     const int nthreads = std::max(1, std::min({nseg, 16, hw > 0 ? static_cast<int>(hw) : 1}));
     std::vector<std::thread> pool;
-    for (int t = 1; t < nthreads; ++t) pool.emplace_back(work);
+    for (int t = 1; t < nthreads; ++t) {
+      try {
+        pool.emplace_back(work);
+      } catch (const std::exception&) {
+        break;  // fewer threads: the others take the remaining segments
+      }
+    }
     work();
     for (auto& t : pool) t.join();
     auto* b = new sbk::AmgBatch();
@@ -694,7 +712,12 @@ int sparlab_system_solve_free_block_device(sparlab_system* s, const double* bloc
   a.n = n;
   sbk::cg_layout(a, s->part_work);
   if (s->preconditioner == SPARLAB_PRECONDITIONER_MULTIGRID) {
-    const int prepared = prepare_multigrid(s, block_values, symmetric, st);
+    int prepared;
+    try {
+      prepared = prepare_multigrid(s, block_values, symmetric, st);
+    } catch (const std::exception& ex) {
+      return sbk::capi_fail(SPARLAB_E_DEVICE, std::string("the multigrid hierarchy: host failure: ") + ex.what());
+    }
     if (prepared != SPARLAB_OK) return prepared;
     a.external = 1;
     a.precond = sbk::amg_apply;
@@ -782,7 +805,12 @@ int sparlab_system_solve_free_block_segments_device(sparlab_system* s, const dou
     return sbk::capi_fail(SPARLAB_E_DEVICE, "the segments' check failed");
   if (bad) return sbk::capi_fail(SPARLAB_E_ARGUMENT, "the free-free block couples two segments");
   if (s->preconditioner == SPARLAB_PRECONDITIONER_MULTIGRID) {
-    const int prepared = prepare_segment_multigrid(s, rows, block_values, symmetric, st);
+    int prepared;
+    try {
+      prepared = prepare_segment_multigrid(s, rows, block_values, symmetric, st);
+    } catch (const std::exception& ex) {
+      return sbk::capi_fail(SPARLAB_E_DEVICE, std::string("the multigrid hierarchies: host failure: ") + ex.what());
+    }
     if (prepared != SPARLAB_OK) return prepared;
     a.external = 1;
     a.precond = sbk::amg_apply;
