@@ -490,16 +490,19 @@ __global__ __launch_bounds__(256) void pre_step_kernel(AmgGroup G, double* __res
   if (!group_block(G, gb)) return;
   const AmgLevel L = gb.e->L;
   double* __restrict__ x = level_x(*gb.e, x0);
+  const double* __restrict__ d0 = L.d0;
+  const double* __restrict__ dinv = L.dinv;
+  double* __restrict__ r = L.r;
   const double c1 = L.coef[kAmgC1], c2 = L.coef[kAmgC2];
   const RowLoop q = row_loop(gb);
   for (long long base = q.first; base < L.n; base += q.stride) {
     const long long i = base + q.g;
-    const double v = row_dot(L.ptr, L.index, L.values, L.d0, i, L.n, q.l);
+    const double v = row_dot(L.ptr, L.index, L.values, d0, i, L.n, q.l);
     if (q.l != 0 || i >= L.n) continue;
-    const double ri = fma(-L.dinv[i], v, L.r[i]);
-    const double d1 = fma(c1, L.d0[i], c2 * ri);
-    L.r[i] = ri;
-    x[i] = L.d0[i] + d1;
+    const double ri = fma(-dinv[i], v, r[i]);
+    const double d1 = fma(c1, d0[i], c2 * ri);
+    r[i] = ri;
+    x[i] = d0[i] + d1;
   }
 }
 
@@ -511,11 +514,12 @@ __global__ __launch_bounds__(256) void residual_kernel(AmgGroup G, const double*
   const AmgLevel L = gb.e->L;
   const double* __restrict__ b = level_b(*gb.e, b0);
   const double* __restrict__ x = level_x(*gb.e, const_cast<double*>(x0));
+  double* __restrict__ r = L.r;
   const RowLoop q = row_loop(gb);
   for (long long base = q.first; base < L.n; base += q.stride) {
     const long long i = base + q.g;
     const double v = row_dot(L.ptr, L.index, L.values, x, i, L.n, q.l);
-    if (q.l == 0 && i < L.n) L.r[i] = b[i] - v;
+    if (q.l == 0 && i < L.n) r[i] = b[i] - v;
   }
 }
 
@@ -525,10 +529,11 @@ __global__ __launch_bounds__(256) void restrict_kernel(AmgGroup G) {
   if (!group_block(G, gb)) return;
   const AmgLevel L = gb.e->L;
   double* __restrict__ nb = gb.e->N->b;
+  const double* __restrict__ r = L.r;
   const RowLoop q = row_loop(gb);
   for (long long base = q.first; base < L.nc; base += q.stride) {
     const long long i = base + q.g;
-    const double v = row_dot(L.pt_ptr, L.pt_index, L.pt_values, L.r, i, L.nc, q.l);
+    const double v = row_dot(L.pt_ptr, L.pt_index, L.pt_values, r, i, L.nc, q.l);
     if (q.l == 0 && i < L.nc) nb[i] = v;
   }
 }
@@ -556,15 +561,18 @@ __global__ __launch_bounds__(256) void post_first_kernel(AmgGroup G, const doubl
   const AmgLevel L = gb.e->L;
   const double* __restrict__ b = level_b(*gb.e, b0);
   const double* __restrict__ x = level_x(*gb.e, const_cast<double*>(x0));
+  const double* __restrict__ dinv = L.dinv;
+  double* __restrict__ r = L.r;
+  double* __restrict__ d0 = L.d0;
   const double theta_inv = L.coef[kAmgThetaInv];
   const RowLoop q = row_loop(gb);
   for (long long base = q.first; base < L.n; base += q.stride) {
     const long long i = base + q.g;
     const double v = row_dot(L.ptr, L.index, L.values, x, i, L.n, q.l);
     if (q.l != 0 || i >= L.n) continue;
-    const double ri = L.dinv[i] * (b[i] - v);
-    L.r[i] = ri;
-    L.d0[i] = theta_inv * ri;
+    const double ri = dinv[i] * (b[i] - v);
+    r[i] = ri;
+    d0[i] = theta_inv * ri;
   }
 }
 
@@ -574,16 +582,19 @@ __global__ __launch_bounds__(256) void post_step_kernel(AmgGroup G, double* __re
   if (!group_block(G, gb)) return;
   const AmgLevel L = gb.e->L;
   double* __restrict__ x = level_x(*gb.e, x0);
+  const double* __restrict__ d0 = L.d0;
+  const double* __restrict__ dinv = L.dinv;
+  double* __restrict__ r = L.r;
   const double c1 = L.coef[kAmgC1], c2 = L.coef[kAmgC2];
   const RowLoop q = row_loop(gb);
   for (long long base = q.first; base < L.n; base += q.stride) {
     const long long i = base + q.g;
-    const double v = row_dot(L.ptr, L.index, L.values, L.d0, i, L.n, q.l);
+    const double v = row_dot(L.ptr, L.index, L.values, d0, i, L.n, q.l);
     if (q.l != 0 || i >= L.n) continue;
-    const double ri = fma(-L.dinv[i], v, L.r[i]);
-    const double d1 = fma(c1, L.d0[i], c2 * ri);
-    L.r[i] = ri;
-    x[i] += L.d0[i] + d1;
+    const double ri = fma(-dinv[i], v, r[i]);
+    const double d1 = fma(c1, d0[i], c2 * ri);
+    r[i] = ri;
+    x[i] += d0[i] + d1;
   }
 }
 
