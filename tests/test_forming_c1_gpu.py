@@ -12,11 +12,12 @@ node's spring is the reference's kappa_j A_j = s E sqrt(A_j)
 A_j the node's share of its top faces' area, exactly a quarter of each flat
 bilinear face's, which the reference's 3 x 3 face quadrature integrates
 exactly), s = 1/7 so that an interior node's spring is k = 1e4 N/mm. Then the
-release as the reference runs one (src/fem/Forming.cpp:801-900): the tool
-removed, the clamp replaced by 3-2-1 supports (a bottom corner node fixed in
-x, y, z, a second in y, z, a third in z), and the imbalance of the formed
-state on the new free DOFs ramped out over 10 increments; at the end the
-statically determinate supports carry no more than round-off.
+deck's two release steps as the reference runs them (src/fem/Forming.cpp:
+801-900: a step's start imbalance on its free DOFs ramped out over 10
+increments): the unload (the tool removed, the clamp held), then the release
+onto 3-2-1 supports (a bottom corner node fixed in x, y, z, a second in y, z,
+a third in z; the clamp let go), at whose end the statically determinate
+supports carry no more than round-off.
 
 The Newton driver is test code (the reference's driver is src/fem/
 Forming.cpp:891-1140 and stays out of the device path): residual = system
@@ -282,15 +283,13 @@ def height_maps(coords, top, u_d, u_o):
     return z_d.cpu().numpy(), hit_d.cpu().numpy(), z_o, hit_o
 
 
-def release_321(dev, ora, u_d, u_o, coords, outer, inner, centre0, bar, solve=None):
-    """The release (Forming.cpp:801-900, a release step of 10 increments):
-    the tool removed, the 3-2-1 supports' partition, the imbalance of the
-    formed state on its free DOFs, R_0f, ramped out: increment i solves
-    R_f(u) = (1 - i / 10) R_0f. Both sides through the same loop (the device
-    side by `solve` when given); residual histories compared while above
-    `bar` of an increment's first residual. Returns the released u of both
-    and the device side's support reactions."""
-    free, fixed = supports_321(coords)
+def release_step(dev, ora, u_d, u_o, free, outer, inner, centre0, bar, solve=None):
+    """A release step (Forming.cpp:801-900, 10 increments, no tool): the
+    imbalance of the start state on the step's free DOFs, R_0f, ramped out:
+    increment i solves R_f(u) = (1 - i / 10) R_0f. Both sides through the
+    same loop (the device side by `solve` when given); residual histories
+    compared while above `bar` of an increment's first residual. Returns the
+    released u of both sides and the device side's start imbalance."""
     r0_d, _ = dev.evaluate(u_d, centre0, centre0, u_d, tool_on=False)
     r0_o, _ = ora.evaluate(u_o, centre0, centre0, u_o, tool_on=False)
     assert np.abs(r0_d[free] - r0_o[free]).max() <= 1e-9 * np.abs(r0_o[free]).max()
@@ -305,12 +304,22 @@ def release_321(dev, ora, u_d, u_o, coords, outer, inner, centre0, bar, solve=No
         for a, b in zip(h_d, h_o):
             if b > bar * h_o[0]:
                 assert abs(a - b) <= 1e-6 * b, ("release", i, h_d, h_o)
-    # the supports are statically determinate: with no load left they carry
-    # round-off, and the part springs back free of them
+    r_d, _ = dev.evaluate(u_d, centre0, centre0, u_d, tool_on=False)
+    assert np.abs(r_d[free]).max() <= 1e-6 * np.abs(r0_d).max()
+    return u_d, u_o, r0_d
+
+
+def unload_and_release(dev, ora, u_d, u_o, coords, clamp_free, outer, inner, centre0, bar, unload_solve=None,
+                       release_solve=None):
+    """The deck's unload (the tool removed, the clamp held) and its release
+    onto 3-2-1 supports; at the end the statically determinate supports
+    carry round-off only, and the part has sprung back free of them."""
+    u_d, u_o, _ = release_step(dev, ora, u_d, u_o, clamp_free, outer, inner, centre0, bar, unload_solve)
+    free, fixed = supports_321(coords)
+    u_d, u_o, r0_d = release_step(dev, ora, u_d, u_o, free, outer, inner, centre0, bar, release_solve)
     r_d, _ = dev.evaluate(u_d, centre0, centre0, u_d, tool_on=False)
     reactions = r_d[fixed]
     assert np.abs(reactions).max() <= 1e-6 * np.abs(r0_d).max(), reactions
-    assert np.abs(r_d[free]).max() <= 1e-6 * np.abs(r0_d).max()
     return u_d, u_o
 
 
@@ -340,7 +349,7 @@ def test_c1_newton_histories_and_release_heights():
     assert u_o[3 * top + 2].min() < -5e-4
     # release onto 3-2-1 supports; the formerly clamped rim springs back
     rim = np.isclose(np.abs(coords[top, 0]), 0.01) | np.isclose(np.abs(coords[top, 1]), 0.01)
-    u_d, u_o = release_321(dev, ora, u_d, u_o, coords, outer, inner, centre0, 1e-7)
+    u_d, u_o = unload_and_release(dev, ora, u_d, u_o, coords, free, outer, inner, centre0, 1e-7)
     assert np.abs(u_o[3 * top[rim] + 2]).max() > 1e-6  # the rim moved
     heights_d = coords[top, 2] + u_d[3 * top + 2]
     heights_o = coords[top, 2] + u_o[3 * top + 2]
@@ -360,8 +369,8 @@ def test_c1_with_the_device_block_solve(precond):
     the block solve to 1e-10, BiCGStab for the loading (frictional contact
     with its exact tangent makes the tangent non-symmetric; the reference then
     factorises with LU) and conjugate gradients for the release (the released
-    blank's tangent is symmetric; the reference: LLT) on the 3-2-1 supports'
-    partition. The oracle side keeps
+    blank's tangent is symmetric; the reference: LLT) of the unload (clamp held)
+    and of the release on the 3-2-1 supports' partition. The oracle side keeps
     scipy's direct solve. Bars: the same iteration counts and step lengths,
     residual norms within 1e-6 relative while above 1e-6 of the step's first
     residual (each iterative correction leaves a residual of 1e-10 of its
@@ -391,9 +400,13 @@ def test_c1_with_the_device_block_solve(precond):
             if b > 1e-6 * h_o[0]:
                 assert abs(a - b) <= 1e-6 * b, (inc, h_d, h_o)
         centre0 = centre
-    release_solve = dev.device_solver(supports_321(coords)[0], 1e-10, outer, inner, symmetric=True,
-                                      precond=precond)
-    u_d, u_o = release_321(dev, ora, u_d, u_o, coords, outer, inner, centre0, 1e-6, solve=release_solve)
+    unload_solve = dev.device_solver(free, 1e-10, outer, inner, symmetric=True, precond=precond)
+    u_d, u_o, _ = release_step(dev, ora, u_d, u_o, free, outer, inner, centre0, 1e-6, unload_solve)
+    release_free, fixed = supports_321(coords)
+    release_solve = dev.device_solver(release_free, 1e-10, outer, inner, symmetric=True, precond=precond)
+    u_d, u_o, r0_d = release_step(dev, ora, u_d, u_o, release_free, outer, inner, centre0, 1e-6, release_solve)
+    r_d, _ = dev.evaluate(u_d, centre0, centre0, u_d, tool_on=False)
+    assert np.abs(r_d[fixed]).max() <= 1e-6 * np.abs(r0_d).max()
     assert any(not sym for sym, _, _ in dev.solves) and any(sym for sym, _, _ in dev.solves)
     assert all(rel <= 1e-10 for _, _, rel in dev.solves)
     heights_d = coords[top, 2] + u_d[3 * top + 2]
