@@ -133,7 +133,7 @@ int sparlab_state_fields(const sparlab_material* material);
  *   yielding, alpha_in as alpha_out and state_in as state_out (its trial is
  *   rejected, as after the reference's SolverError).
  * workspace: sparlab_element_workspace_doubles(num_elements, rule) doubles
- *   of device scratch (each point's stress and tangent of its final local
+ *   of device scratch, 16-byte aligned (each point's stress and tangent of its final local
  *   pass and each element's K_aa factors, passed from the local solve to the
  *   condensation); the reference keeps the same per-call data in a
  *   thread_local workspace (src/fem/IncompatibleModes.cpp:66-71). Calls that

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -487,6 +488,9 @@ int capi_check_element_batch(const sparlab_material* material, const sparlab_rul
 int capi_launch_element_batch(int want_tangent, const sparlab_element_batch* b, double* workspace, int stages,
                               void* stream, ElementArgs& a) {
   if (b->num_elements == 0) return SPARLAB_OK;
+  // the condensation copies the workspace's records in 16-byte chunks
+  if (reinterpret_cast<std::uintptr_t>(workspace) % 16 != 0)
+    return fail(SPARLAB_E_ARGUMENT, "the workspace must be 16-byte aligned");
   a.coords = b->coords;
   a.conn = b->conn;
   a.u = b->u;

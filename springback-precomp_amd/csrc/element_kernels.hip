@@ -962,34 +962,72 @@ __global__ __launch_bounds__(256) void cost_scatter_kernel(const int* __restrict
 // (ElementArgs::tangent_column_major).
 constexpr int kRhs = 25;
 
-// One point's records as a team stages them in LDS: the geometry record
-// then the point record, lane a of the team loading entries a, a + 8, ...
+// One point's records as the teams of a wavefront stage them in LDS: the
+// geometry record (44 doubles, 22 chunks of 16 bytes) then the point record
+// (42, 21 chunks), copied straight from global memory to LDS by 16-byte
+// global_load_lds (no registers, no LDS stores): in copy j, lane a of team t
+// copies its team's chunk 8 j + a into byte 16 (8 t + a) of the stage's row
+// j (64 lanes x 16 bytes). Six copies per point; lanes past chunk 42 copy it
+// again into their own unused slots, so every lane issues the same six. Both
+// records start 16-byte aligned (the workspace is, and each record holds a
+// whole number of chunks).
 constexpr int kStagePoint = kGeomRecord + kPointRecord;
-constexpr int kStageSlice = (kStagePoint + kTeam - 1) / kTeam;
-constexpr int kStageStride = kStageSlice * kTeam;
+constexpr int kStageChunks = kStagePoint / 2;   // 43
+constexpr int kStageCopies = (kStageChunks + kTeam - 1) / kTeam;  // 6
+constexpr int kStageRow = 2 * 64;               // doubles in one copy's row
+constexpr int kStageDoubles = kStageCopies * kStageRow;
+constexpr int kStageDepth = 3;                  // points q, q + 1, q + 2
+static_assert(kGeomRecord % 2 == 0 && kPointRecord % 2 == 0, "records of whole 16-byte chunks");
 
-__device__ __forceinline__ void stage_fetch(const ElementArgs& A, long long q, int a, double v[kStageSlice]) {
+// Entry k of team t's staged records (k < kGeomRecord: the geometry record;
+// OFF = kGeomRecord reads the point record): chunk k / 2 in row k / 16, at
+// lane 8 t + (k % 16) / 2. `col` points at the team's column, stage + 16 t.
+template <int OFF>
+struct StagedRecord {
+  const double* col;
+  __device__ __forceinline__ double operator[](int k) const {
+    const int i = k + OFF;
+    return col[(i >> 4) * kStageRow + (i & 15)];
+  }
+};
+
+typedef __attribute__((address_space(1))) void GlobalVoid;
+typedef __attribute__((address_space(3))) void LdsVoid;
+
+// Lane a of its team: the six copies of point q's records into `stage`
+// (the wavefront's stage, uniform across the lanes).
+__device__ __forceinline__ void stage_issue(const ElementArgs& A, long long q, int a, double* stage) {
   const double* gr = A.geom_ws + q * kGeomRecord;
   const double* pr = A.point_ws + q * kPointRecord;
 #pragma unroll
-  for (int j = 0; j < kStageSlice; ++j) {
-    const int i = a + kTeam * j;
-    if (kTeam * (j + 1) <= kGeomRecord) v[j] = gr[i];
-    else if (kTeam * j >= kGeomRecord && kTeam * (j + 1) <= kStagePoint) v[j] = pr[i - kGeomRecord];
-    else v[j] = i < kGeomRecord ? gr[i] : (i < kStagePoint ? pr[i - kGeomRecord] : 0.0);
+  for (int j = 0; j < kStageCopies; ++j) {
+    int c = kTeam * j + a;
+    if (c >= kStageChunks) c = kStageChunks - 1;
+    const double* src = c < kGeomRecord / 2 ? gr + 2 * c : pr + 2 * (c - kGeomRecord / 2);
+    // address-space casts (C style: the generic pointers name global and LDS memory)
+    __builtin_amdgcn_global_load_lds((GlobalVoid*)(src), (LdsVoid*)(stage + j * kStageRow), 16, 0, 0);
   }
 }
 
-__device__ __forceinline__ void stage_store(double* st, int a, const double v[kStageSlice]) {
-#pragma unroll
-  for (int j = 0; j < kStageSlice; ++j) st[a + kTeam * j] = v[j];
+// Waits until at most `later` points' copies (issued after the one needed
+// now) are in flight: vmcnt <= 6 later (s_waitcnt: expcnt and lgkmcnt not
+// waited on). Stores count in vmcnt too: a point loop starts from vmcnt 0
+// and issues no other vector memory instruction.
+__device__ __forceinline__ void stage_wait(int later) {
+  static_assert(kStageCopies == 6, "the vmcnt thresholds below assume six copies per point");
+  if (later >= 2) __builtin_amdgcn_s_waitcnt(0x0F7C);       // vmcnt(12)
+  else if (later == 1) __builtin_amdgcn_s_waitcnt(0x0F76);  // vmcnt(6)
+  else __builtin_amdgcn_s_waitcnt(0x0F70);                  // vmcnt(0)
 }
+
+// Waits for the wavefront's LDS accesses (lgkmcnt(0); vmcnt and expcnt not
+// waited on): before a copy overwrites a stage the team has read.
+__device__ __forceinline__ void stage_reads_done() { __builtin_amdgcn_s_waitcnt(0xC07F); }
 
 struct CondenseSlot {
   LuSlot lu;
   LuPerm perm;
   double region[9 * kRhs];
-  double stage[2][kStageStride];  // the records of points q and q + 1
 };
 
 // One team (8 lanes) per element. Lane p first forms the geometry of point p
@@ -1004,6 +1042,12 @@ template <int KIN, bool TANGENT, bool SYM, int TW>
 __global__ __launch_bounds__(64) void condense_team_kernel(ElementArgs A) {
   static_assert(TW == kTeam, "the team condensation takes rules of up to 8 points");
   __shared__ CondenseSlot slots[teams_per_wave<TW>()];
+  // three stages as three LDS objects, each named statically where it is
+  // copied to or read: the compiler's wait before an LDS read then covers
+  // the copies into that stage only, not every copy in flight
+  __shared__ __attribute__((aligned(16))) double stage0[kStageDoubles];
+  __shared__ __attribute__((aligned(16))) double stage1[kStageDoubles];
+  __shared__ __attribute__((aligned(16))) double stage2[kStageDoubles];
   const int lane = threadIdx.x;
   const int t = lane / TW;
   const int p = lane % TW;
@@ -1060,20 +1104,41 @@ __global__ __launch_bounds__(64) void condense_team_kernel(ElementArgs A) {
   if (p == 0) lu_perm(T.lu.rt, T.lu.ct, T.perm);
   // the row blocks of the 8 nodes: lane a of the team
   const int a = p;
-  // Every point loop reads the records through the team's LDS stage: while
-  // the team works on point q from LDS, its lanes' loads of point q + 1's
-  // records (lane a: entries a, a + 8, ...) are in flight.
+  // Every point loop reads the records from the wavefront's LDS stages:
+  // while the team works on point q, the copies of the next one or two
+  // points are in flight (each lane's own; the team's lanes wait for their
+  // copies, then a fence shows them to the whole team).
   auto point_loop = [&](auto&& body) {
-    double pre[kStageSlice];
-    stage_fetch(A, q0, a, pre);
-    stage_store(T.stage[0], a, pre);
-    team_sync();
-    for (int q = 0; q < n; ++q) {
-      if (q + 1 < n) stage_fetch(A, q0 + q + 1, a, pre);
-      const double* st = T.stage[q & 1];
-      body(st, st + kGeomRecord);
-      if (q + 1 < n) stage_store(T.stage[(q + 1) & 1], a, pre);
+    stage_wait(0);  // the stores before the loop
+    stage_issue(A, q0, a, stage0);
+    if (n > 1) stage_issue(A, q0 + 1, a, stage1);
+    // Point p is copied into stage p % 3, two points ahead, over stage
+    // (p - 3) % 3, which every lane has read by then. The compiler waits for
+    // every copy in flight before the first read of stage 0 in a pass of
+    // three points (it cannot tell the stages' copies apart there), so the
+    // copy of point q + 2 is issued after point q's reads in that step: the
+    // wait then covers point q + 1's copy, issued a whole point earlier.
+    // `later`: copies issued after point q's when q is read.
+    auto step = [&](int q, const double* cur, double* ahead, bool issue_first) {
+      if (issue_first && q + 2 < n) {
+        stage_reads_done();
+        stage_issue(A, q0 + q + 2, a, ahead);
+      }
+      const int hi = issue_first ? q + 3 : q + 2;  // points issued so far
+      stage_wait((hi < n ? hi : n) - 1 - q);
       team_sync();
+      const double* col = cur + 16 * t;
+      body(StagedRecord<0>{col}, StagedRecord<kGeomRecord>{col});
+      team_sync();
+      if (!issue_first && q + 2 < n) {
+        stage_reads_done();
+        stage_issue(A, q0 + q + 2, a, ahead);
+      }
+    };
+    for (int q = 0; q < n; q += kStageDepth) {
+      step(q, stage0, stage2, false);
+      if (q + 1 < n) step(q + 1, stage1, stage0, true);
+      if (q + 2 < n) step(q + 2, stage2, stage1, true);
     }
   };
   double ru[3], kua[3][9];
@@ -1085,7 +1150,7 @@ __global__ __launch_bounds__(64) void condense_team_kernel(ElementArgs A) {
 #pragma unroll
       for (int j = 0; j < 9; ++j) kua[k][j] = 0.0;
     }
-    point_loop([&](const double* gr, const double* pr) { coupling_point<KIN>(gr, pr, a, ru, kua); });
+    point_loop([&](const auto& gr, const auto& pr) { coupling_point<KIN>(gr, pr, a, ru, kua); });
   }
   if constexpr (TANGENT) {
     // the coupling columns 3a..3a+2: K_ua^T, or K_au with
@@ -1104,7 +1169,7 @@ __global__ __launch_bounds__(64) void condense_team_kernel(ElementArgs A) {
 #pragma unroll
         for (int nn = 0; nn < 9; ++nn) { kua[k][nn] = 0.0; col[k][nn] = 0.0; }
       }
-      point_loop([&](const double* gr, const double* pr) {
+      point_loop([&](const auto& gr, const auto& pr) {
         const double w = gr[kGrW];
         double bu[6][3];
         node_operator(gr, a, bu);
@@ -1145,12 +1210,12 @@ __global__ __launch_bounds__(64) void condense_team_kernel(ElementArgs A) {
           }
         }
         if constexpr (KIN != kSmall) {
-          const double* g = gr + kGrG;
           double gsa[3];
           stress_times_gradient(gr, pr, a, gsa);
 #pragma unroll
           for (int bm = 0; bm < 3; ++bm) {
-            const double geo = gsa[0] * g[0 * 11 + 8 + bm] + gsa[1] * g[1 * 11 + 8 + bm] + gsa[2] * g[2 * 11 + 8 + bm];
+            const double geo = gsa[0] * gr[kGrG + 0 * 11 + 8 + bm] + gsa[1] * gr[kGrG + 1 * 11 + 8 + bm] +
+                               gsa[2] * gr[kGrG + 2 * 11 + 8 + bm];
             const double v = w * geo;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -1194,7 +1259,7 @@ __global__ __launch_bounds__(64) void condense_team_kernel(ElementArgs A) {
     for (int k = 0; k < 3; ++k)
 #pragma unroll
       for (int j = 0; j < 12; ++j) kuu[k][j] = 0.0;
-    point_loop([&](const double* gr, const double* pr) {
+    point_loop([&](const auto& gr, const auto& pr) {
       const double w = gr[kGrW];
       double bu[6][3];
       node_operator(gr, a, bu);
@@ -1216,13 +1281,13 @@ __global__ __launch_bounds__(64) void condense_team_kernel(ElementArgs A) {
           }
         }
       if constexpr (KIN != kSmall) {
-        const double* g = gr + kGrG;
         double gsa[3];
         stress_times_gradient(gr, pr, a, gsa);
 #pragma unroll
         for (int bb = 0; bb < 4; ++bb) {
           const int b = 4 * half + bb;
-          const double geo = gsa[0] * g[0 * 11 + b] + gsa[1] * g[1 * 11 + b] + gsa[2] * g[2 * 11 + b];
+          const double geo =
+              gsa[0] * gr[kGrG + 0 * 11 + b] + gsa[1] * gr[kGrG + 1 * 11 + b] + gsa[2] * gr[kGrG + 2 * 11 + b];
           const double v = w * geo;
 #pragma unroll
           for (int k = 0; k < 3; ++k) kuu[k][3 * bb + k] = kuu[k][3 * bb + k] + v;

@@ -22,22 +22,26 @@ constexpr int kGrF = 0;
 constexpr int kGrG = 9;
 constexpr int kGrW = 42;
 
+// The helpers below read records through R: a pointer to the record, or
+// any reader with operator[] over the record's entries (the team
+// condensation reads them from its LDS stage, StagedRecord).
+
 // B_NL(F, [G | G~]) entry (c, 3a + k) with F and G read from a geometry record.
-__device__ __forceinline__ double bnl_rec(const double* gr, int c, int a, int k) {
-  const double* F = gr + kGrF;
-  const double* g = gr + kGrG;
+template <class R>
+__device__ __forceinline__ double bnl_rec(const R& gr, int c, int a, int k) {
   switch (c) {
-    case 0: return F[k * 3 + 0] * g[0 * 11 + a];
-    case 1: return F[k * 3 + 1] * g[1 * 11 + a];
-    case 2: return F[k * 3 + 2] * g[2 * 11 + a];
-    case 3: return F[k * 3 + 0] * g[1 * 11 + a] + F[k * 3 + 1] * g[0 * 11 + a];
-    case 4: return F[k * 3 + 1] * g[2 * 11 + a] + F[k * 3 + 2] * g[1 * 11 + a];
-    default: return F[k * 3 + 2] * g[0 * 11 + a] + F[k * 3 + 0] * g[2 * 11 + a];
+    case 0: return gr[kGrF + k * 3 + 0] * gr[kGrG + 0 * 11 + a];
+    case 1: return gr[kGrF + k * 3 + 1] * gr[kGrG + 1 * 11 + a];
+    case 2: return gr[kGrF + k * 3 + 2] * gr[kGrG + 2 * 11 + a];
+    case 3: return gr[kGrF + k * 3 + 0] * gr[kGrG + 1 * 11 + a] + gr[kGrF + k * 3 + 1] * gr[kGrG + 0 * 11 + a];
+    case 4: return gr[kGrF + k * 3 + 1] * gr[kGrG + 2 * 11 + a] + gr[kGrF + k * 3 + 2] * gr[kGrG + 1 * 11 + a];
+    default: return gr[kGrF + k * 3 + 2] * gr[kGrG + 0 * 11 + a] + gr[kGrF + k * 3 + 0] * gr[kGrG + 2 * 11 + a];
   }
 }
 
 // z = B_u[:, 3a..3a+2]^T mat of one point (mat row-major in the point record).
-__device__ __forceinline__ void node_times_tangent(const double bu[6][3], const double* pr, double z[3][6]) {
+template <class R>
+__device__ __forceinline__ void node_times_tangent(const double bu[6][3], const R& pr, double z[3][6]) {
 #pragma unroll
   for (int d = 0; d < 6; ++d) {
     double mc[6];
@@ -53,7 +57,8 @@ __device__ __forceinline__ void node_times_tangent(const double bu[6][3], const 
   }
 }
 
-__device__ __forceinline__ void node_operator(const double* gr, int a, double bu[6][3]) {
+template <class R>
+__device__ __forceinline__ void node_operator(const R& gr, int a, double bu[6][3]) {
 #pragma unroll
   for (int c = 0; c < 6; ++c)
 #pragma unroll
@@ -61,23 +66,21 @@ __device__ __forceinline__ void node_operator(const double* gr, int a, double bu
 }
 
 // (S G)_a: row a of G^T S (3), S tensorial Voigt.
-__device__ __forceinline__ void stress_times_gradient(const double* gr, const double* pr, int a, double gsa[3]) {
-  const double* g = gr + kGrG;
-  const double* S = pr + kWsStress;
+template <class R, class P>
+__device__ __forceinline__ void stress_times_gradient(const R& gr, const P& pr, int a, double gsa[3]) {
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
-    const double S3j0 = j == 0 ? S[0] : (j == 1 ? S[3] : S[5]);
-    const double S3j1 = j == 0 ? S[3] : (j == 1 ? S[1] : S[4]);
-    const double S3j2 = j == 0 ? S[5] : (j == 1 ? S[4] : S[2]);
-    gsa[j] = g[0 * 11 + a] * S3j0 + g[1 * 11 + a] * S3j1 + g[2 * 11 + a] * S3j2;
+    const double S3j0 = j == 0 ? pr[kWsStress + 0] : (j == 1 ? pr[kWsStress + 3] : pr[kWsStress + 5]);
+    const double S3j1 = j == 0 ? pr[kWsStress + 3] : (j == 1 ? pr[kWsStress + 1] : pr[kWsStress + 4]);
+    const double S3j2 = j == 0 ? pr[kWsStress + 5] : (j == 1 ? pr[kWsStress + 4] : pr[kWsStress + 2]);
+    gsa[j] = gr[kGrG + 0 * 11 + a] * S3j0 + gr[kGrG + 1 * 11 + a] * S3j1 + gr[kGrG + 2 * 11 + a] * S3j2;
   }
 }
 
 // One point's share of lane a's rows of f_u and K_ua (3 x 9), from its
 // geometry record gr and point record pr.
-template <int KIN>
-__device__ __forceinline__ void coupling_point(const double* gr, const double* pr, int a, double ru[3],
-                                               double kua[3][9]) {
+template <int KIN, class R, class P>
+__device__ __forceinline__ void coupling_point(const R& gr, const P& pr, int a, double ru[3], double kua[3][9]) {
   const double w = gr[kGrW];
   double bu[6][3];
   node_operator(gr, a, bu);
@@ -105,12 +108,12 @@ __device__ __forceinline__ void coupling_point(const double* gr, const double* p
   }
   if constexpr (KIN != kSmall) {
     // geometric (G_aug^T S G_aug) (x) I: block (a, 8 + bm)
-    const double* g = gr + kGrG;
     double gsa[3];
     stress_times_gradient(gr, pr, a, gsa);
 #pragma unroll
     for (int bm = 0; bm < 3; ++bm) {
-      const double geo = gsa[0] * g[0 * 11 + 8 + bm] + gsa[1] * g[1 * 11 + 8 + bm] + gsa[2] * g[2 * 11 + 8 + bm];
+      const double geo = gsa[0] * gr[kGrG + 0 * 11 + 8 + bm] + gsa[1] * gr[kGrG + 1 * 11 + 8 + bm] +
+                         gsa[2] * gr[kGrG + 2 * 11 + 8 + bm];
       const double v = w * geo;
 #pragma unroll
       for (int k = 0; k < 3; ++k) kua[k][3 * bm + k] = kua[k][3 * bm + k] + v;
@@ -129,8 +132,11 @@ __device__ __forceinline__ void coupling_rows(const ElementArgs& A, long long q0
 #pragma unroll
     for (int j = 0; j < 9; ++j) kua[k][j] = 0.0;
   }
-  for (int q = 0; q < n; ++q)
-    coupling_point<KIN>(A.geom_ws + (q0 + q) * kGeomRecord, A.point_ws + (q0 + q) * kPointRecord, a, ru, kua);
+  for (int q = 0; q < n; ++q) {
+    const double* gr = A.geom_ws + (q0 + q) * kGeomRecord;
+    const double* pr = A.point_ws + (q0 + q) * kPointRecord;
+    coupling_point<KIN>(gr, pr, a, ru, kua);
+  }
 }
 
 }  // namespace sbk

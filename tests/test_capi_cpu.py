@@ -160,6 +160,23 @@ def test_workspace_size_mirrors_the_library():
             assert got == G.workspace_doubles(ne, points_rule.points), (ne, points_rule.points)
 
 
+def test_element_batch_refuses_a_misaligned_workspace():
+    """The condensation copies workspace records in 16-byte chunks: a caller's
+    workspace off a 16-byte boundary is refused before any device call (the
+    pointers below are never dereferenced)."""
+    import ctypes
+    lib = G.lib()
+    mat = G.Material(200e9, 0.3, yield_stress=250e6)
+    rule = G.Rule(2, 0, 2)
+    fake = 1 << 20
+    b = G.CElementBatch(*([ctypes.c_void_p(fake)] * 3), 8, 1, *([ctypes.c_void_p(fake)] * 11),
+                        ctypes.c_void_p(fake + 8), None)
+    lib.sparlab_last_error.restype = ctypes.c_char_p
+    rc = lib.sparlab_elements_evaluate_device(ctypes.byref(mat.c), ctypes.byref(rule.c), 2, 1, ctypes.byref(b), None)
+    assert rc == -1
+    assert b"16-byte aligned" in lib.sparlab_last_error()
+
+
 def test_oracle_stiffness_gross_exact_on_integers():
     """stiffness_gross (NonlinearSystem.hpp:55-67) of the oracle: with small
     integer values and displacements every product and sum is exact in FP64,
