@@ -190,6 +190,12 @@ class Material:
         self.c = c
 
     @property
+    def plastic(self):
+        """The material yields (a positive yield stress: the library's rule,
+        Plasticity::enabled in the reference): its elements are elastoplastic."""
+        return self.c.yield_stress > 0.0
+
+    @property
     def yield_matrix(self):
         return np.array(self.c.yield_matrix[:]).reshape(6, 6)
 

@@ -37,8 +37,9 @@ import os
 
 import numpy as np
 
-from . import BatchedSystem, Material, NonlinearSystem, Rule, FINITE, FINITE_LOGARITHMIC, SMALL_STRAIN
+from . import BatchedSystem, Material, NonlinearSystem, Rule, FINITE, FINITE_LOGARITHMIC, SMALL_STRAIN, lib
 from . import workloads as W
+from .results import ResultDirectory, element_values
 
 
 class DeckError(ValueError):
@@ -165,11 +166,12 @@ class DeckModel:
     (coords (n, 3), conn (e, 8), src/mesh/StructuredMesh.cpp:62-113), the
     material, the integration rule and the forming kinematics."""
 
-    def __init__(self, name, coords, conn, material, rule, kinematics, material_record):
+    def __init__(self, name, coords, conn, material, rule, kinematics, material_record, mesh_source=""):
         self.name = name
         self.coords, self.conn = coords, conn
         self.material, self.rule, self.kinematics = material, rule, kinematics
         self.material_record = material_record
+        self.mesh_source = mesh_source  # Configuration::describe_mesh (src/io/Config.cpp:346-358)
 
     def system(self):
         """The model's device NonlinearSystem."""
@@ -211,7 +213,7 @@ def deck_model(deck):
         raise DeckError("the deck has no \"material\"")
     material = deck_material(d["material"])
     return DeckModel(d.get("name", "deck"), coords, conn, material, deck_rule(model), KINEMATICS[kin],
-                     d["material"])
+                     d["material"], f"structured_hex {int(mesh['nx'])} x {int(mesh['ny'])} x {int(mesh['nz'])}")
 
 
 class DeckBatch:
@@ -266,6 +268,36 @@ class DeckBatch:
     def commit(self):
         for _, b in self.groups:
             b.commit()
+
+    def write_results(self, directories, us, step="form", termination="", **fields):
+        """Every deck's result directory in sparlab_form's output contract
+        (sparlab_gpu.results; read by python/precomp/fea/results.py:320-365):
+        one completed step named `step` with deck i's node table at us[i]
+        and its element table from the end-of-step stress recovery at us[i]
+        on the committed history (src/fem/Forming.cpp:1255-1281; each group's
+        union system recovers the stresses of all its decks at once), then
+        summary.json. `fields` go into each step's summary entry. Returns
+        the summaries."""
+        import torch
+        if len(directories) != len(self.models) or len(us) != len(self.models):
+            raise DeckError(f"{len(directories)} directories and {len(us)} displacements for "
+                            f"{len(self.models)} decks")
+        host = [(us[i].detach().cpu().numpy() if isinstance(us[i], torch.Tensor) else np.asarray(us[i],
+                                                                                              dtype=np.float64))
+                .reshape(-1) for i in range(len(self.models))]
+        version = lib().sparlab_version().decode()
+        summaries = [None] * len(self.models)
+        for members, b in self.groups:
+            stress = b.system.stress(np.concatenate([host[i] for i in members]))
+            eps, vm = element_values(stress, self.models[members[0]].material.plastic)
+            for p, i in enumerate(members):
+                m = self.models[i]
+                e0, e1 = b.element_offset[p], b.element_offset[p + 1]
+                d = ResultDirectory(directories[i], m.coords, m.conn, case=m.name, version=version,
+                                    rule_points=m.rule.points, mesh_source=m.mesh_source)
+                d.add_step(step, host[i], plastic_strain=eps[e0:e1], von_mises=vm[e0:e1], **fields)
+                summaries[i] = d.close(termination=termination)
+        return summaries
 
 
 def shape_sensitivity(system, u, values, free, dj_du, tolerance=1e-10, max_iterations=200000):

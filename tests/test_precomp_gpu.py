@@ -6,7 +6,10 @@ deck with the incompatible-mode element), two batched systems (the first two
 share material, rule and kinematics); every deck's f_int, tangent values and
 scalars equal those of its own NonlinearSystem, bit for bit (energy and
 gross within 1e-14: their sums run in another tree), over two increments
-with a commit between.
+with a commit between; then every deck's result directory written from the
+batch (write_results: node table, element table from the end-of-step stress
+recovery, summary) equals the one written from its own system, byte for
+byte.
 
 shape_sensitivity: dJ/dX at an equilibrium state (J = g . u on the free
 DOFs; a plate clamped at x = 0 with the tip's deflection prescribed, under
@@ -26,6 +29,7 @@ import pytest
 
 import sparlab_gpu as G
 from sparlab_gpu import precomp as P
+from sparlab_gpu import results as R
 from sparlab_gpu import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -63,7 +67,7 @@ def displacements(models, scale):
     return us
 
 
-def test_deck_batch_equals_each_deck_alone(device):
+def test_deck_batch_equals_each_deck_alone(device, tmp_path):
     import torch
     batch = P.DeckBatch(decks())
     assert len(batch.groups) == 2 and sorted(len(g[0]) for g in batch.groups) == [1, 2]
@@ -85,6 +89,25 @@ def test_deck_batch_equals_each_deck_alone(device):
             s.commit()
         batch.commit()
     assert scalars[0]["yielding"] > 0
+    # every deck's result directory from the batch equals the one written
+    # from its own system, file for file and byte for byte
+    dirs = [tmp_path / f"batch_{i}" for i in range(len(plain))]
+    summaries = batch.write_results(dirs, us, termination="reached the end of its window", increments=2)
+    version = G.lib().sparlab_version().decode()
+    for i, (m, s) in enumerate(zip(batch.models, plain)):
+        eps, vm = R.element_values(s.stress(us[i]), m.material.plastic)
+        d = R.ResultDirectory(tmp_path / f"plain_{i}", m.coords, m.conn, case=m.name, version=version,
+                              rule_points=m.rule.points, mesh_source=m.mesh_source)
+        d.add_step("form", us[i], plastic_strain=eps, von_mises=vm, increments=2)
+        assert d.close(termination="reached the end of its window") == summaries[i]
+        names = sorted(os.listdir(dirs[i]))
+        assert names == sorted(os.listdir(tmp_path / f"plain_{i}"))
+        assert names == ["step_1_form_elements.csv", "step_1_form_nodes.csv", "summary.json", "tool_forces.csv"]
+        for name in names:
+            assert (dirs[i] / name).read_bytes() == (tmp_path / f"plain_{i}" / name).read_bytes(), (i, name)
+    assert summaries[0]["mesh"]["source"] == "structured_hex 48 x 48 x 1"
+    rows = (dirs[0] / "step_1_form_elements.csv").read_text().split()[1:]
+    assert len(rows) == batch.models[0].conn.shape[0] and any(float(r.split(",")[1]) > 0.0 for r in rows)
 
 
 def plate():
