@@ -171,6 +171,7 @@ __device__ int return_radial(const DevMaterial& m, const double strain[6], Point
 // ---------------------------------------------------------------------------
 struct GenPoint {
   double eq;        // sigma_bar(xi)
+  double inv_eq;    // 1 / sigma_bar
   double m[6];      // P xi / sigma_bar (engineering)
   double b, db;
   double back_rate[6];
@@ -185,8 +186,11 @@ __device__ __forceinline__ void decay(const DevMaterial& mat, int i, double dl, 
     return;
   }
   if (mat.exponential) {
-    theta = exp(-gm * dl);
-    chi = -expm1(-gm * dl) / gm;
+    // exp(-gm dl) as 1 + expm1(-gm dl): one transcendental for both, theta
+    // within an absolute 1e-16 (it scales backstresses in sums of stresses)
+    const double em1 = expm1(-gm * dl);
+    theta = 1.0 + em1;
+    chi = -em1 * (1.0 / gm);  // 1 / gm: the same on every iteration
     dtheta = -gm * theta;
     dchi = theta;
   } else {
@@ -217,8 +221,11 @@ __device__ __forceinline__ bool gen_evaluate(const DevMaterial& mat, const Point
   for (int i = 1; i < 6; ++i) q = q + xi[i] * px[i];
   if (!(q > 0.0) || !isfinite(q)) return false;
   e.eq = sqrt(q);
+  // one division for the seven quotients by sigma_bar (m and, in jac_setup,
+  // r): within an ulp of dividing each
+  e.inv_eq = 1.0 / e.eq;
 #pragma unroll
-  for (int i = 0; i < 6; ++i) e.m[i] = px[i] / e.eq;
+  for (int i = 0; i < 6; ++i) e.m[i] = px[i] * e.inv_eq;
   double xt[6];
 #pragma unroll
   for (int i = 0; i < 6; ++i) { xt[i] = s_tr[i]; e.back_rate[i] = 0.0; }
@@ -303,16 +310,16 @@ __device__ __forceinline__ void a_inverse(const DevMaterial& mat, const JacSolve
 
 __device__ __forceinline__ void jac_setup(const DevMaterial& mat, const GenPoint& e, double dl, double abar,
                                           JacSolver& J) {
-  const double r = e.b / e.eq;
+  const double r = e.b * e.inv_eq;
 #pragma unroll
   for (int i = 0; i < 6; ++i) J.m[i] = e.m[i];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) J.d[k] = 1.0 / (1.0 + r * mat.L[k]);
+  for (int k = 0; k < 6; ++k) J.d[k] = recip_positive(1.0 + r * mat.L[k]);  // r >= 0, L_k >= 0
   double u[6];
 #pragma unroll
   for (int i = 0; i < 6; ++i) u[i] = r * (i < 3 ? e.m[i] : e.m[i] * 0.5);
   b_inverse(mat, J.d, u, J.z);
-  J.sm = 1.0 / (1.0 - dot6(J.m, J.z));
+  J.sm = recip_positive(1.0 - dot6(J.m, J.z));  // 1 - m^T B^-1 u > 0 (A's eigenvalues >= 1)
   double c[6];
 #pragma unroll
   for (int i = 0; i < 6; ++i) c[i] = e.back_rate[i] + e.db * (i < 3 ? e.m[i] : e.m[i] * 0.5);
