@@ -283,7 +283,7 @@ __device__ void point_parameters(const AdjointArgs& Q, const double x0[8][3], co
   double lp[6][6];
   if constexpr (KIN == kLog) log_projection(L, lp);
   JacSolver J;
-  jac_setup(mat, e, dl, st.abar, J);
+  jac_setup(mat, e, J);
   const double g2 = 2.0 * mat.shear;
   for (int k = 0; k < kAdjointParams; ++k) {
     const bool hill = k < 3;

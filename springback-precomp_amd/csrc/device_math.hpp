@@ -72,6 +72,14 @@ __device__ __forceinline__ double yield_stress(const DevMaterial& m, double a) {
 __device__ __forceinline__ double yield_slope(const DevMaterial& m, double a) {
   return m.hard + m.qsat * m.delta * exp(-m.delta * a);
 }
+// The yield stress and its slope at a from one exponential: the stress as
+// yield_stress, the slope with exp(-delta a) = 1 + expm1(-delta a) (within an
+// absolute 1e-16).
+__device__ __forceinline__ void hardening_at(const DevMaterial& m, double a, double& sy, double& slope) {
+  const double em1 = expm1(-m.delta * a);
+  sy = m.sy0 + m.hard * a + m.qsat * -em1;
+  slope = m.hard + m.qsat * m.delta * (1.0 + em1);
+}
 __device__ __forceinline__ double isotropic_energy(const DevMaterial& m, double a) {
   double out = 0.5 * m.hard * a * a;
   if (m.qsat != 0.0) out = out + m.qsat * (a + expm1(-m.delta * a) / m.delta);

@@ -174,6 +174,7 @@ struct GenPoint {
   double inv_eq;    // 1 / sigma_bar
   double m[6];      // P xi / sigma_bar (engineering)
   double b, db;
+  double slope;     // the yield stress's slope at abar + dl
   double back_rate[6];
   double res[7];
 };
@@ -223,7 +224,7 @@ __device__ __forceinline__ bool gen_evaluate(const DevMaterial& mat, const Point
   e.eq = sqrt(q);
   // one division for the seven quotients by sigma_bar (m and, in jac_setup,
   // r): within an ulp of dividing each
-  e.inv_eq = 1.0 / e.eq;
+  e.inv_eq = recip_positive(e.eq);  // sigma_bar = sqrt(q > 0) is a normal number
 #pragma unroll
   for (int i = 0; i < 6; ++i) e.m[i] = px[i] * e.inv_eq;
   double xt[6];
@@ -245,7 +246,9 @@ __device__ __forceinline__ bool gen_evaluate(const DevMaterial& mat, const Point
   }
 #pragma unroll
   for (int i = 0; i < 6; ++i) e.res[i] = xi[i] - xt[i] + e.b * (i < 3 ? e.m[i] : e.m[i] * 0.5);
-  e.res[6] = e.eq - yield_stress(mat, c.abar + dl);
+  double sy;
+  hardening_at(mat, c.abar + dl, sy, e.slope);
+  e.res[6] = e.eq - sy;
   return true;
 }
 
@@ -308,8 +311,8 @@ __device__ __forceinline__ void a_inverse(const DevMaterial& mat, const JacSolve
   for (int i = 0; i < 6; ++i) out[i] = out[i] + J.z[i] * f;
 }
 
-__device__ __forceinline__ void jac_setup(const DevMaterial& mat, const GenPoint& e, double dl, double abar,
-                                          JacSolver& J) {
+// e: gen_evaluate's at the point's (xi, dl)
+__device__ __forceinline__ void jac_setup(const DevMaterial& mat, const GenPoint& e, JacSolver& J) {
   const double r = e.b * e.inv_eq;
 #pragma unroll
   for (int i = 0; i < 6; ++i) J.m[i] = e.m[i];
@@ -324,7 +327,7 @@ __device__ __forceinline__ void jac_setup(const DevMaterial& mat, const GenPoint
 #pragma unroll
   for (int i = 0; i < 6; ++i) c[i] = e.back_rate[i] + e.db * (i < 3 ? e.m[i] : e.m[i] * 0.5);
   a_inverse(mat, J, c, J.wc);
-  J.schur = 1.0 / (dot6(J.m, J.wc) + yield_slope(mat, abar + dl));
+  J.schur = 1.0 / (dot6(J.m, J.wc) + e.slope);
 }
 
 // [x; y] = J^-1 [b1; b2]
@@ -357,7 +360,7 @@ __device__ __forceinline__ void gen_tangent(const DevMaterial& mat, const PointS
                                             double dl, double C[6][6]) {
   const double g = mat.shear;
   JacSolver J;
-  jac_setup(mat, e, dl, c.abar, J);
+  jac_setup(mat, e, J);
   elastic_C(mat.bulk, g, C);
   const double g2 = 2.0 * g;
   const double s = 2.0 * g * dl / e.eq;
@@ -468,7 +471,7 @@ __device__ __forceinline__ bool gen_newton(const DevMaterial& mat, const PointSt
     double step[7][1];
     {
       JacSolver J;
-      jac_setup(mat, e, dl, st.abar, J);
+      jac_setup(mat, e, J);
       double b1[6], x[6], y;
 #pragma unroll
       for (int i = 0; i < 6; ++i) b1[i] = -e.res[i];
@@ -503,7 +506,7 @@ __device__ __forceinline__ bool gen_newton(const DevMaterial& mat, const PointSt
     double step[7][1];
     {
       JacSolver J;
-      jac_setup(mat, e, dl, st.abar, J);
+      jac_setup(mat, e, J);
       double b1[6], x[6], y;
 #pragma unroll
       for (int i = 0; i < 6; ++i) b1[i] = -e.res[i];
