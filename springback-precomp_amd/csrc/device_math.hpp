@@ -41,6 +41,17 @@ __device__ __forceinline__ double recip_positive(double x) {
   return r;
 }
 
+// 1 / sqrt(x) for a positive normal x (here x in [1, 2]): the hardware's
+// estimate (v_rsq_f64) refined by two Newton-Raphson steps for x r^2 = 1,
+// r <- r + r (1/2 - (x/2) r^2); within an ulp of the rounded 1 / sqrt(x).
+__device__ __forceinline__ double rsqrt_positive(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  r = fma(r, fma(-h * r, r, 0.5), r);
+  r = fma(r, fma(-h * r, r, 0.5), r);
+  return r;
+}
+
 // ---------------------------------------------------------------------------
 // Material, broadcast to every lane as a kernel argument
 // ---------------------------------------------------------------------------
@@ -255,12 +266,12 @@ __device__ __forceinline__ void jacobi_rotate(double a[3][3], double v[3][3], in
     t = apq / h;
   } else {
     const double theta = 0.5 * h / apq;
-    t = 1.0 / (fabs(theta) + sqrt(1.0 + theta * theta));
+    t = recip_positive(fabs(theta) + sqrt(1.0 + theta * theta));  // argument >= 1
     if (theta < 0.0) t = -t;
   }
-  const double c = 1.0 / sqrt(1.0 + t * t);
+  const double c = rsqrt_positive(1.0 + t * t);  // |t| <= 1: argument in [1, 2]
   const double s = t * c;
-  const double tau = s / (1.0 + c);
+  const double tau = s * recip_positive(1.0 + c);
   a[p][p] = a[p][p] - t * apq;
   a[q][q] = a[q][q] + t * apq;
   a[p][q] = 0.0;
