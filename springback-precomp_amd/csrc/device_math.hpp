@@ -333,10 +333,11 @@ struct LogData {
 
 __device__ __forceinline__ double log_divided(double mx, double my) {
   if (mx > my) { double t = mx; mx = my; my = t; }
-  const double y = 1.0 + 2.0 * my;
+  const double y = 1.0 + 2.0 * my;  // a principal value of C: positive
   const double d = 2.0 * (mx - my);
-  if (d == 0.0) return 1.0 / y;
-  return log1p(d / y) / d;
+  const double iy = recip_positive(y);
+  if (d == 0.0) return iy;
+  return log1p(d * iy) / d;
 }
 
 // The principal-axes map of tensorial Voigt components, t(Q^T A Q) = M t(A).
@@ -430,16 +431,21 @@ __device__ __forceinline__ double log_divided2(const LogData& L, int a, int b, i
   const double y = 1.0 + 2.0 * mj;
   const double spread = 2.0 * (mi - mk);
   if (spread > 5.0e-3 * y) return (fij - fjk) / spread;
-  const double u = 2.0 * (mi - mj) / y;
-  const double w = 2.0 * (mk - mj) / y;
+  // the series' quotients as products with 1 / y (y = 1 + 2 mu_j, a
+  // principal value of C: positive) and with 1 / (n + 2) rounded, each within
+  // an ulp of the quotient
+  constexpr double kInv[6] = {0.0, 1.0 / 3.0, 1.0 / 4.0, 1.0 / 5.0, 1.0 / 6.0, 1.0 / 7.0};
+  const double iy = recip_positive(y);
+  const double u = 2.0 * (mi - mj) * iy;
+  const double w = 2.0 * (mk - mj) * iy;
   double h = 1.0, wn = 1.0, sum = -0.5;
 #pragma unroll
   for (int n = 1; n <= 5; ++n) {
     wn = wn * w;
     h = u * h + wn;
-    sum = sum + ((n % 2 == 1) ? 1.0 : -1.0) * h / static_cast<double>(n + 2);
+    sum = sum + ((n % 2 == 1) ? 1.0 : -1.0) * h * kInv[n];
   }
-  return sum / (y * y);
+  return sum * (iy * iy);
 }
 
 // The stress conjugate to E (S = P^T T, logarithmic_stress) and the tangent

@@ -298,7 +298,8 @@ __device__ inline bool llt9(double* w) {
         for (int j = 1; j < k; ++j) s = s + w[i * 9 + j] * w[k * 9 + j];
         w[i * 9 + k] = w[i * 9 + k] - s;
       }
-    for (int i = k + 1; i < 9; ++i) w[i * 9 + k] = w[i * 9 + k] / x;
+    const double ix = 1.0 / x;  // one division per column; each quotient within an ulp
+    for (int i = k + 1; i < 9; ++i) w[i * 9 + k] = w[i * 9 + k] * ix;
   }
   return true;
 }
