@@ -41,7 +41,7 @@ __device__ __forceinline__ double recip_positive(double x) {
   return r;
 }
 
-// 1 / sqrt(x) for a positive normal x (here x in [1, 2]): the hardware's
+// 1 / sqrt(x) for a positive normal x (not below 1e-290): the hardware's
 // estimate (v_rsq_f64) refined by two Newton-Raphson steps for x r^2 = 1,
 // r <- r + r (1/2 - (x/2) r^2); within an ulp of the rounded 1 / sqrt(x).
 __device__ __forceinline__ double rsqrt_positive(double x) {
@@ -266,7 +266,8 @@ __device__ __forceinline__ void jacobi_rotate(double a[3][3], double v[3][3], in
     t = apq / h;
   } else {
     const double theta = 0.5 * h / apq;
-    t = recip_positive(fabs(theta) + sqrt(1.0 + theta * theta));  // argument >= 1
+    const double q = 1.0 + theta * theta;  // >= 1
+    t = recip_positive(fabs(theta) + q * rsqrt_positive(q));
     if (theta < 0.0) t = -t;
   }
   const double c = rsqrt_positive(1.0 + t * t);  // |t| <= 1: argument in [1, 2]

@@ -221,10 +221,17 @@ __device__ __forceinline__ bool gen_evaluate(const DevMaterial& mat, const Point
 #pragma unroll
   for (int i = 1; i < 6; ++i) q = q + xi[i] * px[i];
   if (!(q > 0.0) || !isfinite(q)) return false;
-  e.eq = sqrt(q);
-  // one division for the seven quotients by sigma_bar (m and, in jac_setup,
-  // r): within an ulp of dividing each
-  e.inv_eq = recip_positive(e.eq);  // sigma_bar = sqrt(q > 0) is a normal number
+  // sigma_bar and its reciprocal from one inverse square root (each within an
+  // ulp); the reciprocal serves the seven quotients by sigma_bar (m and, in
+  // jac_setup, r). A q below 1e-290 (no stress state reaches it) takes the
+  // IEEE operations.
+  if (q >= 1.0e-290) {
+    e.inv_eq = rsqrt_positive(q);
+    e.eq = q * e.inv_eq;
+  } else {
+    e.eq = sqrt(q);
+    e.inv_eq = 1.0 / e.eq;
+  }
 #pragma unroll
   for (int i = 0; i < 6; ++i) e.m[i] = px[i] * e.inv_eq;
   double xt[6];
@@ -363,7 +370,7 @@ __device__ __forceinline__ void gen_tangent(const DevMaterial& mat, const PointS
   jac_setup(mat, e, J);
   elastic_C(mat.bulk, g, C);
   const double g2 = 2.0 * g;
-  const double s = 2.0 * g * dl / e.eq;
+  const double s = 2.0 * g * dl * e.inv_eq;
 #pragma unroll 1
   for (int j = 0; j < 6; ++j) {
     double b1[6];
