@@ -74,7 +74,26 @@ def build_adapter_test(verbose=False):
     return ADAPTER_TEST
 
 
+MATH_PROBE = os.path.join(ROOT, "tests", "cpp", "_build", "libdevice_math_probe.so")
+
+
+def build_math_probe(verbose=False):
+    """The test probe of tests/cpp/device_math_probe.hip (the device math
+    helpers beside the IEEE operations), with the library's device flags."""
+    os.makedirs(os.path.dirname(MATH_PROBE), exist_ok=True)
+    tmp = MATH_PROBE + ".tmp"
+    cmd = [HIPCC, *FLAGS, "-shared", "-o", tmp, os.path.join(ROOT, "tests", "cpp", "device_math_probe.hip")]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"the device math probe did not build:\n{res.stderr}")
+    os.replace(tmp, MATH_PROBE)
+    if verbose:
+        print(f"built {MATH_PROBE}")
+    return MATH_PROBE
+
+
 if __name__ == "__main__":
     build(verbose=True)
     build_adapter_test(verbose=True)
+    build_math_probe(verbose=True)
     sys.exit(0)
