@@ -53,7 +53,7 @@ __device__ __forceinline__ double* level_x(const AmgGroupEntry& e, double* x) {
 }
 
 constexpr int kWaves = 4;
-constexpr int kLanesPerRow = 32;
+constexpr int kLanesPerRow = 16;
 constexpr int kGroups = 64 / kLanesPerRow;
 constexpr int kRowsPerBlock = kWaves * kGroups;
 constexpr int kCap = 512;  // columns of C a product workgroup holds at a time
@@ -61,12 +61,12 @@ constexpr int kCap = 512;  // columns of C a product workgroup holds at a time
 unsigned int grid_of(long long n, long long per) { return static_cast<unsigned int>((n + per - 1) / per); }
 
 // sum over the row of values[k] x[index[k]] (a row >= n sums to 0), in every
-// lane of the row's 32-lane group: lane l adds entries l, l + 32, ... in
+// lane of the row's 16-lane group: lane l adds entries l, l + 16, ... in
 // order, then a butterfly over the group. The lane's first kSlots entries
 // have every index and value load issued before their gathers of x, and
 // those before the sums (a row of level 0 holds at most 81 entries); longer
 // rows finish in the loop after, in the same order.
-constexpr int kSlots = 3;
+constexpr int kSlots = 4;
 __device__ __forceinline__ double row_dot(const int* __restrict__ ptr, const int* __restrict__ index,
                                           const double* __restrict__ values, const double* __restrict__ x,
                                           long long row, long long n, int l) {
