@@ -22,7 +22,7 @@ namespace sbk {
 namespace {
 
 constexpr int kWaves = 4;          // wavefronts per product block
-constexpr int kLanesPerRow = 32;   // a row's lanes in the product (rows of the block hold up to 81 entries)
+constexpr int kLanesPerRow = 16;   // a row's lanes in the product (rows of the block hold up to 81 entries)
 constexpr int kGroups = 64 / kLanesPerRow;  // rows per wavefront at a time
 constexpr int kSlots = (81 + kLanesPerRow - 1) / kLanesPerRow;  // a lane's entries of a full row
 
