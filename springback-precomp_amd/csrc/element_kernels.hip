@@ -1342,9 +1342,9 @@ constexpr int kChunk = 2;
 constexpr int kAug = 33;  // 11 augmented nodes x 3 components
 constexpr int kOps = kChunk * 6 * kAug;
 
-template <int NP>
+// The geometry records (per point: F (9), [G | G~] (33), w) are the
+// workgroup's dynamic LDS, rule.n of them (the launch's size), after these.
 struct CondenseLds {
-  double geo[NP][kGeomRecord];  // per point: F (9), [G | G~] (33), w
   double ops[2 * kOps];         // B_q, then W_q, of the chunk; afterwards K* (576) and K_ua (216)
   double rec[kChunk][kPointRecord];  // the chunk's point records: S, mat
   double sg[kChunk][kAug];      // w_q S_q g_b, b = 0..10
@@ -1355,8 +1355,9 @@ struct CondenseLds {
 
 template <int KIN, bool TANGENT, bool SYM, int TW>
 __global__ __launch_bounds__(64) void condense_wide_kernel(ElementArgs A) {
-  constexpr int NP = kMaxPoints;
-  __shared__ CondenseLds<NP> L;
+  __shared__ CondenseLds L;
+  extern __shared__ __attribute__((aligned(16))) double condense_wide_geo[];
+  double(*const geos)[kGeomRecord] = reinterpret_cast<double(*)[kGeomRecord]>(condense_wide_geo);
   const int l = threadIdx.x;
   const long long e = blockIdx.x;
   if (e >= A.ne) return;
@@ -1385,7 +1386,7 @@ __global__ __launch_bounds__(64) void condense_wide_kernel(ElementArgs A) {
 #pragma unroll
       for (int j = 0; j < 3; ++j) gm[i][j] = pg.g[i][8 + j];
     mode_kinematics<KIN>(pg.hu, gm, alpha, h, F);
-    double* gr = L.geo[l];
+    double* gr = geos[l];
 #pragma unroll
     for (int i = 0; i < 9; ++i) gr[kGrF + i] = F[i / 3][i % 3];
 #pragma unroll
@@ -1428,7 +1429,7 @@ __global__ __launch_bounds__(64) void condense_wide_kernel(ElementArgs A) {
       L.rec[t / kPointRecord][t % kPointRecord] = A.point_ws[(q0 + c0) * kPointRecord + t];
     for (int t = l; t < cn * 6 * kAug; t += 64) {
       const int q = t / (6 * kAug), r = t % (6 * kAug);
-      B[t] = bnl_rec(L.geo[c0 + q], r / kAug, (r % kAug) / 3, (r % kAug) % 3);
+      B[t] = bnl_rec(geos[c0 + q], r / kAug, (r % kAug) / 3, (r % kAug) % 3);
     }
     __syncthreads();
     // W_q = w_q mat_q B_q (mat row-major in the record), w_q S_q g_b
@@ -1439,14 +1440,14 @@ __global__ __launch_bounds__(64) void condense_wide_kernel(ElementArgs A) {
       double s = m[0] * bq[0];
 #pragma unroll
       for (int d = 1; d < 6; ++d) s = s + m[d] * bq[d * kAug];
-      W[t] = L.geo[c0 + q][kGrW] * s;
+      W[t] = geos[c0 + q][kGrW] * s;
     }
     if constexpr (KIN != kSmall) {
       for (int t = l; t < cn * 11; t += 64) {
         const int q = t / 11, node = t % 11;
         double gsa[3];
-        stress_times_gradient(L.geo[c0 + q], L.rec[q], node, gsa);
-        const double w = L.geo[c0 + q][kGrW];
+        stress_times_gradient(geos[c0 + q], L.rec[q], node, gsa);
+        const double w = geos[c0 + q][kGrW];
 #pragma unroll
         for (int j = 0; j < 3; ++j) L.sg[q][3 * node + j] = w * gsa[j];
       }
@@ -1455,7 +1456,7 @@ __global__ __launch_bounds__(64) void condense_wide_kernel(ElementArgs A) {
     for (int q = 0; q < cn; ++q) {
       const double* bq = B + q * 6 * kAug;
       const double* wq = W + q * 6 * kAug;
-      const double* g = L.geo[c0 + q] + kGrG;
+      const double* g = geos[c0 + q] + kGrG;
       if constexpr (TANGENT) {
         double s[3][3];
 #pragma unroll
@@ -1503,7 +1504,7 @@ __global__ __launch_bounds__(64) void condense_wide_kernel(ElementArgs A) {
         }
       }
       if (fu) {
-        const double w = L.geo[c0 + q][kGrW];
+        const double w = geos[c0 + q][kGrW];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
           double s = bq[3 * fa + i] * L.rec[q][kWsStress];
@@ -1841,7 +1842,9 @@ void launch_condense_one(const ElementArgs& a, hipStream_t stream) {
     hipLaunchKernelGGL((condense_team_kernel<KIN, TANGENT, SYM, TW>), dim3(blocks), dim3(64), 0, stream, a);
   } else {
     const unsigned int blocks = static_cast<unsigned int>(a.ne);  // an element per workgroup
-    hipLaunchKernelGGL((condense_wide_kernel<KIN, TANGENT, SYM, TW>), dim3(blocks), dim3(64), 0, stream, a);
+    // the geometry records in dynamic LDS: the rule's points, not the most a rule may hold
+    const size_t geo_bytes = static_cast<size_t>(a.rule.n) * kGeomRecord * sizeof(double);
+    hipLaunchKernelGGL((condense_wide_kernel<KIN, TANGENT, SYM, TW>), dim3(blocks), dim3(64), geo_bytes, stream, a);
   }
 }
 
