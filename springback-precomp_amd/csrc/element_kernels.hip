@@ -1234,10 +1234,7 @@ __global__ __launch_bounds__(64) void condense_team_kernel(ElementArgs A) {
   team_sync();  // the permutations and the right-hand sides
   // K_aa^-1 [K_au | r_a] in place: lane a its three columns, lane 0 also r_a
   if (p == 0) lu_solve_column(T.lu.a, T.perm, T.region, kRhs, 24);
-  if constexpr (TANGENT) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) lu_solve_column(T.lu.a, T.perm, T.region, kRhs, 3 * a + k);
-  }
+  if constexpr (TANGENT) lu_solve_column3(T.lu.a, T.perm, T.region, kRhs, 3 * a);
   team_sync();
   // f* = f_u - K_ua K_aa^-1 r_a
 #pragma unroll

@@ -240,6 +240,44 @@ __device__ inline void lu_solve_column(const double* f, const LuPerm& P, double*
   for (int j = 0; j < 9; ++j) m[P.qcol[j] * ld + col] = c[j];
 }
 
+// lu_solve_column on the three adjacent columns col, col + 1, col + 2: each
+// column's operations the same and in the same order, but every
+// column loaded before any is stored, so that the three substitution chains
+// overlap (stores through the permutations would otherwise order each
+// column's loads after the previous column's stores) and each factor entry
+// is read once for the three.
+__device__ inline void lu_solve_column3(const double* f, const LuPerm& P, double* m, int ld, int col) {
+  double c[3][9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c[k][i] = m[P.prow[i] * ld + col + k];
+#pragma unroll
+  for (int i = 1; i < 9; ++i)
+#pragma unroll
+    for (int kk = 0; kk < i; ++kk) {
+      const double fik = f[i * 9 + kk];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) c[k][i] = c[k][i] - fik * c[k][kk];
+    }
+#pragma unroll
+  for (int i = 8; i >= 0; --i) {
+#pragma unroll
+    for (int kk = i + 1; kk < 9; ++kk) {
+      const double fik = f[i * 9 + kk];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) c[k][i] = c[k][i] - fik * c[k][kk];
+    }
+    const double d = f[i * 9 + i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c[k][i] = c[k][i] / d;
+  }
+#pragma unroll
+  for (int j = 0; j < 9; ++j)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) m[P.qcol[j] * ld + col + k] = c[k][j];
+}
+
 // Solve A^T x = b with the factors of A (P A Q = L U, the row
 // transpositions rt applied in order, the column ones ct): x = P^T L^-T U^-T
 // Q^T b (every lane, redundantly).
